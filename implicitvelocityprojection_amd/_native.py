@@ -1,0 +1,120 @@
+"""ctypes binding of libivp.so (the C ABI declared in include/ivp.h).
+
+The library must have been built (``python -m implicitvelocityprojection_amd.build``); there is
+no fallback: a missing or unloadable library raises ImportError.
+"""
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libivp.so")
+
+IVP_OK = 0
+IVP_NOT_CONVERGED = 1
+IVP_NOT_PD = 2
+IVP_NEIGHBOR_OVERFLOW = 3
+IVP_ERR_ARG = -1
+IVP_ERR_HIP = -2
+IVP_ERR_CAPACITY = -3
+IVP_ERR_STATE = -4
+IVP_ERR_GRID = -5
+
+FIELD_X, FIELD_V, FIELD_DENSITY, FIELD_K, FIELD_DIVERGENCE = 0, 1, 2, 3, 4
+
+# kernel classes of ivp_profile_read / ivp_profile_enable masks
+KCLASSES = ["sort", "build", "ip_pack", "ip_rhs", "cg_J", "cg_JT", "cg_update", "cg_p", "visc", "w18_Av",
+            "w18_update", "w18_p", "w18_pre", "glue"]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("support_radius", ctypes.c_float), ("domain_start", ctypes.c_float * 3),
+                ("domain_size", ctypes.c_float * 3), ("cache_size", ctypes.c_int), ("material_fluid", ctypes.c_int)]
+
+
+class IPParams(ctypes.Structure):
+    _fields_ = [("omega", ctypes.c_float), ("tol_opt", ctypes.c_double), ("max_iteration_opt", ctypes.c_int),
+                ("use_fixed_iters", ctypes.c_int), ("tol_pcg", ctypes.c_double), ("max_iteration_pcg", ctypes.c_int)]
+
+
+class IPStats(ctypes.Structure):
+    _fields_ = [("capacity", ctypes.c_int), ("density_errors", ctypes.POINTER(ctypes.c_double)),
+                ("pcg_iters", ctypes.POINTER(ctypes.c_int)), ("n_density_errors", ctypes.c_int),
+                ("n_pcg", ctypes.c_int), ("outer_iters", ctypes.c_int), ("final_density_error", ctypes.c_double),
+                ("pAp_bad", ctypes.c_double), ("pcg_iter_bad", ctypes.c_int), ("pairs_traversed", ctypes.c_longlong),
+                ("rows_traversed", ctypes.c_longlong), ("cg_launch_chunks", ctypes.c_int)]
+
+
+class CGStats(ctypes.Structure):
+    _fields_ = [("cg_iters", ctypes.c_int), ("residual_norm2", ctypes.c_double)]
+
+
+class StepParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_float), ("gravity", ctypes.c_float * 3), ("viscosity_method", ctypes.c_int),
+                ("viscosity_coeff", ctypes.c_float), ("ip", IPParams)]
+
+
+class KernelTiming(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_longlong), ("total_ms", ctypes.c_double),
+                ("bytes", ctypes.c_double)]
+
+
+# every symbol include/ivp.h declares, with its ctypes signature
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+SIGNATURES = {
+    "ivp_version": (ctypes.c_char_p, []),
+    "ivp_status_string": (ctypes.c_char_p, [_I]),
+    "ivp_create": (_I, [ctypes.POINTER(Config), _I, _P, ctypes.POINTER(_P)]),
+    "ivp_destroy": (_I, [_P]),
+    "ivp_set_particles": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "ivp_write": (_I, [_P, _I, _P]),
+    "ivp_read": (_I, [_P, _I, _P]),
+    "ivp_neighbors_build": (_I, [_P]),
+    "ivp_neighbors_export": (_I, [_P, _I, _P, _P, _P]),
+    "ivp_ip_solve": (_I, [_P, _F, ctypes.POINTER(IPParams), ctypes.POINTER(IPStats)]),
+    "ivp_viscosity_explicit": (_I, [_P, _F, _F]),
+    "ivp_viscosity_weiler18": (_I, [_P, _F, _F, ctypes.POINTER(CGStats)]),
+    "ivp_apply_gravity": (_I, [_P, ctypes.POINTER(_F), _F]),
+    "ivp_advect_boundary": (_I, [_P, _F]),
+    "ivp_step": (_I, [_P, ctypes.POINTER(StepParams), ctypes.POINTER(IPStats), ctypes.POINTER(CGStats)]),
+    "ivp_profile_enable": (_I, [_P, _I]),
+    "ivp_profile_read": (_I, [_P, ctypes.POINTER(KernelTiming), _I]),
+    "ivp_profile_reset": (_I, [_P]),
+    "ivp_counts": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_longlong)]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libivp.so and bind every entry point.  Raises ImportError if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m implicitvelocityprojection_amd.build`")
+    try:
+        lib = ctypes.CDLL(LIB_PATH)
+    except OSError as e:
+        raise ImportError(f"cannot load {LIB_PATH}: {e}") from e
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class IVPError(RuntimeError):
+    def __init__(self, status, where=""):
+        self.status = status
+        msg = load().ivp_status_string(status).decode()
+        super().__init__(f"{where}: {msg} (status {status})" if where else f"{msg} (status {status})")
+
+
+def check(status, where=""):
+    if status < 0:
+        raise IVPError(status, where)
+    return status

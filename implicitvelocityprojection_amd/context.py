@@ -1,0 +1,191 @@
+"""Thin object wrapper over the libivp C ABI, taking torch device tensors as buffers.
+
+torch is used only for device memory and the stream: every array handed to the library is a
+contiguous CUDA (ROCm) tensor whose ``data_ptr()`` goes through ctypes unchanged.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("libivp buffers must be device tensors")
+    if not t.is_contiguous():
+        raise ValueError("libivp buffers must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class IPParams:
+    """IP solver knobs (pressure_base.py:314-317, ip.py:228-229)."""
+
+    def __init__(self, omega=10.0, tol_opt=1e-3, max_iteration_opt=1000, use_fixed_iters=False, tol_pcg=1e-6,
+                 max_iteration_pcg=1000):
+        self.omega = float(omega)
+        self.tol_opt = float(tol_opt)
+        self.max_iteration_opt = int(max_iteration_opt)
+        self.use_fixed_iters = bool(use_fixed_iters)
+        self.tol_pcg = float(tol_pcg)
+        self.max_iteration_pcg = int(max_iteration_pcg)
+
+    def c(self):
+        return N.IPParams(self.omega, self.tol_opt, self.max_iteration_opt, int(self.use_fixed_iters), self.tol_pcg,
+                          self.max_iteration_pcg)
+
+
+class IPResult:
+    """Statistics of one IP solve."""
+
+    def __init__(self, status, st, errs, its):
+        self.status = status
+        self.density_errors = [float(e) for e in errs[:min(st.n_density_errors, len(errs))]]
+        self.pcg_iters = [int(i) for i in its[:min(st.n_pcg, len(its))]]
+        self.outer_iters = st.outer_iters
+        self.final_density_error = st.final_density_error
+        self.pAp_bad = st.pAp_bad
+        self.pcg_iter_bad = st.pcg_iter_bad
+        self.pairs_traversed = st.pairs_traversed
+        self.rows_traversed = st.rows_traversed
+        self.cg_launch_chunks = st.cg_launch_chunks
+
+    @property
+    def last_err(self):
+        return self.density_errors[-1] if self.density_errors else 0.0
+
+
+class Context:
+    """One libivp context: a particle store plus neighbour rows and solver buffers."""
+
+    def __init__(self, support_radius, domain_start, domain_size, max_particles, cache_size=60, material_fluid=1,
+                 stream=None):
+        self.lib = N.load()
+        cfg = N.Config()
+        cfg.support_radius = float(np.float32(support_radius))
+        cfg.domain_start[:] = [float(v) for v in np.asarray(domain_start, np.float32)]
+        cfg.domain_size[:] = [float(v) for v in np.asarray(domain_size, np.float32)]
+        cfg.cache_size = int(cache_size)
+        cfg.material_fluid = int(material_fluid)
+        self.cache_size = int(cache_size)
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        h = ctypes.c_void_p()
+        N.check(self.lib.ivp_create(ctypes.byref(cfg), int(max_particles), ctypes.c_void_p(self.stream.cuda_stream),
+                                    ctypes.byref(h)), "ivp_create")
+        self.h = h
+        self.max_particles = int(max_particles)
+
+    # -- lifecycle ---------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ivp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def counts(self):
+        n, nd = ctypes.c_int(), ctypes.c_int()
+        N.check(self.lib.ivp_counts(self.h, ctypes.byref(n), ctypes.byref(nd), None))
+        return n.value, nd.value
+
+    def pair_count(self):
+        """Dynamic-dynamic neighbour pairs stored by the last build."""
+        p = ctypes.c_longlong()
+        N.check(self.lib.ivp_counts(self.h, None, None, ctypes.byref(p)))
+        return p.value
+
+    # -- particle store ----------------------------------------------------------------------
+    def set_particles(self, x, v, m, m_inv, density0, is_dynamic, material):
+        n = int(x.shape[0])
+        N.check(self.lib.ivp_set_particles(self.h, n, _ptr(x), _ptr(v), _ptr(m), _ptr(m_inv), _ptr(density0),
+                                           _ptr(is_dynamic), _ptr(material)), "ivp_set_particles")
+
+    def write(self, field, t):
+        N.check(self.lib.ivp_write(self.h, field, _ptr(t)), "ivp_write")
+
+    def read(self, field, out):
+        N.check(self.lib.ivp_read(self.h, field, _ptr(out)), "ivp_read")
+        return out
+
+    # -- neighbours ---------------------------------------------------------------------------
+    def neighbors_build(self):
+        """Returns the status (IVP_OK or IVP_NEIGHBOR_OVERFLOW)."""
+        return N.check(self.lib.ivp_neighbors_build(self.h), "ivp_neighbors_build")
+
+    def neighbors_export(self, count, ids, gradW, cache_size=None):
+        C = self.cache_size if cache_size is None else int(cache_size)
+        N.check(self.lib.ivp_neighbors_export(self.h, C, _ptr(count), _ptr(ids), _ptr(gradW)), "ivp_neighbors_export")
+
+    # -- solvers -------------------------------------------------------------------------------
+    def _stats(self, cap):
+        errs = np.zeros(cap, np.float64)
+        its = np.zeros(cap, np.int32)
+        st = N.IPStats()
+        st.capacity = cap
+        st.density_errors = errs.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        st.pcg_iters = its.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        return st, errs, its
+
+    def ip_solve(self, dt, params: IPParams):
+        st, errs, its = self._stats(params.max_iteration_opt + 2)
+        prm = params.c()
+        status = N.check(self.lib.ivp_ip_solve(self.h, ctypes.c_float(dt), ctypes.byref(prm), ctypes.byref(st)),
+                         "ivp_ip_solve")
+        return IPResult(status, st, errs, its)
+
+    def viscosity_explicit(self, coeff, dt):
+        N.check(self.lib.ivp_viscosity_explicit(self.h, ctypes.c_float(coeff), ctypes.c_float(dt)),
+                "ivp_viscosity_explicit")
+
+    def viscosity_weiler18(self, coeff, dt):
+        cs = N.CGStats()
+        N.check(self.lib.ivp_viscosity_weiler18(self.h, ctypes.c_float(coeff), ctypes.c_float(dt), ctypes.byref(cs)),
+                "ivp_viscosity_weiler18")
+        return cs.cg_iters
+
+    def apply_gravity(self, g, dt):
+        ga = (ctypes.c_float * 3)(*[float(v) for v in g])
+        N.check(self.lib.ivp_apply_gravity(self.h, ga, ctypes.c_float(dt)), "ivp_apply_gravity")
+
+    def advect_boundary(self, dt):
+        N.check(self.lib.ivp_advect_boundary(self.h, ctypes.c_float(dt)), "ivp_advect_boundary")
+
+    def step(self, dt, gravity, viscosity_method, viscosity_coeff, params: IPParams):
+        sp = N.StepParams()
+        sp.dt = float(dt)
+        sp.gravity[:] = [float(v) for v in gravity]
+        sp.viscosity_method = int(viscosity_method)
+        sp.viscosity_coeff = float(viscosity_coeff)
+        sp.ip = params.c()
+        st, errs, its = self._stats(params.max_iteration_opt + 2)
+        cs = N.CGStats()
+        status = N.check(self.lib.ivp_step(self.h, ctypes.byref(sp), ctypes.byref(st), ctypes.byref(cs)), "ivp_step")
+        return IPResult(status, st, errs, its), cs.cg_iters
+
+    # -- instrumentation ----------------------------------------------------------------------
+    def profile_enable(self, mask=-1):
+        N.check(self.lib.ivp_profile_enable(self.h, int(mask)))
+
+    def profile_reset(self):
+        N.check(self.lib.ivp_profile_reset(self.h))
+
+    def profile_read(self):
+        arr = (N.KernelTiming * len(N.KCLASSES))()
+        n = N.check(self.lib.ivp_profile_read(self.h, arr, len(N.KCLASSES)))
+        return {arr[i].name.decode(): dict(launches=arr[i].launches, total_ms=arr[i].total_ms, bytes=arr[i].bytes)
+                for i in range(n)}
+
+
+def kclass_mask(*names):
+    m = 0
+    for nm in names:
+        m |= 1 << N.KCLASSES.index(nm)
+    return m

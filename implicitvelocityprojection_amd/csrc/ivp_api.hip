@@ -1,0 +1,374 @@
+// ivp_api.hip -- the extern "C" entry points of libivp.so (declared in include/ivp.h):
+// context lifecycle, device memory, the particle store interface and instrumentation.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "ivp_internal.h"
+
+using namespace ivp;
+
+int ivp_hip_fail(hipError_t e, const char* what, const char* file, int line) {
+    if (getenv("IVP_VERBOSE")) fprintf(stderr, "libivp: %s failed at %s:%d: %s\n", what, file, line, hipGetErrorString(e));
+    return IVP_ERR_HIP;
+}
+
+namespace {
+
+template <class T>
+int dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    IVP_HIP(hipMalloc((void**)p, count * sizeof(T)));
+    return IVP_OK;
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void free_all(ivp_ctx* c) {
+    dfree(c->pos); dfree(c->pos_b); dfree(c->vel); dfree(c->vel_b); dfree(c->minv); dfree(c->minv_b);
+    dfree(c->rho0); dfree(c->rho0_b); dfree(c->mat); dfree(c->mat_b); dfree(c->uid); dfree(c->uid_b);
+    dfree(c->inv); dfree(c->density); dfree(c->kk); dfree(c->Ss); dfree(c->divg); dfree(c->divg_b);
+    dfree(c->rowcnt); dfree(c->rowtot); dfree(c->nb); dfree(c->tile_len); dfree(c->dstart); dfree(c->sstart);
+    dfree(c->ccount); dfree(c->key); dfree(c->tmp_src); dfree(c->order); dfree(c->scan_bsum);
+    dfree(c->d4); dfree(c->q4); dfree(c->b); dfree(c->sol); dfree(c->r); dfree(c->Ap); dfree(c->p4); dfree(c->y2);
+    dfree(c->cc); dfree(c->sfac); dfree(c->nbact);
+    dfree(c->w_pinv); dfree(c->w_p4); dfree(c->w_v4); dfree(c->w_dv); dfree(c->w_r); dfree(c->w_z);
+    dfree(c->w_Ap); dfree(c->w_rhs); dfree(c->w_fac); dfree(c->v4);
+    dfree(c->partials); dfree(c->state); dfree(c->err_flag);
+    if (c->state_host) (void)hipHostFree(c->state_host);
+    if (c->err_host) (void)hipHostFree(c->err_host);
+    if (c->count_host) (void)hipHostFree(c->count_host);
+    c->state_host = nullptr;
+    c->err_host = nullptr;
+    c->count_host = nullptr;
+    for (auto& e : c->prof_pending) {
+        (void)hipEventDestroy(e.start);
+        (void)hipEventDestroy(e.stop);
+    }
+    c->prof_pending.clear();
+    for (auto e : c->prof_free) (void)hipEventDestroy(e);
+    c->prof_free.clear();
+}
+
+int alloc_all(ivp_ctx* c) {
+    const size_t cap = (size_t)c->cap;
+    const size_t tiles = (size_t)blocks_for(c->cap) * WAVES_PER_BLOCK;
+    c->nd_alloc_tiles = (int)tiles;
+    int rc = IVP_OK;
+#define A(p, n) if ((rc = dalloc(&c->p, (n))) != IVP_OK) return rc
+    A(pos, cap); A(pos_b, cap); A(vel, 3 * cap); A(vel_b, 3 * cap); A(minv, cap); A(minv_b, cap);
+    A(rho0, cap); A(rho0_b, cap); A(mat, cap); A(mat_b, cap); A(uid, cap); A(uid_b, cap); A(inv, cap);
+    A(density, cap); A(kk, cap); A(Ss, 3 * cap); A(divg, cap); A(divg_b, cap); A(rowcnt, cap); A(rowtot, cap);
+    A(tile_len, tiles);
+    A(dstart, (size_t)c->ncell + 1); A(sstart, (size_t)c->ncell + 1); A(ccount, (size_t)c->ncell);
+    A(key, cap); A(tmp_src, cap + 1); A(order, cap);
+    size_t scan_n = std::max((size_t)c->ncell, cap) + 1;
+    c->nscan_blocks = (int)((scan_n + 2047) / 2048) + 2;
+    A(scan_bsum, (size_t)c->nscan_blocks + 1);
+    A(d4, cap); A(q4, cap); A(b, 3 * cap); A(sol, 3 * cap); A(r, 3 * cap); A(Ap, 3 * cap); A(p4, cap); A(y2, cap);
+    A(cc, cap); A(sfac, cap); A(nbact, cap);
+    A(w_pinv, 9 * cap); A(w_p4, cap); A(w_v4, cap); A(w_dv, 3 * cap); A(w_r, 3 * cap); A(w_z, 3 * cap);
+    A(w_Ap, 3 * cap); A(w_rhs, 3 * cap); A(w_fac, cap); A(v4, cap);
+    c->max_blocks = blocks_for(c->cap) + 1;
+    A(partials, 4 * (size_t)c->max_blocks);
+    A(state, 1); A(err_flag, 1);
+#undef A
+    IVP_HIP(hipHostMalloc((void**)&c->state_host, sizeof(SolverState), hipHostMallocDefault));
+    IVP_HIP(hipHostMalloc((void**)&c->err_host, sizeof(int), hipHostMallocDefault));
+    IVP_HIP(hipHostMalloc((void**)&c->count_host, 4 * sizeof(int), hipHostMallocDefault));
+    memset(c->state_host, 0, sizeof(SolverState));
+    IVP_HIP(hipMemsetAsync(c->ccount, 0, sizeof(int) * (size_t)c->ncell, c->stream));
+    IVP_HIP(hipMemsetAsync(c->state, 0, sizeof(SolverState), c->stream));
+    IVP_HIP(hipMemsetAsync(c->tile_len, 0, sizeof(int) * tiles, c->stream));
+    IVP_HIP(hipStreamSynchronize(c->stream));
+    return IVP_OK;
+}
+
+int ensure_rows(ivp_ctx* c) {
+    size_t tiles = (size_t)blocks_for(std::max(c->nd, 1)) * WAVES_PER_BLOCK;
+    static_assert(sizeof(uint32_t) == 4, "");
+    size_t words = tiles * (size_t)c->C * 256u;
+    if (c->nb) return IVP_OK;
+    return dalloc(&c->nb, words);
+}
+
+}  // namespace
+
+namespace ivp {
+
+// Event timing of selected kernel classes.  Launches tagged with a CG iteration index are kept
+// only if that iteration ran (prof_collect), so trailing no-op launches of the last CG chunk are
+// not counted.
+int prof_begin(ivp_ctx* c, int kclass, int tag, double bytes, ProfEvent* ev) {
+    ev->start = nullptr;
+    if (!c->prof || !(c->prof_mask & (1u << kclass))) return IVP_OK;
+    auto take = [&](hipEvent_t* e) -> int {
+        if (!c->prof_free.empty()) {
+            *e = c->prof_free.back();
+            c->prof_free.pop_back();
+            return IVP_OK;
+        }
+        IVP_HIP(hipEventCreate(e));
+        return IVP_OK;
+    };
+    if (take(&ev->start) != IVP_OK || take(&ev->stop) != IVP_OK) return IVP_ERR_HIP;
+    ev->kclass = kclass;
+    ev->tag = tag;
+    ev->epoch = c->prof_epoch;
+    ev->bytes = bytes;
+    IVP_HIP(hipEventRecord(ev->start, c->stream));
+    return IVP_OK;
+}
+
+int prof_end(ivp_ctx* c, ProfEvent* ev) {
+    if (!ev->start) return IVP_OK;
+    IVP_HIP(hipEventRecord(ev->stop, c->stream));
+    c->prof_pending.push_back(*ev);
+    if (c->prof_pending.size() > 8192) {
+        IVP_HIP(hipStreamSynchronize(c->stream));
+        return prof_collect(c, -1, -1);
+    }
+    return IVP_OK;
+}
+
+// Fold finished events.  Events of `epoch` with tag >= cg_iters_done are dropped.
+int prof_collect(ivp_ctx* c, int cg_iters_done, int epoch) {
+    std::vector<ProfEvent> keep;
+    for (auto& e : c->prof_pending) {
+        if (hipEventQuery(e.stop) != hipSuccess) {
+            keep.push_back(e);
+            continue;
+        }
+        bool counted = e.tag < 0 || cg_iters_done < 0 || e.epoch != epoch || e.tag < cg_iters_done;
+        if (counted) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, e.start, e.stop) == hipSuccess) {
+                c->prof_launches[e.kclass]++;
+                c->prof_ms[e.kclass] += ms;
+                c->prof_bytes[e.kclass] += e.bytes;
+            }
+        }
+        c->prof_free.push_back(e.start);
+        c->prof_free.push_back(e.stop);
+    }
+    c->prof_pending.swap(keep);
+    return IVP_OK;
+}
+
+}  // namespace ivp
+
+static const char* kclass_name(int k) {
+    static const char* names[KC_COUNT] = {"sort", "build", "ip_pack", "ip_rhs", "cg_J", "cg_JT", "cg_update", "cg_p",
+                                          "visc", "w18_Av", "w18_update", "w18_p", "w18_pre", "glue"};
+    return (k >= 0 && k < KC_COUNT) ? names[k] : "?";
+}
+
+extern "C" {
+
+const char* ivp_version(void) { return "ivp 0.1 (gfx950)"; }
+
+const char* ivp_status_string(int s) {
+    switch (s) {
+        case IVP_OK: return "ok";
+        case IVP_NOT_CONVERGED: return "IP outer loop did not reach densityErrorTol within maxOuterIters";
+        case IVP_NOT_PD: return "non-positive definite matrix (p.Ap < 0)";
+        case IVP_NEIGHBOR_OVERFLOW: return "a particle has more neighbours than neighborCacheSize";
+        case IVP_ERR_ARG: return "invalid argument";
+        case IVP_ERR_HIP: return "HIP runtime error";
+        case IVP_ERR_CAPACITY: return "capacity exceeded";
+        case IVP_ERR_STATE: return "call out of order (particles not set or neighbours not built)";
+        case IVP_ERR_GRID: return "grid too large (more than 2048 cells per axis or 2^28 cells)";
+        default: return "unknown status";
+    }
+}
+
+int ivp_create(const ivp_config* cfg, int max_particles, void* stream, ivp_ctx** out) {
+    if (!cfg || !out || max_particles < 0) return IVP_ERR_ARG;
+    *out = nullptr;
+    if (!(cfg->support_radius > 0.0f) || cfg->cache_size <= 0) return IVP_ERR_ARG;
+    ivp_ctx* c = new ivp_ctx();
+    c->cfg = *cfg;
+    c->stream = (hipStream_t)stream;
+    c->h = cfg->support_radius;
+    c->C = cfg->cache_size;
+    c->cap = std::max(max_particles, 1);
+    // uniform grid: cell width w = h (1 + 2^-10), origin two cells below the domain start,
+    // ceil(L / w) + 4 cells per axis
+    float w = c->h * 1.0009765625f;
+    c->grid.inv_w = 1.0f / w;
+    long long ncell = 1;
+    for (int a = 0; a < 3; ++a) {
+        c->grid.origin[a] = cfg->domain_start[a] - 2.0f * w;
+        float cells = ceilf(cfg->domain_size[a] / w);
+        if (!(cells >= 0.0f) || cells > 2044.0f) {
+            delete c;
+            return IVP_ERR_GRID;
+        }
+        c->grid.dims[a] = (int)cells + 4;
+        ncell *= c->grid.dims[a];
+    }
+    if (ncell > (1ll << 28)) {
+        delete c;
+        return IVP_ERR_GRID;
+    }
+    c->ncell = (int)ncell;
+    if (hipGetDevice(&c->device) != hipSuccess) {
+        delete c;
+        return IVP_ERR_HIP;
+    }
+    int rc = alloc_all(c);
+    if (rc != IVP_OK) {
+        free_all(c);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return IVP_OK;
+}
+
+int ivp_destroy(ivp_ctx* c) {
+    if (!c) return IVP_ERR_ARG;
+    (void)hipStreamSynchronize(c->stream);
+    dfree(c->nb);
+    free_all(c);
+    delete c;
+    return IVP_OK;
+}
+
+int ivp_set_particles(ivp_ctx* c, int n, const float* x, const float* v, const float* m, const float* m_inv,
+                      const float* density0, const int* is_dynamic, const int* material) {
+    if (!c) return IVP_ERR_ARG;
+    dfree(c->nb);
+    int rc = grid_load(c, n, x, v, m, m_inv, density0, is_dynamic, material);
+    if (rc != IVP_OK) return rc;
+    return ensure_rows(c);
+}
+
+int ivp_write(ivp_ctx* c, int field, const void* src) {
+    if (!c) return IVP_ERR_ARG;
+    return store_write(c, field, src);
+}
+
+int ivp_read(ivp_ctx* c, int field, void* dst) {
+    if (!c) return IVP_ERR_ARG;
+    return store_read(c, field, dst);
+}
+
+int ivp_neighbors_build(ivp_ctx* c) {
+    if (!c) return IVP_ERR_ARG;
+    int rc = grid_build(c);
+    if (rc != IVP_OK) return rc;
+    IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    IVP_HIP(hipStreamSynchronize(c->stream));
+    return (*c->err_host & 1) ? IVP_NEIGHBOR_OVERFLOW : IVP_OK;
+}
+
+int ivp_neighbors_export(ivp_ctx* c, int cache_size, int* count, int* ids, float* gradW) {
+    if (!c) return IVP_ERR_ARG;
+    return grid_export(c, cache_size, count, ids, gradW);
+}
+
+int ivp_ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats) {
+    if (!c) return IVP_ERR_ARG;
+    return ip_solve(c, dt, prm, stats);
+}
+
+int ivp_viscosity_explicit(ivp_ctx* c, float coeff, float dt) {
+    if (!c) return IVP_ERR_ARG;
+    return visc_explicit(c, coeff, dt);
+}
+
+int ivp_viscosity_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* stats) {
+    if (!c) return IVP_ERR_ARG;
+    return visc_weiler18(c, coeff, dt, stats);
+}
+
+int ivp_apply_gravity(ivp_ctx* c, const float gravity[3], float dt) {
+    if (!c || !gravity) return IVP_ERR_ARG;
+    return glue_gravity(c, gravity, dt);
+}
+
+int ivp_advect_boundary(ivp_ctx* c, float dt) {
+    if (!c) return IVP_ERR_ARG;
+    return glue_advect_boundary(c, dt);
+}
+
+int ivp_step(ivp_ctx* c, const ivp_step_params* prm, ivp_ip_stats* stats, ivp_cg_stats* visc) {
+    if (!c || !prm || !stats) return IVP_ERR_ARG;
+    if (!c->built) return IVP_ERR_STATE;
+    int rc = glue_gravity(c, prm->gravity, prm->dt);
+    if (rc != IVP_OK) return rc;
+    if (visc) { visc->cg_iters = 0; visc->residual_norm2 = 0.0; }
+    // coefficient 0 makes either viscosity model an exact no-op (the rhs v - A v is 0, and the
+    // explicit force is scaled by the coefficient)
+    if (prm->viscosity_coeff != 0.0f) {
+        if (prm->viscosity_method == 1)
+            rc = visc_weiler18(c, prm->viscosity_coeff, prm->dt, visc);
+        else
+            rc = visc_explicit(c, prm->viscosity_coeff, prm->dt);
+        if (rc != IVP_OK) return rc;
+    }
+    rc = ip_solve(c, prm->dt, &prm->ip, stats);
+    if (rc != IVP_OK) return rc;
+    rc = glue_advect_boundary(c, prm->dt);
+    if (rc != IVP_OK) return rc;
+    return grid_build(c);
+}
+
+int ivp_profile_enable(ivp_ctx* c, int enable) {
+    if (!c) return IVP_ERR_ARG;
+    c->prof = enable != 0;
+    c->prof_mask = (unsigned)enable;
+    return IVP_OK;
+}
+
+int ivp_profile_reset(ivp_ctx* c) {
+    if (!c) return IVP_ERR_ARG;
+    IVP_HIP(hipStreamSynchronize(c->stream));
+    prof_collect(c, -1, -1);
+    for (int k = 0; k < KC_COUNT; ++k) {
+        c->prof_launches[k] = 0;
+        c->prof_ms[k] = 0.0;
+        c->prof_bytes[k] = 0.0;
+    }
+    return IVP_OK;
+}
+
+int ivp_profile_read(ivp_ctx* c, ivp_kernel_timing* out, int capacity) {
+    if (!c || (!out && capacity > 0)) return IVP_ERR_ARG;
+    IVP_HIP(hipStreamSynchronize(c->stream));
+    prof_collect(c, -1, -1);
+    int n = 0;
+    for (int k = 0; k < KC_COUNT && n < capacity; ++k, ++n) {
+        memset(out[n].name, 0, sizeof(out[n].name));
+        strncpy(out[n].name, kclass_name(k), sizeof(out[n].name) - 1);
+        out[n].launches = c->prof_launches[k];
+        out[n].total_ms = c->prof_ms[k];
+        out[n].bytes = c->prof_bytes[k];
+    }
+    return n;
+}
+
+int ivp_counts(ivp_ctx* c, int* n_total, int* n_dynamic, long long* dyn_pairs) {
+    if (!c) return IVP_ERR_ARG;
+    if (n_total) *n_total = c->n;
+    if (n_dynamic) *n_dynamic = c->nd;
+    if (dyn_pairs) {
+        *dyn_pairs = 0;
+        if (c->built && c->nd > 0) {
+            IVP_HIP(hipMemcpyAsync(c->state_host, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+            IVP_HIP(hipStreamSynchronize(c->stream));
+            *dyn_pairs = c->state_host->total_pairs;
+        }
+    }
+    return IVP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+// ivp_device.h -- device-side helpers shared by the libivp kernels (gfx950, wave64).
+//
+// Floating-point policy: every kernel file is compiled with -ffp-contract=off, and HIP's
+// float division and sqrtf are correctly rounded, so the per-pair quantities below (kernel
+// values, gradients, distances) are bit-identical to a float32 CPU evaluation of the same
+// expression order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ivp {
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;            // 4 waves; every row kernel maps wave w of a block to tile
+constexpr int WAVES_PER_BLOCK = BLOCK / WAVE;
+
+struct Grid {
+    float origin[3];
+    float inv_w;
+    int dims[3];
+};
+
+// cubic_kernel, scripts/sim/sph_kernel.py:8-26 (same operation order)
+__device__ __forceinline__ float cubic_kernel(float r_norm, float h) {
+    float res = 0.0f;
+    float k = 8.0f / 3.14159265359f;
+    k = k / (h * h * h);
+    float q = r_norm / h;
+    if (q <= 1.0f) {
+        if (q <= 0.5f) {
+            float q2 = q * q;
+            float q3 = q2 * q;
+            res = k * (6.0f * q3 - 6.0f * q2 + 1.0f);
+        } else {
+            float factor = 1.0f - q;
+            res = k * 2.0f * factor * factor * factor;
+        }
+    }
+    return res;
+}
+
+__device__ __forceinline__ float len3(float x, float y, float z) {
+    return sqrtf(x * x + y * y + z * z);
+}
+
+// spiky_kernel_derivative, scripts/sim/sph_kernel.py:52-61 (same operation order)
+__device__ __forceinline__ float3 spiky_grad(float rx, float ry, float rz, float h) {
+    float k = 45.0f / (3.14159265359f * h * h * h * h * h * h);
+    float r_norm = len3(rx, ry, rz);
+    if (r_norm < 1e-6f) r_norm = 1e-6f;
+    float gx = rx / r_norm, gy = ry / r_norm, gz = rz / r_norm;
+    float t = h - r_norm;
+    float s = -k * (t * t);
+    return make_float3(s * gx, s * gy, s * gz);
+}
+
+// Cell coordinate along one axis: floor((x - origin) * inv_w), clamped into the grid.
+// The cell width w = h * (1 + 2^-10) keeps the 3x3x3 stencil complete under rounding.
+__device__ __forceinline__ int cell_coord(const Grid& g, float x, int a) {
+    float t = (x - g.origin[a]) * g.inv_w;
+    t = fminf(fmaxf(t, -1.0f), (float)g.dims[a]);
+    int c = (int)floorf(t);
+    c = c < 0 ? 0 : c;
+    c = c > g.dims[a] - 1 ? g.dims[a] - 1 : c;
+    return c;
+}
+
+__device__ __forceinline__ int cell_key(const Grid& g, int cx, int cy, int cz) {
+    return (cz * g.dims[1] + cy) * g.dims[0] + cx;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8).  Remap so
+// each XCD walks one contiguous range of particle tiles: spatial neighbours (and the particle
+// data they gather) then share that XCD's L2.
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+    int xcd = b & 7, idx = b >> 3;
+    int per = nblocks >> 3, rem = nblocks & 7;
+    int start = xcd * per + (xcd < rem ? xcd : rem);
+    return start + idx;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        int u = __shfl_xor(v, o, WAVE);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+// Sum of one double per thread over a 256-thread block, in a fixed order.  Result valid in
+// thread 0.
+__device__ __forceinline__ double block_sum(double v, double* lds4) {
+    v = wave_sum(v);
+    int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x >> 6;
+    if (lane == 0) lds4[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) r = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
+    __syncthreads();
+    return r;
+}
+
+// Fixed-order sum of n doubles by a 256-thread block (result valid in thread 0).
+__device__ __forceinline__ double block_sum_array(const double* a, int n, double* lds4) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += BLOCK) s += a[i];
+    return block_sum(s, lds4);
+}
+
+}  // namespace ivp

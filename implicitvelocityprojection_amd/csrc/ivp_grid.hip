@@ -1,0 +1,624 @@
+// ivp_grid.hip -- particle store, cell sort, neighbour rows, density and Aii.
+//
+// Replaces the reference's Warp HashGrid build + neighbour cache (scripts/sim/
+// neighbour_search.py:10-155) and PressureBase.compute_density / compute_Aii
+// (scripts/sim/forces/pressure/pressure_base.py:12-87).
+//
+// Sort: a deterministic counting sort over cell keys.  count (atomic), exclusive scan,
+// scatter (atomic slot), then every particle ranks itself inside its cell by user id, so the
+// resulting order -- (cell key, user id) -- depends on the positions only.  Cell keys are
+// x-fastest linear, so the three cells (cx-1..cx+1, cy, cz) of a stencil row are one contiguous
+// particle range: a query visits 9 ranges per population.
+#include "ivp_internal.h"
+#include "ivp_reduce.h"
+
+using namespace ivp;
+
+namespace {
+
+constexpr int SCAN_PER_THREAD = 8;
+constexpr int SCAN_CHUNK = BLOCK * SCAN_PER_THREAD;
+
+__device__ __forceinline__ int wave_incl_scan(int x, int lane) {
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        int y = __shfl_up(x, o, WAVE);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// Exclusive scan of one int per thread over the block; returns the exclusive prefix and the
+// block total.
+__device__ __forceinline__ int block_excl_scan(int v, int* total) {
+    __shared__ int wsum[WAVES_PER_BLOCK];
+    int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x >> 6;
+    int incl = wave_incl_scan(v, lane);
+    if (lane == WAVE - 1) wsum[w] = incl;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < WAVES_PER_BLOCK; ++q) {
+        if (q < w) off += wsum[q];
+        tot += wsum[q];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + incl - v;
+}
+
+// start[i] = sum(cnt[0..i)) within the block's chunk; bsum[block] = chunk total
+__global__ __launch_bounds__(BLOCK) void k_scan1(int n, const int* __restrict__ cnt, int* __restrict__ start,
+                                                 int* __restrict__ bsum) {
+    int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER_THREAD;
+    int v[SCAN_PER_THREAD];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_PER_THREAD; ++q) {
+        int idx = base + q;
+        v[q] = idx < n ? cnt[idx] : 0;
+        sum += v[q];
+    }
+    int tot;
+    int run = block_excl_scan(sum, &tot);
+#pragma unroll
+    for (int q = 0; q < SCAN_PER_THREAD; ++q) {
+        int idx = base + q;
+        if (idx < n) start[idx] = run;
+        run += v[q];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// single block: exclusive scan of the nb chunk totals, bsum[nb] = grand total
+__global__ __launch_bounds__(BLOCK) void k_scan2(int nb, int* __restrict__ bsum) {
+    __shared__ int carry_s;
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += BLOCK) {
+        int idx = base + threadIdx.x;
+        int v = idx < nb ? bsum[idx] : 0;
+        int tot;
+        int ex = block_excl_scan(v, &tot);
+        int carry = carry_s;
+        if (idx < nb) bsum[idx] = carry + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry_s = carry + tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[nb] = carry_s;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan3(int n, int* __restrict__ start, const int* __restrict__ bsum, int nb) {
+    int base = blockIdx.x * SCAN_CHUNK;
+    int add = bsum[blockIdx.x];
+    for (int q = threadIdx.x; q < SCAN_CHUNK; q += BLOCK) {
+        int idx = base + q;
+        if (idx < n) start[idx] += add;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) start[n] = bsum[nb];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_keys(int count, const float4* __restrict__ pos, Grid g,
+                                                int* __restrict__ key, int* __restrict__ ccount) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= count) return;
+    float4 p = pos[i];
+    int k = cell_key(g, cell_coord(g, p.x, 0), cell_coord(g, p.y, 1), cell_coord(g, p.z, 2));
+    key[i] = k;
+    atomicAdd(&ccount[k], 1);
+}
+
+// slot = start[key] + (remaining count - 1); leaves ccount at zero for the next sort
+__global__ __launch_bounds__(BLOCK) void k_scatter(int count, const int* __restrict__ key,
+                                                   const int* __restrict__ start, int* __restrict__ ccount,
+                                                   int* __restrict__ tmp_src) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= count) return;
+    int k = key[i];
+    int slot = start[k] + atomicSub(&ccount[k], 1) - 1;
+    tmp_src[slot] = i;
+}
+
+// final position = cell start + number of cell members with a smaller user id
+__global__ __launch_bounds__(BLOCK) void k_rank(int count, const int* __restrict__ key,
+                                                const int* __restrict__ start, const int* __restrict__ tmp_src,
+                                                const int* __restrict__ uid, int* __restrict__ order) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= count) return;
+    int k = key[i];
+    int lo = start[k], hi = start[k + 1];
+    int u = uid[i];
+    int rank = 0;
+    for (int s = lo; s < hi; ++s) rank += uid[tmp_src[s]] < u ? 1 : 0;
+    order[lo + rank] = i;
+}
+
+struct GatherArgs {
+    int count, offset;
+    const int* order;
+    const float4* pos; float4* pos_b;
+    const float* vel; float* vel_b;
+    const float* minv; float* minv_b;
+    const float* rho0; float* rho0_b;
+    const int* mat; int* mat_b;
+    const int* uid; int* uid_b;
+    int* inv;
+    const float* divg; float* divg_b;
+};
+
+template <bool VEL>
+__global__ __launch_bounds__(BLOCK) void k_gather(GatherArgs a) {
+    int s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= a.count) return;
+    int src = a.offset + a.order[s];
+    int dst = a.offset + s;
+    a.pos_b[dst] = a.pos[src];
+    if (VEL) {
+        a.vel_b[3 * dst + 0] = a.vel[3 * src + 0];
+        a.vel_b[3 * dst + 1] = a.vel[3 * src + 1];
+        a.vel_b[3 * dst + 2] = a.vel[3 * src + 2];
+        a.divg_b[dst] = a.divg[src];
+    }
+    a.minv_b[dst] = a.minv[src];
+    a.rho0_b[dst] = a.rho0[src];
+    a.mat_b[dst] = a.mat[src];
+    int u = a.uid[src];
+    a.uid_b[dst] = u;
+    a.inv[u] = dst;
+}
+
+// ---- neighbour enumeration ---------------------------------------------------------------
+// Visits, in order, every particle j of one population (dynamic or static) with j != self and
+// sqrtf(dx*dx + dy*dy + dz*dz) < h -- the reference predicate, neighbour_search.py:34-44 --
+// over the 9 contiguous (cx-1..cx+1) runs of the 3x3 (dz, dy) stencil.  Within a population the
+// visit order is ascending (cell key, user id).
+template <class F>
+__device__ __forceinline__ void visit(const Grid& g, const int* __restrict__ start, const float4* __restrict__ ppos,
+                                      int self, float4 pi, float h, F&& f) {
+    int cx = cell_coord(g, pi.x, 0), cy = cell_coord(g, pi.y, 1), cz = cell_coord(g, pi.z, 2);
+    int x0 = cx > 0 ? cx - 1 : 0;
+    int x1 = cx < g.dims[0] - 1 ? cx + 1 : g.dims[0] - 1;
+    for (int dz = -1; dz <= 1; ++dz) {
+        int zz = cz + dz;
+        if (zz < 0 || zz >= g.dims[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int yy = cy + dy;
+            if (yy < 0 || yy >= g.dims[1]) continue;
+            int s0 = start[cell_key(g, x0, yy, zz)];
+            int s1 = start[cell_key(g, x1, yy, zz) + 1];
+            for (int j = s0; j < s1; ++j) {
+                if (j == self) continue;
+                float4 pj = ppos[j];
+                float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                float rn = len3(rx, ry, rz);
+                if (rn < h) f(j, pj, rx, ry, rz, rn);
+            }
+        }
+    }
+}
+
+struct BuildArgs {
+    int nd, C;
+    float h;
+    Grid g;
+    const float4* pos;
+    const float* minv;
+    const int* dstart;
+    const int* sstart;
+    uint32_t* nb;
+    int* tile_len;
+    int* rowcnt;
+    int* rowtot;
+    float* density;
+    float* kk;
+    float* Ss;
+    int* err_flag;
+    double* partials;
+    int stride;
+};
+
+// One thread per dynamic row, one wave per 64-row tile.  Writes the row's dynamic neighbours
+// (id, gradW) into the tile, pads the tile, and accumulates density (pressure_base.py:12-46),
+// Aii (pressure_base.py:49-87) and S_i over the row in reference order: dynamic neighbours,
+// then static neighbours.
+__global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
+    int blk = xcd_block(blockIdx.x, gridDim.x);
+    int i = blk * BLOCK + threadIdx.x;
+    int lane = threadIdx.x & (WAVE - 1);
+    int tile = i >> 6;
+    bool valid = i < a.nd;
+    int cnt = 0;
+    const float h = a.h;
+    if (valid) {
+        float4 pi = a.pos[i];
+        float rho = 0.0f;
+        rho = rho + pi.w * cubic_kernel(0.0f, h);
+        float aii = 0.0f;
+        float jx = 0.0f, jy = 0.0f, jz = 0.0f;
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        int tot = 0;
+        const int C = a.C;
+        uint32_t* tbase = a.nb + (size_t)tile * (size_t)C * 256u + lane;
+        visit(a.g, a.dstart, a.pos, i, pi, h, [&](int j, float4 pj, float rx, float ry, float rz, float rn) {
+            float3 gw = spiky_grad(rx, ry, rz, h);
+            if (cnt < C) {
+                uint32_t* rec = tbase + (size_t)cnt * 256u;
+                rec[0] = (uint32_t)j;
+                rec[64] = __float_as_uint(gw.x);
+                rec[128] = __float_as_uint(gw.y);
+                rec[192] = __float_as_uint(gw.z);
+                cnt++;
+            }
+            tot++;
+            rho += pj.w * cubic_kernel(rn, h);
+            float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
+            aii += (Jx * Jx + Jy * Jy + Jz * Jz) * a.minv[j];
+            jx -= Jx; jy -= Jy; jz -= Jz;
+        });
+        visit(a.g, a.sstart, a.pos + a.nd, -1, pi, h, [&](int, float4 pj, float rx, float ry, float rz, float rn) {
+            float3 gw = spiky_grad(rx, ry, rz, h);
+            tot++;
+            rho += pj.w * cubic_kernel(rn, h);
+            float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
+            jx -= Jx; jy -= Jy; jz -= Jz;
+            sx += Jx; sy += Jy; sz += Jz;
+        });
+        aii += (jx * jx + jy * jy + jz * jz) * a.minv[i];
+        a.kk[i] = 1.0f / (aii + 1e-3f);
+        a.density[i] = rho;
+        a.Ss[3 * i + 0] = sx;
+        a.Ss[3 * i + 1] = sy;
+        a.Ss[3 * i + 2] = sz;
+        a.rowcnt[i] = cnt;
+        a.rowtot[i] = tot;
+        if (tot > C) atomicOr(a.err_flag, 1);
+    }
+    double v[1] = {(double)cnt};
+    block_partials<1>(v, a.partials, a.stride);
+    int tl = wave_max(cnt);
+    if (lane == 0) a.tile_len[tile] = tl;
+    if (valid) {
+        uint32_t* tbase = a.nb + (size_t)tile * (size_t)a.C * 256u + lane;
+        for (int k = cnt; k < tl; ++k) {
+            uint32_t* rec = tbase + (size_t)k * 256u;
+            rec[0] = (uint32_t)i;
+            rec[64] = 0u;
+            rec[128] = 0u;
+            rec[192] = 0u;
+        }
+    }
+}
+
+struct ExportArgs {
+    int n, nd, C;
+    float h;
+    Grid g;
+    const float4* pos;
+    const int* dstart;
+    const int* sstart;
+    const int* uid;
+    int* count;
+    int* ids;
+    float* gradW;
+};
+
+// The reference cache layout (every row, user ids, count capped at C), same enumeration as
+// k_build.
+__global__ __launch_bounds__(BLOCK) void k_export(ExportArgs a) {
+    int s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= a.n) return;
+    float4 pi = a.pos[s];
+    int u = a.uid[s];
+    size_t base = (size_t)u * (size_t)a.C;
+    int cnt = 0;
+    bool is_dyn = s < a.nd;
+    auto put = [&](int jint, float rx, float ry, float rz) {
+        if (cnt < a.C) {
+            float3 gw = spiky_grad(rx, ry, rz, a.h);
+            a.ids[base + cnt] = a.uid[jint];
+            a.gradW[3 * (base + cnt) + 0] = gw.x;
+            a.gradW[3 * (base + cnt) + 1] = gw.y;
+            a.gradW[3 * (base + cnt) + 2] = gw.z;
+        }
+        cnt++;
+    };
+    visit(a.g, a.dstart, a.pos, is_dyn ? s : -1, pi, a.h,
+          [&](int j, float4, float rx, float ry, float rz, float) { put(j, rx, ry, rz); });
+    visit(a.g, a.sstart, a.pos + a.nd, is_dyn ? -1 : s - a.nd, pi, a.h,
+          [&](int j, float4, float rx, float ry, float rz, float) { put(a.nd + j, rx, ry, rz); });
+    a.count[u] = cnt < a.C ? cnt : a.C;
+}
+
+// ---- particle store loading ----------------------------------------------------------------
+
+__global__ __launch_bounds__(BLOCK) void k_dyn_flags(int n, const int* __restrict__ is_dynamic,
+                                                     const int* __restrict__ material, int fluid,
+                                                     int* __restrict__ flag, int* __restrict__ err) {
+    int u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= n) return;
+    int dyn = is_dynamic[u] != 0 ? 1 : 0;
+    flag[u] = dyn;
+    if (dyn && material[u] != fluid) atomicOr(err, 2);
+}
+
+struct StageArgs {
+    int n, nd;
+    const int* dpos;
+    const int* is_dynamic;
+    const float* x; const float* v; const float* m; const float* m_inv;
+    const float* density0; const int* material;
+    float4* pos; float* vel; float* minv; float* rho0; int* mat; int* uid; int* inv;
+    float* density; float* kk; float* divg;
+};
+
+__global__ __launch_bounds__(BLOCK) void k_stage(StageArgs a) {
+    int u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= a.n) return;
+    bool dyn = a.is_dynamic[u] != 0;
+    int s = dyn ? a.dpos[u] : a.nd + (u - a.dpos[u]);
+    a.pos[s] = make_float4(a.x[3 * u + 0], a.x[3 * u + 1], a.x[3 * u + 2], a.m[u]);
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    if (dyn) { vx = a.v[3 * u + 0]; vy = a.v[3 * u + 1]; vz = a.v[3 * u + 2]; }
+    a.vel[3 * s + 0] = vx; a.vel[3 * s + 1] = vy; a.vel[3 * s + 2] = vz;
+    a.minv[s] = a.m_inv[u];
+    a.rho0[s] = a.density0[u];
+    a.mat[s] = a.material[u];
+    a.uid[s] = u;
+    a.inv[u] = s;
+    a.density[s] = a.density0[u];
+    a.kk[s] = 0.0f;
+    a.divg[s] = 0.0f;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_write_x(int nd, const float* __restrict__ x, const int* __restrict__ uid,
+                                                   float4* __restrict__ pos) {
+    int s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= nd) return;
+    int u = uid[s];
+    float4 p = pos[s];
+    p.x = x[3 * u + 0]; p.y = x[3 * u + 1]; p.z = x[3 * u + 2];
+    pos[s] = p;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_write_v(int nd, const float* __restrict__ v, const int* __restrict__ uid,
+                                                   float* __restrict__ vel) {
+    int s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= nd) return;
+    int u = uid[s];
+    vel[3 * s + 0] = v[3 * u + 0];
+    vel[3 * s + 1] = v[3 * u + 1];
+    vel[3 * s + 2] = v[3 * u + 2];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_read_x(int n, const float4* __restrict__ pos, const int* __restrict__ uid,
+                                                  float* __restrict__ x) {
+    int s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= n) return;
+    int u = uid[s];
+    float4 p = pos[s];
+    x[3 * u + 0] = p.x; x[3 * u + 1] = p.y; x[3 * u + 2] = p.z;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_read_v(int n, int nd, const float* __restrict__ vel,
+                                                  const int* __restrict__ uid, float* __restrict__ v) {
+    int s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= n) return;
+    int u = uid[s];
+    bool dyn = s < nd;
+    v[3 * u + 0] = dyn ? vel[3 * s + 0] : 0.0f;
+    v[3 * u + 1] = dyn ? vel[3 * s + 1] : 0.0f;
+    v[3 * u + 2] = dyn ? vel[3 * s + 2] : 0.0f;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_read_f(int n, const float* __restrict__ f, const int* __restrict__ uid,
+                                                  float* __restrict__ out) {
+    int s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= n) return;
+    out[uid[s]] = f[s];
+}
+
+int scan(ivp_ctx* c, const int* cnt, int* start, int n) {
+    int nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    if (nb + 1 > c->nscan_blocks) return IVP_ERR_CAPACITY;
+    if (nb == 0) {
+        IVP_HIP(hipMemsetAsync(start, 0, sizeof(int), c->stream));
+        return IVP_OK;
+    }
+    hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(BLOCK), 0, c->stream, n, cnt, start, c->scan_bsum);
+    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(BLOCK), 0, c->stream, nb, c->scan_bsum);
+    hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(BLOCK), 0, c->stream, n, start, c->scan_bsum, nb);
+    IVP_HIP(hipGetLastError());
+    return IVP_OK;
+}
+
+template <class T>
+void swap_ptr(T*& a, T*& b) {
+    T* t = a;
+    a = b;
+    b = t;
+}
+
+}  // namespace
+
+namespace ivp {
+
+// Sort one population [offset, offset + count) of the store by (cell key, user id) and fill
+// its cell table `start_table` (ncell + 1 entries, relative to offset).
+int grid_sort_population(ivp_ctx* c, int offset, int count, int* start_table, bool dynamic) {
+    if (count <= 0) {
+        IVP_HIP(hipMemsetAsync(start_table, 0, sizeof(int) * (size_t)(c->ncell + 1), c->stream));
+        return IVP_OK;
+    }
+    ProfEvent ev;
+    prof_begin(c, KC_SORT, -1, (double)count * (16 + 4 + 4 + 4 + 4 + 4 + 4 + 16 + (dynamic ? 24 : 0) + 24), &ev);
+    int nb = blocks_for(count);
+    hipLaunchKernelGGL(k_keys, dim3(nb), dim3(BLOCK), 0, c->stream, count, c->pos + offset, c->grid, c->key,
+                       c->ccount);
+    int rc = scan(c, c->ccount, start_table, c->ncell);
+    if (rc != IVP_OK) return rc;
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(BLOCK), 0, c->stream, count, c->key, start_table, c->ccount,
+                       c->tmp_src);
+    hipLaunchKernelGGL(k_rank, dim3(nb), dim3(BLOCK), 0, c->stream, count, c->key, start_table, c->tmp_src,
+                       c->uid + offset, c->order);
+    GatherArgs ga{count, offset, c->order, c->pos, c->pos_b, c->vel, c->vel_b, c->minv, c->minv_b,
+                  c->rho0, c->rho0_b, c->mat, c->mat_b, c->uid, c->uid_b, c->inv, c->divg, c->divg_b};
+    if (dynamic)
+        hipLaunchKernelGGL(k_gather<true>, dim3(nb), dim3(BLOCK), 0, c->stream, ga);
+    else
+        hipLaunchKernelGGL(k_gather<false>, dim3(nb), dim3(BLOCK), 0, c->stream, ga);
+    IVP_HIP(hipGetLastError());
+    prof_end(c, &ev);
+    swap_ptr(c->pos, c->pos_b);
+    swap_ptr(c->minv, c->minv_b);
+    swap_ptr(c->rho0, c->rho0_b);
+    swap_ptr(c->mat, c->mat_b);
+    swap_ptr(c->uid, c->uid_b);
+    if (dynamic) {
+        swap_ptr(c->vel, c->vel_b);
+        swap_ptr(c->divg, c->divg_b);
+    }
+    return IVP_OK;
+}
+
+int grid_load(ivp_ctx* c, int n, const float* x, const float* v, const float* m, const float* m_inv,
+              const float* density0, const int* is_dynamic, const int* material) {
+    if (n < 0 || n > c->cap) return IVP_ERR_CAPACITY;
+    if (n > 0 && (!x || !v || !m || !m_inv || !density0 || !is_dynamic || !material)) return IVP_ERR_ARG;
+    c->loaded = false;
+    c->built = false;
+    c->n = n;
+    if (n == 0) {
+        c->nd = c->ns = 0;
+        c->loaded = true;
+        return IVP_OK;
+    }
+    int nb = blocks_for(n);
+    IVP_HIP(hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream));
+    // flags -> exclusive prefix = rank of each dynamic particle
+    hipLaunchKernelGGL(k_dyn_flags, dim3(nb), dim3(BLOCK), 0, c->stream, n, is_dynamic, material,
+                       c->cfg.material_fluid, c->order, c->err_flag);
+    int rc = scan(c, c->order, c->tmp_src, n);
+    if (rc != IVP_OK) return rc;
+    IVP_HIP(hipMemcpyAsync(c->count_host, c->tmp_src + n, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    IVP_HIP(hipStreamSynchronize(c->stream));
+    c->nd = *c->count_host;
+    c->ns = n - c->nd;
+    c->all_dyn_fluid = (*c->err_host & 2) == 0;
+    StageArgs sa{n, c->nd, c->tmp_src, is_dynamic, x, v, m, m_inv, density0, material,
+                 c->pos, c->vel, c->minv, c->rho0, c->mat, c->uid, c->inv, c->density, c->kk, c->divg};
+    hipLaunchKernelGGL(k_stage, dim3(nb), dim3(BLOCK), 0, c->stream, sa);
+    IVP_HIP(hipGetLastError());
+    // static particles: sort once.  The sort gathers [nd, n) into the reorder buffers and swaps,
+    // so afterwards the staged dynamic segment [0, nd) lives in the reorder buffers and the
+    // sorted static segment in the primary ones: copy each segment across so both copies of
+    // every per-particle array are complete and identical.
+    rc = grid_sort_population(c, c->nd, c->ns, c->sstart, false);
+    if (rc != IVP_OK) return rc;
+    size_t off = (size_t)c->nd, cnt = (size_t)c->ns;
+    if (cnt > 0) {
+        size_t nd = off;
+        auto mirror = [&](void* prim, void* alt, size_t elem) -> int {
+            char* p = (char*)prim;
+            char* b = (char*)alt;
+            if (nd > 0) IVP_HIP(hipMemcpyAsync(p, b, nd * elem, hipMemcpyDeviceToDevice, c->stream));
+            IVP_HIP(hipMemcpyAsync(b + nd * elem, p + nd * elem, cnt * elem, hipMemcpyDeviceToDevice, c->stream));
+            return IVP_OK;
+        };
+        if ((rc = mirror(c->pos, c->pos_b, sizeof(float4))) != IVP_OK) return rc;
+        if ((rc = mirror(c->minv, c->minv_b, sizeof(float))) != IVP_OK) return rc;
+        if ((rc = mirror(c->rho0, c->rho0_b, sizeof(float))) != IVP_OK) return rc;
+        if ((rc = mirror(c->mat, c->mat_b, sizeof(int))) != IVP_OK) return rc;
+        if ((rc = mirror(c->uid, c->uid_b, sizeof(int))) != IVP_OK) return rc;
+        // vel and divg were not swapped by the static sort: primary holds everything
+        IVP_HIP(hipMemcpyAsync(c->vel_b, c->vel, 3 * (nd + cnt) * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        IVP_HIP(hipMemcpyAsync(c->divg_b, c->divg, (nd + cnt) * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        // static density stays rho0 (compute_density skips static rows, pressure_base.py:28-29);
+        // static k and divergence are 0 in any order
+        IVP_HIP(hipMemcpyAsync(c->density + off, c->rho0 + off, cnt * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    }
+    c->loaded = true;
+    return IVP_OK;
+}
+
+int grid_build(ivp_ctx* c) {
+    if (!c->loaded) return IVP_ERR_STATE;
+    if (c->nd > 0) {
+        int rc = grid_sort_population(c, 0, c->nd, c->dstart, true);
+        if (rc != IVP_OK) return rc;
+    } else {
+        IVP_HIP(hipMemsetAsync(c->dstart, 0, sizeof(int) * (size_t)(c->ncell + 1), c->stream));
+    }
+    IVP_HIP(hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream));
+    if (c->nd > 0) {
+        BuildArgs ba{c->nd, c->C, c->h, c->grid, c->pos, c->minv, c->dstart, c->sstart, c->nb, c->tile_len,
+                     c->rowcnt, c->rowtot, c->density, c->kk, c->Ss, c->err_flag, c->partials, c->max_blocks};
+        ProfEvent ev;
+        prof_begin(c, KC_BUILD, -1, 0.0, &ev);
+        hipLaunchKernelGGL(k_build, dim3(blocks_for(c->nd)), dim3(BLOCK), 0, c->stream, ba);
+        IVP_HIP(hipGetLastError());
+        prof_end(c, &ev);
+        int rc = finalize<FIN_BUILD>(c, blocks_for(c->nd));
+        if (rc != IVP_OK) return rc;
+    }
+    c->built = true;
+    return IVP_OK;
+}
+
+int grid_export(ivp_ctx* c, int cache_size, int* count, int* ids, float* gradW) {
+    if (!c->built) return IVP_ERR_STATE;
+    if (cache_size <= 0 || !count || !ids || !gradW) return IVP_ERR_ARG;
+    if (c->n == 0) return IVP_OK;
+    ExportArgs ea{c->n, c->nd, cache_size, c->h, c->grid, c->pos, c->dstart, c->sstart, c->uid, count, ids, gradW};
+    hipLaunchKernelGGL(k_export, dim3(blocks_for(c->n)), dim3(BLOCK), 0, c->stream, ea);
+    IVP_HIP(hipGetLastError());
+    return IVP_OK;
+}
+
+int store_write(ivp_ctx* c, int field, const void* src) {
+    if (!c->loaded) return IVP_ERR_STATE;
+    if (!src) return IVP_ERR_ARG;
+    if (c->nd == 0) return IVP_OK;
+    int nb = blocks_for(c->nd);
+    if (field == IVP_FIELD_X) {
+        hipLaunchKernelGGL(k_write_x, dim3(nb), dim3(BLOCK), 0, c->stream, c->nd, (const float*)src, c->uid, c->pos);
+        c->built = false;
+    } else if (field == IVP_FIELD_V) {
+        hipLaunchKernelGGL(k_write_v, dim3(nb), dim3(BLOCK), 0, c->stream, c->nd, (const float*)src, c->uid, c->vel);
+    } else {
+        return IVP_ERR_ARG;
+    }
+    IVP_HIP(hipGetLastError());
+    return IVP_OK;
+}
+
+int store_read(ivp_ctx* c, int field, void* dst) {
+    if (!c->loaded) return IVP_ERR_STATE;
+    if (!dst) return IVP_ERR_ARG;
+    if (c->n == 0) return IVP_OK;
+    int nb = blocks_for(c->n);
+    switch (field) {
+        case IVP_FIELD_X:
+            hipLaunchKernelGGL(k_read_x, dim3(nb), dim3(BLOCK), 0, c->stream, c->n, c->pos, c->uid, (float*)dst);
+            break;
+        case IVP_FIELD_V:
+            hipLaunchKernelGGL(k_read_v, dim3(nb), dim3(BLOCK), 0, c->stream, c->n, c->nd, c->vel, c->uid, (float*)dst);
+            break;
+        case IVP_FIELD_DENSITY:
+            hipLaunchKernelGGL(k_read_f, dim3(nb), dim3(BLOCK), 0, c->stream, c->n, c->density, c->uid, (float*)dst);
+            break;
+        case IVP_FIELD_K:
+            hipLaunchKernelGGL(k_read_f, dim3(nb), dim3(BLOCK), 0, c->stream, c->n, c->kk, c->uid, (float*)dst);
+            break;
+        case IVP_FIELD_DIVERGENCE:
+            hipLaunchKernelGGL(k_read_f, dim3(nb), dim3(BLOCK), 0, c->stream, c->n, c->divg, c->uid, (float*)dst);
+            break;
+        default:
+            return IVP_ERR_ARG;
+    }
+    IVP_HIP(hipGetLastError());
+    return IVP_OK;
+}
+
+}  // namespace ivp

@@ -1,0 +1,198 @@
+// ivp_internal.h -- the library context and the launch functions each kernel file provides.
+//
+// Particle store ("internal order"): dynamic particles occupy [0, nd) sorted by
+// (cell key, user id) -- re-sorted at every neighbour rebuild -- and static particles occupy
+// [nd, n), sorted once by ivp_set_particles (they never move: framework.py:130-133 zeroes
+// their velocity and advects them by 0).  uid[s] is the user index of internal particle s and
+// inv[u] its inverse.
+//
+// Neighbour rows exist for dynamic particles only and hold their DYNAMIC neighbours; the
+// static neighbours of a row enter the operators only through S_i = sum_static m_j gradW_ij,
+// density and Aii, all accumulated during the build.  Rows are stored in wave tiles:
+// tile t = rows [64t, 64t+64), slot k, component c (0 = neighbour id, 1..3 = gradW xyz):
+//     nb[((t * C + k) * 4 + c) * 64 + lane]
+// so every (tile, slot) is one contiguous 1 KiB record read by one wave.  Slots past a row's
+// count up to the tile length hold (id = the row itself, gradW = 0), which adds exactly 0 to
+// every operator, so the hot loops run a wave-uniform trip count.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ivp.h"
+#include "ivp_device.h"
+
+#define IVP_HIP(call)                                   \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) return ivp_hip_fail(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+int ivp_hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+namespace ivp {
+
+// Device-resident solver state.  Reducing kernels write per-block partials; a one-block
+// finalize kernel folds them in a fixed order and updates these fields.
+struct SolverState {
+    double rr_old;      // r.r (IP) / r.z (Weiler18) of the current iteration
+    double rr_new;
+    double pAp;
+    double alpha;
+    double beta;
+    double err_sum;     // sum of max(f / rho0, 0) over dynamic particles
+    double tol;         // CG stop threshold on r.r
+    double rr_last;     // r.r at the last completed iteration
+    double rr0;         // r.r before the first iteration
+    int iter;           // CG iterations completed
+    int max_iter;
+    int active;         // 1 while CG continues
+    int status;         // 0 ok, IVP_NOT_PD
+    int n_active_rows;  // rows with dfdt > 0 (per outer iteration)
+    int n_touched_rows; // rows whose J^T loop runs (active or with an active neighbour)
+    long long touched_pairs;
+    long long active_pairs;
+    long long total_pairs;  // stored (dynamic-dynamic) neighbour pairs of the last build
+};
+
+enum KernelClass {
+    KC_SORT = 0,
+    KC_BUILD,
+    KC_IP_PACK,
+    KC_IP_RHS,
+    KC_CG_J,
+    KC_CG_JT,
+    KC_CG_UPDATE,
+    KC_CG_P,
+    KC_VISC,
+    KC_W18_AV,
+    KC_W18_UPDATE,
+    KC_W18_P,
+    KC_W18_PRE,
+    KC_GLUE,
+    KC_COUNT
+};
+
+struct ProfEvent {
+    hipEvent_t start, stop;
+    int kclass;
+    int tag;       // CG iteration index the launch belongs to (-1: always counted)
+    int epoch;     // CG solve the launch belongs to
+    double bytes;
+};
+
+}  // namespace ivp
+
+struct ivp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    ivp_config cfg{};
+    float h = 0.0f;
+    int C = 60;
+    ivp::Grid grid{};
+    int ncell = 0;
+    int cap = 0;
+    int n = 0, nd = 0, ns = 0;
+    int nd_alloc_tiles = 0;
+    bool loaded = false;
+    bool built = false;
+    bool all_dyn_fluid = true;
+
+    // particle store (internal order) and reorder buffers
+    float4* pos = nullptr;  float4* pos_b = nullptr;    // x, y, z, m
+    float* vel = nullptr;   float* vel_b = nullptr;     // 3 * cap
+    float* minv = nullptr;  float* minv_b = nullptr;
+    float* rho0 = nullptr;  float* rho0_b = nullptr;
+    int* mat = nullptr;     int* mat_b = nullptr;
+    int* uid = nullptr;     int* uid_b = nullptr;
+    int* inv = nullptr;
+    float* density = nullptr;
+    float* kk = nullptr;
+    float* Ss = nullptr;      // 3 * cap
+    float* divg = nullptr;  float* divg_b = nullptr;
+    int* rowcnt = nullptr;
+    int* rowtot = nullptr;
+
+    // neighbour rows
+    uint32_t* nb = nullptr;
+    int* tile_len = nullptr;
+
+    // cell tables and sort scratch
+    int* dstart = nullptr;  // ncell + 1
+    int* sstart = nullptr;  // ncell + 1
+    int* ccount = nullptr;  // ncell, all zero between sorts
+    int* key = nullptr;
+    int* tmp_src = nullptr;
+    int* order = nullptr;
+    int* scan_bsum = nullptr;
+    int nscan_blocks = 0;
+
+    // IP solver vectors (dynamic rows)
+    float4* d4 = nullptr;   // displacement d, w = m
+    float4* q4 = nullptr;   // (f k, m, dfdt k, 0)
+    float* b = nullptr;  float* sol = nullptr;  float* r = nullptr;  float* Ap = nullptr;  // 3 * cap
+    float4* p4 = nullptr;   // CG direction, w = m
+    float2* y2 = nullptr;   // (dfdt k J p, m)
+    float* cc = nullptr;    // constraint c = rho - rho0
+    float* sfac = nullptr;  // dfdt * k
+    int* nbact = nullptr;   // row has an active particle in its closed neighbourhood
+
+    // Weiler18 vectors
+    float* w_pinv = nullptr;  // 9 * cap
+    float4* w_p4 = nullptr;   // (p, m/rho)
+    float4* w_v4 = nullptr;   // (v, m/rho)
+    float* w_dv = nullptr; float* w_r = nullptr; float* w_z = nullptr;
+    float* w_Ap = nullptr; float* w_rhs = nullptr;
+    float* w_fac = nullptr;   // dt * mu_i / rho_i
+
+    // explicit viscosity
+    float4* v4 = nullptr;     // (v, rho)
+
+    // reductions
+    double* partials = nullptr;   // 4 * max_blocks
+    int max_blocks = 0;
+    ivp::SolverState* state = nullptr;      // device
+    ivp::SolverState* state_host = nullptr; // pinned
+    int* err_flag = nullptr;                // device: bit 0 neighbour overflow
+    int* err_host = nullptr;                // pinned
+    int* count_host = nullptr;              // pinned scratch
+
+    // profiling
+    bool prof = false;
+    unsigned prof_mask = 0xffffffffu;
+    std::vector<ivp::ProfEvent> prof_pending;
+    std::vector<hipEvent_t> prof_free;
+    long long prof_launches[ivp::KC_COUNT] = {};
+    double prof_ms[ivp::KC_COUNT] = {};
+    double prof_bytes[ivp::KC_COUNT] = {};
+    int prof_epoch = 0;
+};
+
+namespace ivp {
+
+// profiling helpers (ivp_api.hip)
+int prof_begin(ivp_ctx* c, int kclass, int tag, double bytes, ProfEvent* ev);
+int prof_end(ivp_ctx* c, ProfEvent* ev);
+int prof_collect(ivp_ctx* c, int cg_iters_done, int epoch);
+
+// ivp_grid.hip
+int grid_sort_population(ivp_ctx* c, int offset, int count, int* start_table, bool dynamic);
+int grid_build(ivp_ctx* c);
+int grid_export(ivp_ctx* c, int cache_size, int* count, int* ids, float* gradW);
+int grid_load(ivp_ctx* c, int n, const float* x, const float* v, const float* m, const float* m_inv,
+              const float* density0, const int* is_dynamic, const int* material);
+int store_write(ivp_ctx* c, int field, const void* src);
+int store_read(ivp_ctx* c, int field, void* dst);
+
+// ivp_solve.hip
+int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* st);
+int visc_explicit(ivp_ctx* c, float coeff, float dt);
+int visc_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* st);
+int glue_gravity(ivp_ctx* c, const float g[3], float dt);
+int glue_advect_boundary(ivp_ctx* c, float dt);
+
+inline int blocks_for(int n) { return (n + BLOCK - 1) / BLOCK; }
+
+}  // namespace ivp

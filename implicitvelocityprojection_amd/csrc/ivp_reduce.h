@@ -1,0 +1,124 @@
+// ivp_reduce.h -- grid reductions: per-block partials plus a one-block finalize kernel that
+// folds them in a fixed order (deterministic) and advances the device-resident solver state.
+#pragma once
+
+#include "ivp_internal.h"
+
+namespace ivp {
+
+// Per-block partial sums of up to 3 doubles (fixed order inside the block).  The grid totals are
+// formed by k_finalize, launched next on the stream: the kernel boundary publishes the partials,
+// so no device-scope fence (an L2 write-back on a multi-XCD part) is issued per block.
+template <int NV>
+__device__ __forceinline__ void block_partials(const double (&v)[NV], double* partials, int stride) {
+    __shared__ double lds[WAVES_PER_BLOCK];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double s = block_sum(v[q], lds);
+        if (threadIdx.x == 0) partials[q * stride + blockIdx.x] = s;
+    }
+}
+
+enum FinOp { FIN_BUILD, FIN_PACK, FIN_RHS, FIN_CG_INIT, FIN_CG_PAP, FIN_CG_UPDATE, FIN_W18_INIT, FIN_W18_PAP, FIN_W18_UPDATE };
+
+struct FinArgs {
+    const double* partials;
+    int stride;
+    int nblocks;
+    SolverState* st;
+    double tol;
+    int max_iter;
+};
+
+// One block: fold the partials of the previous kernel in a fixed order and advance the solver
+// state (the scalar host logic of ip.py:296-327, pressure_base.py:405-423, weiler18.py:330-365).
+template <int OP>
+__global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
+    __shared__ double lds[WAVES_PER_BLOCK];
+    constexpr int NV = OP == FIN_PACK ? 3 : (OP == FIN_RHS || OP == FIN_W18_UPDATE) ? 2 : 1;
+    SolverState* st = a.st;
+    if (OP == FIN_CG_PAP || OP == FIN_CG_UPDATE || OP == FIN_W18_PAP || OP == FIN_W18_UPDATE) {
+        if (st->active == 0) return;
+    }
+    double t[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < NV; ++q) t[q] = block_sum_array(a.partials + q * a.stride, a.nblocks, lds);
+    if (threadIdx.x != 0) return;
+    if (OP == FIN_BUILD) {
+        st->total_pairs = (long long)t[0];
+    } else if (OP == FIN_PACK) {
+        st->err_sum = t[0];
+        st->n_active_rows = (int)t[1];
+        st->active_pairs = (long long)t[2];
+    } else if (OP == FIN_RHS) {
+        st->n_touched_rows = (int)t[0];
+        st->touched_pairs = (long long)t[1];
+    } else if (OP == FIN_CG_INIT || OP == FIN_W18_INIT) {
+        // IP: CG runs when r.r > 1e-30 (ip.py:302); Weiler18: when r.z > 1e-12 (weiler18.py:337)
+        double floor_ = OP == FIN_CG_INIT ? 1e-30 : 1e-12;
+        st->rr_old = t[0];
+        st->rr_new = t[0];
+        st->rr_last = t[0];
+        st->rr0 = t[0];
+        st->tol = a.tol;
+        st->max_iter = a.max_iter;
+        st->iter = 0;
+        st->status = 0;
+        st->active = (t[0] > floor_ && a.max_iter > 0) ? 1 : 0;
+    } else if (OP == FIN_CG_PAP) {
+        // ip.py:307-311
+        st->pAp = t[0];
+        if (t[0] < 0.0) {
+            st->status = IVP_NOT_PD;
+            st->active = 0;
+        } else {
+            st->alpha = st->rr_old / (t[0] > 1e-30 ? t[0] : 1e-30);
+        }
+    } else if (OP == FIN_W18_PAP) {
+        // weiler18.py:338-342: pAp <= 0 ends the iteration
+        st->pAp = t[0];
+        if (t[0] <= 0.0) {
+            st->active = 0;
+        } else {
+            st->alpha = st->rr_old / (t[0] > 1e-12 ? t[0] : 1e-12);
+        }
+    } else if (OP == FIN_CG_UPDATE) {
+        // ip.py:317-327
+        int it = st->iter + 1;
+        st->iter = it;
+        st->rr_new = t[0];
+        st->rr_last = t[0];
+        bool stop = (t[0] < st->tol && it >= 2) || it >= st->max_iter;
+        if (stop) {
+            st->active = 0;
+        } else {
+            double ro = st->rr_old;
+            st->beta = t[0] / (ro > 1e-30 ? ro : 1e-30);
+            st->rr_old = t[0];
+        }
+    } else if (OP == FIN_W18_UPDATE) {
+        // weiler18.py:350-365 (t[0] = r.r, t[1] = r.z)
+        int it = st->iter + 1;
+        st->iter = it;
+        st->rr_last = t[0];
+        bool stop = (t[0] < st->tol && it >= 2) || it >= st->max_iter;
+        if (stop) {
+            st->active = 0;
+        } else {
+            double ro = st->rr_old;
+            st->beta = t[1] / (ro > 1e-12 ? ro : 1e-12);
+            st->rr_old = t[1];
+        }
+    }
+}
+
+
+template <int OP>
+inline int finalize(ivp_ctx* c, int nblocks, double tol = 0.0, int max_iter = 0) {
+    FinArgs fa{c->partials, c->max_blocks, nblocks, c->state, tol, max_iter};
+    hipLaunchKernelGGL(k_finalize<OP>, dim3(1), dim3(BLOCK), 0, c->stream, fa);
+    IVP_HIP(hipGetLastError());
+    return IVP_OK;
+}
+
+}  // namespace ivp

@@ -1,0 +1,963 @@
+// ivp_solve.hip -- IP velocity projection, Weiler 2018 implicit viscosity, explicit viscosity
+// and the step glue, on the tiled neighbour rows built by ivp_grid.hip.
+//
+// Reference: scripts/sim/forces/pressure/ip.py (IP.solve, IP.PCG, five kernels),
+// scripts/sim/forces/pressure/pressure_base.py (J, J^T, errors, divergence),
+// scripts/sim/math_utils.py (vector ops, dots), scripts/sim/forces/non_pressure/viscosity/
+// {viscosity,weiler18}.py, scripts/framework.py and scripts/sim/boundary.py.
+//
+// Operator form.  The reference applies A = M + omega J^T diag(dfdt k) J with a scatter
+// (compute_Ax_part2, ip.py:83-121: two float atomics per pair).  Neighbour rows here are
+// complete (no truncation: an overflowing row is an error) and therefore symmetric, and
+// gradW_ji = -gradW_ij exactly, so the scatter equals the gather
+//     (A p)_i = m_i p_i + omega * ( sum_{j dyn} (m_j y_i + m_i y_j) gradW_ij + y_i S_i ),
+//     y = diag(dfdt k) J p,  S_i = sum_{j static} m_j gradW_ij,
+// i.e. one J pass and one J^T pass, both pure gathers with no atomics.  Rows whose closed
+// neighbourhood holds no particle with dfdt > 0 contribute exactly 0 and skip their loop.
+//
+// Control.  CG dot products are reduced in FP64 on the device; alpha, beta and the stop test
+// (ip.py:302-327) are evaluated by a one-block finalize kernel after each reduction, and the host enqueues
+// CG iterations in chunks, synchronising once per chunk instead of twice per iteration.
+#include <math.h>
+
+#include <algorithm>
+
+#include "ivp_internal.h"
+#include "ivp_reduce.h"
+
+using namespace ivp;
+
+namespace {
+
+struct Rows {
+    const uint32_t* nb;
+    const int* tile_len;
+    int C;
+    int nd;
+};
+
+__device__ __forceinline__ int row_index() {
+    int blk = xcd_block(blockIdx.x, gridDim.x);
+    return blk * BLOCK + threadIdx.x;
+}
+
+__device__ __forceinline__ const uint32_t* row_base(const Rows& R, int i) {
+    return R.nb + (size_t)(i >> 6) * (size_t)R.C * 256u + (threadIdx.x & (WAVE - 1));
+}
+
+__device__ __forceinline__ void load_pair(const uint32_t* e, int& j, float& gx, float& gy, float& gz) {
+    j = (int)__builtin_nontemporal_load(e);
+    gx = __uint_as_float(__builtin_nontemporal_load(e + 64));
+    gy = __uint_as_float(__builtin_nontemporal_load(e + 128));
+    gz = __uint_as_float(__builtin_nontemporal_load(e + 192));
+}
+
+// ---- IP ------------------------------------------------------------------------------------
+
+// compute_constraint (pressure_base.py:169-184) and d = dt v (ip.py:343)
+__global__ __launch_bounds__(BLOCK) void k_ip_init(int nd, float dt, const float* __restrict__ density,
+                                                   const float* __restrict__ rho0, const float* __restrict__ vel,
+                                                   const float4* __restrict__ pos, float* __restrict__ cc,
+                                                   float4* __restrict__ d4) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nd) return;
+    cc[i] = density[i] - rho0[i];
+    d4[i] = make_float4(dt * vel[3 * i + 0], dt * vel[3 * i + 1], dt * vel[3 * i + 2], pos[i].w);
+}
+
+// J pass, form of ip_pack_fused / compute_J_x_kernel (ip.py:150-163, pressure_base.py:111-127):
+// sum_j m_j dot(u_i - [dyn_j] u_j, gradW_ij), static part as dot(u_i, S_i).
+__device__ __forceinline__ float J_row_mdot(const Rows& R, int i, float4 ui, const float4* __restrict__ u4,
+                                            const float* __restrict__ Ss) {
+    const uint32_t* e = row_base(R, i);
+    int tl = R.tile_len[i >> 6];
+    float acc = 0.0f;
+#pragma unroll 4
+    for (int k = 0; k < tl; ++k) {
+        int j; float gx, gy, gz;
+        load_pair(e + (size_t)k * 256u, j, gx, gy, gz);
+        float4 uj = u4[j];
+        float dx = ui.x - uj.x, dy = ui.y - uj.y, dz = ui.z - uj.z;
+        acc += uj.w * (dx * gx + dy * gy + dz * gz);
+    }
+    return acc + (ui.x * Ss[3 * i + 0] + ui.y * Ss[3 * i + 1] + ui.z * Ss[3 * i + 2]);
+}
+
+// J pass, form of compute_Jx_pcg (ip.py:51-58): sum_j dot(p_i - [dyn_j] p_j, m_j gradW_ij)
+__device__ __forceinline__ float J_row_dotm(const Rows& R, int i, float4 pi, const float4* __restrict__ p4,
+                                            const float* __restrict__ Ss) {
+    const uint32_t* e = row_base(R, i);
+    int tl = R.tile_len[i >> 6];
+    float acc = 0.0f;
+#pragma unroll 4
+    for (int k = 0; k < tl; ++k) {
+        int j; float gx, gy, gz;
+        load_pair(e + (size_t)k * 256u, j, gx, gy, gz);
+        float4 pj = p4[j];
+        float wx = pj.w * gx, wy = pj.w * gy, wz = pj.w * gz;
+        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        acc += dx * wx + dy * wy + dz * wz;
+    }
+    return acc + (pi.x * Ss[3 * i + 0] + pi.y * Ss[3 * i + 1] + pi.z * Ss[3 * i + 2]);
+}
+
+struct PackArgs {
+    Rows R;
+    const float4* d4;
+    const float* Ss;
+    const float* cc;
+    const float* kk;
+    const float* rho0;
+    const int* rowcnt;
+    float4* q4;      // (f k, m, dfdt k, 0) gathered by the rhs pass
+    float* sfac;     // dfdt k
+    double* partials;
+    int stride;
+    SolverState* st;
+};
+
+// ip_pack_fused (ip.py:124-171) + coef_wise_div + compute_avg_density_error
+// (math_utils.py:76-91, pressure_base.py:187-203, 413-423).
+__global__ __launch_bounds__(BLOCK) void k_ip_pack(PackArgs a) {
+    int i = row_index();
+    double err = 0.0, nact = 0.0, npairs = 0.0;
+    if (i < a.R.nd) {
+        float4 di = a.d4[i];
+        float t = J_row_mdot(a.R, i, di, a.d4, a.Ss) + a.cc[i];
+        float f = fmaxf(t, 0.0f);
+        float dfdt = t >= 1e-6f ? 1.0f : 0.0f;
+        float k = a.kk[i];
+        float s = dfdt * k;
+        a.sfac[i] = s;
+        a.q4[i] = make_float4(f * k, di.w, s, 0.0f);
+        float denom = a.rho0[i];
+        if (denom < 1e-6f) denom = 1e-6f;
+        err = (double)fmaxf(f / denom, 0.0f);
+        if (s != 0.0f) {
+            nact = 1.0;
+            npairs = (double)a.rowcnt[i];
+        }
+    }
+    double v[3] = {err, nact, npairs};
+    block_partials<3>(v, a.partials, a.stride);
+}
+
+struct RhsArgs {
+    Rows R;
+    const float4* q4;
+    const float* Ss;
+    float omega;
+    float* b;
+    int* nbact;
+    const int* rowcnt;
+    double* partials;
+    int stride;
+    SolverState* st;
+};
+
+// ip_rhs_fused (ip.py:174-215): b = omega J^T (f k); also marks rows whose CG J^T loop can be
+// non-zero (own dfdt > 0 or a dynamic neighbour with dfdt > 0).
+__global__ __launch_bounds__(BLOCK) void k_ip_rhs(RhsArgs a) {
+    int i = row_index();
+    double touched = 0.0, tpairs = 0.0;
+    if (i < a.R.nd) {
+        float4 qi = a.q4[i];
+        float fki = qi.x, mi = qi.y;
+        bool act = qi.z != 0.0f;
+        float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+        const uint32_t* e = row_base(a.R, i);
+        int tl = a.R.tile_len[i >> 6];
+#pragma unroll 4
+        for (int k = 0; k < tl; ++k) {
+            int j; float gx, gy, gz;
+            load_pair(e + (size_t)k * 256u, j, gx, gy, gz);
+            float4 qj = a.q4[j];
+            float s = qj.y * fki + mi * qj.x;
+            rx += s * gx; ry += s * gy; rz += s * gz;
+            act = act || (qj.z != 0.0f);
+        }
+        rx += fki * a.Ss[3 * i + 0];
+        ry += fki * a.Ss[3 * i + 1];
+        rz += fki * a.Ss[3 * i + 2];
+        a.b[3 * i + 0] = a.omega * rx;
+        a.b[3 * i + 1] = a.omega * ry;
+        a.b[3 * i + 2] = a.omega * rz;
+        a.nbact[i] = act ? 1 : 0;
+        if (act) {
+            touched = 1.0;
+            tpairs = (double)a.rowcnt[i];
+        }
+    }
+    double v[2] = {touched, tpairs};
+    block_partials<2>(v, a.partials, a.stride);
+}
+
+struct CGInitArgs {
+    int nd;
+    const float* b;
+    const float4* pos;
+    float* sol;
+    float* r;
+    float4* p4;
+    float2* y2;
+    double tol;
+    int max_iter;
+    double* partials;
+    int stride;
+    SolverState* st;
+};
+
+// IP.PCG prologue (ip.py:292-304): x = 0, r = b, rr = r.r, p = r
+__global__ __launch_bounds__(BLOCK) void k_cg_init(CGInitArgs a) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    double rr = 0.0;
+    if (i < a.nd) {
+        float bx = a.b[3 * i + 0], by = a.b[3 * i + 1], bz = a.b[3 * i + 2];
+        float m = a.pos[i].w;
+        a.sol[3 * i + 0] = 0.0f; a.sol[3 * i + 1] = 0.0f; a.sol[3 * i + 2] = 0.0f;
+        a.r[3 * i + 0] = bx; a.r[3 * i + 1] = by; a.r[3 * i + 2] = bz;
+        a.p4[i] = make_float4(bx, by, bz, m);
+        a.y2[i] = make_float2(0.0f, m);
+        rr = (double)(bx * bx + by * by + bz * bz);
+    }
+    double v[1] = {rr};
+    block_partials<1>(v, a.partials, a.stride);
+}
+
+struct CGJArgs {
+    Rows R;
+    const float4* p4;
+    const float* Ss;
+    const float* sfac;
+    float2* y2;
+    const SolverState* st;
+};
+
+// compute_Jx_pcg (ip.py:19-60): y_i = dfdt_i k_i (J p)_i on rows with dfdt > 0
+__global__ __launch_bounds__(BLOCK) void k_cg_J(CGJArgs a) {
+    if (a.st->active == 0) return;
+    int i = row_index();
+    if (i >= a.R.nd) return;
+    float s = a.sfac[i];
+    if (s == 0.0f) return;
+    float4 pi = a.p4[i];
+    float acc = J_row_dotm(a.R, i, pi, a.p4, a.Ss);
+    a.y2[i] = make_float2(acc * s, pi.w);
+}
+
+struct CGJTArgs {
+    Rows R;
+    const float4* p4;
+    const float2* y2;
+    const float* Ss;
+    const int* nbact;
+    float omega;
+    float* Ap;
+    double* partials;
+    int stride;
+    SolverState* st;
+};
+
+// compute_Ax_part1 + compute_Ax_part2 (ip.py:63-121) in gather form, fused with p.Ap
+// (ip.py:307) and alpha (ip.py:308-311)
+__global__ __launch_bounds__(BLOCK) void k_cg_JT(CGJTArgs a) {
+    SolverState* st = a.st;
+    if (st->active == 0) return;
+    int i = row_index();
+    double pap = 0.0;
+    if (i < a.R.nd) {
+        float4 pi = a.p4[i];
+        float mi = pi.w;
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        if (a.nbact[i] != 0) {
+            float yi = a.y2[i].x;
+            const uint32_t* e = row_base(a.R, i);
+            int tl = a.R.tile_len[i >> 6];
+#pragma unroll 4
+            for (int k = 0; k < tl; ++k) {
+                int j; float gx, gy, gz;
+                load_pair(e + (size_t)k * 256u, j, gx, gy, gz);
+                float2 yj = a.y2[j];
+                float s = yj.y * yi + mi * yj.x;
+                ax += s * gx; ay += s * gy; az += s * gz;
+            }
+            ax += yi * a.Ss[3 * i + 0];
+            ay += yi * a.Ss[3 * i + 1];
+            az += yi * a.Ss[3 * i + 2];
+        }
+        float Apx = mi * pi.x + a.omega * ax;
+        float Apy = mi * pi.y + a.omega * ay;
+        float Apz = mi * pi.z + a.omega * az;
+        a.Ap[3 * i + 0] = Apx; a.Ap[3 * i + 1] = Apy; a.Ap[3 * i + 2] = Apz;
+        pap = (double)(pi.x * Apx + pi.y * Apy + pi.z * Apz);
+    }
+    double v[1] = {pap};
+    block_partials<1>(v, a.partials, a.stride);
+}
+
+struct CGUpdArgs {
+    int nd;
+    float* sol;
+    float* r;
+    const float4* p4;
+    const float* Ap;
+    double* partials;
+    int stride;
+    SolverState* st;
+};
+
+// pcg_update_x_r_scalar (math_utils.py:180-195) + r.r + stop test and beta (ip.py:314-327)
+__global__ __launch_bounds__(BLOCK) void k_cg_update(CGUpdArgs a) {
+    SolverState* st = a.st;
+    if (st->active == 0) return;
+    float alpha = (float)st->alpha;
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    double rr = 0.0;
+    if (i < a.nd) {
+        float4 p = a.p4[i];
+        float x0 = a.sol[3 * i + 0] + alpha * p.x;
+        float x1 = a.sol[3 * i + 1] + alpha * p.y;
+        float x2 = a.sol[3 * i + 2] + alpha * p.z;
+        float r0 = a.r[3 * i + 0] - alpha * a.Ap[3 * i + 0];
+        float r1 = a.r[3 * i + 1] - alpha * a.Ap[3 * i + 1];
+        float r2 = a.r[3 * i + 2] - alpha * a.Ap[3 * i + 2];
+        a.sol[3 * i + 0] = x0; a.sol[3 * i + 1] = x1; a.sol[3 * i + 2] = x2;
+        a.r[3 * i + 0] = r0; a.r[3 * i + 1] = r1; a.r[3 * i + 2] = r2;
+        rr = (double)(r0 * r0 + r1 * r1 + r2 * r2);
+    }
+    double v[1] = {rr};
+    block_partials<1>(v, a.partials, a.stride);
+}
+
+// p = r + beta p (add_vec3, ip.py:323-326)
+__global__ __launch_bounds__(BLOCK) void k_cg_p(int nd, const float* __restrict__ r, float4* __restrict__ p4,
+                                                const SolverState* __restrict__ st) {
+    if (st->active == 0) return;
+    float beta = (float)st->beta;
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nd) return;
+    float4 p = p4[i];
+    p.x = r[3 * i + 0] + beta * p.x;
+    p.y = r[3 * i + 1] + beta * p.y;
+    p.z = r[3 * i + 2] + beta * p.z;
+    p4[i] = p;
+}
+
+// d -= sol (add_vec3 with scale -1, ip.py:389)
+__global__ __launch_bounds__(BLOCK) void k_ip_dupdate(int nd, float4* __restrict__ d4, const float* __restrict__ sol) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nd) return;
+    float4 d = d4[i];
+    d.x = d.x + -1.0f * sol[3 * i + 0];
+    d.y = d.y + -1.0f * sol[3 * i + 1];
+    d.z = d.z + -1.0f * sol[3 * i + 2];
+    d4[i] = d;
+}
+
+// v = (1/dt) d (ip.py:398), staged with m for the divergence pass
+__global__ __launch_bounds__(BLOCK) void k_ip_finish(int nd, float inv_dt, const float4* __restrict__ d4,
+                                                     float* __restrict__ vel, float4* __restrict__ v4) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nd) return;
+    float4 d = d4[i];
+    float vx = inv_dt * d.x, vy = inv_dt * d.y, vz = inv_dt * d.z;
+    vel[3 * i + 0] = vx; vel[3 * i + 1] = vy; vel[3 * i + 2] = vz;
+    v4[i] = make_float4(vx, vy, vz, d.w);
+}
+
+// compute_divergence (pressure_base.py:449-457, 240-254): max(J v, 0)
+__global__ __launch_bounds__(BLOCK) void k_ip_div(Rows R, const float4* __restrict__ v4, const float* __restrict__ Ss,
+                                                  float* __restrict__ divg) {
+    int i = row_index();
+    if (i >= R.nd) return;
+    divg[i] = fmaxf(J_row_mdot(R, i, v4[i], v4, Ss), 0.0f);
+}
+
+// ---- explicit viscosity (viscosity.py:9-108), Jacobi reads ---------------------------------
+
+__global__ __launch_bounds__(BLOCK) void k_v4(int nd, const float* __restrict__ vel, const float* __restrict__ density,
+                                              float4* __restrict__ v4) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nd) return;
+    v4[i] = make_float4(vel[3 * i + 0], vel[3 * i + 1], vel[3 * i + 2], density[i]);
+}
+
+template <bool ALLFLUID>
+__global__ __launch_bounds__(BLOCK) void k_visc(Rows R, float h, float coeff, float dt, int fluid,
+                                                const float4* __restrict__ pos, const float4* __restrict__ v4,
+                                                const float* __restrict__ rho0, const int* __restrict__ mat,
+                                                float* __restrict__ vel) {
+    int i = row_index();
+    if (i >= R.nd) return;
+    if (!ALLFLUID && mat[i] != fluid) return;
+    float4 xi = pos[i];
+    float4 vi = v4[i];
+    float rho_i = vi.w, rho0_i = rho0[i];
+    float eps2 = (0.01f * h) * (0.01f * h);
+    float k_visc = 2.0f * (3.0f + 2.0f);
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    const uint32_t* e = row_base(R, i);
+    int tl = R.tile_len[i >> 6];
+#pragma unroll 2
+    for (int k = 0; k < tl; ++k) {
+        int j; float gx, gy, gz;
+        load_pair(e + (size_t)k * 256u, j, gx, gy, gz);
+        if (!ALLFLUID && mat[j] != fluid) continue;
+        float4 xj = pos[j];
+        float4 vj = v4[j];
+        float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        float rn = len3(rx, ry, rz);
+        float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+        float vxy = vx * rx + vy * ry + vz * rz;
+        float rho_j = vj.w;
+        float K = 2.0f * rho0_i / (rho_i + rho_j);
+        float s = k_visc * coeff * (xj.w / rho_j) * (vxy / (rn * rn + eps2));
+        ax = ax + K * (s * gx);
+        ay = ay + K * (s * gy);
+        az = az + K * (s * gz);
+    }
+    vel[3 * i + 0] = vi.x + ax * dt;
+    vel[3 * i + 1] = vi.y + ay * dt;
+    vel[3 * i + 2] = vi.z + az * dt;
+}
+
+// ---- Weiler 2018 (weiler18.py) -------------------------------------------------------------
+
+// (v, m / rho) for the operator gathers and factor = dt mu_i / rho_i, mu_i = coeff rho0_i
+__global__ __launch_bounds__(BLOCK) void k_w18_prep(int nd, float coeff, float dt, const float* __restrict__ vel,
+                                                    const float4* __restrict__ pos, const float* __restrict__ density,
+                                                    const float* __restrict__ rho0, float4* __restrict__ v4,
+                                                    float* __restrict__ fac) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nd) return;
+    float rho = density[i];
+    v4[i] = make_float4(vel[3 * i + 0], vel[3 * i + 1], vel[3 * i + 2], pos[i].w / rho);
+    float mu = coeff * rho0[i];
+    fac[i] = dt * mu / rho;
+}
+
+__device__ __forceinline__ void inverse33(const float a[9], float out[9]) {
+    float c00 = a[4] * a[8] - a[5] * a[7];
+    float c01 = a[5] * a[6] - a[3] * a[8];
+    float c02 = a[3] * a[7] - a[4] * a[6];
+    float det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    if (det == 0.0f) {
+        for (int q = 0; q < 9; ++q) out[q] = 0.0f;
+        return;
+    }
+    float id = 1.0f / det;
+    out[0] = c00 * id;
+    out[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    out[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    out[3] = c01 * id;
+    out[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    out[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    out[6] = c02 * id;
+    out[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    out[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+}
+
+// compute_precond_weiler18 (weiler18.py:21-85): inverse of I - factor_i L_ii
+template <bool ALLFLUID>
+__global__ __launch_bounds__(BLOCK) void k_w18_pre(Rows R, float h, int fluid, const float4* __restrict__ pos,
+                                                   const float4* __restrict__ v4, const float* __restrict__ fac,
+                                                   const int* __restrict__ mat, float* __restrict__ pinv) {
+    int i = row_index();
+    if (i >= R.nd) return;
+    float* P = pinv + 9 * (size_t)i;
+    if (!ALLFLUID && mat[i] != fluid) {
+        for (int q = 0; q < 9; ++q) P[q] = (q % 4) == 0 ? 1.0f : 0.0f;
+        return;
+    }
+    float4 xi = pos[i];
+    float eps2 = 0.01f * h * h;
+    float dd = 10.0f;
+    float L[9];
+    for (int q = 0; q < 9; ++q) L[q] = 0.0f;
+    const uint32_t* e = row_base(R, i);
+    int tl = R.tile_len[i >> 6];
+    for (int k = 0; k < tl; ++k) {
+        int j; float g[3];
+        load_pair(e + (size_t)k * 256u, j, g[0], g[1], g[2]);
+        if (!ALLFLUID && mat[j] != fluid) continue;
+        float4 xj = pos[j];
+        float x[3] = {xi.x - xj.x, xi.y - xj.y, xi.z - xj.z};
+        float r_sq = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+        float c = dd * v4[j].w / (r_sq + eps2);
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) L[3 * p + q] = L[3 * p + q] + c * (g[p] * x[q]);
+    }
+    float f = fac[i];
+    float M[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) M[q] = ((q % 4) == 0 ? 1.0f : 0.0f) - f * L[q];
+    float out[9];
+    inverse33(M, out);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) P[q] = out[q];
+}
+
+// (A u)_i = u_i - factor_i * 10 sum_fluid (m_j/rho_j) (u_ij . x_ij) / (r^2 + 0.01 h^2) gradW_ij
+// (compute_Av_weiler18, weiler18.py:103-172)
+template <bool ALLFLUID>
+__device__ __forceinline__ float3 w18_Av_row(const Rows& R, int i, float h, int fluid, const float4* __restrict__ pos,
+                                             const float4* __restrict__ u4, const float* __restrict__ fac,
+                                             const int* __restrict__ mat, float4 ui) {
+    if (!ALLFLUID && mat[i] != fluid) return make_float3(ui.x, ui.y, ui.z);
+    float4 xi = pos[i];
+    float eps2 = 0.01f * h * h;
+    float dd = 10.0f;
+    float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+    const uint32_t* e = row_base(R, i);
+    int tl = R.tile_len[i >> 6];
+#pragma unroll 2
+    for (int k = 0; k < tl; ++k) {
+        int j; float gx, gy, gz;
+        load_pair(e + (size_t)k * 256u, j, gx, gy, gz);
+        if (!ALLFLUID && mat[j] != fluid) continue;
+        float4 xj = pos[j];
+        float4 uj = u4[j];
+        float x0 = xi.x - xj.x, x1 = xi.y - xj.y, x2 = xi.z - xj.z;
+        float v0 = ui.x - uj.x, v1 = ui.y - uj.y, v2 = ui.z - uj.z;
+        float r_sq = x0 * x0 + x1 * x1 + x2 * x2;
+        float vx = v0 * x0 + v1 * x1 + v2 * x2;
+        float s = dd * uj.w * (vx / (r_sq + eps2));
+        lx += s * gx; ly += s * gy; lz += s * gz;
+    }
+    float f = fac[i];
+    return make_float3(ui.x - f * lx, ui.y - f * ly, ui.z - f * lz);
+}
+
+struct W18Args {
+    Rows R;
+    float h;
+    int fluid;
+    const float4* pos;
+    const float* fac;
+    const int* mat;
+    const float* pinv;
+    float4* v4;   // operator input (v for the rhs, p for CG), w = m / rho
+    float* out;   // rhs or Ap
+    float* dv; float* r; float* z;
+    double* partials;
+    int stride;
+    SolverState* st;
+};
+
+// rhs = v - A v (weiler18.py:390-393), then the PCG prologue (weiler18.py:320-335):
+// dv = 0, r = rhs, z = P r, p = z, rz = r.z
+template <bool ALLFLUID>
+__global__ __launch_bounds__(BLOCK) void k_w18_rhs(W18Args a) {
+    int i = row_index();
+    if (i >= a.R.nd) return;
+    float4 vi = a.v4[i];
+    float3 Av = w18_Av_row<ALLFLUID>(a.R, i, a.h, a.fluid, a.pos, a.v4, a.fac, a.mat, vi);
+    a.out[3 * i + 0] = vi.x + -1.0f * Av.x;
+    a.out[3 * i + 1] = vi.y + -1.0f * Av.y;
+    a.out[3 * i + 2] = vi.z + -1.0f * Av.z;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_w18_init(W18Args a, float4* __restrict__ p4, double tol, int max_iter) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    double rz = 0.0;
+    if (i < a.R.nd) {
+        const float* P = a.pinv + 9 * (size_t)i;
+        float r0 = a.out[3 * i + 0], r1 = a.out[3 * i + 1], r2 = a.out[3 * i + 2];
+        float z0 = P[0] * r0 + P[1] * r1 + P[2] * r2;
+        float z1 = P[3] * r0 + P[4] * r1 + P[5] * r2;
+        float z2 = P[6] * r0 + P[7] * r1 + P[8] * r2;
+        a.dv[3 * i + 0] = 0.0f; a.dv[3 * i + 1] = 0.0f; a.dv[3 * i + 2] = 0.0f;
+        a.r[3 * i + 0] = r0; a.r[3 * i + 1] = r1; a.r[3 * i + 2] = r2;
+        a.z[3 * i + 0] = z0; a.z[3 * i + 1] = z1; a.z[3 * i + 2] = z2;
+        p4[i] = make_float4(z0, z1, z2, a.v4[i].w);
+        rz = (double)(r0 * z0 + r1 * z1 + r2 * z2);
+    }
+    double v[1] = {rz};
+    block_partials<1>(v, a.partials, a.stride);
+}
+
+// Ap = A p, p.Ap; pAp <= 0 stops the iteration (weiler18.py:337-342)
+template <bool ALLFLUID>
+__global__ __launch_bounds__(BLOCK) void k_w18_Av(W18Args a) {
+    SolverState* st = a.st;
+    if (st->active == 0) return;
+    int i = row_index();
+    double pap = 0.0;
+    if (i < a.R.nd) {
+        float4 pi = a.v4[i];
+        float3 Ap = w18_Av_row<ALLFLUID>(a.R, i, a.h, a.fluid, a.pos, a.v4, a.fac, a.mat, pi);
+        a.out[3 * i + 0] = Ap.x; a.out[3 * i + 1] = Ap.y; a.out[3 * i + 2] = Ap.z;
+        pap = (double)(pi.x * Ap.x + pi.y * Ap.y + pi.z * Ap.z);
+    }
+    double v[1] = {pap};
+    block_partials<1>(v, a.partials, a.stride);
+}
+
+// dv += alpha p, r -= alpha Ap, r.r, z = P r, r.z, stop test and beta (weiler18.py:344-365)
+__global__ __launch_bounds__(BLOCK) void k_w18_update(W18Args a) {
+    SolverState* st = a.st;
+    if (st->active == 0) return;
+    float alpha = (float)st->alpha;
+    float nalpha = (float)(-st->alpha);
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    double rr = 0.0, rz = 0.0;
+    if (i < a.R.nd) {
+        float4 p = a.v4[i];
+        a.dv[3 * i + 0] = a.dv[3 * i + 0] + alpha * p.x;
+        a.dv[3 * i + 1] = a.dv[3 * i + 1] + alpha * p.y;
+        a.dv[3 * i + 2] = a.dv[3 * i + 2] + alpha * p.z;
+        float r0 = a.r[3 * i + 0] + nalpha * a.out[3 * i + 0];
+        float r1 = a.r[3 * i + 1] + nalpha * a.out[3 * i + 1];
+        float r2 = a.r[3 * i + 2] + nalpha * a.out[3 * i + 2];
+        a.r[3 * i + 0] = r0; a.r[3 * i + 1] = r1; a.r[3 * i + 2] = r2;
+        const float* P = a.pinv + 9 * (size_t)i;
+        float z0 = P[0] * r0 + P[1] * r1 + P[2] * r2;
+        float z1 = P[3] * r0 + P[4] * r1 + P[5] * r2;
+        float z2 = P[6] * r0 + P[7] * r1 + P[8] * r2;
+        a.z[3 * i + 0] = z0; a.z[3 * i + 1] = z1; a.z[3 * i + 2] = z2;
+        rr = (double)(r0 * r0 + r1 * r1 + r2 * r2);
+        rz = (double)(r0 * z0 + r1 * z1 + r2 * z2);
+    }
+    double v[2] = {rr, rz};
+    block_partials<2>(v, a.partials, a.stride);
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(BLOCK) void k_w18_p(int nd, const float* __restrict__ z, float4* __restrict__ p4,
+                                                 const SolverState* __restrict__ st) {
+    if (st->active == 0) return;
+    float beta = (float)st->beta;
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nd) return;
+    float4 p = p4[i];
+    p.x = z[3 * i + 0] + beta * p.x;
+    p.y = z[3 * i + 1] + beta * p.y;
+    p.z = z[3 * i + 2] + beta * p.z;
+    p4[i] = p;
+}
+
+// v += dv (weiler18.py:399-401)
+__global__ __launch_bounds__(BLOCK) void k_w18_apply(int nd, const float* __restrict__ dv, float* __restrict__ vel) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= 3 * nd) return;
+    vel[i] = vel[i] + 1.0f * dv[i];
+}
+
+// ---- glue ----------------------------------------------------------------------------------
+
+// apply_gravity (framework.py:26-41)
+__global__ __launch_bounds__(BLOCK) void k_gravity(int nd, float gx, float gy, float gz, float* __restrict__ vel) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nd) return;
+    vel[3 * i + 0] = vel[3 * i + 0] + gx;
+    vel[3 * i + 1] = vel[3 * i + 1] + gy;
+    vel[3 * i + 2] = vel[3 * i + 2] + gz;
+}
+
+// advect_position (framework.py:117-136) + enforce_boundary (boundary.py:27-91)
+__global__ __launch_bounds__(BLOCK) void k_advect_boundary(int nd, float dt, float L0, float L1, float L2, float pad,
+                                                           float4* __restrict__ pos, float* __restrict__ vel) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nd) return;
+    float4 p = pos[i];
+    float v[3] = {vel[3 * i + 0], vel[3 * i + 1], vel[3 * i + 2]};
+    float x[3] = {p.x + dt * v[0], p.y + dt * v[1], p.z + dt * v[2]};
+    float L[3] = {L0, L1, L2};
+    float nrm[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        if (x[q] > L[q] - pad) { nrm[q] = nrm[q] + 1.0f; x[q] = L[q] - pad; }
+        if (x[q] <= pad) { nrm[q] = nrm[q] + -1.0f; x[q] = pad; }
+    }
+    float len = len3(nrm[0], nrm[1], nrm[2]);
+    if (len > 1e-6f) {
+        float c0 = nrm[0] / len, c1 = nrm[1] / len, c2 = nrm[2] / len;
+        float s = (1.0f + 0.5f) * (v[0] * c0 + v[1] * c1 + v[2] * c2);
+        vel[3 * i + 0] = v[0] - s * c0;
+        vel[3 * i + 1] = v[1] - s * c1;
+        vel[3 * i + 2] = v[2] - s * c2;
+    }
+    pos[i] = make_float4(x[0], x[1], x[2], p.w);
+}
+
+Rows rows_of(ivp_ctx* c) { return Rows{c->nb, c->tile_len, c->C, c->nd}; }
+
+int read_state(ivp_ctx* c) {
+    IVP_HIP(hipMemcpyAsync(c->state_host, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+    IVP_HIP(hipStreamSynchronize(c->stream));
+    return IVP_OK;
+}
+
+// Enqueue CG iterations in chunks until the device-side state says stop.  `launch(tag)`
+// enqueues one iteration.  Returns the iteration count in *iters.
+template <class L>
+int run_chunks(ivp_ctx* c, int max_iter, int guess, L&& launch, int* iters, int* chunks) {
+    int issued = 0;
+    int chunk = std::max(2, std::min(guess, 64));
+    double rr0 = c->state_host->rr0;
+    for (;;) {
+        int todo = std::min(chunk, max_iter - issued);
+        if (todo <= 0) break;
+        for (int q = 0; q < todo; ++q) {
+            int rc = launch(issued + q);
+            if (rc != IVP_OK) return rc;
+        }
+        issued += todo;
+        int rc = read_state(c);
+        if (rc != IVP_OK) return rc;
+        (*chunks)++;
+        const SolverState& s = *c->state_host;
+        if (s.active == 0) break;
+        // geometric extrapolation of r.r to the tolerance
+        double rate = s.iter > 0 && rr0 > 0.0 && s.rr_last > 0.0 ? pow(s.rr_last / rr0, 1.0 / s.iter) : 0.5;
+        int est = 8;
+        if (rate > 0.0 && rate < 0.999 && s.rr_last > s.tol)
+            est = (int)ceil(log(s.tol / s.rr_last) / log(rate)) + 1;
+        chunk = std::max(1, std::min(est, 64));
+    }
+    *iters = c->state_host->iter;
+    return IVP_OK;
+}
+
+}  // namespace
+
+namespace ivp {
+
+int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats) {
+    if (!c->built) return IVP_ERR_STATE;
+    if (!prm || !stats) return IVP_ERR_ARG;
+    ivp_ip_stats& st = *stats;
+    st.n_density_errors = 0;
+    st.n_pcg = 0;
+    st.outer_iters = 0;
+    st.final_density_error = 0.0;
+    st.pAp_bad = 0.0;
+    st.pcg_iter_bad = 0;
+    st.pairs_traversed = 0;
+    st.rows_traversed = 0;
+    st.cg_launch_chunks = 0;
+    const int nd = c->nd;
+    if (nd == 0) return IVP_OK;
+    const int nb = blocks_for(nd);
+    const int stride = c->max_blocks;
+    Rows R = rows_of(c);
+    float4* d4 = c->d4;
+    hipLaunchKernelGGL(k_ip_init, dim3(nb), dim3(BLOCK), 0, c->stream, nd, dt, c->density, c->rho0, c->vel, c->pos,
+                       c->cc, d4);
+    IVP_HIP(hipGetLastError());
+    int opt_iter = 0;
+    double err = 0.0;
+    static int cg_guess = 4;
+    for (;;) {
+        PackArgs pa{R, d4, c->Ss, c->cc, c->kk, c->rho0, c->rowcnt, c->q4, c->sfac, c->partials, stride, c->state};
+        ProfEvent ev;
+        prof_begin(c, KC_IP_PACK, -1, (double)nd * (16 + 4 + 4 + 4 + 12 + 4 + 16 + 4), &ev);
+        hipLaunchKernelGGL(k_ip_pack, dim3(nb), dim3(BLOCK), 0, c->stream, pa);
+        IVP_HIP(hipGetLastError());
+        prof_end(c, &ev);
+        int rc = finalize<FIN_PACK>(c, nb);
+        if (rc != IVP_OK) return rc;
+        IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        rc = read_state(c);
+        if (rc != IVP_OK) return rc;
+        if (*c->err_host & 1) return IVP_NEIGHBOR_OVERFLOW;
+        err = c->state_host->err_sum / (double)nd;
+        if (st.n_density_errors < st.capacity && st.density_errors) st.density_errors[st.n_density_errors] = err;
+        st.n_density_errors++;
+        bool converged = err < prm->tol_opt && !prm->use_fixed_iters;
+        if ((converged && opt_iter >= 1) || opt_iter >= prm->max_iteration_opt) break;
+        long long act_pairs = c->state_host->active_pairs;
+        int act_rows = c->state_host->n_active_rows;
+
+        RhsArgs ra{R, c->q4, c->Ss, prm->omega, c->b, c->nbact, c->rowcnt,
+                   c->partials, stride, c->state};
+        prof_begin(c, KC_IP_RHS, -1, (double)nd * (16 + 12 + 12 + 4 + 4), &ev);
+        hipLaunchKernelGGL(k_ip_rhs, dim3(nb), dim3(BLOCK), 0, c->stream, ra);
+        IVP_HIP(hipGetLastError());
+        prof_end(c, &ev);
+        if ((rc = finalize<FIN_RHS>(c, nb)) != IVP_OK) return rc;
+        CGInitArgs ia{nd, c->b, c->pos, c->sol, c->r, c->p4, c->y2, prm->tol_pcg, prm->max_iteration_pcg,
+                      c->partials, stride, c->state};
+        hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(BLOCK), 0, c->stream, ia);
+        IVP_HIP(hipGetLastError());
+        if ((rc = finalize<FIN_CG_INIT>(c, nb, prm->tol_pcg, prm->max_iteration_pcg)) != IVP_OK) return rc;
+        rc = read_state(c);
+        if (rc != IVP_OK) return rc;
+        long long touch_pairs = c->state_host->touched_pairs;
+        int touch_rows = c->state_host->n_touched_rows;
+        int iters = 0;
+        if (c->state_host->active) {
+            c->prof_epoch++;
+            int epoch = c->prof_epoch;
+            // algorithmic bytes per launch: pair records 16 B, per-row terms as in the kernels
+            double bJ = 16.0 * (double)act_pairs + (double)act_rows * (4 + 16 + 12 + 8) + (double)nd * 4;
+            double bJT = 16.0 * (double)touch_pairs + (double)touch_rows * (8 + 12) + (double)nd * (16 + 4 + 12);
+            double bU = (double)nd * (12 + 12 + 16 + 12 + 12 + 12);
+            double bP = (double)nd * (12 + 16 + 16);
+            CGJArgs ja{R, c->p4, c->Ss, c->sfac, c->y2, c->state};
+            CGJTArgs ta{R, c->p4, c->y2, c->Ss, c->nbact, prm->omega, c->Ap, c->partials, stride, c->state};
+            CGUpdArgs ua{nd, c->sol, c->r, c->p4, c->Ap, c->partials, stride, c->state};
+            auto launch = [&](int tag) -> int {
+                ProfEvent e1;
+                prof_begin(c, KC_CG_J, tag, bJ, &e1);
+                hipLaunchKernelGGL(k_cg_J, dim3(nb), dim3(BLOCK), 0, c->stream, ja);
+                prof_end(c, &e1);
+                prof_begin(c, KC_CG_JT, tag, bJT, &e1);
+                hipLaunchKernelGGL(k_cg_JT, dim3(nb), dim3(BLOCK), 0, c->stream, ta);
+                prof_end(c, &e1);
+                int frc = finalize<FIN_CG_PAP>(c, nb);
+                if (frc != IVP_OK) return frc;
+                prof_begin(c, KC_CG_UPDATE, tag, bU, &e1);
+                hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(BLOCK), 0, c->stream, ua);
+                prof_end(c, &e1);
+                if ((frc = finalize<FIN_CG_UPDATE>(c, nb)) != IVP_OK) return frc;
+                prof_begin(c, KC_CG_P, tag, bP, &e1);
+                hipLaunchKernelGGL(k_cg_p, dim3(nb), dim3(BLOCK), 0, c->stream, nd, c->r, c->p4, c->state);
+                prof_end(c, &e1);
+                hipError_t e = hipGetLastError();
+                return e == hipSuccess ? IVP_OK : ivp_hip_fail(e, "cg launch", __FILE__, __LINE__);
+            };
+            rc = run_chunks(c, prm->max_iteration_pcg, cg_guess, launch, &iters, &st.cg_launch_chunks);
+            if (rc != IVP_OK) return rc;
+            prof_collect(c, iters, epoch);
+            cg_guess = std::max(2, iters);
+            st.pairs_traversed += (long long)iters * (act_pairs + touch_pairs);
+            st.rows_traversed += (long long)iters * (act_rows + touch_rows);
+            if (c->state_host->status == IVP_NOT_PD) {
+                st.pAp_bad = c->state_host->pAp;
+                st.pcg_iter_bad = iters;
+                return IVP_NOT_PD;
+            }
+        }
+        if (st.n_pcg < st.capacity && st.pcg_iters) st.pcg_iters[st.n_pcg] = iters;
+        st.n_pcg++;
+        hipLaunchKernelGGL(k_ip_dupdate, dim3(nb), dim3(BLOCK), 0, c->stream, nd, d4, c->sol);
+        IVP_HIP(hipGetLastError());
+        opt_iter++;
+    }
+    st.outer_iters = opt_iter;
+    if (!prm->use_fixed_iters && err >= prm->tol_opt && opt_iter >= prm->max_iteration_opt) return IVP_NOT_CONVERGED;
+    float inv_dt = (float)(1.0 / (double)dt);
+    hipLaunchKernelGGL(k_ip_finish, dim3(nb), dim3(BLOCK), 0, c->stream, nd, inv_dt, d4, c->vel, c->p4);
+    // final_density_error (pressure_base.py:438-447) evaluates avg(max((J d + c)/rho0, 0)) at the
+    // d of the last loop evaluation, with the same per-row arithmetic: it equals that err.
+    st.final_density_error = err;
+    hipLaunchKernelGGL(k_ip_div, dim3(nb), dim3(BLOCK), 0, c->stream, R, c->p4, c->Ss, c->divg);
+    IVP_HIP(hipGetLastError());
+    return IVP_OK;
+}
+
+int visc_explicit(ivp_ctx* c, float coeff, float dt) {
+    if (!c->built) return IVP_ERR_STATE;
+    const int nd = c->nd;
+    if (nd == 0) return IVP_OK;
+    const int nb = blocks_for(nd);
+    hipLaunchKernelGGL(k_v4, dim3(nb), dim3(BLOCK), 0, c->stream, nd, c->vel, c->density, c->v4);
+    ProfEvent ev;
+    prof_begin(c, KC_VISC, -1, 0.0, &ev);
+    if (c->all_dyn_fluid)
+        hipLaunchKernelGGL(k_visc<true>, dim3(nb), dim3(BLOCK), 0, c->stream, rows_of(c), c->h, coeff, dt,
+                           c->cfg.material_fluid, c->pos, c->v4, c->rho0, c->mat, c->vel);
+    else
+        hipLaunchKernelGGL(k_visc<false>, dim3(nb), dim3(BLOCK), 0, c->stream, rows_of(c), c->h, coeff, dt,
+                           c->cfg.material_fluid, c->pos, c->v4, c->rho0, c->mat, c->vel);
+    IVP_HIP(hipGetLastError());
+    prof_end(c, &ev);
+    return IVP_OK;
+}
+
+int visc_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* out) {
+    if (!c->built) return IVP_ERR_STATE;
+    if (out) { out->cg_iters = 0; out->residual_norm2 = 0.0; }
+    const int nd = c->nd;
+    if (nd == 0) return IVP_OK;
+    const int nb = blocks_for(nd);
+    const int stride = c->max_blocks;
+    Rows R = rows_of(c);
+    hipLaunchKernelGGL(k_w18_prep, dim3(nb), dim3(BLOCK), 0, c->stream, nd, coeff, dt, c->vel, c->pos, c->density,
+                       c->rho0, c->w_v4, c->w_fac);
+    ProfEvent ev;
+    prof_begin(c, KC_W18_PRE, -1, 0.0, &ev);
+    if (c->all_dyn_fluid)
+        hipLaunchKernelGGL(k_w18_pre<true>, dim3(nb), dim3(BLOCK), 0, c->stream, R, c->h, c->cfg.material_fluid,
+                           c->pos, c->w_v4, c->w_fac, c->mat, c->w_pinv);
+    else
+        hipLaunchKernelGGL(k_w18_pre<false>, dim3(nb), dim3(BLOCK), 0, c->stream, R, c->h, c->cfg.material_fluid,
+                           c->pos, c->w_v4, c->w_fac, c->mat, c->w_pinv);
+    prof_end(c, &ev);
+    W18Args wa{R, c->h, c->cfg.material_fluid, c->pos, c->w_fac, c->mat, c->w_pinv, c->w_v4, c->w_rhs,
+               c->w_dv, c->w_r, c->w_z, c->partials, stride, c->state};
+    if (c->all_dyn_fluid)
+        hipLaunchKernelGGL(k_w18_rhs<true>, dim3(nb), dim3(BLOCK), 0, c->stream, wa);
+    else
+        hipLaunchKernelGGL(k_w18_rhs<false>, dim3(nb), dim3(BLOCK), 0, c->stream, wa);
+    hipLaunchKernelGGL(k_w18_init, dim3(nb), dim3(BLOCK), 0, c->stream, wa, c->w_p4, 1e-5, 1000);
+    IVP_HIP(hipGetLastError());
+    int rc = finalize<FIN_W18_INIT>(c, nb, 1e-5, 1000);
+    if (rc != IVP_OK) return rc;
+    rc = read_state(c);
+    if (rc != IVP_OK) return rc;
+    int iters = 0, chunks = 0;
+    if (c->state_host->active) {
+        W18Args cg = wa;
+        cg.v4 = c->w_p4;
+        cg.out = c->w_Ap;
+        auto launch = [&](int tag) -> int {
+            ProfEvent e1;
+            prof_begin(c, KC_W18_AV, tag, 0.0, &e1);
+            if (c->all_dyn_fluid)
+                hipLaunchKernelGGL(k_w18_Av<true>, dim3(nb), dim3(BLOCK), 0, c->stream, cg);
+            else
+                hipLaunchKernelGGL(k_w18_Av<false>, dim3(nb), dim3(BLOCK), 0, c->stream, cg);
+            prof_end(c, &e1);
+            int frc = finalize<FIN_W18_PAP>(c, nb);
+            if (frc != IVP_OK) return frc;
+            prof_begin(c, KC_W18_UPDATE, tag, 0.0, &e1);
+            hipLaunchKernelGGL(k_w18_update, dim3(nb), dim3(BLOCK), 0, c->stream, cg);
+            prof_end(c, &e1);
+            if ((frc = finalize<FIN_W18_UPDATE>(c, nb)) != IVP_OK) return frc;
+            prof_begin(c, KC_W18_P, tag, 0.0, &e1);
+            hipLaunchKernelGGL(k_w18_p, dim3(nb), dim3(BLOCK), 0, c->stream, nd, c->w_z, c->w_p4, c->state);
+            prof_end(c, &e1);
+            hipError_t e = hipGetLastError();
+            return e == hipSuccess ? IVP_OK : ivp_hip_fail(e, "w18 launch", __FILE__, __LINE__);
+        };
+        c->prof_epoch++;
+        int epoch = c->prof_epoch;
+        static int w_guess = 4;
+        rc = run_chunks(c, 1000, w_guess, launch, &iters, &chunks);
+        if (rc != IVP_OK) return rc;
+        prof_collect(c, iters, epoch);
+        w_guess = std::max(2, iters);
+    }
+    hipLaunchKernelGGL(k_w18_apply, dim3(blocks_for(3 * nd)), dim3(BLOCK), 0, c->stream, nd, c->w_dv, c->vel);
+    IVP_HIP(hipGetLastError());
+    if (out) {
+        out->cg_iters = iters;
+        out->residual_norm2 = c->state_host->rr_last;
+    }
+    return IVP_OK;
+}
+
+int glue_gravity(ivp_ctx* c, const float g[3], float dt) {
+    if (!c->loaded) return IVP_ERR_STATE;
+    if (c->nd == 0) return IVP_OK;
+    hipLaunchKernelGGL(k_gravity, dim3(blocks_for(c->nd)), dim3(BLOCK), 0, c->stream, c->nd, g[0] * dt, g[1] * dt,
+                       g[2] * dt, c->vel);
+    IVP_HIP(hipGetLastError());
+    return IVP_OK;
+}
+
+int glue_advect_boundary(ivp_ctx* c, float dt) {
+    if (!c->loaded) return IVP_ERR_STATE;
+    if (c->nd == 0) return IVP_OK;
+    hipLaunchKernelGGL(k_advect_boundary, dim3(blocks_for(c->nd)), dim3(BLOCK), 0, c->stream, c->nd, dt,
+                       c->cfg.domain_size[0], c->cfg.domain_size[1], c->cfg.domain_size[2], c->h, c->pos, c->vel);
+    IVP_HIP(hipGetLastError());
+    c->built = false;
+    return IVP_OK;
+}
+
+}  // namespace ivp

@@ -1,0 +1,255 @@
+"""Scene construction: the procedural subset of the reference's ``SceneLoader``
+(scripts/io/scene_loader.py) plus the synthetic benchmark scenes.
+
+Supported scene JSON blocks (reference semantics):
+  * ``FluidBlocks``      scene_loader.py:90-102, 319-336 and the lattice of
+                         ``SimulationData.add_cube`` (simulation_data.py:290-356):
+                         ``np.arange(start, end, d)`` per axis, ``indexing='ij'``.
+  * ``GeneratedBodies``  ``"type": "cube"`` hollow shells, scene_loader.py:132-140, 402-419,
+                         447-492 (every face is a full grid, so edge/corner particles repeat).
+  * ``boundaryLayers``   domain walls, scene_loader.py:188-194, 421-426, 494-545.
+
+Extension keys (not in the reference), used by the synthetic scenes:
+  * FluidBlocks ``"jitter"`` (fraction of the particle diameter d) and ``"jitterSeed"``: every
+    coordinate gets ``numpy.random.default_rng(seed).uniform(-jitter*d, +jitter*d)`` added in
+    float64 before the float32 store.
+
+Mesh-based blocks (``FluidBodies``, ``RigidBodies``, ``SolidBodies``, ``ScheduledSpawns``) need
+mesh voxelisation (trimesh in the reference) and raise ``NotImplementedError``.
+"""
+
+from dataclasses import dataclass
+from functools import reduce
+
+import numpy as np
+
+from .config import SimConfig
+
+MATERIAL_RIGID = 0
+MATERIAL_FLUID = 1
+
+
+@dataclass
+class ParticleSet:
+    """Particles in the reference's insertion order (scene_loader.py:217-429)."""
+    x: np.ndarray           # (n, 3) float32
+    v: np.ndarray           # (n, 3) float32
+    density0: np.ndarray    # (n,) float32
+    material: np.ndarray    # (n,) int32
+    is_dynamic: np.ndarray  # (n,) int32
+    object_id: np.ndarray   # (n,) int32
+
+    @property
+    def n(self) -> int:
+        return int(self.x.shape[0])
+
+
+def cube_lattice(lower_corner, cube_size, diam):
+    """Lattice of ``SimulationData.add_cube`` (simulation_data.py:316-328)."""
+    axes = [np.arange(lower_corner[i], lower_corner[i] + cube_size[i], diam) for i in range(3)]
+    n = reduce(lambda a, b: a * b, [len(a) for a in axes])
+    pos = np.array(np.meshgrid(*axes, sparse=False, indexing="ij"), dtype=np.float32)
+    return pos.reshape(-1, n).transpose().copy()
+
+
+def cube_shell(translation, size, layers, diam):
+    """``SceneLoader._generate_body`` for ``type == "cube"`` (scene_loader.py:447-492)."""
+    translation = np.array(translation, dtype=np.float32)
+    size = np.array(size, dtype=np.float32)
+    half = size / 2.0
+    start = translation - half
+    end = translation + half
+    out = []
+    for layer in range(int(layers)):
+        offset = layer * diam
+        s = start - offset
+        e = end + offset
+        xs = np.arange(s[0], e[0] + diam * 0.5, diam)
+        ys = np.arange(s[1], e[1] + diam * 0.5, diam)
+        zs = np.arange(s[2], e[2] + diam * 0.5, diam)
+        gx, gz = np.meshgrid(xs, zs, indexing="ij")
+        out.append(np.stack([gx.ravel(), np.full(gx.size, s[1]), gz.ravel()], axis=1))
+        out.append(np.stack([gx.ravel(), np.full(gx.size, e[1]), gz.ravel()], axis=1))
+        gy, gz2 = np.meshgrid(ys, zs, indexing="ij")
+        out.append(np.stack([np.full(gy.size, s[0]), gy.ravel(), gz2.ravel()], axis=1))
+        out.append(np.stack([np.full(gy.size, e[0]), gy.ravel(), gz2.ravel()], axis=1))
+        gx2, gy2 = np.meshgrid(xs, ys, indexing="ij")
+        out.append(np.stack([gx2.ravel(), gy2.ravel(), np.full(gx2.size, s[2])], axis=1))
+        out.append(np.stack([gx2.ravel(), gy2.ravel(), np.full(gx2.size, e[2])], axis=1))
+    return np.concatenate(out, axis=0).astype(np.float32)
+
+
+def boundary_walls(domain_start, domain_size, num_layers, diam):
+    """``SceneLoader._generate_boundary_walls`` (scene_loader.py:494-545)."""
+    start = np.array(domain_start, dtype=np.float32)
+    end = start + np.array(domain_size, dtype=np.float32)
+    out = []
+    for layer in range(int(num_layers)):
+        offset = layer * diam
+        xs = np.arange(start[0] - offset, end[0] + offset + diam * 0.5, diam)
+        ys = np.arange(start[1] - offset, end[1] + offset + diam * 0.5, diam)
+        zs = np.arange(start[2] - offset, end[2] + offset + diam * 0.5, diam)
+        gx, gz = np.meshgrid(xs, zs, indexing="ij")
+        out.append(np.stack([gx.ravel(), np.full(gx.size, start[1] - offset), gz.ravel()], axis=1))
+        out.append(np.stack([gx.ravel(), np.full(gx.size, end[1] + offset), gz.ravel()], axis=1))
+        gy, gz2 = np.meshgrid(ys, zs, indexing="ij")
+        out.append(np.stack([np.full(gy.size, start[0] - offset), gy.ravel(), gz2.ravel()], axis=1))
+        out.append(np.stack([np.full(gy.size, end[0] + offset), gy.ravel(), gz2.ravel()], axis=1))
+        gx2, gy2 = np.meshgrid(xs, ys, indexing="ij")
+        out.append(np.stack([gx2.ravel(), gy2.ravel(), np.full(gx2.size, start[2] - offset)], axis=1))
+        out.append(np.stack([gx2.ravel(), gy2.ravel(), np.full(gx2.size, end[2] + offset)], axis=1))
+    return np.concatenate(out, axis=0).astype(np.float32)
+
+
+def load_particles(config: SimConfig) -> ParticleSet:
+    """Particles of a scene, in the reference's insertion order: fluid blocks, generated bodies,
+    boundary walls (scene_loader.py:319-426)."""
+    for key, getter in (("FluidBodies", config.get_fluid_bodies), ("RigidBodies", config.get_rigid_bodies),
+                        ("SolidBodies", config.get_solid_bodies),
+                        ("ScheduledSpawns", config.get_scheduled_spawns)):
+        if getter():
+            raise NotImplementedError(f"scene block {key!r} needs mesh voxelisation, which is not supported")
+    diam = config.particle_diameter
+    xs, vs, rho0, mat, dyn, oid = [], [], [], [], [], []
+
+    def push(pos, vel, density, material, is_dynamic, obj):
+        n = pos.shape[0]
+        xs.append(pos.astype(np.float32))
+        vs.append(np.broadcast_to(np.asarray(vel, dtype=np.float32), (n, 3)).copy())
+        rho0.append(np.full(n, density, dtype=np.float32))
+        mat.append(np.full(n, material, dtype=np.int32))
+        dyn.append(np.full(n, is_dynamic, dtype=np.int32))
+        oid.append(np.full(n, obj, dtype=np.int32))
+
+    object_ids = []
+    for fluid in config.get_fluid_blocks():
+        object_ids.append(int(fluid["objectId"]))
+        t = np.array(fluid.get("translation", [0, 0, 0]), dtype=np.float32)
+        s = np.array(fluid.get("scale", [1, 1, 1]), dtype=np.float32)
+        start = np.array(fluid["start"], dtype=np.float32) + t
+        size = (np.array(fluid["end"], dtype=np.float32) - np.array(fluid["start"], dtype=np.float32)) * s
+        end = start + size
+        pos = cube_lattice(tuple(start), tuple(end - start), diam)
+        jitter = float(fluid.get("jitter", 0.0))
+        if jitter > 0.0:
+            rng = np.random.default_rng(int(fluid.get("jitterSeed", 0)))
+            amp = jitter * diam
+            pos = (pos.astype(np.float64) + rng.uniform(-amp, amp, size=pos.shape)).astype(np.float32)
+        push(pos, fluid.get("velocity", [0.0, 0.0, 0.0]), float(fluid.get("density", 1000.0)),
+             MATERIAL_FLUID, 1, int(fluid["objectId"]))
+    for gb in config.get_generated_bodies():
+        object_ids.append(int(gb["objectId"]))
+        if gb.get("type", "cube") != "cube":
+            raise ValueError(f"Unknown generated body type: {gb.get('type')}")
+        pos = cube_shell(gb.get("translation", [0, 0, 0]), gb.get("size", [1, 1, 1]), int(gb.get("layers", 2)), diam)
+        push(pos, [0.0, 0.0, 0.0], float(gb.get("density", 1000.0)), MATERIAL_RIGID,
+             int(bool(gb.get("isDynamic", False))), int(gb["objectId"]))
+    layers = int(config.get_cfg("boundaryLayers", 0))
+    if layers > 0:
+        pos = boundary_walls(config.domain_start, config.domain_size, layers, diam)
+        num_objects = (max(object_ids) + 1) if object_ids else 1
+        push(pos, [0.0, 0.0, 0.0], float(config.density0), MATERIAL_RIGID, 0, num_objects)
+    if not xs:
+        z = np.zeros((0, 3), np.float32)
+        return ParticleSet(z, z.copy(), np.zeros(0, np.float32), np.zeros(0, np.int32),
+                           np.zeros(0, np.int32), np.zeros(0, np.int32))
+    return ParticleSet(np.concatenate(xs), np.concatenate(vs), np.concatenate(rho0),
+                       np.concatenate(mat), np.concatenate(dyn), np.concatenate(oid))
+
+
+# ----------------------------------------------------------------------------------------------
+# Synthetic benchmark scenes
+# ----------------------------------------------------------------------------------------------
+
+# (fluid counts nx, ny, nz; box L; particle radius; margin; fluid placement; dt; steps;
+#  viscosity method, coefficient; seed)
+_SYNTHETIC = {
+    0: dict(n=(16, 32, 16), L=(0.64, 1.0, 0.64), r=0.01, o=0.1, place="centred", dt=1e-3, steps=100,
+            visc=(0, 0.0), seed=0, name="column_8k"),
+    1: dict(n=(40, 64, 40), L=(1.6, 3.0, 1.6), r=0.01, o=0.1, place="centred", dt=1e-3, steps=200,
+            visc=(0, 0.01), seed=1, name="column_102k"),
+    2: dict(n=(100, 100, 100), L=(4.0, 1.5, 1.0), r=0.005, o=0.05, place="corner", dt=5e-4, steps=100,
+            visc=(1, 0.01), seed=2, name="dambreak_1m"),
+}
+
+
+def synthetic_scene(cfg_id: int, scale=None) -> dict:
+    """Scene JSON dict of a synthetic benchmark scene.
+
+    A fluid block of nx*ny*nz particles sits at voxel centres (k + 1/2) d with a uniform jitter of
+    +-0.01 d, resting on the floor of a static two-layer particle box whose inner faces are at 0
+    and L (shifted by the margin o inside the domain [0, L + 2o], so the domain clamp never
+    engages).  ``scale`` optionally replaces the fluid counts (for reduced test sizes).
+      0: 16x32x16 = 8192 particles, box 0.64 x 1.0 x 0.64 m, r = 0.01, dt = 1 ms, no viscosity.
+      1: 40x64x40 = 102400 particles, box 1.6 x 3.0 x 1.6 m, r = 0.01, dt = 1 ms,
+         explicit viscosity 0.01.
+      2: 100^3 = 1M particles in the corner [0,1]^3 of a 4.0 x 1.5 x 1.0 m box, r = 0.005,
+         dt = 0.5 ms, implicit (Weiler 2018) viscosity 0.01.
+    y is the gravity axis.
+    """
+    s = _SYNTHETIC[cfg_id]
+    n = tuple(scale) if scale is not None else s["n"]
+    r = s["r"]
+    d = 2.0 * r
+    L = s["L"]
+    o = s["o"]
+    if s["place"] == "centred":
+        off = ((L[0] - n[0] * d) / 2.0, 0.0, (L[2] - n[2] * d) / 2.0)
+    else:
+        off = (0.0, 0.0, 0.0)
+    start = [o + off[a] + 0.5 * d for a in range(3)]
+    # end sits half a spacing past the last centre so np.arange yields exactly n points
+    end = [start[a] + (n[a] - 0.5) * d for a in range(3)]
+    visc_method, visc_coeff = s["visc"]
+    return {
+        "Configuration": {
+            "domainStart": [0.0, 0.0, 0.0],
+            "domainEnd": [L[0] + 2 * o, L[1] + 2 * o, L[2] + 2 * o],
+            "particleRadius": r,
+            "density0": 1000.0,
+            "gravitation": [0.0, -9.81, 0.0],
+            "timeStepSize": s["dt"],
+            "viscosityMethod": visc_method,
+            "viscosityCoeff": visc_coeff,
+            "surfaceTensionMethod": 0,
+            "surfaceTensionCoeff": 0.0,
+            "pressureMethod": 1,
+            "densityErrorTol": 1e-3,
+            "maxOuterIters": 100,
+            "omega": 10.0,
+            "tol_pcg": 1e-6,
+            "neighborCacheSize": 60,
+            "steps": s["steps"],
+            "name": s["name"],
+        },
+        "FluidBlocks": [{
+            "objectId": 0,
+            "start": start,
+            "end": end,
+            "velocity": [0.0, 0.0, 0.0],
+            "density": 1000.0,
+            "jitter": 0.01,
+            "jitterSeed": s["seed"],
+        }],
+        "GeneratedBodies": [{
+            "objectId": 1,
+            "type": "cube",
+            "translation": [o + L[0] / 2.0, o + L[1] / 2.0, o + L[2] / 2.0],
+            "size": list(L),
+            "layers": 2,
+            "density": 1000.0,
+            "isDynamic": False,
+        }],
+    }
+
+
+def particle_masses(config: SimConfig, ps: ParticleSet):
+    """Mass convention of ``add_particle`` (simulation_data.py:40-57, 137):
+    m = 0.8 d^3 rho0; static particles m *= 10 and m_inv = 0; dynamic m_inv = 1 / m."""
+    v0 = np.float32(0.8 * config.particle_diameter ** 3)
+    m = (v0 * ps.density0).astype(np.float32)
+    dyn = ps.is_dynamic == 1
+    m_inv = np.zeros_like(m)
+    m_inv[dyn] = np.float32(1.0) / m[dyn]
+    m[~dyn] = m[~dyn] * np.float32(10.0)
+    return m, m_inv

@@ -1,0 +1,174 @@
+/*
+ * ivp.h -- C ABI of libivp.so, the MI355X (gfx950) implementation of the reference's
+ * Implicit Velocity Projection hot path (medialab-ku/ImplicitVelocityProjection).
+ *
+ * Every pointer argument named x, v, m, ... is a DEVICE pointer to a contiguous array in the
+ * caller's particle order ("user order"), laid out like the reference's Warp arrays:
+ * vec3 fields are float[3*n], scalar fields float[n] / int[n].  Host structs are plain PODs.
+ * All work is enqueued on the stream given to ivp_create; entry points that return solver
+ * statistics synchronise that stream once (the reference synchronises on every dot product).
+ *
+ * Return values: IVP_OK, a positive solver status, or a negative error code.
+ * ivp_status_string() turns either into text.
+ */
+#ifndef IVP_H
+#define IVP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------- */
+#define IVP_OK                     0
+#define IVP_NOT_CONVERGED          1  /* pressure_base.py:487-495 check_converged raise      */
+#define IVP_NOT_PD                 2  /* ip.py:308-309 "non-positive definite matrix"         */
+#define IVP_NEIGHBOR_OVERFLOW      3  /* a row has more than cache_size neighbours; the
+                                         reference truncates silently (neighbour_search.py:45) */
+#define IVP_ERR_ARG               -1
+#define IVP_ERR_HIP               -2
+#define IVP_ERR_CAPACITY          -3
+#define IVP_ERR_STATE             -4
+#define IVP_ERR_GRID              -5
+
+/* ---- configuration ----------------------------------------------------------------------- */
+typedef struct ivp_ctx ivp_ctx;
+
+/* SimConfig fields the hot path reads (scripts/io/config_builder.py:10-100) */
+typedef struct {
+    float support_radius;    /* h = float32(4 * particleRadius) (config_builder.py:52)     */
+    float domain_start[3];   /* domainStart                                                */
+    float domain_size[3];    /* domainEnd - domainStart, float32 (simulation_data.py:124)  */
+    int cache_size;          /* neighborCacheSize (simulation_data.py:157), default 60     */
+    int material_fluid;      /* SimulationData.material_fluid (simulation_data.py:128) = 1 */
+} ivp_config;
+
+/* IP solver parameters, read at every ivp_ip_solve call (the GUI edits them live):
+ * pressure_base.py:314-317, ip.py:228-229 */
+typedef struct {
+    float omega;             /* omega              */
+    double tol_opt;          /* densityErrorTol    */
+    int max_iteration_opt;   /* maxOuterIters      */
+    int use_fixed_iters;     /* fixedOuterIters    */
+    double tol_pcg;          /* tol_pcg            */
+    int max_iteration_pcg;   /* max_iteration_pcg  */
+} ivp_ip_params;
+
+/* Per-solve statistics (pressure_base.py:332-339, ip.py:365-404).  The caller owns the two
+ * host arrays; entries beyond `capacity` are counted but not stored. */
+typedef struct {
+    int capacity;
+    double *density_errors;      /* stats_density_errors, one per outer-loop evaluation   */
+    int *pcg_iters;              /* stats_pcg_iters, one per update                        */
+    int n_density_errors;
+    int n_pcg;
+    int outer_iters;             /* stats_outer_iters                                      */
+    double final_density_error;  /* stats_final_density_error                              */
+    double pAp_bad;              /* p^T A p when IVP_NOT_PD is returned                    */
+    int pcg_iter_bad;            /* CG iteration at which IVP_NOT_PD was detected          */
+    long long pairs_traversed;   /* neighbour pairs read by the CG operator passes         */
+    long long rows_traversed;    /* rows whose neighbour loop ran in the CG operator passes */
+    int cg_launch_chunks;        /* host synchronisations spent on CG termination          */
+} ivp_ip_stats;
+
+/* Weiler18 solve statistics */
+typedef struct {
+    int cg_iters;
+    double residual_norm2;
+} ivp_cg_stats;
+
+/* One Framework.forward step (framework.py:423-544), velocity-based branch */
+typedef struct {
+    float dt;                /* timeStepSize                                       */
+    float gravity[3];        /* gravitation                                        */
+    int viscosity_method;    /* 0 explicit (Jacobi), 1 Weiler18 implicit           */
+    float viscosity_coeff;   /* viscosityCoeff; 0 with method 0 skips the pass     */
+    ivp_ip_params ip;
+} ivp_step_params;
+
+/* Fields readable with ivp_read / writable with ivp_write */
+enum {
+    IVP_FIELD_X = 0,          /* float[3n] positions            */
+    IVP_FIELD_V = 1,          /* float[3n] velocities           */
+    IVP_FIELD_DENSITY = 2,    /* float[n]  density              */
+    IVP_FIELD_K = 3,          /* float[n]  1/(Aii + eps)        */
+    IVP_FIELD_DIVERGENCE = 4  /* float[n]  max(J v, 0)          */
+};
+
+/* ---- lifecycle ---------------------------------------------------------------------------- */
+const char *ivp_version(void);
+const char *ivp_status_string(int status);
+
+/* SimulationData.allocate + NeighbourSearch.__init__ (simulation_data.py:165-212,
+ * neighbour_search.py:93-114).  `stream` is a hipStream_t (NULL: the null stream). */
+int ivp_create(const ivp_config *cfg, int max_particles, void *stream, ivp_ctx **out);
+int ivp_destroy(ivp_ctx *ctx);
+
+/* SimulationData.add_particles (simulation_data.py:214-287) for the whole particle set at once.
+ * m and m_inv follow add_particle (simulation_data.py:40-57).  Dynamic particles are stored
+ * first, sorted by cell; static particles are sorted once here and never move. */
+int ivp_set_particles(ivp_ctx *ctx, int n, const float *x, const float *v, const float *m,
+                      const float *m_inv, const float *density0, const int *is_dynamic,
+                      const int *material);
+
+/* Copy a user-order field in or out of the library's particle store.  ivp_write accepts
+ * IVP_FIELD_X (dynamic particles only; static particles keep the positions given to
+ * ivp_set_particles) and IVP_FIELD_V. */
+int ivp_write(ivp_ctx *ctx, int field, const void *src);
+int ivp_read(ivp_ctx *ctx, int field, void *dst);
+
+/* ---- neighbour search + density + Aii ---------------------------------------------------- */
+/* NeighbourSearch.build + build_cache (neighbour_search.py:116-155) fused with
+ * PressureBase.compute_density and compute_Aii (pressure_base.py:12-87, 341-365):
+ * cell keys, counting sort, reorder, neighbour rows, density, k = 1/(Aii + 1e-3). */
+int ivp_neighbors_build(ivp_ctx *ctx);
+
+/* The reference's neighbour cache layout (simulation_data.py:206-209): count[n] (capped at
+ * cache_size), ids[n*cache_size], gradW[3*n*cache_size], user particle ids, every row
+ * (static rows included).  Row order: dynamic neighbours by ascending (cell key, id), then
+ * static neighbours by ascending (cell key, id). */
+int ivp_neighbors_export(ivp_ctx *ctx, int cache_size, int *count, int *ids, float *gradW);
+
+/* ---- solvers ----------------------------------------------------------------------------- */
+/* IP.solve (ip.py:331-404): reads the stored v, density, k; writes v and divergence. */
+int ivp_ip_solve(ivp_ctx *ctx, float dt, const ivp_ip_params *prm, ivp_ip_stats *stats);
+
+/* Viscosity.solve (viscosity.py:134-164) with out-of-place (Jacobi) reads. */
+int ivp_viscosity_explicit(ivp_ctx *ctx, float coeff, float dt);
+
+/* Weiler18.solve (weiler18.py:371-401), block-Jacobi PCG, cold start. */
+int ivp_viscosity_weiler18(ivp_ctx *ctx, float coeff, float dt, ivp_cg_stats *stats);
+
+/* apply_gravity (framework.py:26-41) */
+int ivp_apply_gravity(ivp_ctx *ctx, const float gravity[3], float dt);
+
+/* advect_position + enforce_boundary (framework.py:117-136, boundary.py:27-91) */
+int ivp_advect_boundary(ivp_ctx *ctx, float dt);
+
+/* Framework.forward (framework.py:423-544), velocity-based branch with IP: gravity,
+ * viscosity, IP solve, advect, boundary, neighbour rebuild, density, Aii. */
+int ivp_step(ivp_ctx *ctx, const ivp_step_params *prm, ivp_ip_stats *stats, ivp_cg_stats *visc);
+
+/* ---- instrumentation ---------------------------------------------------------------------- */
+/* Per-kernel-class event timing on the context's stream.  Only launches that did work are
+ * counted (CG launches after convergence inside a launch chunk are excluded). */
+typedef struct {
+    char name[32];
+    long long launches;
+    double total_ms;
+    double bytes;            /* algorithmic bytes moved by the counted launches */
+} ivp_kernel_timing;
+
+/* enable: 0 off, otherwise a bit mask of kernel classes (bit k = class k of ivp_profile_read;
+ * -1 selects all) */
+int ivp_profile_enable(ivp_ctx *ctx, int enable);
+int ivp_profile_read(ivp_ctx *ctx, ivp_kernel_timing *out, int capacity);
+int ivp_profile_reset(ivp_ctx *ctx);
+
+/* Particle counts of the loaded set (n_total, n_dynamic) and the number of dynamic-dynamic
+ * neighbour pairs stored by the last build (synchronises the stream when requested). */
+int ivp_counts(ivp_ctx *ctx, int *n_total, int *n_dynamic, long long *dyn_pairs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,343 @@
+"""Python front end of the CPU oracle (test infrastructure only).
+
+The oracle is the checker for the HIP product path: only tests/, ``__graft_entry__.smoke()`` and
+the cpu_baseline leg of bench.py import it.  It wraps ``oracle/_build/libivp_oracle.so`` (built
+from oracle/ivp_oracle.c by oracle/Makefile) with numpy arrays in the reference's layout and
+composes the reference's step (scripts/framework.py:423-544, velocity-based branch) from the
+restated kernels.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libivp_oracle.so")
+_lib = None
+
+ORC_OK, ORC_NOT_CONVERGED, ORC_NOT_PD = 0, 1, 2
+VISC_INPLACE, VISC_JACOBI = 0, 1
+
+
+class OrcGrid(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_float * 3), ("inv_w", ctypes.c_float), ("dims", ctypes.c_int * 3)]
+
+
+class OrcIPParams(ctypes.Structure):
+    _fields_ = [("omega", ctypes.c_float), ("tol_opt", ctypes.c_double), ("max_iteration_opt", ctypes.c_int),
+                ("use_fixed_iters", ctypes.c_int), ("tol_pcg", ctypes.c_double),
+                ("max_iteration_pcg", ctypes.c_int)]
+
+
+class OrcIPStats(ctypes.Structure):
+    _fields_ = [("capacity", ctypes.c_int), ("density_errors", ctypes.POINTER(ctypes.c_double)),
+                ("pcg_iters", ctypes.POINTER(ctypes.c_int)), ("n_density_errors", ctypes.c_int),
+                ("n_pcg", ctypes.c_int), ("outer_iters", ctypes.c_int),
+                ("final_density_error", ctypes.c_double), ("final_err_loop", ctypes.c_double),
+                ("pAp_bad", ctypes.c_double), ("pcg_iter_bad", ctypes.c_int)]
+
+
+def build(force=False):
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "-s"], cwd=_HERE, check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _lib.orc_cubic_kernel.restype = ctypes.c_float
+        _lib.orc_cubic_kernel.argtypes = [ctypes.c_float, ctypes.c_float]
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def cubic_kernel(r, h):
+    return float(lib().orc_cubic_kernel(ctypes.c_float(r), ctypes.c_float(h)))
+
+
+def spiky_grad(r, h):
+    r = _f32(r)
+    out = np.zeros(3, np.float32)
+    lib().orc_spiky_grad(_p(r), ctypes.c_float(h), _p(out))
+    return out
+
+
+def grid_init(domain_start, domain_size, h):
+    g = OrcGrid()
+    ds = _f32(domain_start)
+    dz = _f32(domain_size)
+    lib().orc_grid_init(_p(ds), _p(dz), ctypes.c_float(h), ctypes.byref(g))
+    return g
+
+
+def cell_keys(grid, x):
+    x = _f32(x)
+    key = np.zeros(x.shape[0], np.int32)
+    lib().orc_cell_keys(ctypes.byref(grid), ctypes.c_int(x.shape[0]), _p(x), _p(key))
+    return key
+
+
+def build_neighbors(grid, x, is_dynamic, h, cache_size):
+    """Reference-layout neighbour cache: (count [n], ids [n*C], gradW [n*C, 3], max_total)."""
+    x = _f32(x)
+    dyn = _i32(is_dynamic)
+    n = x.shape[0]
+    count = np.zeros(n, np.int32)
+    ids = np.zeros(n * cache_size, np.int32)
+    gradW = np.zeros((n * cache_size, 3), np.float32)
+    mx = ctypes.c_int(0)
+    rc = lib().orc_build_neighbors(ctypes.byref(grid), ctypes.c_int(n), _p(x), _p(dyn), ctypes.c_float(h),
+                                   ctypes.c_int(cache_size), _p(count), _p(ids), _p(gradW), ctypes.byref(mx))
+    if rc != 0:
+        raise MemoryError("oracle neighbour build failed")
+    return count, ids, gradW, int(mx.value)
+
+
+def brute_neighbors(x, h, cap):
+    x = _f32(x)
+    n = x.shape[0]
+    count = np.zeros(n, np.int32)
+    ids = np.zeros(n * cap, np.int32)
+    lib().orc_brute_neighbors(ctypes.c_int(n), _p(x), ctypes.c_float(h), ctypes.c_int(cap), _p(count), _p(ids))
+    return count, ids
+
+
+def density(x, m, is_dynamic, h, C, count, ids, out=None):
+    x, m, dyn, count, ids = _f32(x), _f32(m), _i32(is_dynamic), _i32(count), _i32(ids)
+    out = np.zeros(x.shape[0], np.float32) if out is None else out
+    lib().orc_density(ctypes.c_int(x.shape[0]), _p(x), _p(m), _p(dyn), ctypes.c_float(h), ctypes.c_int(C),
+                      _p(count), _p(ids), _p(out))
+    return out
+
+
+def aii(m, m_inv, is_dynamic, C, count, ids, gradW):
+    m, m_inv, dyn = _f32(m), _f32(m_inv), _i32(is_dynamic)
+    count, ids, gradW = _i32(count), _i32(ids), _f32(gradW)
+    out = np.zeros(m.shape[0], np.float32)
+    lib().orc_aii(ctypes.c_int(m.shape[0]), _p(m), _p(m_inv), _p(dyn), ctypes.c_int(C), _p(count), _p(ids),
+                  _p(gradW), _p(out))
+    return out
+
+
+def J(xv, m, is_dynamic, C, count, ids, gradW):
+    xv, m, dyn = _f32(xv), _f32(m), _i32(is_dynamic)
+    out = np.zeros(m.shape[0], np.float32)
+    lib().orc_J(ctypes.c_int(m.shape[0]), _p(xv), _p(m), _p(dyn), ctypes.c_int(C), _p(_i32(count)),
+                _p(_i32(ids)), _p(_f32(gradW)), _p(out))
+    return out
+
+
+def JT(y, m, is_dynamic, C, count, ids, gradW):
+    y, m, dyn = _f32(y), _f32(m), _i32(is_dynamic)
+    out = np.zeros((m.shape[0], 3), np.float32)
+    lib().orc_JT(ctypes.c_int(m.shape[0]), _p(y), _p(m), _p(dyn), ctypes.c_int(C), _p(_i32(count)),
+                 _p(_i32(ids)), _p(_f32(gradW)), _p(out))
+    return out
+
+
+def ip_apply_A(m, is_dynamic, C, count, ids, gradW, k, dfdt, omega, p):
+    m, dyn, k, dfdt, p = _f32(m), _i32(is_dynamic), _f32(k), _f32(dfdt), _f32(p)
+    out = np.zeros((m.shape[0], 3), np.float32)
+    lib().orc_ip_apply_A(ctypes.c_int(m.shape[0]), _p(m), _p(dyn), ctypes.c_int(C), _p(_i32(count)),
+                         _p(_i32(ids)), _p(_f32(gradW)), _p(k), _p(dfdt), ctypes.c_float(omega), _p(p), _p(out))
+    return out
+
+
+class IPResult:
+    def __init__(self, status, stats, density_errors, pcg_iters):
+        self.status = status
+        self.outer_iters = stats.outer_iters
+        self.final_density_error = stats.final_density_error
+        self.density_errors = density_errors
+        self.pcg_iters = pcg_iters
+        self.pAp_bad = stats.pAp_bad
+        self.pcg_iter_bad = stats.pcg_iter_bad
+        self.last_err = stats.final_err_loop
+
+
+def ip_solve(v, m, dens, density0, is_dynamic, C, count, ids, gradW, k, dt, omega=10.0, tol_opt=1e-3,
+             max_iteration_opt=1000, use_fixed_iters=False, tol_pcg=1e-6, max_iteration_pcg=1000):
+    """IP.solve (ip.py:331-404) on copies; returns (v_new, divergence, IPResult)."""
+    v = _f32(v).copy()
+    n = v.shape[0]
+    div = np.zeros(n, np.float32)
+    prm = OrcIPParams(omega, tol_opt, max_iteration_opt, int(bool(use_fixed_iters)), tol_pcg, max_iteration_pcg)
+    cap = max_iteration_opt + 2
+    errs = np.zeros(cap, np.float64)
+    its = np.zeros(cap, np.int32)
+    st = OrcIPStats()
+    st.capacity = cap
+    st.density_errors = errs.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    st.pcg_iters = its.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    status = lib().orc_ip_solve(ctypes.c_int(n), _p(v), _p(_f32(m)), _p(_f32(dens)), _p(_f32(density0)),
+                                _p(_i32(is_dynamic)), ctypes.c_int(C), _p(_i32(count)), _p(_i32(ids)),
+                                _p(_f32(gradW)), _p(_f32(k)), ctypes.c_float(dt), ctypes.byref(prm), _p(div),
+                                ctypes.byref(st))
+    res = IPResult(status, st, list(errs[:min(st.n_density_errors, cap)]), list(its[:min(st.n_pcg, cap)]))
+    return v, div, res
+
+
+def viscosity(x, v, m, dens, density0, material, is_dynamic, h, C, count, ids, gradW, coeff, dt,
+              mode=VISC_JACOBI, material_fluid=1):
+    v = _f32(v).copy()
+    lib().orc_viscosity(ctypes.c_int(v.shape[0]), _p(_f32(x)), _p(v), _p(_f32(m)), _p(_f32(dens)),
+                        _p(_f32(density0)), _p(_i32(material)), _p(_i32(is_dynamic)), ctypes.c_float(h),
+                        ctypes.c_int(C), _p(_i32(count)), _p(_i32(ids)), _p(_f32(gradW)), ctypes.c_float(coeff),
+                        ctypes.c_float(dt), ctypes.c_int(material_fluid), ctypes.c_int(mode))
+    return v
+
+
+def inverse33(a):
+    a = _f32(a).reshape(9)
+    out = np.zeros(9, np.float32)
+    lib().orc_inverse33(_p(a), _p(out))
+    return out.reshape(3, 3)
+
+
+def w18_apply(x, m, dens, density0, material, is_dynamic, h, C, count, ids, gradW, coeff, dt, vin=None,
+              material_fluid=1):
+    """(A v_in, block-Jacobi inverses) of the Weiler18 operator (weiler18.py:21-172)."""
+    n = _f32(x).shape[0]
+    Av = np.zeros((n, 3), np.float32)
+    P = np.zeros((n, 9), np.float32)
+    vin_a = _f32(vin) if vin is not None else None
+    lib().orc_w18_apply_A(ctypes.c_int(n), _p(_f32(x)), _p(_f32(m)), _p(_f32(dens)), _p(_f32(density0)),
+                          _p(_i32(material)), _p(_i32(is_dynamic)), ctypes.c_float(h), ctypes.c_int(C),
+                          _p(_i32(count)), _p(_i32(ids)), _p(_f32(gradW)), ctypes.c_float(coeff),
+                          ctypes.c_float(dt), ctypes.c_int(material_fluid),
+                          _p(vin_a) if vin_a is not None else None, _p(Av) if vin_a is not None else None,
+                          _p(P))
+    return Av, P.reshape(n, 3, 3)
+
+
+def weiler18(x, v, m, dens, density0, material, is_dynamic, h, C, count, ids, gradW, coeff, dt,
+             material_fluid=1):
+    v = _f32(v).copy()
+    it = lib().orc_weiler18(ctypes.c_int(v.shape[0]), _p(_f32(x)), _p(v), _p(_f32(m)), _p(_f32(dens)),
+                            _p(_f32(density0)), _p(_i32(material)), _p(_i32(is_dynamic)), ctypes.c_float(h),
+                            ctypes.c_int(C), _p(_i32(count)), _p(_i32(ids)), _p(_f32(gradW)),
+                            ctypes.c_float(coeff), ctypes.c_float(dt), ctypes.c_int(material_fluid))
+    return v, int(it)
+
+
+def gravity(v, is_dynamic, g, dt):
+    v = _f32(v).copy()
+    lib().orc_gravity(ctypes.c_int(v.shape[0]), _p(v), _p(_i32(is_dynamic)), _p(_f32(g)), ctypes.c_float(dt))
+    return v
+
+
+def advect(x, v, is_dynamic, dt):
+    x, v = _f32(x).copy(), _f32(v).copy()
+    lib().orc_advect(ctypes.c_int(x.shape[0]), _p(x), None, _p(v), _p(_i32(is_dynamic)), ctypes.c_float(dt))
+    return x, v
+
+
+def enforce_boundary(x, v, is_dynamic, domain_size, padding):
+    x, v = _f32(x).copy(), _f32(v).copy()
+    lib().orc_enforce_boundary(ctypes.c_int(x.shape[0]), _p(x), _p(v), _p(_i32(is_dynamic)),
+                               _p(_f32(domain_size)), ctypes.c_float(padding))
+    return x, v
+
+
+# ----------------------------------------------------------------------------------------------
+# Whole-step composition (Framework, scripts/framework.py)
+# ----------------------------------------------------------------------------------------------
+
+class OracleSim:
+    """The reference's velocity-based IP step on numpy state, built from the restated kernels.
+
+    State arrays are in the reference's particle order.  ``visc_mode`` selects how the in-place
+    explicit viscosity kernel (viscosity.py:9-108) is executed: VISC_JACOBI (every row reads the
+    pre-step velocities) or VISC_INPLACE (serial index order)."""
+
+    class Diverged(RuntimeError):
+        pass
+
+    def __init__(self, config, particles, visc_mode=VISC_JACOBI):
+        from implicitvelocityprojection_amd.scenes import particle_masses
+        self.cfg = config
+        self.h = float(np.float32(config.support_radius))
+        self.C = int(config.get_cfg("neighborCacheSize", 60))
+        self.x = particles.x.copy()
+        self.v = particles.v.copy()
+        self.density0 = particles.density0.copy()
+        self.density = particles.density0.copy()
+        self.is_dynamic = particles.is_dynamic.copy()
+        self.material = particles.material.copy()
+        self.m, self.m_inv = particle_masses(config, particles)
+        self.k = np.zeros(particles.n, np.float32)
+        self.divergence = np.zeros(particles.n, np.float32)
+        self.grid = grid_init(config.domain_start, config.domain_size, self.h)
+        self.dt = float(config.dt)
+        self.g = np.array(config.gravity, np.float32)
+        self.visc_mode = visc_mode
+        self.ip_kw = dict(
+            omega=float(config.get_cfg("omega", 10.0)),
+            tol_opt=float(config.get_cfg("densityErrorTol", 1e-3)),
+            max_iteration_opt=int(config.get_cfg("maxOuterIters", 1000)),
+            use_fixed_iters=bool(config.get_cfg("fixedOuterIters", False)),
+            tol_pcg=float(config.get_cfg("tol_pcg", 1e-6)),
+        )
+        self.last_ip = None
+        self.last_visc_iters = 0
+        self.max_total = 0
+
+    def rebuild(self):
+        """Framework._rebuild_neighbors_and_density (framework.py:282-290)."""
+        self.count, self.ids, self.gradW, self.max_total = build_neighbors(
+            self.grid, self.x, self.is_dynamic, self.h, self.C)
+        density(self.x, self.m, self.is_dynamic, self.h, self.C, self.count, self.ids, out=self.density)
+        self.k = aii(self.m, self.m_inv, self.is_dynamic, self.C, self.count, self.ids, self.gradW)
+
+    def non_pressure(self):
+        """compute_non_pressure_forces (framework.py:400-421); surface tension at coefficient 0
+        is an exact no-op and is skipped."""
+        self.v = gravity(self.v, self.is_dynamic, self.g, self.dt)
+        method = int(self.cfg.viscosity_method)
+        coeff = float(self.cfg.viscosity_coeff)
+        if method == 1:
+            self.v, self.last_visc_iters = weiler18(self.x, self.v, self.m, self.density, self.density0,
+                                                    self.material, self.is_dynamic, self.h, self.C, self.count,
+                                                    self.ids, self.gradW, coeff, self.dt)
+        elif coeff != 0.0:
+            self.v = viscosity(self.x, self.v, self.m, self.density, self.density0, self.material,
+                               self.is_dynamic, self.h, self.C, self.count, self.ids, self.gradW, coeff,
+                               self.dt, mode=self.visc_mode)
+
+    def pressure(self):
+        v, div, res = ip_solve(self.v, self.m, self.density, self.density0, self.is_dynamic, self.C,
+                               self.count, self.ids, self.gradW, self.k, self.dt, **self.ip_kw)
+        self.last_ip = res
+        if res.status == ORC_NOT_PD:
+            raise self.Diverged(f"[IP] non-positive definite matrix at PCG iter {res.pcg_iter_bad} "
+                                f"(pAp={res.pAp_bad:.2e})")
+        if res.status == ORC_NOT_CONVERGED:
+            raise self.Diverged(f"[IP] does not converge! density err={res.last_err:.2e} "
+                                f"(tol={self.ip_kw['tol_opt']:.2e}) after {res.outer_iters} iters")
+        self.v, self.divergence = v, div
+
+    def step(self):
+        """Framework.forward (framework.py:423-544), velocity-based branch, fixed dt."""
+        self.non_pressure()
+        self.pressure()
+        self.x, self.v = advect(self.x, self.v, self.is_dynamic, self.dt)
+        self.x, self.v = enforce_boundary(self.x, self.v, self.is_dynamic,
+                                          np.array(self.cfg.domain_size, np.float32), self.h)
+        self.rebuild()

@@ -1,0 +1,150 @@
+"""GPU parity of libivp against the CPU oracle on the same inputs (teacher-forced states).
+
+Bars: neighbour caches (counts, ids in row order, gradW bits), density and k are bit-exact;
+IP / viscosity velocities within 1e-5 relative (max-norm); CG iteration counts within +-1;
+final density error at or below densityErrorTol.
+"""
+
+import numpy as np
+import pytest
+
+from tests.helpers import gpu_context, ip_kwargs, oracle_at, rel_inf, scene
+from oracle import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+V_TOL = 1e-5
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _export(ctx, n, C):
+    count = torch.zeros(n, dtype=torch.int32, device="cuda")
+    ids = torch.zeros(n * C, dtype=torch.int32, device="cuda")
+    gradW = torch.zeros(n * C * 3, dtype=torch.float32, device="cuda")
+    ctx.neighbors_export(count, ids, gradW)
+    torch.cuda.synchronize()
+    return _np(count), _np(ids), _np(gradW).reshape(-1, 3)
+
+
+@pytest.mark.parametrize("step", [0, 12])
+def test_neighbors_density_k_bit_exact(step):
+    sim = oracle_at(0, step)
+    n = sim.x.shape[0]
+    ctx = gpu_context(sim, torch)
+    assert ctx.neighbors_build() == 0
+    count, ids, gradW = _export(ctx, n, sim.C)
+    np.testing.assert_array_equal(count, sim.count)
+    C = sim.C
+    for i in range(n):
+        c = count[i]
+        assert np.array_equal(ids[i * C:i * C + c], sim.ids[i * C:i * C + c]), f"row {i}"
+    mask = np.zeros(n * C, bool)
+    for i in range(n):
+        mask[i * C:i * C + count[i]] = True
+    assert np.array_equal(gradW[mask].view(np.uint32), sim.gradW[mask].view(np.uint32))
+    dens = ctx.read(2, torch.zeros(n, device="cuda"))
+    k = ctx.read(3, torch.zeros(n, device="cuda"))
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(dens).view(np.uint32), sim.density.view(np.uint32))
+    assert np.array_equal(_np(k).view(np.uint32), sim.k.view(np.uint32))
+    ctx.close()
+
+
+@pytest.mark.parametrize("step", [0, 5, 20])
+def test_ip_solve_matches_oracle(step):
+    sim = oracle_at(0, step)
+    # IP input: velocities after gravity (viscosity is off in this scene)
+    v_in = O.gravity(sim.v, sim.is_dynamic, sim.g, sim.dt)
+    kw = ip_kwargs(sim.cfg)
+    v_o, div_o, res_o = O.ip_solve(v_in, sim.m, sim.density, sim.density0, sim.is_dynamic, sim.C, sim.count,
+                                   sim.ids, sim.gradW, sim.k, sim.dt, **kw)
+    assert res_o.status == O.ORC_OK
+    ctx = gpu_context(sim, torch)
+    assert ctx.neighbors_build() == 0
+    n = sim.x.shape[0]
+    ctx.write(1, torch.from_numpy(v_in).cuda())
+    from implicitvelocityprojection_amd.context import IPParams
+    res_g = ctx.ip_solve(sim.dt, IPParams(**kw))
+    v_g = _np(ctx.read(1, torch.zeros((n, 3), device="cuda")))
+    div_g = _np(ctx.read(4, torch.zeros(n, device="cuda")))
+    dyn = sim.is_dynamic == 1
+    assert res_g.status == 0
+    assert res_g.outer_iters == res_o.outer_iters
+    assert len(res_g.pcg_iters) == len(res_o.pcg_iters)
+    for a, b in zip(res_g.pcg_iters, res_o.pcg_iters):
+        assert abs(a - b) <= 1, (res_g.pcg_iters, res_o.pcg_iters)
+    assert res_g.final_density_error <= kw["tol_opt"]
+    assert rel_inf(v_g[dyn], v_o[dyn]) <= V_TOL, rel_inf(v_g[dyn], v_o[dyn])
+    assert rel_inf(div_g, div_o) <= 1e-4
+    ctx.close()
+
+
+def test_full_step_matches_oracle():
+    sim = oracle_at(0, 8)
+    ctx = gpu_context(sim, torch)
+    assert ctx.neighbors_build() == 0
+    from implicitvelocityprojection_amd.context import IPParams
+    kw = ip_kwargs(sim.cfg)
+    res_g, _ = ctx.step(sim.dt, sim.g, sim.cfg.viscosity_method, sim.cfg.viscosity_coeff, IPParams(**kw))
+    sim.step()
+    n = sim.x.shape[0]
+    x_g = _np(ctx.read(0, torch.zeros((n, 3), device="cuda")))
+    v_g = _np(ctx.read(1, torch.zeros((n, 3), device="cuda")))
+    dyn = sim.is_dynamic == 1
+    assert rel_inf(v_g[dyn], sim.v[dyn]) <= V_TOL
+    assert rel_inf(x_g[dyn], sim.x[dyn]) <= V_TOL
+    assert np.array_equal(x_g[~dyn], sim.x[~dyn])
+    assert res_g.outer_iters == sim.last_ip.outer_iters
+    ctx.close()
+
+
+def test_viscosity_explicit_jacobi_matches_oracle():
+    sim = oracle_at(0, 10)
+    coeff = 0.01
+    v_o = O.viscosity(sim.x, sim.v, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h, sim.C,
+                      sim.count, sim.ids, sim.gradW, coeff, sim.dt, mode=O.VISC_JACOBI)
+    ctx = gpu_context(sim, torch)
+    assert ctx.neighbors_build() == 0
+    ctx.viscosity_explicit(coeff, sim.dt)
+    n = sim.x.shape[0]
+    v_g = _np(ctx.read(1, torch.zeros((n, 3), device="cuda")))
+    dyn = sim.is_dynamic == 1
+    assert rel_inf(v_g[dyn] - sim.v[dyn], v_o[dyn] - sim.v[dyn]) <= 1e-4
+    assert rel_inf(v_g[dyn], v_o[dyn]) <= V_TOL
+    ctx.close()
+
+
+def test_weiler18_matches_oracle():
+    sim = oracle_at(0, 10)
+    coeff = 0.05
+    v_o, it_o = O.weiler18(sim.x, sim.v, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h,
+                           sim.C, sim.count, sim.ids, sim.gradW, coeff, sim.dt)
+    ctx = gpu_context(sim, torch)
+    assert ctx.neighbors_build() == 0
+    it_g = ctx.viscosity_weiler18(coeff, sim.dt)
+    n = sim.x.shape[0]
+    v_g = _np(ctx.read(1, torch.zeros((n, 3), device="cuda")))
+    dyn = sim.is_dynamic == 1
+    assert abs(it_g - it_o) <= 1, (it_g, it_o)
+    assert rel_inf(v_g[dyn], v_o[dyn]) <= V_TOL
+    ctx.close()
+
+
+def test_deterministic_rerun():
+    sim = oracle_at(0, 6)
+    from implicitvelocityprojection_amd.context import IPParams
+    kw = ip_kwargs(sim.cfg)
+    outs = []
+    for _ in range(2):
+        ctx = gpu_context(sim, torch)
+        assert ctx.neighbors_build() == 0
+        for _ in range(3):
+            ctx.step(sim.dt, sim.g, 0, 0.0, IPParams(**kw))
+        n = sim.x.shape[0]
+        outs.append(_np(ctx.read(1, torch.zeros((n, 3), device="cuda"))))
+        ctx.close()
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
