@@ -1,0 +1,185 @@
+"""Benchmark: the reference's velocity-based IP step on the 1M-particle dam break, one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Workload ("dambreak_1m"): a 100^3 fluid block (r = 0.005 m, jitter +-0.01 d, seed 2) in the corner
+of a 4.0 x 1.5 x 1.0 m static two-layer particle box, implicit (Weiler 2018) viscosity 0.01,
+dt = 0.5 ms, densityErrorTol 1e-3, maxOuterIters 100, omega 10, tol_pcg 1e-6.  One step is the
+reference's Framework.forward: gravity, Weiler18 viscosity, IP solve, advection, domain clamp,
+neighbour rebuild, density and Aii -- all of it on the GPU, inputs resident in HBM.
+
+Prints one JSON line.  `value` is dynamic-particle-steps per second over the K timed steps.
+`roofline` reports the kernel with the largest share of the timed region, its algorithmic bytes
+per launch (byte models below) over its average launch time, measured with HIP events on the
+library's stream.  `cpu_baseline` is the CPU oracle (a serial float32 C restatement of the
+reference kernels) timed on a few steps of the same workload, starting from the GPU state at the
+start of the timed region.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s + CG iters/s at 1M particles, ≤0.1% density err; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def byte_models(nd, ns, pairs):
+    """Algorithmic bytes per launch of the row kernels: every stored dynamic pair record once
+    (id + gradW = 16 B) and every per-particle array once."""
+    return {
+        "build": 16.0 * pairs + 44.0 * nd + 16.0 * ns,   # write pairs; read x,m; write rho,k,S,counts
+        "ip_pack": 16.0 * pairs + 64.0 * nd,
+        "ip_rhs": 16.0 * pairs + 48.0 * nd,
+        "w18_Av": 16.0 * pairs + 48.0 * nd,              # pairs; x,m; (v, m/rho); factor; write Av
+        "w18_pre": 16.0 * pairs + 56.0 * nd,             # pairs; x,m; m/rho; factor; write 3x3 inverse
+        "w18_update": 124.0 * nd,
+        "w18_p": 44.0 * nd,
+        "visc": 16.0 * pairs + 48.0 * nd,
+        "cg_update": 76.0 * nd,
+        "cg_p": 44.0 * nd,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the IP hot path here is a single-device build "
+              "(the multi-GPU decomposition is a design note, not an implementation)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from implicitvelocityprojection_amd.config import SimConfig
+    from implicitvelocityprojection_amd.context import Context, IPParams, kclass_mask
+    from implicitvelocityprojection_amd.scenes import load_particles, particle_masses, synthetic_scene
+
+    scene = synthetic_scene(2)
+    cfg = SimConfig.from_dict(scene)
+    ps = load_particles(cfg)
+    m, m_inv = particle_masses(cfg, ps)
+    h = float(np.float32(cfg.support_radius))
+    dev = torch.device("cuda:0")
+    t = lambda a, dt=torch.float32: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)
+    ctx = Context(h, cfg.domain_start, cfg.domain_size, ps.n, cache_size=int(cfg.get_cfg("neighborCacheSize", 60)))
+    ctx.set_particles(t(ps.x), t(ps.v), t(m), t(m_inv), t(ps.density0), t(ps.is_dynamic, torch.int32),
+                      t(ps.material, torch.int32))
+    if ctx.neighbors_build() != 0:
+        raise RuntimeError("neighbour overflow at load")
+    n, nd = ctx.counts()
+    ns = n - nd
+    prm = IPParams(omega=cfg.get_cfg("omega", 10.0), tol_opt=cfg.get_cfg("densityErrorTol", 1e-3),
+                   max_iteration_opt=cfg.get_cfg("maxOuterIters", 100), tol_pcg=cfg.get_cfg("tol_pcg", 1e-6))
+
+    def step():
+        res, vit = ctx.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, prm)
+        if res.status != 0:
+            raise RuntimeError(f"IP solve status {res.status}")
+        return res, vit
+
+    # warm-up, with every kernel class timed, to find the dominant kernel
+    ctx.profile_enable(-1)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    prof = ctx.profile_read()
+    pairs = ctx.pair_count()
+    models = byte_models(nd, ns, pairs)
+    dominant = max(prof, key=lambda k: prof[k]["total_ms"])
+
+    # state at the start of the timed region (for the CPU baseline)
+    x0 = ctx.read(0, torch.zeros((n, 3), device=dev))
+    v0 = ctx.read(1, torch.zeros((n, 3), device=dev))
+
+    # timed region: events on the dominant kernel only
+    ctx.profile_reset()
+    ctx.profile_enable(kclass_mask(dominant))
+    pairs_sum = 0
+    cg_iters = 0
+    solve_ms = 0.0
+    visc_iters = 0
+    outer = 0
+    max_err = 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res, vit = step()
+        cg_iters += sum(res.pcg_iters)
+        solve_ms += res.solve_ms
+        visc_iters += vit
+        outer += res.outer_iters
+        max_err = max(max_err, res.final_density_error)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    ctx.profile_enable(0)
+    pd = prof[dominant]
+    avg_ms = pd["total_ms"] / max(pd["launches"], 1)
+    if pd["bytes"] > 0:
+        bytes_per_launch = pd["bytes"] / max(pd["launches"], 1)
+    else:
+        bytes_per_launch = models.get(dominant, 0.0)
+    pairs_end = ctx.pair_count()
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        sim = O.OracleSim(cfg, ps)
+        sim.x = x0.cpu().numpy().astype(np.float32)
+        sim.v = v0.cpu().numpy().astype(np.float32)
+        sim.rebuild()
+        tc = time.perf_counter()
+        for _ in range(args.cpu_steps):
+            sim.step()
+        tcpu = time.perf_counter() - tc
+        cpu = {"value": nd * args.cpu_steps / tcpu, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of dambreak_1m from the GPU state at the start of the timed "
+                         f"region, serial float32 C oracle ({tcpu:.1f} s)"}
+
+    out = {
+        "metric": METRIC,
+        "value": nd * args.steps / wall,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "dambreak_1m", "fluid_particles": nd, "static_particles": ns,
+                   "dt": cfg.dt, "viscosity": "Weiler18 0.01", "densityErrorTol": prm.tol_opt,
+                   "maxOuterIters": prm.max_iteration_opt, "neighbour_pairs": pairs_end},
+        "cg_iters_per_s": cg_iters / (solve_ms * 1e-3) if solve_ms > 0 else 0.0,
+        "cg_iters": cg_iters,
+        "ip_outer_iters": outer,
+        "weiler18_cg_iters": visc_iters,
+        "ip_solve_ms_per_step": solve_ms / args.steps,
+        "max_final_density_error": max_err,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
+                     "avg_launch_us": avg_ms * 1e3, "launches": pd["launches"],
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -42,7 +42,8 @@ class IPStats(ctypes.Structure):
                 ("pcg_iters", ctypes.POINTER(ctypes.c_int)), ("n_density_errors", ctypes.c_int),
                 ("n_pcg", ctypes.c_int), ("outer_iters", ctypes.c_int), ("final_density_error", ctypes.c_double),
                 ("pAp_bad", ctypes.c_double), ("pcg_iter_bad", ctypes.c_int), ("pairs_traversed", ctypes.c_longlong),
-                ("rows_traversed", ctypes.c_longlong), ("cg_launch_chunks", ctypes.c_int)]
+                ("rows_traversed", ctypes.c_longlong), ("cg_launch_chunks", ctypes.c_int),
+                ("solve_ms", ctypes.c_double)]
 
 
 class CGStats(ctypes.Structure):

@@ -53,6 +53,7 @@ class IPResult:
         self.pairs_traversed = st.pairs_traversed
         self.rows_traversed = st.rows_traversed
         self.cg_launch_chunks = st.cg_launch_chunks
+        self.solve_ms = st.solve_ms
 
     @property
     def last_err(self):

@@ -21,6 +21,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <chrono>
 
 #include "ivp_internal.h"
 #include "ivp_reduce.h"
@@ -725,9 +726,21 @@ int run_chunks(ivp_ctx* c, int max_iter, int guess, L&& launch, int* iters, int*
 
 namespace ivp {
 
+static int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats);
+
+// IP.solve with its wall-clock cost (stats_total_cost_ms, ip.py:335, 403-404: the reference
+// synchronises before reading the clock, and so does this).
 int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats) {
     if (!c->built) return IVP_ERR_STATE;
     if (!prm || !stats) return IVP_ERR_ARG;
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = ip_solve_body(c, dt, prm, stats);
+    if (rc >= 0) IVP_HIP(hipStreamSynchronize(c->stream));
+    stats->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+static int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats) {
     ivp_ip_stats& st = *stats;
     st.n_density_errors = 0;
     st.n_pcg = 0;

@@ -68,6 +68,7 @@ typedef struct {
     long long pairs_traversed;   /* neighbour pairs read by the CG operator passes         */
     long long rows_traversed;    /* rows whose neighbour loop ran in the CG operator passes */
     int cg_launch_chunks;        /* host synchronisations spent on CG termination          */
+    double solve_ms;             /* host wall time of the solve, synchronisations included */
 } ivp_ip_stats;
 
 /* Weiler18 solve statistics */
