@@ -19,6 +19,7 @@ IVP_ERR_HIP = -2
 IVP_ERR_CAPACITY = -3
 IVP_ERR_STATE = -4
 IVP_ERR_GRID = -5
+IVP_ERR_INTERNAL = -6
 
 FIELD_X, FIELD_V, FIELD_DENSITY, FIELD_K, FIELD_DIVERGENCE = 0, 1, 2, 3, 4
 

@@ -185,6 +185,7 @@ const char* ivp_status_string(int s) {
         case IVP_ERR_CAPACITY: return "capacity exceeded";
         case IVP_ERR_STATE: return "call out of order (particles not set or neighbours not built)";
         case IVP_ERR_GRID: return "grid too large (more than 2048 cells per axis or 2^28 cells)";
+        case IVP_ERR_INTERNAL: return "internal consistency check failed in a kernel";
         default: return "unknown status";
     }
 }
@@ -267,6 +268,7 @@ int ivp_neighbors_build(ivp_ctx* c) {
     if (rc != IVP_OK) return rc;
     IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     IVP_HIP(hipStreamSynchronize(c->stream));
+    if (*c->err_host & 4) return IVP_ERR_INTERNAL;
     return (*c->err_host & 1) ? IVP_NEIGHBOR_OVERFLOW : IVP_OK;
 }
 

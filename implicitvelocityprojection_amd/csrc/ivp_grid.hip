@@ -198,9 +198,39 @@ __device__ __forceinline__ void visit(const Grid& g, const int* __restrict__ sta
     }
 }
 
+// Candidate scan of one population: calls f(j) for every j != self with
+// sqrtf(dx*dx + dy*dy + dz*dz) < h, in the order of `visit`.  d^2 >= h2_hi (a bound a few ulps
+// above h^2) already implies sqrtf(d^2) >= h, so the correctly rounded square root is only
+// evaluated for the candidates below it.
+template <class F>
+__device__ __forceinline__ void scan_ids(const Grid& g, const int* __restrict__ start, const float4* __restrict__ ppos,
+                                         int self, float4 pi, float h, float h2_hi, F&& f) {
+    int cx = cell_coord(g, pi.x, 0), cy = cell_coord(g, pi.y, 1), cz = cell_coord(g, pi.z, 2);
+    int x0 = cx > 0 ? cx - 1 : 0;
+    int x1 = cx < g.dims[0] - 1 ? cx + 1 : g.dims[0] - 1;
+    for (int dz = -1; dz <= 1; ++dz) {
+        int zz = cz + dz;
+        if (zz < 0 || zz >= g.dims[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int yy = cy + dy;
+            if (yy < 0 || yy >= g.dims[1]) continue;
+            int s0 = start[cell_key(g, x0, yy, zz)];
+            int s1 = start[cell_key(g, x1, yy, zz) + 1];
+            for (int j = s0; j < s1; ++j) {
+                float4 pj = ppos[j];
+                float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                float d2 = rx * rx + ry * ry + rz * rz;
+                if (d2 < h2_hi && j != self) {
+                    if (sqrtf(d2) < h) f(j);
+                }
+            }
+        }
+    }
+}
+
 struct BuildArgs {
-    int nd, C;
-    float h;
+    int nd, ns, C;
+    float h, h2_hi;
     Grid g;
     const float4* pos;
     const float* minv;
@@ -218,52 +248,92 @@ struct BuildArgs {
     int stride;
 };
 
-// One thread per dynamic row, one wave per 64-row tile.  Writes the row's dynamic neighbours
-// (id, gradW) into the tile, pads the tile, and accumulates density (pressure_base.py:12-46),
-// Aii (pressure_base.py:49-87) and S_i over the row in reference order: dynamic neighbours,
-// then static neighbours.
+// One thread per dynamic row, one wave per 64-row tile, two phases.
+//  A: candidate scan; the ids of dynamic neighbours go to slots 0, 1, ... of the row and the
+//     ids of static neighbours to slots C-1, C-2, ... (an overflowing row sets the error flag).
+//  B: the wave walks its slots in lockstep, computing gradW (stored next to the id, coalesced),
+//     density (pressure_base.py:12-46), Aii (pressure_base.py:49-87) and S_i, accumulated in the
+//     reference row order: dynamic neighbours, then static neighbours.  Dynamic slots from the
+//     row's count up to the tile length are padded with (id = row, gradW = 0).
 __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
     int blk = xcd_block(blockIdx.x, gridDim.x);
     int i = blk * BLOCK + threadIdx.x;
     int lane = threadIdx.x & (WAVE - 1);
     int tile = i >> 6;
     bool valid = i < a.nd;
-    int cnt = 0;
     const float h = a.h;
+    const int C = a.C;
+    uint32_t* tbase = a.nb + (size_t)tile * (size_t)C * 256u + lane;
+    int cnt = 0, cs = 0, tot = 0;
+    float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (valid) {
-        float4 pi = a.pos[i];
-        float rho = 0.0f;
-        rho = rho + pi.w * cubic_kernel(0.0f, h);
-        float aii = 0.0f;
-        float jx = 0.0f, jy = 0.0f, jz = 0.0f;
-        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-        int tot = 0;
-        const int C = a.C;
-        uint32_t* tbase = a.nb + (size_t)tile * (size_t)C * 256u + lane;
-        visit(a.g, a.dstart, a.pos, i, pi, h, [&](int j, float4 pj, float rx, float ry, float rz, float rn) {
-            float3 gw = spiky_grad(rx, ry, rz, h);
-            if (cnt < C) {
-                uint32_t* rec = tbase + (size_t)cnt * 256u;
-                rec[0] = (uint32_t)j;
-                rec[64] = __float_as_uint(gw.x);
-                rec[128] = __float_as_uint(gw.y);
-                rec[192] = __float_as_uint(gw.z);
-                cnt++;
-            }
+        pi = a.pos[i];
+        scan_ids(a.g, a.dstart, a.pos, i, pi, h, a.h2_hi, [&](int j) {
+            if (tot < C) tbase[(size_t)cnt * 256u] = (uint32_t)j;
+            if (tot < C) cnt++;
             tot++;
-            rho += pj.w * cubic_kernel(rn, h);
-            float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
-            aii += (Jx * Jx + Jy * Jy + Jz * Jz) * a.minv[j];
-            jx -= Jx; jy -= Jy; jz -= Jz;
         });
-        visit(a.g, a.sstart, a.pos + a.nd, -1, pi, h, [&](int, float4 pj, float rx, float ry, float rz, float rn) {
-            float3 gw = spiky_grad(rx, ry, rz, h);
+        scan_ids(a.g, a.sstart, a.pos + a.nd, -1, pi, h, a.h2_hi, [&](int j) {
+            if (tot < C) tbase[(size_t)(C - 1 - cs) * 256u] = (uint32_t)j;
+            if (tot < C) cs++;
             tot++;
-            rho += pj.w * cubic_kernel(rn, h);
-            float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
-            jx -= Jx; jy -= Jy; jz -= Jz;
-            sx += Jx; sy += Jy; sz += Jz;
         });
+        if (tot > C) atomicOr(a.err_flag, 1);
+    }
+    int tl = wave_max(cnt);
+    int ts = wave_max(cs);
+    float rho = 0.0f;
+    rho = rho + pi.w * cubic_kernel(0.0f, h);
+    float aii = 0.0f;
+    float jx = 0.0f, jy = 0.0f, jz = 0.0f;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (int k = 0; k < tl; ++k) {
+        if (!valid || k >= cnt) continue;
+        uint32_t* rec = tbase + (size_t)k * 256u;
+        int j = (int)rec[0];
+        if ((unsigned)j >= (unsigned)a.nd) {  // never taken: phase A stored a dynamic id here
+            atomicOr(a.err_flag, 4);
+            continue;
+        }
+        float4 pj = a.pos[j];
+        float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        float rn = len3(rx, ry, rz);
+        float3 gw = spiky_grad(rx, ry, rz, h);
+        rec[64] = __float_as_uint(gw.x);
+        rec[128] = __float_as_uint(gw.y);
+        rec[192] = __float_as_uint(gw.z);
+        rho += pj.w * cubic_kernel(rn, h);
+        float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
+        aii += (Jx * Jx + Jy * Jy + Jz * Jz) * a.minv[j];
+        jx -= Jx; jy -= Jy; jz -= Jz;
+    }
+    for (int k = 0; k < ts; ++k) {
+        if (!valid || k >= cs) continue;
+        int j = (int)tbase[(size_t)(C - 1 - k) * 256u];
+        if ((unsigned)j >= (unsigned)a.ns) {  // never taken: phase A stored a static id here
+            atomicOr(a.err_flag, 4);
+            continue;
+        }
+        float4 pj = a.pos[a.nd + j];
+        float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        float rn = len3(rx, ry, rz);
+        float3 gw = spiky_grad(rx, ry, rz, h);
+        rho += pj.w * cubic_kernel(rn, h);
+        float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
+        jx -= Jx; jy -= Jy; jz -= Jz;
+        sx += Jx; sy += Jy; sz += Jz;
+    }
+    // Pad only after the static ids (slots C-1-cs .. C-1) have been consumed: the padded range
+    // [cnt, tl) follows the wave's longest row and can reach into them.
+    for (int k = 0; k < tl; ++k) {
+        if (!valid || k < cnt) continue;
+        uint32_t* rec = tbase + (size_t)k * 256u;
+        rec[0] = (uint32_t)i;
+        rec[64] = 0u;
+        rec[128] = 0u;
+        rec[192] = 0u;
+    }
+    if (valid) {
         aii += (jx * jx + jy * jy + jz * jz) * a.minv[i];
         a.kk[i] = 1.0f / (aii + 1e-3f);
         a.density[i] = rho;
@@ -272,22 +342,10 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
         a.Ss[3 * i + 2] = sz;
         a.rowcnt[i] = cnt;
         a.rowtot[i] = tot;
-        if (tot > C) atomicOr(a.err_flag, 1);
     }
     double v[1] = {(double)cnt};
     block_partials<1>(v, a.partials, a.stride);
-    int tl = wave_max(cnt);
     if (lane == 0) a.tile_len[tile] = tl;
-    if (valid) {
-        uint32_t* tbase = a.nb + (size_t)tile * (size_t)a.C * 256u + lane;
-        for (int k = cnt; k < tl; ++k) {
-            uint32_t* rec = tbase + (size_t)k * 256u;
-            rec[0] = (uint32_t)i;
-            rec[64] = 0u;
-            rec[128] = 0u;
-            rec[192] = 0u;
-        }
-    }
 }
 
 struct ExportArgs {
@@ -552,7 +610,8 @@ int grid_build(ivp_ctx* c) {
     }
     IVP_HIP(hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream));
     if (c->nd > 0) {
-        BuildArgs ba{c->nd, c->C, c->h, c->grid, c->pos, c->minv, c->dstart, c->sstart, c->nb, c->tile_len,
+        float h2_hi = (float)((double)c->h * (double)c->h * (1.0 + 1.0 / 1048576.0));
+        BuildArgs ba{c->nd, c->ns, c->C, c->h, h2_hi, c->grid, c->pos, c->minv, c->dstart, c->sstart, c->nb, c->tile_len,
                      c->rowcnt, c->rowtot, c->density, c->kk, c->Ss, c->err_flag, c->partials, c->max_blocks};
         ProfEvent ev;
         prof_begin(c, KC_BUILD, -1, 0.0, &ev);

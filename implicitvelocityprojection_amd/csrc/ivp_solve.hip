@@ -775,6 +775,7 @@ static int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_
         IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         rc = read_state(c);
         if (rc != IVP_OK) return rc;
+        if (*c->err_host & 4) return IVP_ERR_INTERNAL;
         if (*c->err_host & 1) return IVP_NEIGHBOR_OVERFLOW;
         err = c->state_host->err_sum / (double)nd;
         if (st.n_density_errors < st.capacity && st.density_errors) st.density_errors[st.n_density_errors] = err;

@@ -29,6 +29,7 @@ extern "C" {
 #define IVP_ERR_CAPACITY          -3
 #define IVP_ERR_STATE             -4
 #define IVP_ERR_GRID              -5
+#define IVP_ERR_INTERNAL          -6  /* a kernel's own consistency guard fired              */
 
 /* ---- configuration ----------------------------------------------------------------------- */
 typedef struct ivp_ctx ivp_ctx;

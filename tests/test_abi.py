@@ -31,5 +31,5 @@ def test_status_strings_without_gpu():
     from implicitvelocityprojection_amd import _native as N
     lib = N.load()
     assert lib.ivp_version().decode().startswith("ivp")
-    for s in (0, 1, 2, 3, -1, -2, -3, -4, -5):
+    for s in (0, 1, 2, 3, -1, -2, -3, -4, -5, -6):
         assert lib.ivp_status_string(s)
