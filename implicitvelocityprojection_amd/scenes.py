@@ -173,13 +173,14 @@ _SYNTHETIC = {
 }
 
 
-def synthetic_scene(cfg_id: int, scale=None) -> dict:
+def synthetic_scene(cfg_id: int, scale=None, box=None) -> dict:
     """Scene JSON dict of a synthetic benchmark scene.
 
     A fluid block of nx*ny*nz particles sits at voxel centres (k + 1/2) d with a uniform jitter of
     +-0.01 d, resting on the floor of a static two-layer particle box whose inner faces are at 0
     and L (shifted by the margin o inside the domain [0, L + 2o], so the domain clamp never
-    engages).  ``scale`` optionally replaces the fluid counts (for reduced test sizes).
+    engages).  ``scale`` optionally replaces the fluid counts and ``box`` the box size L (for
+    reduced test sizes).
       0: 16x32x16 = 8192 particles, box 0.64 x 1.0 x 0.64 m, r = 0.01, dt = 1 ms, no viscosity.
       1: 40x64x40 = 102400 particles, box 1.6 x 3.0 x 1.6 m, r = 0.01, dt = 1 ms,
          explicit viscosity 0.01.
@@ -191,7 +192,7 @@ def synthetic_scene(cfg_id: int, scale=None) -> dict:
     n = tuple(scale) if scale is not None else s["n"]
     r = s["r"]
     d = 2.0 * r
-    L = s["L"]
+    L = tuple(box) if box is not None else s["L"]
     o = s["o"]
     if s["place"] == "centred":
         off = ((L[0] - n[0] * d) / 2.0, 0.0, (L[2] - n[2] * d) / 2.0)

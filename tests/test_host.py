@@ -1,0 +1,79 @@
+"""Host-side logic: scene construction (scene_loader.py / simulation_data.py semantics), scene
+JSON parsing (config_builder.py) and the mass convention (simulation_data.py:40-57, 137)."""
+
+import json
+
+import numpy as np
+import pytest
+
+from implicitvelocityprojection_amd.config import SimConfig
+from implicitvelocityprojection_amd.scenes import (boundary_walls, cube_lattice, cube_shell, load_particles,
+                                                   particle_masses, synthetic_scene)
+
+
+def test_synthetic_scene_sizes():
+    counts = {0: (8192, 18780), 1: (102400, 126620), 2: (1000000, 470460)}
+    for cid, (nf, ns) in counts.items():
+        if cid == 2:
+            sc = synthetic_scene(2)
+            assert sc["FluidBlocks"][0]["start"][0] == pytest.approx(0.05 + 0.005)
+            continue
+        ps = load_particles(SimConfig.from_dict(synthetic_scene(cid)))
+        assert int((ps.is_dynamic == 1).sum()) == nf and int((ps.is_dynamic == 0).sum()) == ns
+
+
+def test_fluid_lattice_at_voxel_centres_with_bounded_jitter():
+    cfg = SimConfig.from_dict(synthetic_scene(0))
+    ps = load_particles(cfg)
+    x = ps.x[ps.is_dynamic == 1].astype(np.float64)
+    d = 0.02
+    ideal = cube_lattice((0.1 + 0.16 + 0.01, 0.1 + 0.01, 0.1 + 0.16 + 0.01), (0.32 - 0.01, 0.64 - 0.01, 0.32 - 0.01), d)
+    assert ideal.shape == x.shape
+    assert np.abs(x - ideal).max() <= 0.01 * d * (1 + 1e-3)
+    assert np.abs(x - ideal).max() > 0.009 * d
+    assert np.all(ps.v == 0.0)
+
+
+def test_cube_shell_repeats_edges_like_reference():
+    pts = cube_shell((0.5, 0.5, 0.5), (0.2, 0.2, 0.2), 1, 0.1)
+    # 3 points per axis, 6 full 3x3 faces: 54 points, 26 distinct (edges and corners repeat)
+    assert pts.shape == (54, 3)
+    assert len({tuple(p) for p in np.round(pts, 6)}) == 26
+    pts2 = cube_shell((0.5, 0.5, 0.5), (0.2, 0.2, 0.2), 2, 0.1)
+    assert pts2.shape == (54 + 6 * 25, 3)
+
+
+def test_boundary_walls_layers():
+    pts = boundary_walls((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 2, 0.5)
+    assert pts.shape[0] == 6 * 9 + 6 * 25
+    assert pts.min() == pytest.approx(-0.5) and pts.max() == pytest.approx(1.5)
+
+
+def test_config_parsing_and_get_cfg(tmp_path):
+    scene = {"Configuration": {"domainStart": [0.0, 0.0, 0.0], "domainEnd": [2.0, 3.0, 4.0], "particleRadius": 0.02,
+                               "timeStepSize": 0.002, "density0": 1000, "gravitation": [0.0, -9.81, 0.0],
+                               "viscosityMethod": 1, "viscosityCoeff": 0.1, "pressureMethod": 1,
+                               "densityErrorTol": 1e-3, "omega": 10, "tol_pcg": 1e-6}}
+    p = tmp_path / "s.json"
+    p.write_text(json.dumps(scene))
+    cfg = SimConfig.from_json(str(p))
+    assert cfg.domain_size == (2.0, 3.0, 4.0)
+    assert cfg.support_radius == pytest.approx(0.08) and cfg.particle_diameter == pytest.approx(0.04)
+    assert cfg.viscosity_method == 1 and cfg.dt == 0.002
+    assert cfg.get_cfg("maxOuterIters", 1000) == 1000 and cfg.get_cfg("omega") == 10
+
+
+def test_mass_convention():
+    cfg = SimConfig.from_dict(synthetic_scene(0))
+    ps = load_particles(cfg)
+    m, minv = particle_masses(cfg, ps)
+    dyn = ps.is_dynamic == 1
+    m0 = np.float32(np.float32(0.8 * 0.02 ** 3) * np.float32(1000.0))
+    assert np.all(m[dyn] == m0) and np.all(m[~dyn] == m0 * np.float32(10.0))
+    assert np.all(minv[dyn] == np.float32(1.0) / m0) and np.all(minv[~dyn] == 0.0)
+
+
+def test_mesh_blocks_are_rejected():
+    cfg = SimConfig.from_dict({"Configuration": {}, "RigidBodies": [{"objectId": 1, "geometryFile": "x.obj"}]})
+    with pytest.raises(NotImplementedError):
+        load_particles(cfg)

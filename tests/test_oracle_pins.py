@@ -1,0 +1,498 @@
+"""Pins of the CPU oracle against facts that do not come from the oracle's own formulas:
+closed forms and normalisation of the SPH kernels, brute-force neighbour sets, lattice sums,
+dense linear algebra (J assembled from unit probes, its adjoint, A = M + omega J^T D J, the
+Weiler18 operator and its block diagonal), and the conservation / invariance properties of the
+viscosity models.  The reference ships no tests or golden vectors for this path, so these pins
+are what ties the restatement to the mathematics it restates.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from implicitvelocityprojection_amd.config import SimConfig
+from implicitvelocityprojection_amd.scenes import load_particles, particle_masses, synthetic_scene
+from oracle import oracle as O
+from tests.helpers import oracle_states, scene
+
+H = 0.04
+PI32 = float(np.float32(3.14159265359))
+
+
+# ---------------------------------------------------------------------------------------------
+# SPH kernels (sph_kernel.py:8-26, 52-61)
+# ---------------------------------------------------------------------------------------------
+
+def test_cubic_kernel_closed_form_and_support():
+    assert O.cubic_kernel(0.0, H) == pytest.approx(8.0 / (math.pi * H ** 3), rel=1e-6)
+    assert O.cubic_kernel(H, H) == 0.0
+    assert O.cubic_kernel(1.5 * H, H) == 0.0
+    # the two branches meet at q = 1/2: k(6/8 - 6/4 + 1) = 2k(1/2)^3 = k/4
+    k = 8.0 / (math.pi * H ** 3)
+    assert O.cubic_kernel(0.5 * H, H) == pytest.approx(k / 4.0, rel=1e-6)
+    assert O.cubic_kernel(0.5 * H * (1 + 1e-6), H) == pytest.approx(k / 4.0, rel=1e-4)
+
+
+def test_cubic_kernel_normalised():
+    # int W dV = 4 pi int_0^h W(r) r^2 dr = 1
+    r = np.linspace(0.0, H, 20001)
+    w = np.array([O.cubic_kernel(float(x), H) for x in r], dtype=np.float64)
+    integral = np.trapezoid(4.0 * math.pi * r * r * w, r)
+    assert integral == pytest.approx(1.0, abs=2e-4)
+
+
+def _spiky(r):
+    # Mueller et al. 2003 spiky kernel, normalised: 15/(pi h^6) (h - r)^3
+    return 15.0 / (math.pi * H ** 6) * (H - r) ** 3
+
+
+def test_spiky_gradient_is_gradient_of_normalised_spiky():
+    r = np.linspace(0.0, H, 20001)
+    assert np.trapezoid(4.0 * math.pi * r * r * _spiky(r), r) == pytest.approx(1.0, abs=1e-4)
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        rn = rng.uniform(0.05 * H, 0.95 * H)
+        vec = (d * rn).astype(np.float32)
+        g = O.spiky_grad(vec, H).astype(np.float64)
+        eps = 1e-7
+        dwdr = (_spiky(rn + eps) - _spiky(rn - eps)) / (2 * eps)
+        np.testing.assert_allclose(g, dwdr * d, rtol=2e-4, atol=1e-3)
+
+
+def test_spiky_gradient_antisymmetric_bitwise_and_floored_at_origin():
+    rng = np.random.default_rng(4)
+    for _ in range(100):
+        v = rng.uniform(-H, H, size=3).astype(np.float32)
+        a = O.spiky_grad(v, H)
+        b = O.spiky_grad(-v, H)
+        assert np.array_equal((-a).view(np.uint32), b.view(np.uint32))
+    np.testing.assert_array_equal(O.spiky_grad(np.zeros(3, np.float32), H), np.zeros(3, np.float32))
+
+
+# ---------------------------------------------------------------------------------------------
+# Tiny scenes
+# ---------------------------------------------------------------------------------------------
+
+def tiny(n=(6, 6, 6), jitter=0.01, seed=5, L=(0.2, 0.2, 0.2), vel=(0.0, 0.0, 0.0)):
+    sc = synthetic_scene(0, scale=n)
+    o = 0.05
+    sc["Configuration"]["domainEnd"] = [L[0] + 2 * o, L[1] + 2 * o, L[2] + 2 * o]
+    d = 0.02
+    off = [(L[0] - n[0] * d) / 2, 0.0, (L[2] - n[2] * d) / 2]
+    start = [o + off[a] + 0.5 * d for a in range(3)]
+    sc["FluidBlocks"][0].update(start=start, end=[start[a] + (n[a] - 0.5) * d for a in range(3)], jitter=jitter,
+                                jitterSeed=seed, velocity=list(vel))
+    sc["GeneratedBodies"][0].update(translation=[o + L[a] / 2 for a in range(3)], size=list(L))
+    cfg = SimConfig.from_dict(sc)
+    return cfg, load_particles(cfg)
+
+
+def tiny_sim(**kw):
+    cfg, ps = tiny(**kw)
+    sim = O.OracleSim(cfg, ps)
+    sim.rebuild()
+    return sim
+
+
+# ---------------------------------------------------------------------------------------------
+# Neighbour cache (neighbour_search.py:10-51)
+# ---------------------------------------------------------------------------------------------
+
+def _rows(count, ids, C):
+    return [ids[i * C:i * C + count[i]] for i in range(len(count))]
+
+
+@pytest.mark.parametrize("which", ["tiny", "cfg0", "cfg0_flowing", "random"])
+def test_neighbour_sets_equal_brute_force(which):
+    if which == "tiny":
+        sim = tiny_sim()
+        x, dyn = sim.x, sim.is_dynamic
+    elif which == "cfg0":
+        cfg, ps = scene(0)
+        x, dyn = ps.x, ps.is_dynamic
+    elif which == "cfg0_flowing":
+        cfg, ps = scene(0)
+        x, _ = oracle_states(0, 25)[25]
+        dyn = ps.is_dynamic
+    else:
+        rng = np.random.default_rng(11)
+        x = rng.uniform(0.05, 0.35, size=(3000, 3)).astype(np.float32)
+        dyn = (rng.uniform(size=3000) < 0.7).astype(np.int32)
+    C = 60
+    g = O.grid_init((0.0, 0.0, 0.0), (0.4, 1.2, 0.4) if which != "tiny" else (0.3, 0.3, 0.3), H)
+    if which.startswith("cfg0"):
+        g = O.grid_init((0.0, 0.0, 0.0), (0.84, 1.2, 0.84), H)
+    count, ids, gradW, mx = O.build_neighbors(g, x, dyn, H, C)
+    cap = 128
+    bc, bids = O.brute_neighbors(x, H, cap)
+    assert mx <= C and bc.max() <= cap
+    np.testing.assert_array_equal(count, bc)
+    for i in range(x.shape[0]):
+        a = np.sort(ids[i * C:i * C + count[i]])
+        b = bids[i * cap:i * cap + bc[i]]
+        assert np.array_equal(a, b), i
+
+
+def test_neighbour_order_and_cached_gradients():
+    sim = tiny_sim()
+    C = sim.C
+    key = O.cell_keys(sim.grid, sim.x)
+    dyn = sim.is_dynamic
+    for i in range(sim.x.shape[0]):
+        row = sim.ids[i * C:i * C + sim.count[i]]
+        is_dyn = dyn[row] == 1
+        # dynamic first, then static
+        if is_dyn.any() and (~is_dyn).any():
+            assert np.nonzero(is_dyn)[0].max() < np.nonzero(~is_dyn)[0].min()
+        for part in (row[is_dyn], row[~is_dyn]):
+            kk = list(zip(key[part], part))
+            assert kk == sorted(kk)
+        for s, j in enumerate(row):
+            g = O.spiky_grad(sim.x[i] - sim.x[j], H)
+            assert np.array_equal(g.view(np.uint32), sim.gradW[i * C + s].view(np.uint32))
+
+
+def test_neighbour_cache_truncates_like_reference():
+    rng = np.random.default_rng(2)
+    x = rng.uniform(0.1, 0.12, size=(200, 3)).astype(np.float32)  # everything within h
+    dyn = np.ones(200, np.int32)
+    g = O.grid_init((0.0, 0.0, 0.0), (0.3, 0.3, 0.3), H)
+    count, ids, gradW, mx = O.build_neighbors(g, x, dyn, H, 60)
+    assert mx == 199 and count.max() == 60
+
+
+def test_cell_keys_formula():
+    g = O.grid_init((0.0, 0.0, 0.0), (1.0, 2.0, 3.0), H)
+    w = np.float32(H) * np.float32(1.0009765625)
+    assert g.inv_w == pytest.approx(float(np.float32(1.0) / w))
+    assert list(g.dims) == [int(math.ceil(np.float32(1.0) / w)) + 4, int(math.ceil(np.float32(2.0) / w)) + 4,
+                            int(math.ceil(np.float32(3.0) / w)) + 4]
+    x = np.array([[0.0, 0.0, 0.0], [0.5, 1.0, 1.5], [-5.0, 99.0, 1.0]], np.float32)
+    k = O.cell_keys(g, x)
+    dims = list(g.dims)
+
+    def cc(v, a):
+        t = (np.float32(v) - np.float32(g.origin[a])) * np.float32(g.inv_w)
+        return min(max(int(math.floor(min(max(t, -1.0), dims[a]))), 0), dims[a] - 1)
+
+    for i in range(3):
+        c = [cc(x[i, a], a) for a in range(3)]
+        assert k[i] == (c[2] * dims[1] + c[1]) * dims[0] + c[0]
+
+
+# ---------------------------------------------------------------------------------------------
+# Density and Aii (pressure_base.py:12-87)
+# ---------------------------------------------------------------------------------------------
+
+def test_density_two_particles_closed_form():
+    x = np.array([[0.1, 0.1, 0.1], [0.1 + 0.013, 0.1, 0.1]], np.float32)
+    m = np.array([0.0064, 0.0064], np.float32)
+    dyn = np.ones(2, np.int32)
+    g = O.grid_init((0.0, 0.0, 0.0), (0.3, 0.3, 0.3), H)
+    count, ids, gradW, _ = O.build_neighbors(g, x, dyn, H, 60)
+    rho = O.density(x, m, dyn, H, 60, count, ids)
+    r = float(np.linalg.norm(x[0].astype(np.float64) - x[1]))
+    k = 8.0 / (math.pi * H ** 3)
+    q = r / H
+    w = k * (6 * q ** 3 - 6 * q ** 2 + 1) if q <= 0.5 else 2 * k * (1 - q) ** 3
+    assert rho[0] == pytest.approx(0.0064 * (k + w), rel=1e-6)
+
+
+def test_density_interior_lattice_sum():
+    sim = tiny_sim(n=(10, 10, 10), jitter=0.0, L=(0.4, 0.4, 0.4))
+    d = 0.02
+    # an interior particle of an unjittered lattice sees the full lattice sum
+    offs = np.array([(a, b, c) for a in range(-2, 3) for b in range(-2, 3) for c in range(-2, 3)], np.float64) * d
+    rr = np.linalg.norm(offs, axis=1)
+    k = 8.0 / (math.pi * H ** 3)
+    q = rr / H
+    w = np.where(q <= 0.5, k * (6 * q ** 3 - 6 * q ** 2 + 1), np.where(q <= 1.0, 2 * k * (1 - q) ** 3, 0.0))
+    m = 0.8 * d ** 3 * 1000.0
+    expect = m * w.sum()
+    dyn = np.nonzero(sim.is_dynamic == 1)[0]
+    c = sim.x[dyn].mean(axis=0)
+    centre = dyn[np.argmin(np.linalg.norm(sim.x[dyn] - c, axis=1))]
+    assert sim.density[centre] == pytest.approx(expect, rel=1e-5)
+    assert expect == pytest.approx(799.98, abs=0.01)
+
+
+def _dense_J(sim):
+    """J assembled column by column from unit probes of the oracle's J d (dynamic columns)."""
+    dyn = np.nonzero(sim.is_dynamic == 1)[0]
+    n = sim.x.shape[0]
+    cols = []
+    for q in dyn:
+        for a in range(3):
+            e = np.zeros((n, 3), np.float32)
+            e[q, a] = 1.0
+            cols.append(O.J(e, sim.m, sim.is_dynamic, sim.C, sim.count, sim.ids, sim.gradW).astype(np.float64))
+    Jfull = np.stack(cols, axis=1)            # rows: all particles (static rows are 0)
+    return Jfull[dyn], dyn
+
+
+def test_aii_is_diagonal_of_J_Minv_JT():
+    sim = tiny_sim()
+    J, dyn = _dense_J(sim)
+    minv = np.repeat(sim.m_inv[dyn].astype(np.float64), 3)
+    diag = np.einsum("ij,j,ij->i", J, minv, J)
+    np.testing.assert_allclose(1.0 / sim.k[dyn].astype(np.float64) - 1e-3, diag, rtol=2e-4)
+    assert np.all(sim.k[sim.is_dynamic == 0] == 0.0)
+
+
+def test_J_translation_invariance_in_the_bulk():
+    sim = tiny_sim(n=(8, 8, 8))
+    n = sim.x.shape[0]
+    d = np.tile(np.array([[0.3, -0.2, 0.1]], np.float32), (n, 1))
+    d[sim.is_dynamic == 0] = 0.0
+    Jd = O.J(d, sim.m, sim.is_dynamic, sim.C, sim.count, sim.ids, sim.gradW)
+    dyn = np.nonzero(sim.is_dynamic == 1)[0]
+    for i in dyn:
+        row = sim.ids[i * sim.C:i * sim.C + sim.count[i]]
+        if np.all(sim.is_dynamic[row] == 1):          # no static neighbour: J d = 0
+            assert abs(Jd[i]) < 1e-3 * np.abs(sim.m[row, None] * sim.gradW[i * sim.C:i * sim.C + sim.count[i]]).sum()
+
+
+def test_JT_is_adjoint_of_J():
+    sim = tiny_sim()
+    n = sim.x.shape[0]
+    rng = np.random.default_rng(7)
+    x = rng.normal(size=(n, 3)).astype(np.float32)
+    y = rng.normal(size=n).astype(np.float32)
+    y[sim.is_dynamic == 0] = 0.0
+    x[sim.is_dynamic == 0] = 0.0
+    Jx = O.J(x, sim.m, sim.is_dynamic, sim.C, sim.count, sim.ids, sim.gradW).astype(np.float64)
+    JTy = O.JT(y, sim.m, sim.is_dynamic, sim.C, sim.count, sim.ids, sim.gradW).astype(np.float64)
+    lhs = float(np.dot(Jx, y))
+    rhs = float(np.sum(x.astype(np.float64) * JTy))
+    assert lhs == pytest.approx(rhs, rel=1e-4)
+
+
+def test_ip_operator_equals_dense_M_plus_omega_JT_D_J_and_is_spd():
+    sim = tiny_sim(n=(6, 6, 6))
+    J, dyn = _dense_J(sim)
+    n = sim.x.shape[0]
+    rng = np.random.default_rng(9)
+    dfdt = (rng.uniform(size=n) < 0.6).astype(np.float32)
+    dfdt[sim.is_dynamic == 0] = 0.0
+    omega = 10.0
+    D = (dfdt[dyn] * sim.k[dyn]).astype(np.float64)
+    M = np.repeat(sim.m[dyn].astype(np.float64), 3)
+    A = np.diag(M) + omega * J.T @ (D[:, None] * J)
+    p = rng.normal(size=(n, 3)).astype(np.float32)
+    p[sim.is_dynamic == 0] = 0.0
+    Ap = O.ip_apply_A(sim.m, sim.is_dynamic, sim.C, sim.count, sim.ids, sim.gradW, sim.k, dfdt, omega, p)
+    np.testing.assert_allclose(Ap[dyn].reshape(-1), A @ p[dyn].reshape(-1).astype(np.float64), rtol=2e-4,
+                               atol=2e-4 * np.abs(A @ p[dyn].reshape(-1)).max())
+    assert np.allclose(A, A.T, rtol=1e-8, atol=1e-12 * np.abs(A).max())
+    assert np.linalg.eigvalsh(A).min() > 0.0
+
+
+# ---------------------------------------------------------------------------------------------
+# IP.solve (ip.py:283-404)
+# ---------------------------------------------------------------------------------------------
+
+def _compressed(sim, factor=0.9):
+    """Shrink the fluid towards its centre so that it is over-dense and IP has work to do."""
+    dyn = sim.is_dynamic == 1
+    c = sim.x[dyn].mean(axis=0)
+    sim.x[dyn] = (c + (sim.x[dyn] - c) * factor).astype(np.float32)
+    sim.rebuild()
+    return sim
+
+
+def test_ip_single_update_matches_dense_solve():
+    sim = _compressed(tiny_sim(n=(6, 6, 6)), 0.85)
+    J, dyn = _dense_J(sim)
+    n = sim.x.shape[0]
+    dt = 1e-3
+    rng = np.random.default_rng(1)
+    v = rng.normal(scale=0.05, size=(n, 3)).astype(np.float32)
+    v[sim.is_dynamic == 0] = 0.0
+    kw = dict(omega=10.0, tol_opt=1e-12, max_iteration_opt=1, use_fixed_iters=True, tol_pcg=1e-24,
+              max_iteration_pcg=1000)
+    v_new, _, res = O.ip_solve(v, sim.m, sim.density, sim.density0, sim.is_dynamic, sim.C, sim.count, sim.ids,
+                               sim.gradW, sim.k, dt, **kw)
+    assert res.status == O.ORC_OK and res.outer_iters == 1
+    # one update: t = J d + c, f = max(t, 0), D = [t >= 1e-6] k, (M + w J^T D J) s = w J^T (f k), d -= s
+    d = (np.float32(dt) * v[dyn]).reshape(-1).astype(np.float64)
+    c = (sim.density[dyn] - sim.density0[dyn]).astype(np.float64)
+    t = J @ d + c
+    f = np.maximum(t, 0.0)
+    Dv = (t >= 1e-6) * sim.k[dyn].astype(np.float64)
+    M = np.repeat(sim.m[dyn].astype(np.float64), 3)
+    A = np.diag(M) + 10.0 * J.T @ (Dv[:, None] * J)
+    b = 10.0 * J.T @ (f * sim.k[dyn])
+    s = np.linalg.solve(A, b)
+    v_expect = (d - s) / dt
+    np.testing.assert_allclose(v_new[dyn].reshape(-1), v_expect, rtol=1e-4, atol=1e-4 * np.abs(v_expect).max())
+
+
+def test_ip_stop_rules():
+    # f == 0 everywhere (a sparse fluid): one update is still applied, with 0 CG iterations
+    sim = tiny_sim(n=(4, 4, 4))
+    sim.x[sim.is_dynamic == 1] += np.float32(0.0)
+    n = sim.x.shape[0]
+    dens = np.full(n, 100.0, np.float32)
+    v = np.zeros((n, 3), np.float32)
+    kw = dict(omega=10.0, tol_opt=1e-3, max_iteration_opt=10)
+    v2, _, res = O.ip_solve(v, sim.m, dens, sim.density0, sim.is_dynamic, sim.C, sim.count, sim.ids, sim.gradW,
+                            sim.k, 1e-3, **kw)
+    assert res.status == O.ORC_OK and res.outer_iters == 1 and res.pcg_iters == [0]
+    assert res.density_errors == [0.0, 0.0]
+    # unreachable tolerance with a cap of 1 update: the reference raises (check_converged)
+    sim = _compressed(tiny_sim(n=(6, 6, 6)), 0.85)
+    kw = dict(omega=10.0, tol_opt=1e-12, max_iteration_opt=1)
+    _, _, res = O.ip_solve(np.zeros((sim.x.shape[0], 3), np.float32), sim.m, sim.density, sim.density0,
+                           sim.is_dynamic, sim.C, sim.count, sim.ids, sim.gradW, sim.k, 1e-3, **kw)
+    assert res.status == O.ORC_NOT_CONVERGED and res.outer_iters == 1
+    # fixed iterations accept the result
+    _, _, res = O.ip_solve(np.zeros((sim.x.shape[0], 3), np.float32), sim.m, sim.density, sim.density0,
+                           sim.is_dynamic, sim.C, sim.count, sim.ids, sim.gradW, sim.k, 1e-3,
+                           **dict(kw, use_fixed_iters=True, max_iteration_opt=2))
+    assert res.status == O.ORC_OK and res.outer_iters == 2
+    # a negative omega makes the operator indefinite: p.Ap < 0 is reported
+    _, _, res = O.ip_solve(np.zeros((sim.x.shape[0], 3), np.float32), sim.m, sim.density, sim.density0,
+                           sim.is_dynamic, sim.C, sim.count, sim.ids, sim.gradW, sim.k, 1e-3,
+                           omega=-1e6, tol_opt=1e-3, max_iteration_opt=5)
+    assert res.status == O.ORC_NOT_PD and res.pAp_bad < 0.0
+
+
+def test_ip_converges_below_tolerance_and_leaves_static_untouched():
+    sim = _compressed(tiny_sim(n=(8, 8, 8)), 0.85)
+    n = sim.x.shape[0]
+    v = np.zeros((n, 3), np.float32)
+    v2, div, res = O.ip_solve(v, sim.m, sim.density, sim.density0, sim.is_dynamic, sim.C, sim.count, sim.ids,
+                              sim.gradW, sim.k, 1e-3, omega=10.0, tol_opt=1e-3, max_iteration_opt=100)
+    assert res.status == O.ORC_OK
+    assert res.final_density_error < 1e-3 and res.final_density_error == res.last_err
+    assert np.all(v2[sim.is_dynamic == 0] == 0.0) and np.all(div[sim.is_dynamic == 0] == 0.0)
+    assert np.all(div >= 0.0)
+
+
+# ---------------------------------------------------------------------------------------------
+# Step glue (framework.py:26-41, 117-136; boundary.py:27-91)
+# ---------------------------------------------------------------------------------------------
+
+def test_free_fall_closed_form():
+    v = np.zeros((1, 3), np.float32)
+    dyn = np.ones(1, np.int32)
+    g = np.array([0.0, -9.81, 0.0], np.float32)
+    for _ in range(10):
+        v = O.gravity(v, dyn, g, 1e-3)
+    assert v[0, 1] == pytest.approx(-9.81e-2, rel=1e-5) and v[0, 0] == 0.0
+
+
+def test_boundary_clamp_and_reflection():
+    L = np.array([1.0, 1.0, 1.0], np.float32)
+    pad = 0.04
+    x = np.array([[1.2, 0.5, 0.01], [0.5, 0.5, 0.5]], np.float32)
+    v = np.array([[2.0, 1.0, -1.0], [1.0, 1.0, 1.0]], np.float32)
+    x2, v2 = O.enforce_boundary(x, v, np.ones(2, np.int32), L, pad)
+    assert x2[0, 0] == pytest.approx(1.0 - pad) and x2[0, 2] == pytest.approx(pad)
+    nrm = np.array([1.0, 0.0, -1.0]) / math.sqrt(2.0)
+    expect = v[0] - 1.5 * np.dot(v[0], nrm) * nrm
+    np.testing.assert_allclose(v2[0], expect, rtol=1e-6)
+    np.testing.assert_array_equal(v2[1], v[1])
+    xs, vs = O.advect(np.array([[0.1, 0.1, 0.1]], np.float32), np.array([[1.0, 1.0, 1.0]], np.float32),
+                      np.zeros(1, np.int32), 1e-3)
+    assert np.all(vs == 0.0) and np.all(xs == np.float32(0.1))
+
+
+# ---------------------------------------------------------------------------------------------
+# Viscosity (viscosity.py:9-108, weiler18.py:21-401)
+# ---------------------------------------------------------------------------------------------
+
+def test_explicit_viscosity_two_particles_closed_form_and_momentum():
+    x = np.array([[0.1, 0.1, 0.1], [0.12, 0.105, 0.1]], np.float32)
+    v = np.array([[0.3, 0.0, 0.1], [-0.2, 0.1, 0.0]], np.float32)
+    m = np.array([0.0064, 0.0064], np.float32)
+    rho = np.array([1000.0, 1000.0], np.float32)
+    mat = np.ones(2, np.int32)
+    dyn = np.ones(2, np.int32)
+    g = O.grid_init((0.0, 0.0, 0.0), (0.3, 0.3, 0.3), H)
+    count, ids, gradW, _ = O.build_neighbors(g, x, dyn, H, 60)
+    coeff, dt = 0.01, 1e-3
+    for mode in (O.VISC_JACOBI,):
+        v2 = O.viscosity(x, v, m, rho, rho, mat, dyn, H, 60, count, ids, gradW, coeff, dt, mode=mode)
+        r = x[0].astype(np.float64) - x[1]
+        vij = v[0].astype(np.float64) - v[1]
+        rn = np.linalg.norm(r)
+        gw = -45.0 / (math.pi * H ** 6) * (H - rn) ** 2 * r / rn
+        acc = 1.0 * 10.0 * coeff * (0.0064 / 1000.0) * np.dot(vij, r) / (rn * rn + (0.01 * H) ** 2) * gw
+        np.testing.assert_allclose(v2[0] - v[0], acc * dt, rtol=1e-4)
+        # equal masses and densities: the pair forces cancel (up to the float32 rounding of v + dv)
+        np.testing.assert_allclose((v2.astype(np.float64) - v).sum(axis=0), 0.0,
+                                   atol=4 * np.finfo(np.float32).eps * np.abs(v).max())
+    # a uniform velocity field is left unchanged (Galilean invariance), in both execution modes
+    sim = tiny_sim()
+    vu = np.tile(np.array([[0.2, -0.1, 0.05]], np.float32), (sim.x.shape[0], 1))
+    for mode in (O.VISC_JACOBI, O.VISC_INPLACE):
+        v2 = O.viscosity(sim.x, vu, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h, sim.C,
+                         sim.count, sim.ids, sim.gradW, 0.05, 1e-3, mode=mode)
+        np.testing.assert_array_equal(v2, vu)
+
+
+def test_inverse33_matches_numpy():
+    rng = np.random.default_rng(12)
+    for _ in range(20):
+        a = (np.eye(3) + 0.3 * rng.normal(size=(3, 3))).astype(np.float32)
+        np.testing.assert_allclose(O.inverse33(a), np.linalg.inv(a.astype(np.float64)), rtol=1e-4, atol=1e-5)
+    np.testing.assert_array_equal(O.inverse33(np.zeros((3, 3), np.float32)), np.zeros((3, 3), np.float32))
+
+
+def _dense_w18(sim, coeff, dt):
+    dyn = np.nonzero((sim.is_dynamic == 1) & (sim.material == 1))[0]
+    n = sim.x.shape[0]
+    cols = []
+    for q in dyn:
+        for a in range(3):
+            e = np.zeros((n, 3), np.float32)
+            e[q, a] = 1.0
+            Av, _ = O.w18_apply(sim.x, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h, sim.C,
+                                sim.count, sim.ids, sim.gradW, coeff, dt, vin=e)
+            cols.append(Av[dyn].reshape(-1).astype(np.float64))
+    return np.stack(cols, axis=1), dyn
+
+
+def test_weiler18_operator_preconditioner_and_solve():
+    sim = tiny_sim(n=(5, 5, 5))
+    coeff, dt = 0.5, 2e-3
+    A, dyn = _dense_w18(sim, coeff, dt)
+    _, P = O.w18_apply(sim.x, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h, sim.C,
+                       sim.count, sim.ids, sim.gradW, coeff, dt)
+    for bi, i in enumerate(dyn):
+        np.testing.assert_allclose(P[i], np.linalg.inv(A[3 * bi:3 * bi + 3, 3 * bi:3 * bi + 3]), rtol=1e-4,
+                                   atol=1e-6)
+    # uniform velocity: A v = v
+    n = sim.x.shape[0]
+    vu = np.tile(np.array([[0.2, -0.1, 0.05]], np.float32), (n, 1))
+    Av, _ = O.w18_apply(sim.x, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h, sim.C,
+                        sim.count, sim.ids, sim.gradW, coeff, dt, vin=vu)
+    np.testing.assert_allclose(Av[dyn], vu[dyn], rtol=1e-5, atol=1e-7)
+    # the solve: v_new = v + dv with A dv = v - A v, i.e. A v_new = v
+    rng = np.random.default_rng(8)
+    v = rng.normal(scale=1.0, size=(n, 3)).astype(np.float32)
+    v[sim.is_dynamic == 0] = 0.0
+    v2, it = O.weiler18(sim.x, v, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h, sim.C,
+                        sim.count, sim.ids, sim.gradW, coeff, dt)
+    assert it >= 2
+    expect = np.linalg.solve(A, v[dyn].reshape(-1).astype(np.float64))
+    np.testing.assert_allclose(v2[dyn].reshape(-1), expect, rtol=0, atol=5e-3 * np.abs(expect).max())
+
+
+# ---------------------------------------------------------------------------------------------
+# Whole step (framework.py:423-544)
+# ---------------------------------------------------------------------------------------------
+
+def test_oracle_column_runs_within_tolerance():
+    cfg, ps = scene(0)
+    sim = O.OracleSim(cfg, ps)
+    sim.rebuild()
+    for _ in range(10):
+        sim.step()
+        assert sim.last_ip.final_density_error <= 1e-3
+    dyn = sim.is_dynamic == 1
+    assert np.all(np.isfinite(sim.x)) and np.all(sim.x[~dyn] == ps.x[~dyn])
