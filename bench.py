@@ -9,9 +9,11 @@ reference's Framework.forward: gravity, Weiler18 viscosity, IP solve, advection,
 neighbour rebuild, density and Aii -- all of it on the GPU, inputs resident in HBM.
 
 Prints one JSON line.  `value` is dynamic-particle-steps per second over the K timed steps.
-`roofline` reports the kernel with the largest share of the timed region, its algorithmic bytes
-per launch (byte models below) over its average launch time, measured with HIP events on the
-library's stream.  `cpu_baseline` is the CPU oracle (a serial float32 C restatement of the
+`roofline` reports the kernel with the largest share of the timed region: its algorithmic bytes
+per launch (the byte model the library attaches to each launch: every stored neighbour-pair
+record once -- 4 B of id, plus 12 B of gradW when gradients are cached -- and every per-particle
+array once) over its average launch time, measured with HIP events that the kernel dispatch
+itself stamps (hipExtLaunchKernelGGL) on the library's stream.  `cpu_baseline` is the CPU oracle (a serial float32 C restatement of the
 reference kernels) timed on a few steps of the same workload, starting from the GPU state at the
 start of the timed region.
 """
@@ -32,21 +34,29 @@ METRIC = "particle-steps/s + CG iters/s at 1M particles, ≤0.1% density err; % 
 HBM_PEAK_GBS = 8000.0
 
 
-def byte_models(nd, ns, pairs):
-    """Algorithmic bytes per launch of the row kernels: every stored dynamic pair record once
-    (id + gradW = 16 B) and every per-particle array once."""
-    return {
-        "build": 16.0 * pairs + 44.0 * nd + 16.0 * ns,   # write pairs; read x,m; write rho,k,S,counts
-        "ip_pack": 16.0 * pairs + 64.0 * nd,
-        "ip_rhs": 16.0 * pairs + 48.0 * nd,
-        "w18_Av": 16.0 * pairs + 48.0 * nd,              # pairs; x,m; (v, m/rho); factor; write Av
-        "w18_pre": 16.0 * pairs + 56.0 * nd,             # pairs; x,m; m/rho; factor; write 3x3 inverse
-        "w18_update": 124.0 * nd,
-        "w18_p": 44.0 * nd,
-        "visc": 16.0 * pairs + 48.0 * nd,
-        "cg_update": 76.0 * nd,
-        "cg_p": 44.0 * nd,
-    }
+def copy_calibration(torch, mib):
+    """A copy kernel of known size: each dispatch reads and writes exactly `mib` MiB."""
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    assert torch.equal(dst, src)
+    print(json.dumps({"copy_calibration_bytes": n * 4, "dispatches": 5}))
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    measurement (scaled by the copy-kernel calibration), or None when none was recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic_dambreak_1m.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = rec.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("traffic_bytes")
 
 
 def main():
@@ -56,6 +66,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--gradient-cache", type=int, default=0,
+                    help="1: neighbour rows cache float32 gradW (16 B per pair); 0: ids only, gradW from positions")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain device-to-device copies of MIB MiB (the reference kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes), then exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the IP hot path here is a single-device build "
@@ -63,6 +78,9 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        copy_calibration(torch, args.copy_calibration)
+        return
     from implicitvelocityprojection_amd.config import SimConfig
     from implicitvelocityprojection_amd.context import Context, IPParams, kclass_mask
     from implicitvelocityprojection_amd.scenes import load_particles, particle_masses, synthetic_scene
@@ -75,6 +93,7 @@ def main():
     dev = torch.device("cuda:0")
     t = lambda a, dt=torch.float32: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)
     ctx = Context(h, cfg.domain_start, cfg.domain_size, ps.n, cache_size=int(cfg.get_cfg("neighborCacheSize", 60)))
+    ctx.gradient_cache = bool(args.gradient_cache)
     ctx.set_particles(t(ps.x), t(ps.v), t(m), t(m_inv), t(ps.density0), t(ps.is_dynamic, torch.int32),
                       t(ps.material, torch.int32))
     if ctx.neighbors_build() != 0:
@@ -96,8 +115,6 @@ def main():
         step()
     torch.cuda.synchronize()
     prof = ctx.profile_read()
-    pairs = ctx.pair_count()
-    models = byte_models(nd, ns, pairs)
     dominant = max(prof, key=lambda k: prof[k]["total_ms"])
 
     # state at the start of the timed region (for the CPU baseline)
@@ -128,10 +145,7 @@ def main():
     ctx.profile_enable(0)
     pd = prof[dominant]
     avg_ms = pd["total_ms"] / max(pd["launches"], 1)
-    if pd["bytes"] > 0:
-        bytes_per_launch = pd["bytes"] / max(pd["launches"], 1)
-    else:
-        bytes_per_launch = models.get(dominant, 0.0)
+    bytes_per_launch = pd["bytes"] / max(pd["launches"], 1)
     pairs_end = ctx.pair_count()
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
 
@@ -164,6 +178,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "dambreak_1m", "fluid_particles": nd, "static_particles": ns,
+                   "gradient_cache": bool(args.gradient_cache),
                    "dt": cfg.dt, "viscosity": "Weiler18 0.01", "densityErrorTol": prm.tol_opt,
                    "maxOuterIters": prm.max_iteration_opt, "neighbour_pairs": pairs_end},
         "cg_iters_per_s": cg_iters / (solve_ms * 1e-3) if solve_ms > 0 else 0.0,
@@ -173,7 +188,7 @@ def main():
         "ip_solve_ms_per_step": solve_ms / args.steps,
         "max_final_density_error": max_err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant), "kernel": dominant,
                      "avg_launch_us": avg_ms * 1e3, "launches": pd["launches"],
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,

@@ -22,6 +22,7 @@ IVP_ERR_GRID = -5
 IVP_ERR_INTERNAL = -6
 
 FIELD_X, FIELD_V, FIELD_DENSITY, FIELD_K, FIELD_DIVERGENCE = 0, 1, 2, 3, 4
+OPT_GRADIENT_CACHE = 1
 
 # kernel classes of ivp_profile_read / ivp_profile_enable masks
 KCLASSES = ["sort", "build", "ip_pack", "ip_rhs", "cg_J", "cg_JT", "cg_update", "cg_p", "visc", "w18_Av",
@@ -71,6 +72,8 @@ SIGNATURES = {
     "ivp_create": (_I, [ctypes.POINTER(Config), _I, _P, ctypes.POINTER(_P)]),
     "ivp_destroy": (_I, [_P]),
     "ivp_set_particles": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "ivp_set_option": (_I, [_P, _I, _I]),
+    "ivp_get_option": (_I, [_P, _I, ctypes.POINTER(_I)]),
     "ivp_write": (_I, [_P, _I, _P]),
     "ivp_read": (_I, [_P, _I, _P]),
     "ivp_neighbors_build": (_I, [_P]),

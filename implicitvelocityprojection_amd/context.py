@@ -103,6 +103,17 @@ class Context:
         N.check(self.lib.ivp_counts(self.h, None, None, ctypes.byref(p)))
         return p.value
 
+    @property
+    def gradient_cache(self):
+        """True when neighbour rows cache float32 gradW per pair (see include/ivp.h)."""
+        v = ctypes.c_int()
+        N.check(self.lib.ivp_get_option(self.h, N.OPT_GRADIENT_CACHE, ctypes.byref(v)))
+        return bool(v.value)
+
+    @gradient_cache.setter
+    def gradient_cache(self, on):
+        N.check(self.lib.ivp_set_option(self.h, N.OPT_GRADIENT_CACHE, int(bool(on))), "ivp_set_option")
+
     # -- particle store ----------------------------------------------------------------------
     def set_particles(self, x, v, m, m_inv, density0, is_dynamic, material):
         n = int(x.shape[0])

@@ -7,6 +7,7 @@
 #include <algorithm>
 
 #include "ivp_internal.h"
+#include "ivp_launch.h"
 
 using namespace ivp;
 
@@ -38,7 +39,7 @@ void free_all(ivp_ctx* c) {
     dfree(c->rowcnt); dfree(c->rowtot); dfree(c->nb); dfree(c->tile_len); dfree(c->dstart); dfree(c->sstart);
     dfree(c->ccount); dfree(c->key); dfree(c->tmp_src); dfree(c->order); dfree(c->scan_bsum);
     dfree(c->d4); dfree(c->q4); dfree(c->b); dfree(c->sol); dfree(c->r); dfree(c->Ap); dfree(c->p4); dfree(c->y2);
-    dfree(c->cc); dfree(c->sfac); dfree(c->nbact);
+    dfree(c->cc); dfree(c->sfac); dfree(c->jd); dfree(c->nbact);
     dfree(c->w_pinv); dfree(c->w_p4); dfree(c->w_v4); dfree(c->w_dv); dfree(c->w_r); dfree(c->w_z);
     dfree(c->w_Ap); dfree(c->w_rhs); dfree(c->w_fac); dfree(c->v4);
     dfree(c->partials); dfree(c->state); dfree(c->err_flag);
@@ -73,7 +74,7 @@ int alloc_all(ivp_ctx* c) {
     c->nscan_blocks = (int)((scan_n + 2047) / 2048) + 2;
     A(scan_bsum, (size_t)c->nscan_blocks + 1);
     A(d4, cap); A(q4, cap); A(b, 3 * cap); A(sol, 3 * cap); A(r, 3 * cap); A(Ap, 3 * cap); A(p4, cap); A(y2, cap);
-    A(cc, cap); A(sfac, cap); A(nbact, cap);
+    A(cc, cap); A(sfac, cap); A(jd, cap); A(nbact, cap);
     A(w_pinv, 9 * cap); A(w_p4, cap); A(w_v4, cap); A(w_dv, 3 * cap); A(w_r, 3 * cap); A(w_z, 3 * cap);
     A(w_Ap, 3 * cap); A(w_rhs, 3 * cap); A(w_fac, cap); A(v4, cap);
     c->max_blocks = blocks_for(c->cap) + 1;
@@ -91,12 +92,18 @@ int alloc_all(ivp_ctx* c) {
     return IVP_OK;
 }
 
+// Neighbour rows sized for the loaded dynamic count.
 int ensure_rows(ivp_ctx* c) {
     size_t tiles = (size_t)blocks_for(std::max(c->nd, 1)) * WAVES_PER_BLOCK;
     static_assert(sizeof(uint32_t) == 4, "");
-    size_t words = tiles * (size_t)c->C * 256u;
+    size_t slots = tiles * (size_t)c->C * 64u;
     if (c->nb) return IVP_OK;
-    return dalloc(&c->nb, words);
+    IVP_TRY(dalloc(&c->nb, slots * (size_t)(c->rec_words / 64)));
+    return IVP_OK;
+}
+
+void free_rows(ivp_ctx* c) {
+    dfree(c->nb);
 }
 
 }  // namespace
@@ -106,16 +113,17 @@ namespace ivp {
 // Event timing of selected kernel classes.  Launches tagged with a CG iteration index are kept
 // only if that iteration ran (prof_collect), so trailing no-op launches of the last CG chunk are
 // not counted.
-int prof_begin(ivp_ctx* c, int kclass, int tag, double bytes, ProfEvent* ev) {
+int prof_take(ivp_ctx* c, int kclass, int tag, double bytes, ProfEvent* ev) {
     ev->start = nullptr;
-    if (!c->prof || !(c->prof_mask & (1u << kclass))) return IVP_OK;
     auto take = [&](hipEvent_t* e) -> int {
         if (!c->prof_free.empty()) {
             *e = c->prof_free.back();
             c->prof_free.pop_back();
             return IVP_OK;
         }
-        IVP_HIP(hipEventCreate(e));
+        // timing-only events: no system-scope fence (cache write-back) around the timed kernel;
+        // they are read only after the stream has been synchronised
+        IVP_HIP(hipEventCreateWithFlags(e, hipEventDisableSystemFence));
         return IVP_OK;
     };
     if (take(&ev->start) != IVP_OK || take(&ev->stop) != IVP_OK) return IVP_ERR_HIP;
@@ -123,13 +131,10 @@ int prof_begin(ivp_ctx* c, int kclass, int tag, double bytes, ProfEvent* ev) {
     ev->tag = tag;
     ev->epoch = c->prof_epoch;
     ev->bytes = bytes;
-    IVP_HIP(hipEventRecord(ev->start, c->stream));
     return IVP_OK;
 }
 
-int prof_end(ivp_ctx* c, ProfEvent* ev) {
-    if (!ev->start) return IVP_OK;
-    IVP_HIP(hipEventRecord(ev->stop, c->stream));
+int prof_push(ivp_ctx* c, ProfEvent* ev) {
     c->prof_pending.push_back(*ev);
     if (c->prof_pending.size() > 8192) {
         IVP_HIP(hipStreamSynchronize(c->stream));
@@ -204,6 +209,12 @@ int ivp_create(const ivp_config* cfg, int max_particles, void* stream, ivp_ctx**
     // ceil(L / w) + 4 cells per axis
     float w = c->h * 1.0009765625f;
     c->grid.inv_w = 1.0f / w;
+    {
+        float h = c->h;
+        c->kspiky = 45.0f / (3.14159265359f * h * h * h * h * h * h);
+    }
+    c->recomp = true;
+    c->rec_words = 64;
     long long ncell = 1;
     for (int a = 0; a < 3; ++a) {
         c->grid.origin[a] = cfg->domain_start[a] - 2.0f * w;
@@ -237,7 +248,7 @@ int ivp_create(const ivp_config* cfg, int max_particles, void* stream, ivp_ctx**
 int ivp_destroy(ivp_ctx* c) {
     if (!c) return IVP_ERR_ARG;
     (void)hipStreamSynchronize(c->stream);
-    dfree(c->nb);
+    free_rows(c);
     free_all(c);
     delete c;
     return IVP_OK;
@@ -246,10 +257,34 @@ int ivp_destroy(ivp_ctx* c) {
 int ivp_set_particles(ivp_ctx* c, int n, const float* x, const float* v, const float* m, const float* m_inv,
                       const float* density0, const int* is_dynamic, const int* material) {
     if (!c) return IVP_ERR_ARG;
-    dfree(c->nb);
+    free_rows(c);
     int rc = grid_load(c, n, x, v, m, m_inv, density0, is_dynamic, material);
     if (rc != IVP_OK) return rc;
     return ensure_rows(c);
+}
+
+int ivp_set_option(ivp_ctx* c, int option, int value) {
+    if (!c) return IVP_ERR_ARG;
+    if (option == IVP_OPT_GRADIENT_CACHE) {
+        bool recomp = value == 0;
+        if (recomp == c->recomp) return IVP_OK;
+        IVP_HIP(hipStreamSynchronize(c->stream));
+        free_rows(c);
+        c->recomp = recomp;
+        c->rec_words = recomp ? 64 : 256;
+        c->built = false;
+        return c->loaded ? ensure_rows(c) : IVP_OK;
+    }
+    return IVP_ERR_ARG;
+}
+
+int ivp_get_option(ivp_ctx* c, int option, int* value) {
+    if (!c || !value) return IVP_ERR_ARG;
+    if (option == IVP_OPT_GRADIENT_CACHE) {
+        *value = c->recomp ? 0 : 1;
+        return IVP_OK;
+    }
+    return IVP_ERR_ARG;
 }
 
 int ivp_write(ivp_ctx* c, int field, const void* src) {

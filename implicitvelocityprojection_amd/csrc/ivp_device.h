@@ -108,11 +108,4 @@ __device__ __forceinline__ double block_sum(double v, double* lds4) {
     return r;
 }
 
-// Fixed-order sum of n doubles by a 256-thread block (result valid in thread 0).
-__device__ __forceinline__ double block_sum_array(const double* a, int n, double* lds4) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += BLOCK) s += a[i];
-    return block_sum(s, lds4);
-}
-
 }  // namespace ivp

@@ -10,6 +10,7 @@
 // x-fastest linear, so the three cells (cx-1..cx+1, cy, cz) of a stencil row are one contiguous
 // particle range: a query visits 9 ranges per population.
 #include "ivp_internal.h"
+#include "ivp_launch.h"
 #include "ivp_reduce.h"
 
 using namespace ivp;
@@ -255,7 +256,9 @@ struct BuildArgs {
 //     density (pressure_base.py:12-46), Aii (pressure_base.py:49-87) and S_i, accumulated in the
 //     reference row order: dynamic neighbours, then static neighbours.  Dynamic slots from the
 //     row's count up to the tile length are padded with (id = row, gradW = 0).
+template <bool RECOMP>
 __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
+    constexpr unsigned REC = RECOMP ? 64u : 256u;   // words per (tile, slot) record
     int blk = xcd_block(blockIdx.x, gridDim.x);
     int i = blk * BLOCK + threadIdx.x;
     int lane = threadIdx.x & (WAVE - 1);
@@ -263,18 +266,18 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
     bool valid = i < a.nd;
     const float h = a.h;
     const int C = a.C;
-    uint32_t* tbase = a.nb + (size_t)tile * (size_t)C * 256u + lane;
+    uint32_t* tbase = a.nb + (size_t)tile * (size_t)C * REC + lane;
     int cnt = 0, cs = 0, tot = 0;
     float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (valid) {
         pi = a.pos[i];
         scan_ids(a.g, a.dstart, a.pos, i, pi, h, a.h2_hi, [&](int j) {
-            if (tot < C) tbase[(size_t)cnt * 256u] = (uint32_t)j;
+            if (tot < C) tbase[(size_t)cnt * REC] = (uint32_t)j;
             if (tot < C) cnt++;
             tot++;
         });
         scan_ids(a.g, a.sstart, a.pos + a.nd, -1, pi, h, a.h2_hi, [&](int j) {
-            if (tot < C) tbase[(size_t)(C - 1 - cs) * 256u] = (uint32_t)j;
+            if (tot < C) tbase[(size_t)(C - 1 - cs) * REC] = (uint32_t)j;
             if (tot < C) cs++;
             tot++;
         });
@@ -289,7 +292,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     for (int k = 0; k < tl; ++k) {
         if (!valid || k >= cnt) continue;
-        uint32_t* rec = tbase + (size_t)k * 256u;
+        uint32_t* rec = tbase + (size_t)k * REC;
         int j = (int)rec[0];
         if ((unsigned)j >= (unsigned)a.nd) {  // never taken: phase A stored a dynamic id here
             atomicOr(a.err_flag, 4);
@@ -299,9 +302,11 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
         float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         float rn = len3(rx, ry, rz);
         float3 gw = spiky_grad(rx, ry, rz, h);
-        rec[64] = __float_as_uint(gw.x);
-        rec[128] = __float_as_uint(gw.y);
-        rec[192] = __float_as_uint(gw.z);
+        if (!RECOMP) {
+            rec[64] = __float_as_uint(gw.x);
+            rec[128] = __float_as_uint(gw.y);
+            rec[192] = __float_as_uint(gw.z);
+        }
         rho += pj.w * cubic_kernel(rn, h);
         float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
         aii += (Jx * Jx + Jy * Jy + Jz * Jz) * a.minv[j];
@@ -309,7 +314,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
     }
     for (int k = 0; k < ts; ++k) {
         if (!valid || k >= cs) continue;
-        int j = (int)tbase[(size_t)(C - 1 - k) * 256u];
+        int j = (int)tbase[(size_t)(C - 1 - k) * REC];
         if ((unsigned)j >= (unsigned)a.ns) {  // never taken: phase A stored a static id here
             atomicOr(a.err_flag, 4);
             continue;
@@ -327,11 +332,13 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
     // [cnt, tl) follows the wave's longest row and can reach into them.
     for (int k = 0; k < tl; ++k) {
         if (!valid || k < cnt) continue;
-        uint32_t* rec = tbase + (size_t)k * 256u;
+        uint32_t* rec = tbase + (size_t)k * REC;
         rec[0] = (uint32_t)i;
-        rec[64] = 0u;
-        rec[128] = 0u;
-        rec[192] = 0u;
+        if (!RECOMP) {
+            rec[64] = 0u;
+            rec[128] = 0u;
+            rec[192] = 0u;
+        }
     }
     if (valid) {
         aii += (jx * jx + jy * jy + jz * jz) * a.minv[i];
@@ -508,25 +515,23 @@ int grid_sort_population(ivp_ctx* c, int offset, int count, int* start_table, bo
         IVP_HIP(hipMemsetAsync(start_table, 0, sizeof(int) * (size_t)(c->ncell + 1), c->stream));
         return IVP_OK;
     }
-    ProfEvent ev;
-    prof_begin(c, KC_SORT, -1, (double)count * (16 + 4 + 4 + 4 + 4 + 4 + 4 + 16 + (dynamic ? 24 : 0) + 24), &ev);
-    int nb = blocks_for(count);
-    hipLaunchKernelGGL(k_keys, dim3(nb), dim3(BLOCK), 0, c->stream, count, c->pos + offset, c->grid, c->key,
-                       c->ccount);
-    int rc = scan(c, c->ccount, start_table, c->ncell);
-    if (rc != IVP_OK) return rc;
-    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(BLOCK), 0, c->stream, count, c->key, start_table, c->ccount,
-                       c->tmp_src);
-    hipLaunchKernelGGL(k_rank, dim3(nb), dim3(BLOCK), 0, c->stream, count, c->key, start_table, c->tmp_src,
-                       c->uid + offset, c->order);
+    const dim3 nb(blocks_for(count));
+    // bytes: keys (read x 16, write key 4), scatter (key 4 + slot 4), rank (key, uid, cell members
+    // ~ 3 x 4), gather (index 4 + every per-particle array read and written)
+    const double per = dynamic ? 4 + 2 * (16 + 12 + 4 + 4 + 4 + 4 + 4) + 4 : 4 + 2 * (16 + 4 + 4 + 4 + 4) + 4;
+    IVP_TRY(launch(c, KC_SORT, -1, 20.0 * count, k_keys, nb, count, (const float4*)(c->pos + offset), c->grid, c->key,
+                   c->ccount));
+    IVP_TRY(scan(c, c->ccount, start_table, c->ncell));
+    IVP_TRY(launch(c, KC_SORT, -1, 12.0 * count, k_scatter, nb, count, (const int*)c->key, (const int*)start_table,
+                   c->ccount, c->tmp_src));
+    IVP_TRY(launch(c, KC_SORT, -1, 20.0 * count, k_rank, nb, count, (const int*)c->key, (const int*)start_table,
+                   (const int*)c->tmp_src, (const int*)(c->uid + offset), c->order));
     GatherArgs ga{count, offset, c->order, c->pos, c->pos_b, c->vel, c->vel_b, c->minv, c->minv_b,
                   c->rho0, c->rho0_b, c->mat, c->mat_b, c->uid, c->uid_b, c->inv, c->divg, c->divg_b};
     if (dynamic)
-        hipLaunchKernelGGL(k_gather<true>, dim3(nb), dim3(BLOCK), 0, c->stream, ga);
+        IVP_TRY(launch(c, KC_SORT, -1, per * count, k_gather<true>, nb, ga));
     else
-        hipLaunchKernelGGL(k_gather<false>, dim3(nb), dim3(BLOCK), 0, c->stream, ga);
-    IVP_HIP(hipGetLastError());
-    prof_end(c, &ev);
+        IVP_TRY(launch(c, KC_SORT, -1, per * count, k_gather<false>, nb, ga));
     swap_ptr(c->pos, c->pos_b);
     swap_ptr(c->minv, c->minv_b);
     swap_ptr(c->rho0, c->rho0_b);
@@ -613,13 +618,16 @@ int grid_build(ivp_ctx* c) {
         float h2_hi = (float)((double)c->h * (double)c->h * (1.0 + 1.0 / 1048576.0));
         BuildArgs ba{c->nd, c->ns, c->C, c->h, h2_hi, c->grid, c->pos, c->minv, c->dstart, c->sstart, c->nb, c->tile_len,
                      c->rowcnt, c->rowtot, c->density, c->kk, c->Ss, c->err_flag, c->partials, c->max_blocks};
-        ProfEvent ev;
-        prof_begin(c, KC_BUILD, -1, 0.0, &ev);
-        hipLaunchKernelGGL(k_build, dim3(blocks_for(c->nd)), dim3(BLOCK), 0, c->stream, ba);
-        IVP_HIP(hipGetLastError());
-        prof_end(c, &ev);
-        int rc = finalize<FIN_BUILD>(c, blocks_for(c->nd));
-        if (rc != IVP_OK) return rc;
+        // bytes (pair count of the previous build): the pair records written, the ids read back
+        // once, x and m of every particle once, the per-row outputs (rho, k, S, counts)
+        double pairs = (double)c->state_host->total_pairs;
+        double bytes = pairs * ((c->recomp ? 4.0 : 16.0) + 4.0) + 16.0 * c->n + 32.0 * c->nd;
+        const dim3 nb(blocks_for(c->nd));
+        if (c->recomp)
+            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<true>, nb, ba));
+        else
+            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<false>, nb, ba));
+        IVP_TRY(finalize<FIN_BUILD>(c, nb.x));
     }
     c->built = true;
     return IVP_OK;

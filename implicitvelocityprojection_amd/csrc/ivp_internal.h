@@ -9,11 +9,13 @@
 // Neighbour rows exist for dynamic particles only and hold their DYNAMIC neighbours; the
 // static neighbours of a row enter the operators only through S_i = sum_static m_j gradW_ij,
 // density and Aii, all accumulated during the build.  Rows are stored in wave tiles:
-// tile t = rows [64t, 64t+64), slot k, component c (0 = neighbour id, 1..3 = gradW xyz):
-//     nb[((t * C + k) * 4 + c) * 64 + lane]
-// so every (tile, slot) is one contiguous 1 KiB record read by one wave.  Slots past a row's
-// count up to the tile length hold (id = the row itself, gradW = 0), which adds exactly 0 to
-// every operator, so the hot loops run a wave-uniform trip count.
+// tile t = rows [64t, 64t+64), slot k, component c (0 = neighbour id, 1..3 = gradW xyz when
+// gradients are cached):
+//     nb[((t * C + k) * R + c) * 64 + lane],   R = 4 (cached gradW) or 1 (ids only)
+// so every (tile, slot) is one contiguous record (1 KiB or 256 B) read by one wave.  Slots past a
+// row's count up to the tile length hold (id = the row itself, gradW = 0): r_ii = 0 gives
+// gradW = 0 either way, which adds exactly 0 to every operator, so the hot loops run a
+// wave-uniform trip count.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -99,6 +101,11 @@ struct ivp_ctx {
     bool loaded = false;
     bool built = false;
     bool all_dyn_fluid = true;
+    // neighbour-row format: recomp = rows hold ids only and the operator passes evaluate gradW
+    // from positions; otherwise rows cache float32 gradW next to each id (rec_words per slot)
+    bool recomp = true;
+    int rec_words = 64;
+    float kspiky = 0.0f;    // 45 / (pi h^6), the spiky gradient constant
 
     // particle store (internal order) and reorder buffers
     float4* pos = nullptr;  float4* pos_b = nullptr;    // x, y, z, m
@@ -137,6 +144,7 @@ struct ivp_ctx {
     float2* y2 = nullptr;   // (dfdt k J p, m)
     float* cc = nullptr;    // constraint c = rho - rho0
     float* sfac = nullptr;  // dfdt * k
+    float* jd = nullptr;    // J d of the last outer iteration
     int* nbact = nullptr;   // row has an active particle in its closed neighbourhood
 
     // Weiler18 vectors
@@ -172,9 +180,10 @@ struct ivp_ctx {
 
 namespace ivp {
 
-// profiling helpers (ivp_api.hip)
-int prof_begin(ivp_ctx* c, int kclass, int tag, double bytes, ProfEvent* ev);
-int prof_end(ivp_ctx* c, ProfEvent* ev);
+// profiling helpers (ivp_api.hip): take a pair of timing events, queue them after the launch,
+// fold finished ones into the per-class totals
+int prof_take(ivp_ctx* c, int kclass, int tag, double bytes, ProfEvent* ev);
+int prof_push(ivp_ctx* c, ProfEvent* ev);
 int prof_collect(ivp_ctx* c, int cg_iters_done, int epoch);
 
 // ivp_grid.hip

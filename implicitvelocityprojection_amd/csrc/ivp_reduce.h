@@ -30,19 +30,59 @@ struct FinArgs {
     int max_iter;
 };
 
+constexpr int FIN_BLOCK = 1024;
+constexpr int FIN_UNROLL = 4;  // the fold below adds exactly four loads per row
+
+// Fixed-order fold of NV rows of n partials by one FIN_BLOCK-thread block (valid in thread 0).
+// Each thread issues NV * FIN_UNROLL independent loads before adding, so the fold costs a few
+// memory latencies instead of one per loop trip.
+template <int NV>
+__device__ __forceinline__ void fold_partials(const double* __restrict__ partials, int stride, int n,
+                                              double (&t)[3]) {
+    __shared__ double lds[NV][FIN_BLOCK / WAVE];
+    double s[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) s[q] = 0.0;
+    for (int base = 0; base < n; base += FIN_UNROLL * FIN_BLOCK) {
+        double v[NV][FIN_UNROLL];
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+#pragma unroll
+            for (int u = 0; u < FIN_UNROLL; ++u) {
+                int idx = base + u * FIN_BLOCK + (int)threadIdx.x;
+                v[q][u] = idx < n ? partials[q * stride + idx] : 0.0;
+            }
+#pragma unroll
+        for (int q = 0; q < NV; ++q) s[q] += (v[q][0] + v[q][1]) + (v[q][2] + v[q][3]);
+    }
+    int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double x = wave_sum(s[q]);
+        if (lane == 0) lds[q][w] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double x = 0.0;
+            for (int k = 0; k < FIN_BLOCK / WAVE; ++k) x += lds[q][k];
+            t[q] = x;
+        }
+    }
+}
+
 // One block: fold the partials of the previous kernel in a fixed order and advance the solver
 // state (the scalar host logic of ip.py:296-327, pressure_base.py:405-423, weiler18.py:330-365).
 template <int OP>
-__global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
-    __shared__ double lds[WAVES_PER_BLOCK];
+__global__ __launch_bounds__(FIN_BLOCK) void k_finalize(FinArgs a) {
     constexpr int NV = OP == FIN_PACK ? 3 : (OP == FIN_RHS || OP == FIN_W18_UPDATE) ? 2 : 1;
     SolverState* st = a.st;
     if (OP == FIN_CG_PAP || OP == FIN_CG_UPDATE || OP == FIN_W18_PAP || OP == FIN_W18_UPDATE) {
         if (st->active == 0) return;
     }
     double t[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int q = 0; q < NV; ++q) t[q] = block_sum_array(a.partials + q * a.stride, a.nblocks, lds);
+    fold_partials<NV>(a.partials, a.stride, a.nblocks, t);
     if (threadIdx.x != 0) return;
     if (OP == FIN_BUILD) {
         st->total_pairs = (long long)t[0];
@@ -116,7 +156,7 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(FinArgs a) {
 template <int OP>
 inline int finalize(ivp_ctx* c, int nblocks, double tol = 0.0, int max_iter = 0) {
     FinArgs fa{c->partials, c->max_blocks, nblocks, c->state, tol, max_iter};
-    hipLaunchKernelGGL(k_finalize<OP>, dim3(1), dim3(BLOCK), 0, c->stream, fa);
+    hipLaunchKernelGGL(k_finalize<OP>, dim3(1), dim3(FIN_BLOCK), 0, c->stream, fa);
     IVP_HIP(hipGetLastError());
     return IVP_OK;
 }

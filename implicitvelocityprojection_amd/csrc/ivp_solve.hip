@@ -15,15 +15,24 @@
 // i.e. one J pass and one J^T pass, both pure gathers with no atomics.  Rows whose closed
 // neighbourhood holds no particle with dfdt > 0 contribute exactly 0 and skip their loop.
 //
+// Pair gradients.  Rows either cache float32 gradW per pair (16 B per pair and pass, the
+// reference's layout) or hold ids only (4 B) with gradW evaluated from the positions in every
+// pass: 1/|r| from the hardware reciprocal square root (v_rsq_f32, 1 ulp), then
+// -45/(pi h^6) (h - |r|)^2 r/|r|.  The evaluation is a function of r_ij = x_i - x_j alone, so
+// gradW_ji = -gradW_ij still holds bitwise; density and Aii always use the reference's exact
+// expression (in the neighbour build).
+//
 // Control.  CG dot products are reduced in FP64 on the device; alpha, beta and the stop test
-// (ip.py:302-327) are evaluated by a one-block finalize kernel after each reduction, and the host enqueues
-// CG iterations in chunks, synchronising once per chunk instead of twice per iteration.
+// (ip.py:302-327) are evaluated by a one-block finalize kernel after each reduction, and the
+// host enqueues CG iterations in chunks, synchronising once per chunk instead of twice per
+// iteration.
 #include <math.h>
 
 #include <algorithm>
 #include <chrono>
 
 #include "ivp_internal.h"
+#include "ivp_launch.h"
 #include "ivp_reduce.h"
 
 using namespace ivp;
@@ -35,6 +44,9 @@ struct Rows {
     const int* tile_len;
     int C;
     int nd;
+    const float4* pos;   // particle positions (x, y, z, m), for evaluated gradients
+    float h;
+    float ks;            // 45 / (pi h^6)
 };
 
 __device__ __forceinline__ int row_index() {
@@ -42,16 +54,49 @@ __device__ __forceinline__ int row_index() {
     return blk * BLOCK + threadIdx.x;
 }
 
-__device__ __forceinline__ const uint32_t* row_base(const Rows& R, int i) {
-    return R.nb + (size_t)(i >> 6) * (size_t)R.C * 256u + (threadIdx.x & (WAVE - 1));
-}
+// One lane's view of its neighbour row.
+template <bool RECOMP>
+struct RowIter {
+    static constexpr unsigned REC = RECOMP ? 64u : 256u;
+    const uint32_t* e;
+    int tl;
+    float4 xi;
+    const float4* __restrict__ pos;
+    float h, ks;
 
-__device__ __forceinline__ void load_pair(const uint32_t* e, int& j, float& gx, float& gy, float& gz) {
-    j = (int)__builtin_nontemporal_load(e);
-    gx = __uint_as_float(__builtin_nontemporal_load(e + 64));
-    gy = __uint_as_float(__builtin_nontemporal_load(e + 128));
-    gz = __uint_as_float(__builtin_nontemporal_load(e + 192));
-}
+    __device__ __forceinline__ RowIter(const Rows& R, int i, float4 xi_)
+        : e(R.nb + (size_t)(i >> 6) * (size_t)R.C * REC + (threadIdx.x & (WAVE - 1))),
+          tl(R.tile_len[i >> 6]), xi(xi_), pos(R.pos), h(R.h), ks(R.ks) {}
+
+    // gradW_ij = s * r_ij for the evaluated form; returns s
+    __device__ __forceinline__ float grad_scale(float rx, float ry, float rz) const {
+        float d2 = rx * rx + ry * ry + rz * rz;
+        float inv = __builtin_amdgcn_rsqf(fmaxf(d2, 1e-12f));   // 1 / max(|r|, 1e-6)
+        float rn = d2 * inv;
+        float t = h - rn;
+        return -ks * (t * t) * inv;
+    }
+
+    // Pair k: neighbour id, its (x, y, z, m) record (loaded when NEEDPJ or when gradW is
+    // evaluated) and gradW_ij.
+    template <bool NEEDPJ>
+    __device__ __forceinline__ int get(int k, float4& pj, float& gx, float& gy, float& gz) const {
+        const uint32_t* r = e + (size_t)k * REC;
+        int j = (int)__builtin_nontemporal_load(r);
+        if (RECOMP) {
+            pj = pos[j];
+            float rx = xi.x - pj.x, ry = xi.y - pj.y, rz = xi.z - pj.z;
+            float s = grad_scale(rx, ry, rz);
+            gx = s * rx; gy = s * ry; gz = s * rz;
+        } else {
+            gx = __uint_as_float(__builtin_nontemporal_load(r + 64));
+            gy = __uint_as_float(__builtin_nontemporal_load(r + 128));
+            gz = __uint_as_float(__builtin_nontemporal_load(r + 192));
+            if (NEEDPJ) pj = pos[j];
+        }
+        return j;
+    }
+};
 
 // ---- IP ------------------------------------------------------------------------------------
 
@@ -68,15 +113,15 @@ __global__ __launch_bounds__(BLOCK) void k_ip_init(int nd, float dt, const float
 
 // J pass, form of ip_pack_fused / compute_J_x_kernel (ip.py:150-163, pressure_base.py:111-127):
 // sum_j m_j dot(u_i - [dyn_j] u_j, gradW_ij), static part as dot(u_i, S_i).
+template <bool RECOMP>
 __device__ __forceinline__ float J_row_mdot(const Rows& R, int i, float4 ui, const float4* __restrict__ u4,
                                             const float* __restrict__ Ss) {
-    const uint32_t* e = row_base(R, i);
-    int tl = R.tile_len[i >> 6];
+    RowIter<RECOMP> it(R, i, RECOMP ? R.pos[i] : make_float4(0, 0, 0, 0));
     float acc = 0.0f;
 #pragma unroll 4
-    for (int k = 0; k < tl; ++k) {
-        int j; float gx, gy, gz;
-        load_pair(e + (size_t)k * 256u, j, gx, gy, gz);
+    for (int k = 0; k < it.tl; ++k) {
+        float4 pj; float gx, gy, gz;
+        int j = it.template get<false>(k, pj, gx, gy, gz);
         float4 uj = u4[j];
         float dx = ui.x - uj.x, dy = ui.y - uj.y, dz = ui.z - uj.z;
         acc += uj.w * (dx * gx + dy * gy + dz * gz);
@@ -85,15 +130,15 @@ __device__ __forceinline__ float J_row_mdot(const Rows& R, int i, float4 ui, con
 }
 
 // J pass, form of compute_Jx_pcg (ip.py:51-58): sum_j dot(p_i - [dyn_j] p_j, m_j gradW_ij)
+template <bool RECOMP>
 __device__ __forceinline__ float J_row_dotm(const Rows& R, int i, float4 pi, const float4* __restrict__ p4,
                                             const float* __restrict__ Ss) {
-    const uint32_t* e = row_base(R, i);
-    int tl = R.tile_len[i >> 6];
+    RowIter<RECOMP> it(R, i, RECOMP ? R.pos[i] : make_float4(0, 0, 0, 0));
     float acc = 0.0f;
 #pragma unroll 4
-    for (int k = 0; k < tl; ++k) {
-        int j; float gx, gy, gz;
-        load_pair(e + (size_t)k * 256u, j, gx, gy, gz);
+    for (int k = 0; k < it.tl; ++k) {
+        float4 pj_pos; float gx, gy, gz;
+        int j = it.template get<false>(k, pj_pos, gx, gy, gz);
         float4 pj = p4[j];
         float wx = pj.w * gx, wy = pj.w * gy, wz = pj.w * gz;
         float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -112,19 +157,22 @@ struct PackArgs {
     const int* rowcnt;
     float4* q4;      // (f k, m, dfdt k, 0) gathered by the rhs pass
     float* sfac;     // dfdt k
+    float* jd;       // J d, kept for the divergence of the final d
     double* partials;
     int stride;
-    SolverState* st;
 };
 
 // ip_pack_fused (ip.py:124-171) + coef_wise_div + compute_avg_density_error
 // (math_utils.py:76-91, pressure_base.py:187-203, 413-423).
+template <bool RECOMP>
 __global__ __launch_bounds__(BLOCK) void k_ip_pack(PackArgs a) {
     int i = row_index();
     double err = 0.0, nact = 0.0, npairs = 0.0;
     if (i < a.R.nd) {
         float4 di = a.d4[i];
-        float t = J_row_mdot(a.R, i, di, a.d4, a.Ss) + a.cc[i];
+        float jd = J_row_mdot<RECOMP>(a.R, i, di, a.d4, a.Ss);
+        a.jd[i] = jd;
+        float t = jd + a.cc[i];
         float f = fmaxf(t, 0.0f);
         float dfdt = t >= 1e-6f ? 1.0f : 0.0f;
         float k = a.kk[i];
@@ -153,11 +201,11 @@ struct RhsArgs {
     const int* rowcnt;
     double* partials;
     int stride;
-    SolverState* st;
 };
 
 // ip_rhs_fused (ip.py:174-215): b = omega J^T (f k); also marks rows whose CG J^T loop can be
 // non-zero (own dfdt > 0 or a dynamic neighbour with dfdt > 0).
+template <bool RECOMP>
 __global__ __launch_bounds__(BLOCK) void k_ip_rhs(RhsArgs a) {
     int i = row_index();
     double touched = 0.0, tpairs = 0.0;
@@ -166,12 +214,11 @@ __global__ __launch_bounds__(BLOCK) void k_ip_rhs(RhsArgs a) {
         float fki = qi.x, mi = qi.y;
         bool act = qi.z != 0.0f;
         float rx = 0.0f, ry = 0.0f, rz = 0.0f;
-        const uint32_t* e = row_base(a.R, i);
-        int tl = a.R.tile_len[i >> 6];
+        RowIter<RECOMP> it(a.R, i, RECOMP ? a.R.pos[i] : make_float4(0, 0, 0, 0));
 #pragma unroll 4
-        for (int k = 0; k < tl; ++k) {
-            int j; float gx, gy, gz;
-            load_pair(e + (size_t)k * 256u, j, gx, gy, gz);
+        for (int k = 0; k < it.tl; ++k) {
+            float4 pj; float gx, gy, gz;
+            int j = it.template get<false>(k, pj, gx, gy, gz);
             float4 qj = a.q4[j];
             float s = qj.y * fki + mi * qj.x;
             rx += s * gx; ry += s * gy; rz += s * gz;
@@ -201,11 +248,8 @@ struct CGInitArgs {
     float* r;
     float4* p4;
     float2* y2;
-    double tol;
-    int max_iter;
     double* partials;
     int stride;
-    SolverState* st;
 };
 
 // IP.PCG prologue (ip.py:292-304): x = 0, r = b, rr = r.r, p = r
@@ -235,6 +279,7 @@ struct CGJArgs {
 };
 
 // compute_Jx_pcg (ip.py:19-60): y_i = dfdt_i k_i (J p)_i on rows with dfdt > 0
+template <bool RECOMP>
 __global__ __launch_bounds__(BLOCK) void k_cg_J(CGJArgs a) {
     if (a.st->active == 0) return;
     int i = row_index();
@@ -242,7 +287,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_J(CGJArgs a) {
     float s = a.sfac[i];
     if (s == 0.0f) return;
     float4 pi = a.p4[i];
-    float acc = J_row_dotm(a.R, i, pi, a.p4, a.Ss);
+    float acc = J_row_dotm<RECOMP>(a.R, i, pi, a.p4, a.Ss);
     a.y2[i] = make_float2(acc * s, pi.w);
 }
 
@@ -256,14 +301,14 @@ struct CGJTArgs {
     float* Ap;
     double* partials;
     int stride;
-    SolverState* st;
+    const SolverState* st;
 };
 
 // compute_Ax_part1 + compute_Ax_part2 (ip.py:63-121) in gather form, fused with p.Ap
-// (ip.py:307) and alpha (ip.py:308-311)
+// (ip.py:307)
+template <bool RECOMP>
 __global__ __launch_bounds__(BLOCK) void k_cg_JT(CGJTArgs a) {
-    SolverState* st = a.st;
-    if (st->active == 0) return;
+    if (a.st->active == 0) return;
     int i = row_index();
     double pap = 0.0;
     if (i < a.R.nd) {
@@ -272,12 +317,11 @@ __global__ __launch_bounds__(BLOCK) void k_cg_JT(CGJTArgs a) {
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         if (a.nbact[i] != 0) {
             float yi = a.y2[i].x;
-            const uint32_t* e = row_base(a.R, i);
-            int tl = a.R.tile_len[i >> 6];
+            RowIter<RECOMP> it(a.R, i, RECOMP ? a.R.pos[i] : make_float4(0, 0, 0, 0));
 #pragma unroll 4
-            for (int k = 0; k < tl; ++k) {
-                int j; float gx, gy, gz;
-                load_pair(e + (size_t)k * 256u, j, gx, gy, gz);
+            for (int k = 0; k < it.tl; ++k) {
+                float4 pj; float gx, gy, gz;
+                int j = it.template get<false>(k, pj, gx, gy, gz);
                 float2 yj = a.y2[j];
                 float s = yj.y * yi + mi * yj.x;
                 ax += s * gx; ay += s * gy; az += s * gz;
@@ -304,14 +348,13 @@ struct CGUpdArgs {
     const float* Ap;
     double* partials;
     int stride;
-    SolverState* st;
+    const SolverState* st;
 };
 
-// pcg_update_x_r_scalar (math_utils.py:180-195) + r.r + stop test and beta (ip.py:314-327)
+// pcg_update_x_r_scalar (math_utils.py:180-195) + r.r (ip.py:314-317)
 __global__ __launch_bounds__(BLOCK) void k_cg_update(CGUpdArgs a) {
-    SolverState* st = a.st;
-    if (st->active == 0) return;
-    float alpha = (float)st->alpha;
+    if (a.st->active == 0) return;
+    float alpha = (float)a.st->alpha;
     int i = blockIdx.x * BLOCK + threadIdx.x;
     double rr = 0.0;
     if (i < a.nd) {
@@ -355,23 +398,17 @@ __global__ __launch_bounds__(BLOCK) void k_ip_dupdate(int nd, float4* __restrict
     d4[i] = d;
 }
 
-// v = (1/dt) d (ip.py:398), staged with m for the divergence pass
+// v = (1/dt) d (ip.py:398) and compute_divergence (pressure_base.py:449-457, 240-254):
+// max(J v, 0).  The outer loop only ends right after a J pass over the final d, which left
+// J d in jd; J is linear, so J v = (1/dt) J d (equal up to float rounding of the terms).
 __global__ __launch_bounds__(BLOCK) void k_ip_finish(int nd, float inv_dt, const float4* __restrict__ d4,
-                                                     float* __restrict__ vel, float4* __restrict__ v4) {
+                                                     const float* __restrict__ jd, float* __restrict__ vel,
+                                                     float* __restrict__ divg) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= nd) return;
     float4 d = d4[i];
-    float vx = inv_dt * d.x, vy = inv_dt * d.y, vz = inv_dt * d.z;
-    vel[3 * i + 0] = vx; vel[3 * i + 1] = vy; vel[3 * i + 2] = vz;
-    v4[i] = make_float4(vx, vy, vz, d.w);
-}
-
-// compute_divergence (pressure_base.py:449-457, 240-254): max(J v, 0)
-__global__ __launch_bounds__(BLOCK) void k_ip_div(Rows R, const float4* __restrict__ v4, const float* __restrict__ Ss,
-                                                  float* __restrict__ divg) {
-    int i = row_index();
-    if (i >= R.nd) return;
-    divg[i] = fmaxf(J_row_mdot(R, i, v4[i], v4, Ss), 0.0f);
+    vel[3 * i + 0] = inv_dt * d.x; vel[3 * i + 1] = inv_dt * d.y; vel[3 * i + 2] = inv_dt * d.z;
+    divg[i] = fmaxf(inv_dt * jd[i], 0.0f);
 }
 
 // ---- explicit viscosity (viscosity.py:9-108), Jacobi reads ---------------------------------
@@ -383,28 +420,25 @@ __global__ __launch_bounds__(BLOCK) void k_v4(int nd, const float* __restrict__ 
     v4[i] = make_float4(vel[3 * i + 0], vel[3 * i + 1], vel[3 * i + 2], density[i]);
 }
 
-template <bool ALLFLUID>
-__global__ __launch_bounds__(BLOCK) void k_visc(Rows R, float h, float coeff, float dt, int fluid,
-                                                const float4* __restrict__ pos, const float4* __restrict__ v4,
-                                                const float* __restrict__ rho0, const int* __restrict__ mat,
-                                                float* __restrict__ vel) {
+template <bool RECOMP, bool ALLFLUID>
+__global__ __launch_bounds__(BLOCK) void k_visc(Rows R, float coeff, float dt, int fluid,
+                                                const float4* __restrict__ v4, const float* __restrict__ rho0,
+                                                const int* __restrict__ mat, float* __restrict__ vel) {
     int i = row_index();
     if (i >= R.nd) return;
     if (!ALLFLUID && mat[i] != fluid) return;
-    float4 xi = pos[i];
+    float4 xi = R.pos[i];
     float4 vi = v4[i];
     float rho_i = vi.w, rho0_i = rho0[i];
-    float eps2 = (0.01f * h) * (0.01f * h);
+    float eps2 = (0.01f * R.h) * (0.01f * R.h);
     float k_visc = 2.0f * (3.0f + 2.0f);
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    const uint32_t* e = row_base(R, i);
-    int tl = R.tile_len[i >> 6];
+    RowIter<RECOMP> it(R, i, xi);
 #pragma unroll 2
-    for (int k = 0; k < tl; ++k) {
-        int j; float gx, gy, gz;
-        load_pair(e + (size_t)k * 256u, j, gx, gy, gz);
+    for (int k = 0; k < it.tl; ++k) {
+        float4 xj; float gx, gy, gz;
+        int j = it.template get<true>(k, xj, gx, gy, gz);
         if (!ALLFLUID && mat[j] != fluid) continue;
-        float4 xj = pos[j];
         float4 vj = v4[j];
         float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         float rn = len3(rx, ry, rz);
@@ -459,10 +493,10 @@ __device__ __forceinline__ void inverse33(const float a[9], float out[9]) {
 }
 
 // compute_precond_weiler18 (weiler18.py:21-85): inverse of I - factor_i L_ii
-template <bool ALLFLUID>
-__global__ __launch_bounds__(BLOCK) void k_w18_pre(Rows R, float h, int fluid, const float4* __restrict__ pos,
-                                                   const float4* __restrict__ v4, const float* __restrict__ fac,
-                                                   const int* __restrict__ mat, float* __restrict__ pinv) {
+template <bool RECOMP, bool ALLFLUID>
+__global__ __launch_bounds__(BLOCK) void k_w18_pre(Rows R, int fluid, const float4* __restrict__ v4,
+                                                   const float* __restrict__ fac, const int* __restrict__ mat,
+                                                   float* __restrict__ pinv) {
     int i = row_index();
     if (i >= R.nd) return;
     float* P = pinv + 9 * (size_t)i;
@@ -470,21 +504,19 @@ __global__ __launch_bounds__(BLOCK) void k_w18_pre(Rows R, float h, int fluid, c
         for (int q = 0; q < 9; ++q) P[q] = (q % 4) == 0 ? 1.0f : 0.0f;
         return;
     }
-    float4 xi = pos[i];
-    float eps2 = 0.01f * h * h;
+    float4 xi = R.pos[i];
+    float eps2 = 0.01f * R.h * R.h;
     float dd = 10.0f;
     float L[9];
     for (int q = 0; q < 9; ++q) L[q] = 0.0f;
-    const uint32_t* e = row_base(R, i);
-    int tl = R.tile_len[i >> 6];
-    for (int k = 0; k < tl; ++k) {
-        int j; float g[3];
-        load_pair(e + (size_t)k * 256u, j, g[0], g[1], g[2]);
-        if (!ALLFLUID && mat[j] != fluid) continue;
-        float4 xj = pos[j];
+    RowIter<RECOMP> it(R, i, xi);
+    for (int k = 0; k < it.tl; ++k) {
+        float4 xj; float g[3];
+        int j = it.template get<true>(k, xj, g[0], g[1], g[2]);
         float x[3] = {xi.x - xj.x, xi.y - xj.y, xi.z - xj.z};
         float r_sq = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
         float c = dd * v4[j].w / (r_sq + eps2);
+        if (!ALLFLUID && mat[j] != fluid) continue;
 #pragma unroll
         for (int p = 0; p < 3; ++p)
 #pragma unroll
@@ -502,23 +534,21 @@ __global__ __launch_bounds__(BLOCK) void k_w18_pre(Rows R, float h, int fluid, c
 
 // (A u)_i = u_i - factor_i * 10 sum_fluid (m_j/rho_j) (u_ij . x_ij) / (r^2 + 0.01 h^2) gradW_ij
 // (compute_Av_weiler18, weiler18.py:103-172)
-template <bool ALLFLUID>
-__device__ __forceinline__ float3 w18_Av_row(const Rows& R, int i, float h, int fluid, const float4* __restrict__ pos,
-                                             const float4* __restrict__ u4, const float* __restrict__ fac,
-                                             const int* __restrict__ mat, float4 ui) {
+template <bool RECOMP, bool ALLFLUID>
+__device__ __forceinline__ float3 w18_Av_row(const Rows& R, int i, int fluid, const float4* __restrict__ u4,
+                                             const float* __restrict__ fac, const int* __restrict__ mat,
+                                             float4 ui) {
     if (!ALLFLUID && mat[i] != fluid) return make_float3(ui.x, ui.y, ui.z);
-    float4 xi = pos[i];
-    float eps2 = 0.01f * h * h;
-    float dd = 10.0f;
+    float4 xi = R.pos[i];
     float lx = 0.0f, ly = 0.0f, lz = 0.0f;
-    const uint32_t* e = row_base(R, i);
-    int tl = R.tile_len[i >> 6];
-#pragma unroll 2
-    for (int k = 0; k < tl; ++k) {
-        int j; float gx, gy, gz;
-        load_pair(e + (size_t)k * 256u, j, gx, gy, gz);
+    float eps2 = 0.01f * R.h * R.h;
+    float dd = 10.0f;
+    RowIter<RECOMP> it(R, i, xi);
+#pragma unroll 4
+    for (int k = 0; k < it.tl; ++k) {
+        float4 xj; float gx, gy, gz;
+        int j = it.template get<true>(k, xj, gx, gy, gz);
         if (!ALLFLUID && mat[j] != fluid) continue;
-        float4 xj = pos[j];
         float4 uj = u4[j];
         float x0 = xi.x - xj.x, x1 = xi.y - xj.y, x2 = xi.z - xj.z;
         float v0 = ui.x - uj.x, v1 = ui.y - uj.y, v2 = ui.z - uj.z;
@@ -533,9 +563,7 @@ __device__ __forceinline__ float3 w18_Av_row(const Rows& R, int i, float h, int 
 
 struct W18Args {
     Rows R;
-    float h;
     int fluid;
-    const float4* pos;
     const float* fac;
     const int* mat;
     const float* pinv;
@@ -544,23 +572,23 @@ struct W18Args {
     float* dv; float* r; float* z;
     double* partials;
     int stride;
-    SolverState* st;
+    const SolverState* st;
 };
 
-// rhs = v - A v (weiler18.py:390-393), then the PCG prologue (weiler18.py:320-335):
-// dv = 0, r = rhs, z = P r, p = z, rz = r.z
-template <bool ALLFLUID>
+// rhs = v - A v (weiler18.py:390-393)
+template <bool RECOMP, bool ALLFLUID>
 __global__ __launch_bounds__(BLOCK) void k_w18_rhs(W18Args a) {
     int i = row_index();
     if (i >= a.R.nd) return;
     float4 vi = a.v4[i];
-    float3 Av = w18_Av_row<ALLFLUID>(a.R, i, a.h, a.fluid, a.pos, a.v4, a.fac, a.mat, vi);
+    float3 Av = w18_Av_row<RECOMP, ALLFLUID>(a.R, i, a.fluid, a.v4, a.fac, a.mat, vi);
     a.out[3 * i + 0] = vi.x + -1.0f * Av.x;
     a.out[3 * i + 1] = vi.y + -1.0f * Av.y;
     a.out[3 * i + 2] = vi.z + -1.0f * Av.z;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_w18_init(W18Args a, float4* __restrict__ p4, double tol, int max_iter) {
+// PCG prologue (weiler18.py:320-335): dv = 0, r = rhs, z = P r, p = z, rz = r.z
+__global__ __launch_bounds__(BLOCK) void k_w18_init(W18Args a, float4* __restrict__ p4) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     double rz = 0.0;
     if (i < a.R.nd) {
@@ -579,16 +607,15 @@ __global__ __launch_bounds__(BLOCK) void k_w18_init(W18Args a, float4* __restric
     block_partials<1>(v, a.partials, a.stride);
 }
 
-// Ap = A p, p.Ap; pAp <= 0 stops the iteration (weiler18.py:337-342)
-template <bool ALLFLUID>
+// Ap = A p, p.Ap (weiler18.py:337-338)
+template <bool RECOMP, bool ALLFLUID>
 __global__ __launch_bounds__(BLOCK) void k_w18_Av(W18Args a) {
-    SolverState* st = a.st;
-    if (st->active == 0) return;
+    if (a.st->active == 0) return;
     int i = row_index();
     double pap = 0.0;
     if (i < a.R.nd) {
         float4 pi = a.v4[i];
-        float3 Ap = w18_Av_row<ALLFLUID>(a.R, i, a.h, a.fluid, a.pos, a.v4, a.fac, a.mat, pi);
+        float3 Ap = w18_Av_row<RECOMP, ALLFLUID>(a.R, i, a.fluid, a.v4, a.fac, a.mat, pi);
         a.out[3 * i + 0] = Ap.x; a.out[3 * i + 1] = Ap.y; a.out[3 * i + 2] = Ap.z;
         pap = (double)(pi.x * Ap.x + pi.y * Ap.y + pi.z * Ap.z);
     }
@@ -596,12 +623,11 @@ __global__ __launch_bounds__(BLOCK) void k_w18_Av(W18Args a) {
     block_partials<1>(v, a.partials, a.stride);
 }
 
-// dv += alpha p, r -= alpha Ap, r.r, z = P r, r.z, stop test and beta (weiler18.py:344-365)
+// dv += alpha p, r -= alpha Ap, r.r, z = P r, r.z (weiler18.py:344-360)
 __global__ __launch_bounds__(BLOCK) void k_w18_update(W18Args a) {
-    SolverState* st = a.st;
-    if (st->active == 0) return;
-    float alpha = (float)st->alpha;
-    float nalpha = (float)(-st->alpha);
+    if (a.st->active == 0) return;
+    float alpha = (float)a.st->alpha;
+    float nalpha = (float)(-a.st->alpha);
     int i = blockIdx.x * BLOCK + threadIdx.x;
     double rr = 0.0, rz = 0.0;
     if (i < a.R.nd) {
@@ -640,9 +666,9 @@ __global__ __launch_bounds__(BLOCK) void k_w18_p(int nd, const float* __restrict
 }
 
 // v += dv (weiler18.py:399-401)
-__global__ __launch_bounds__(BLOCK) void k_w18_apply(int nd, const float* __restrict__ dv, float* __restrict__ vel) {
+__global__ __launch_bounds__(BLOCK) void k_w18_apply(int n3, const float* __restrict__ dv, float* __restrict__ vel) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= 3 * nd) return;
+    if (i >= n3) return;
     vel[i] = vel[i] + 1.0f * dv[i];
 }
 
@@ -683,7 +709,7 @@ __global__ __launch_bounds__(BLOCK) void k_advect_boundary(int nd, float dt, flo
     pos[i] = make_float4(x[0], x[1], x[2], p.w);
 }
 
-Rows rows_of(ivp_ctx* c) { return Rows{c->nb, c->tile_len, c->C, c->nd}; }
+Rows rows_of(ivp_ctx* c) { return Rows{c->nb, c->tile_len, c->C, c->nd, c->pos, c->h, c->kspiky}; }
 
 int read_state(ivp_ctx* c) {
     IVP_HIP(hipMemcpyAsync(c->state_host, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
@@ -691,23 +717,19 @@ int read_state(ivp_ctx* c) {
     return IVP_OK;
 }
 
-// Enqueue CG iterations in chunks until the device-side state says stop.  `launch(tag)`
+// Enqueue CG iterations in chunks until the device-side state says stop.  `launch_it(tag)`
 // enqueues one iteration.  Returns the iteration count in *iters.
 template <class L>
-int run_chunks(ivp_ctx* c, int max_iter, int guess, L&& launch, int* iters, int* chunks) {
+int run_chunks(ivp_ctx* c, int max_iter, int guess, L&& launch_it, int* iters, int* chunks) {
     int issued = 0;
     int chunk = std::max(2, std::min(guess, 64));
     double rr0 = c->state_host->rr0;
     for (;;) {
         int todo = std::min(chunk, max_iter - issued);
         if (todo <= 0) break;
-        for (int q = 0; q < todo; ++q) {
-            int rc = launch(issued + q);
-            if (rc != IVP_OK) return rc;
-        }
+        for (int q = 0; q < todo; ++q) IVP_TRY(launch_it(issued + q));
         issued += todo;
-        int rc = read_state(c);
-        if (rc != IVP_OK) return rc;
+        IVP_TRY(read_state(c));
         (*chunks)++;
         const SolverState& s = *c->state_host;
         if (s.active == 0) break;
@@ -722,59 +744,32 @@ int run_chunks(ivp_ctx* c, int max_iter, int guess, L&& launch, int* iters, int*
     return IVP_OK;
 }
 
-}  // namespace
+// Bytes one pass over the rows reads from the pair stream: id (4 B), plus 12 B of gradW when
+// gradients are cached.
+double pair_bytes(const ivp_ctx* c) { return c->recomp ? 4.0 : 16.0; }
 
-namespace ivp {
-
-static int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats);
-
-// IP.solve with its wall-clock cost (stats_total_cost_ms, ip.py:335, 403-404: the reference
-// synchronises before reading the clock, and so does this).
-int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats) {
-    if (!c->built) return IVP_ERR_STATE;
-    if (!prm || !stats) return IVP_ERR_ARG;
-    auto t0 = std::chrono::steady_clock::now();
-    int rc = ip_solve_body(c, dt, prm, stats);
-    if (rc >= 0) IVP_HIP(hipStreamSynchronize(c->stream));
-    stats->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    return rc;
-}
-
-static int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats) {
+template <bool RECOMP>
+int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats) {
     ivp_ip_stats& st = *stats;
-    st.n_density_errors = 0;
-    st.n_pcg = 0;
-    st.outer_iters = 0;
-    st.final_density_error = 0.0;
-    st.pAp_bad = 0.0;
-    st.pcg_iter_bad = 0;
-    st.pairs_traversed = 0;
-    st.rows_traversed = 0;
-    st.cg_launch_chunks = 0;
     const int nd = c->nd;
     if (nd == 0) return IVP_OK;
-    const int nb = blocks_for(nd);
+    const dim3 nb(blocks_for(nd));
     const int stride = c->max_blocks;
+    const double pb = pair_bytes(c);
     Rows R = rows_of(c);
     float4* d4 = c->d4;
-    hipLaunchKernelGGL(k_ip_init, dim3(nb), dim3(BLOCK), 0, c->stream, nd, dt, c->density, c->rho0, c->vel, c->pos,
-                       c->cc, d4);
-    IVP_HIP(hipGetLastError());
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_init, nb, nd, dt, (const float*)c->density, (const float*)c->rho0,
+                   (const float*)c->vel, (const float4*)c->pos, c->cc, d4));
     int opt_iter = 0;
     double err = 0.0;
     static int cg_guess = 4;
     for (;;) {
-        PackArgs pa{R, d4, c->Ss, c->cc, c->kk, c->rho0, c->rowcnt, c->q4, c->sfac, c->partials, stride, c->state};
-        ProfEvent ev;
-        prof_begin(c, KC_IP_PACK, -1, (double)nd * (16 + 4 + 4 + 4 + 12 + 4 + 16 + 4), &ev);
-        hipLaunchKernelGGL(k_ip_pack, dim3(nb), dim3(BLOCK), 0, c->stream, pa);
-        IVP_HIP(hipGetLastError());
-        prof_end(c, &ev);
-        int rc = finalize<FIN_PACK>(c, nb);
-        if (rc != IVP_OK) return rc;
+        PackArgs pa{R, d4, c->Ss, c->cc, c->kk, c->rho0, c->rowcnt, c->q4, c->sfac, c->jd, c->partials, stride};
+        double total_pairs = (double)c->state_host->total_pairs;
+        IVP_TRY(launch(c, KC_IP_PACK, -1, pb * total_pairs + 68.0 * nd, k_ip_pack<RECOMP>, nb, pa));
+        IVP_TRY(finalize<FIN_PACK>(c, nb.x));
         IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        rc = read_state(c);
-        if (rc != IVP_OK) return rc;
+        IVP_TRY(read_state(c));
         if (*c->err_host & 4) return IVP_ERR_INTERNAL;
         if (*c->err_host & 1) return IVP_NEIGHBOR_OVERFLOW;
         err = c->state_host->err_sum / (double)nd;
@@ -785,56 +780,38 @@ static int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_
         long long act_pairs = c->state_host->active_pairs;
         int act_rows = c->state_host->n_active_rows;
 
-        RhsArgs ra{R, c->q4, c->Ss, prm->omega, c->b, c->nbact, c->rowcnt,
-                   c->partials, stride, c->state};
-        prof_begin(c, KC_IP_RHS, -1, (double)nd * (16 + 12 + 12 + 4 + 4), &ev);
-        hipLaunchKernelGGL(k_ip_rhs, dim3(nb), dim3(BLOCK), 0, c->stream, ra);
-        IVP_HIP(hipGetLastError());
-        prof_end(c, &ev);
-        if ((rc = finalize<FIN_RHS>(c, nb)) != IVP_OK) return rc;
-        CGInitArgs ia{nd, c->b, c->pos, c->sol, c->r, c->p4, c->y2, prm->tol_pcg, prm->max_iteration_pcg,
-                      c->partials, stride, c->state};
-        hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(BLOCK), 0, c->stream, ia);
-        IVP_HIP(hipGetLastError());
-        if ((rc = finalize<FIN_CG_INIT>(c, nb, prm->tol_pcg, prm->max_iteration_pcg)) != IVP_OK) return rc;
-        rc = read_state(c);
-        if (rc != IVP_OK) return rc;
+        RhsArgs ra{R, c->q4, c->Ss, prm->omega, c->b, c->nbact, c->rowcnt, c->partials, stride};
+        IVP_TRY(launch(c, KC_IP_RHS, -1, pb * total_pairs + 48.0 * nd, k_ip_rhs<RECOMP>, nb, ra));
+        IVP_TRY(finalize<FIN_RHS>(c, nb.x));
+        CGInitArgs ia{nd, c->b, c->pos, c->sol, c->r, c->p4, c->y2, c->partials, stride};
+        IVP_TRY(launch(c, -1, -1, 0.0, k_cg_init, nb, ia));
+        IVP_TRY(finalize<FIN_CG_INIT>(c, nb.x, prm->tol_pcg, prm->max_iteration_pcg));
+        IVP_TRY(read_state(c));
         long long touch_pairs = c->state_host->touched_pairs;
         int touch_rows = c->state_host->n_touched_rows;
         int iters = 0;
         if (c->state_host->active) {
             c->prof_epoch++;
             int epoch = c->prof_epoch;
-            // algorithmic bytes per launch: pair records 16 B, per-row terms as in the kernels
-            double bJ = 16.0 * (double)act_pairs + (double)act_rows * (4 + 16 + 12 + 8) + (double)nd * 4;
-            double bJT = 16.0 * (double)touch_pairs + (double)touch_rows * (8 + 12) + (double)nd * (16 + 4 + 12);
+            // algorithmic bytes per launch: the pair stream once per traversed pair, per-row
+            // arrays once (gathered neighbour values counted once per particle)
+            double bJ = pb * (double)act_pairs + (double)act_rows * (16 + 12 + 8) + (double)nd * 4;
+            double bJT = pb * (double)touch_pairs + (double)touch_rows * (8 + 12) + (double)nd * (16 + 4 + 12);
             double bU = (double)nd * (12 + 12 + 16 + 12 + 12 + 12);
             double bP = (double)nd * (12 + 16 + 16);
             CGJArgs ja{R, c->p4, c->Ss, c->sfac, c->y2, c->state};
             CGJTArgs ta{R, c->p4, c->y2, c->Ss, c->nbact, prm->omega, c->Ap, c->partials, stride, c->state};
             CGUpdArgs ua{nd, c->sol, c->r, c->p4, c->Ap, c->partials, stride, c->state};
-            auto launch = [&](int tag) -> int {
-                ProfEvent e1;
-                prof_begin(c, KC_CG_J, tag, bJ, &e1);
-                hipLaunchKernelGGL(k_cg_J, dim3(nb), dim3(BLOCK), 0, c->stream, ja);
-                prof_end(c, &e1);
-                prof_begin(c, KC_CG_JT, tag, bJT, &e1);
-                hipLaunchKernelGGL(k_cg_JT, dim3(nb), dim3(BLOCK), 0, c->stream, ta);
-                prof_end(c, &e1);
-                int frc = finalize<FIN_CG_PAP>(c, nb);
-                if (frc != IVP_OK) return frc;
-                prof_begin(c, KC_CG_UPDATE, tag, bU, &e1);
-                hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(BLOCK), 0, c->stream, ua);
-                prof_end(c, &e1);
-                if ((frc = finalize<FIN_CG_UPDATE>(c, nb)) != IVP_OK) return frc;
-                prof_begin(c, KC_CG_P, tag, bP, &e1);
-                hipLaunchKernelGGL(k_cg_p, dim3(nb), dim3(BLOCK), 0, c->stream, nd, c->r, c->p4, c->state);
-                prof_end(c, &e1);
-                hipError_t e = hipGetLastError();
-                return e == hipSuccess ? IVP_OK : ivp_hip_fail(e, "cg launch", __FILE__, __LINE__);
+            auto one_iter = [&](int tag) -> int {
+                IVP_TRY(launch(c, KC_CG_J, tag, bJ, k_cg_J<RECOMP>, nb, ja));
+                IVP_TRY(launch(c, KC_CG_JT, tag, bJT, k_cg_JT<RECOMP>, nb, ta));
+                IVP_TRY(finalize<FIN_CG_PAP>(c, nb.x));
+                IVP_TRY(launch(c, KC_CG_UPDATE, tag, bU, k_cg_update, nb, ua));
+                IVP_TRY(finalize<FIN_CG_UPDATE>(c, nb.x));
+                return launch(c, KC_CG_P, tag, bP, k_cg_p, nb, nd, (const float*)c->r, c->p4,
+                              (const SolverState*)c->state);
             };
-            rc = run_chunks(c, prm->max_iteration_pcg, cg_guess, launch, &iters, &st.cg_launch_chunks);
-            if (rc != IVP_OK) return rc;
+            IVP_TRY(run_chunks(c, prm->max_iteration_pcg, cg_guess, one_iter, &iters, &st.cg_launch_chunks));
             prof_collect(c, iters, epoch);
             cg_guess = std::max(2, iters);
             st.pairs_traversed += (long long)iters * (act_pairs + touch_pairs);
@@ -847,107 +824,121 @@ static int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_
         }
         if (st.n_pcg < st.capacity && st.pcg_iters) st.pcg_iters[st.n_pcg] = iters;
         st.n_pcg++;
-        hipLaunchKernelGGL(k_ip_dupdate, dim3(nb), dim3(BLOCK), 0, c->stream, nd, d4, c->sol);
-        IVP_HIP(hipGetLastError());
+        IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_dupdate, nb, nd, d4, (const float*)c->sol));
         opt_iter++;
     }
     st.outer_iters = opt_iter;
     if (!prm->use_fixed_iters && err >= prm->tol_opt && opt_iter >= prm->max_iteration_opt) return IVP_NOT_CONVERGED;
     float inv_dt = (float)(1.0 / (double)dt);
-    hipLaunchKernelGGL(k_ip_finish, dim3(nb), dim3(BLOCK), 0, c->stream, nd, inv_dt, d4, c->vel, c->p4);
     // final_density_error (pressure_base.py:438-447) evaluates avg(max((J d + c)/rho0, 0)) at the
     // d of the last loop evaluation, with the same per-row arithmetic: it equals that err.
     st.final_density_error = err;
-    hipLaunchKernelGGL(k_ip_div, dim3(nb), dim3(BLOCK), 0, c->stream, R, c->p4, c->Ss, c->divg);
-    IVP_HIP(hipGetLastError());
-    return IVP_OK;
+    return launch(c, KC_GLUE, -1, 0.0, k_ip_finish, nb, nd, inv_dt, (const float4*)d4, (const float*)c->jd, c->vel,
+                  c->divg);
 }
 
-int visc_explicit(ivp_ctx* c, float coeff, float dt) {
-    if (!c->built) return IVP_ERR_STATE;
+template <bool RECOMP>
+int visc_explicit_body(ivp_ctx* c, float coeff, float dt) {
     const int nd = c->nd;
-    if (nd == 0) return IVP_OK;
-    const int nb = blocks_for(nd);
-    hipLaunchKernelGGL(k_v4, dim3(nb), dim3(BLOCK), 0, c->stream, nd, c->vel, c->density, c->v4);
-    ProfEvent ev;
-    prof_begin(c, KC_VISC, -1, 0.0, &ev);
+    const dim3 nb(blocks_for(nd));
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_v4, nb, nd, (const float*)c->vel, (const float*)c->density, c->v4));
+    double bytes = pair_bytes(c) * (double)c->state_host->total_pairs + 48.0 * nd;
+    Rows R = rows_of(c);
     if (c->all_dyn_fluid)
-        hipLaunchKernelGGL(k_visc<true>, dim3(nb), dim3(BLOCK), 0, c->stream, rows_of(c), c->h, coeff, dt,
-                           c->cfg.material_fluid, c->pos, c->v4, c->rho0, c->mat, c->vel);
-    else
-        hipLaunchKernelGGL(k_visc<false>, dim3(nb), dim3(BLOCK), 0, c->stream, rows_of(c), c->h, coeff, dt,
-                           c->cfg.material_fluid, c->pos, c->v4, c->rho0, c->mat, c->vel);
-    IVP_HIP(hipGetLastError());
-    prof_end(c, &ev);
-    return IVP_OK;
+        return launch(c, KC_VISC, -1, bytes, k_visc<RECOMP, true>, nb, R, coeff, dt, c->cfg.material_fluid,
+                      (const float4*)c->v4, (const float*)c->rho0, (const int*)c->mat, c->vel);
+    return launch(c, KC_VISC, -1, bytes, k_visc<RECOMP, false>, nb, R, coeff, dt, c->cfg.material_fluid,
+                  (const float4*)c->v4, (const float*)c->rho0, (const int*)c->mat, c->vel);
 }
 
-int visc_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* out) {
-    if (!c->built) return IVP_ERR_STATE;
-    if (out) { out->cg_iters = 0; out->residual_norm2 = 0.0; }
+template <bool RECOMP, bool ALLFLUID>
+int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
     const int nd = c->nd;
-    if (nd == 0) return IVP_OK;
-    const int nb = blocks_for(nd);
+    const dim3 nb(blocks_for(nd));
     const int stride = c->max_blocks;
     Rows R = rows_of(c);
-    hipLaunchKernelGGL(k_w18_prep, dim3(nb), dim3(BLOCK), 0, c->stream, nd, coeff, dt, c->vel, c->pos, c->density,
-                       c->rho0, c->w_v4, c->w_fac);
-    ProfEvent ev;
-    prof_begin(c, KC_W18_PRE, -1, 0.0, &ev);
-    if (c->all_dyn_fluid)
-        hipLaunchKernelGGL(k_w18_pre<true>, dim3(nb), dim3(BLOCK), 0, c->stream, R, c->h, c->cfg.material_fluid,
-                           c->pos, c->w_v4, c->w_fac, c->mat, c->w_pinv);
-    else
-        hipLaunchKernelGGL(k_w18_pre<false>, dim3(nb), dim3(BLOCK), 0, c->stream, R, c->h, c->cfg.material_fluid,
-                           c->pos, c->w_v4, c->w_fac, c->mat, c->w_pinv);
-    prof_end(c, &ev);
-    W18Args wa{R, c->h, c->cfg.material_fluid, c->pos, c->w_fac, c->mat, c->w_pinv, c->w_v4, c->w_rhs,
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_w18_prep, nb, nd, coeff, dt, (const float*)c->vel, (const float4*)c->pos,
+                   (const float*)c->density, (const float*)c->rho0, c->w_v4, c->w_fac));
+    const double pairs = (double)c->state_host->total_pairs;
+    const double pb = pair_bytes(c);
+    IVP_TRY(launch(c, KC_W18_PRE, -1, pb * pairs + 56.0 * nd, k_w18_pre<RECOMP, ALLFLUID>, nb, R,
+                   c->cfg.material_fluid, (const float4*)c->w_v4, (const float*)c->w_fac, (const int*)c->mat,
+                   c->w_pinv));
+    W18Args wa{R, c->cfg.material_fluid, c->w_fac, c->mat, c->w_pinv, c->w_v4, c->w_rhs,
                c->w_dv, c->w_r, c->w_z, c->partials, stride, c->state};
-    if (c->all_dyn_fluid)
-        hipLaunchKernelGGL(k_w18_rhs<true>, dim3(nb), dim3(BLOCK), 0, c->stream, wa);
-    else
-        hipLaunchKernelGGL(k_w18_rhs<false>, dim3(nb), dim3(BLOCK), 0, c->stream, wa);
-    hipLaunchKernelGGL(k_w18_init, dim3(nb), dim3(BLOCK), 0, c->stream, wa, c->w_p4, 1e-5, 1000);
-    IVP_HIP(hipGetLastError());
-    int rc = finalize<FIN_W18_INIT>(c, nb, 1e-5, 1000);
-    if (rc != IVP_OK) return rc;
-    rc = read_state(c);
-    if (rc != IVP_OK) return rc;
+    IVP_TRY(launch(c, KC_W18_AV, -1, pb * pairs + 48.0 * nd, k_w18_rhs<RECOMP, ALLFLUID>, nb, wa));
+    IVP_TRY(launch(c, -1, -1, 0.0, k_w18_init, nb, wa, c->w_p4));
+    IVP_TRY(finalize<FIN_W18_INIT>(c, nb.x, 1e-5, 1000));
+    IVP_TRY(read_state(c));
     int iters = 0, chunks = 0;
     if (c->state_host->active) {
         W18Args cg = wa;
         cg.v4 = c->w_p4;
         cg.out = c->w_Ap;
-        auto launch = [&](int tag) -> int {
-            ProfEvent e1;
-            prof_begin(c, KC_W18_AV, tag, 0.0, &e1);
-            if (c->all_dyn_fluid)
-                hipLaunchKernelGGL(k_w18_Av<true>, dim3(nb), dim3(BLOCK), 0, c->stream, cg);
-            else
-                hipLaunchKernelGGL(k_w18_Av<false>, dim3(nb), dim3(BLOCK), 0, c->stream, cg);
-            prof_end(c, &e1);
-            int frc = finalize<FIN_W18_PAP>(c, nb);
-            if (frc != IVP_OK) return frc;
-            prof_begin(c, KC_W18_UPDATE, tag, 0.0, &e1);
-            hipLaunchKernelGGL(k_w18_update, dim3(nb), dim3(BLOCK), 0, c->stream, cg);
-            prof_end(c, &e1);
-            if ((frc = finalize<FIN_W18_UPDATE>(c, nb)) != IVP_OK) return frc;
-            prof_begin(c, KC_W18_P, tag, 0.0, &e1);
-            hipLaunchKernelGGL(k_w18_p, dim3(nb), dim3(BLOCK), 0, c->stream, nd, c->w_z, c->w_p4, c->state);
-            prof_end(c, &e1);
-            hipError_t e = hipGetLastError();
-            return e == hipSuccess ? IVP_OK : ivp_hip_fail(e, "w18 launch", __FILE__, __LINE__);
+        const double bAv = pb * (double)c->state_host->total_pairs + 48.0 * nd;
+        auto one_iter = [&](int tag) -> int {
+            IVP_TRY(launch(c, KC_W18_AV, tag, bAv, k_w18_Av<RECOMP, ALLFLUID>, nb, cg));
+            IVP_TRY(finalize<FIN_W18_PAP>(c, nb.x));
+            IVP_TRY(launch(c, KC_W18_UPDATE, tag, 124.0 * nd, k_w18_update, nb, cg));
+            IVP_TRY(finalize<FIN_W18_UPDATE>(c, nb.x));
+            return launch(c, KC_W18_P, tag, 44.0 * nd, k_w18_p, nb, nd, (const float*)c->w_z, c->w_p4,
+                          (const SolverState*)c->state);
         };
         c->prof_epoch++;
         int epoch = c->prof_epoch;
         static int w_guess = 4;
-        rc = run_chunks(c, 1000, w_guess, launch, &iters, &chunks);
-        if (rc != IVP_OK) return rc;
+        IVP_TRY(run_chunks(c, 1000, w_guess, one_iter, &iters, &chunks));
         prof_collect(c, iters, epoch);
         w_guess = std::max(2, iters);
     }
-    hipLaunchKernelGGL(k_w18_apply, dim3(blocks_for(3 * nd)), dim3(BLOCK), 0, c->stream, nd, c->w_dv, c->vel);
-    IVP_HIP(hipGetLastError());
+    *iters_out = iters;
+    return launch(c, KC_GLUE, -1, 0.0, k_w18_apply, dim3(blocks_for(3 * nd)), 3 * nd, (const float*)c->w_dv, c->vel);
+}
+
+}  // namespace
+
+namespace ivp {
+
+// IP.solve with its wall-clock cost (stats_total_cost_ms, ip.py:335, 403-404: the reference
+// synchronises before reading the clock, and so does this).
+int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats) {
+    if (!c->built) return IVP_ERR_STATE;
+    if (!prm || !stats) return IVP_ERR_ARG;
+    ivp_ip_stats& st = *stats;
+    st.n_density_errors = 0;
+    st.n_pcg = 0;
+    st.outer_iters = 0;
+    st.final_density_error = 0.0;
+    st.pAp_bad = 0.0;
+    st.pcg_iter_bad = 0;
+    st.pairs_traversed = 0;
+    st.rows_traversed = 0;
+    st.cg_launch_chunks = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = c->recomp ? ip_solve_body<true>(c, dt, prm, stats) : ip_solve_body<false>(c, dt, prm, stats);
+    if (rc >= 0) IVP_HIP(hipStreamSynchronize(c->stream));
+    stats->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+int visc_explicit(ivp_ctx* c, float coeff, float dt) {
+    if (!c->built) return IVP_ERR_STATE;
+    if (c->nd == 0) return IVP_OK;
+    return c->recomp ? visc_explicit_body<true>(c, coeff, dt) : visc_explicit_body<false>(c, coeff, dt);
+}
+
+int visc_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* out) {
+    if (!c->built) return IVP_ERR_STATE;
+    if (out) { out->cg_iters = 0; out->residual_norm2 = 0.0; }
+    if (c->nd == 0) return IVP_OK;
+    int iters = 0, rc;
+    if (c->recomp)
+        rc = c->all_dyn_fluid ? weiler18_body<true, true>(c, coeff, dt, &iters)
+                              : weiler18_body<true, false>(c, coeff, dt, &iters);
+    else
+        rc = c->all_dyn_fluid ? weiler18_body<false, true>(c, coeff, dt, &iters)
+                              : weiler18_body<false, false>(c, coeff, dt, &iters);
+    if (rc != IVP_OK) return rc;
     if (out) {
         out->cg_iters = iters;
         out->residual_norm2 = c->state_host->rr_last;
@@ -958,18 +949,15 @@ int visc_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* out) {
 int glue_gravity(ivp_ctx* c, const float g[3], float dt) {
     if (!c->loaded) return IVP_ERR_STATE;
     if (c->nd == 0) return IVP_OK;
-    hipLaunchKernelGGL(k_gravity, dim3(blocks_for(c->nd)), dim3(BLOCK), 0, c->stream, c->nd, g[0] * dt, g[1] * dt,
-                       g[2] * dt, c->vel);
-    IVP_HIP(hipGetLastError());
-    return IVP_OK;
+    return launch(c, KC_GLUE, -1, 0.0, k_gravity, dim3(blocks_for(c->nd)), c->nd, g[0] * dt, g[1] * dt, g[2] * dt,
+                  c->vel);
 }
 
 int glue_advect_boundary(ivp_ctx* c, float dt) {
     if (!c->loaded) return IVP_ERR_STATE;
     if (c->nd == 0) return IVP_OK;
-    hipLaunchKernelGGL(k_advect_boundary, dim3(blocks_for(c->nd)), dim3(BLOCK), 0, c->stream, c->nd, dt,
-                       c->cfg.domain_size[0], c->cfg.domain_size[1], c->cfg.domain_size[2], c->h, c->pos, c->vel);
-    IVP_HIP(hipGetLastError());
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_advect_boundary, dim3(blocks_for(c->nd)), c->nd, dt, c->cfg.domain_size[0],
+                   c->cfg.domain_size[1], c->cfg.domain_size[2], c->h, c->pos, c->vel));
     c->built = false;
     return IVP_OK;
 }

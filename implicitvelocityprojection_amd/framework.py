@@ -1,0 +1,95 @@
+"""Step orchestrator, mirroring the reference's ``Framework`` (scripts/framework.py:143-544) for
+the velocity-based IP solver.
+
+``forward()`` runs one reference step -- gravity, viscosity, IP solve, advection, domain clamp,
+neighbour rebuild, density, Aii -- as a single library call (ivp_step) on the library's particle
+store, then copies x, v, density and divergence back into the SimulationData arrays (skip that
+with ``sync_arrays = False`` when only the final state is read).  ``ps.v`` is read at the start of
+every step, so velocity edits between steps take effect as in the reference; position edits need
+a rebuild (``initialize()``), as the reference's cached neighbour rows do.
+"""
+
+import time
+
+from . import _native as N
+from .neighbour_search import NeighbourSearch
+from .pressure import IP
+from .viscosity import Viscosity, Weiler18
+
+_UNSUPPORTED_PRESSURE = {0: "PRJ", 3: "SISPH", 5: "PBF"}
+
+
+class Framework:
+    def __init__(self, simulation_data, neighbour_search: NeighbourSearch, config):
+        self.ps = simulation_data
+        self.ns = neighbour_search
+        self.cfg = config
+        self.dt = config.dt
+        self.dt_default = config.dt
+        self.use_cfl = bool(config.get_cfg("useCFL", False))
+        if self.use_cfl:
+            raise NotImplementedError("useCFL (adaptive time step) is not part of the IP hot path")
+        self.time = 0.0
+        self.initial_particle_num = simulation_data.particle_num
+        self.stats_pressure_ms = 0.0
+        self.stats_elasticity_ms = 0.0
+        self.g = tuple(float(v) for v in config.gravity)
+        pressure_method = int(config.get_cfg("pressureMethod", 1))
+        if pressure_method in _UNSUPPORTED_PRESSURE:
+            raise NotImplementedError(f"pressureMethod {pressure_method} ({_UNSUPPORTED_PRESSURE[pressure_method]}) "
+                                      "is not part of this build; use pressureMethod 1 (IP)")
+        if pressure_method != 1:
+            raise ValueError(f"Unknown pressureMethod {pressure_method} (valid: [0, 1, 3, 5]).")
+        self.pressure = IP(simulation_data, neighbour_search, config)
+        self.viscosity = (Weiler18 if config.viscosity_method == 1 else Viscosity)(simulation_data, neighbour_search,
+                                                                                   config)
+        self.viscosity_coeff = config.viscosity_coeff
+        self.surface_tension_coeff = config.surface_tension_coeff
+        if self.surface_tension_coeff != 0.0:
+            raise NotImplementedError("surface tension is not part of the IP hot path (set surfaceTensionCoeff 0)")
+        self.pressure_enabled = bool(config.get_cfg("pressureEnabled", True))
+        if not self.pressure_enabled:
+            raise NotImplementedError("pressureEnabled false is not supported")
+        self.sync_arrays = True
+        self.last_visc_iters = 0
+
+    def initialize(self):
+        """framework.py:282-290, 323-330: neighbour rebuild, density and Aii at the current x."""
+        ps = self.ps
+        self.ns.build(ps.x, ps.particle_num)
+        self.ns.build_cache(ps.x, ps.particle_num, ps.cache_size)
+        self.pressure.compute_density()
+        self.pressure.compute_Aii()
+
+    def reset(self):
+        """framework.py:227-280 for the state this build keeps: x = x0, v = v0, density0, t = 0."""
+        ps = self.ps
+        ps.particle_num = self.initial_particle_num
+        n = ps.particle_num
+        ps.x[:n] = ps.x0[:n]
+        ps.x_old[:n] = ps.x0[:n]
+        ps.v[:n] = ps.v0[:n]
+        ps.density.copy_(ps.density0)
+        self.pressure.k.zero_()
+        self.time = 0.0
+        ps._dirty = True
+        self.initialize()
+
+    def forward(self):
+        """One step (framework.py:423-544, velocity-based branch)."""
+        ps = self.ps
+        ps.sync_to_library(positions=False, velocities=True)
+        t0 = time.perf_counter()
+        res, vit = ps.ctx.step(self.dt, self.g, self.cfg.viscosity_method, self.viscosity_coeff,
+                               self.pressure.params())
+        self.stats_pressure_ms = res.solve_ms
+        self.last_visc_iters = vit
+        self.pressure.record(res)
+        ps._cache_stale = True
+        if self.sync_arrays:
+            n = ps.particle_num
+            ps.x_old[:n] = ps.x[:n]
+            ps.sync_from_library(x=True, v=True, density=True, divergence=True)
+            ps.ctx.read(N.FIELD_K, self.pressure.k[:n])
+        self.time += self.dt
+        self.stats_step_ms = (time.perf_counter() - t0) * 1000.0

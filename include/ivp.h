@@ -112,6 +112,15 @@ int ivp_set_particles(ivp_ctx *ctx, int n, const float *x, const float *v, const
                       const float *m_inv, const float *density0, const int *is_dynamic,
                       const int *material);
 
+/* Options.  IVP_OPT_GRADIENT_CACHE selects the neighbour-row format: 1 caches float32 gradW
+ * next to every neighbour id (the reference's neighbor_gradW cache, 16 B per pair and pass);
+ * 0 (default) stores ids only (4 B) and evaluates gradW from the positions in every operator
+ * pass.  Density and Aii use the reference's exact expression either way.  Changing the option
+ * drops the neighbour rows: call ivp_neighbors_build again.  No counterpart in the reference. */
+#define IVP_OPT_GRADIENT_CACHE 1
+int ivp_set_option(ivp_ctx *ctx, int option, int value);
+int ivp_get_option(ivp_ctx *ctx, int option, int *value);
+
 /* Copy a user-order field in or out of the library's particle store.  ivp_write accepts
  * IVP_FIELD_X (dynamic particles only; static particles keep the positions given to
  * ivp_set_particles) and IVP_FIELD_V. */

@@ -44,12 +44,14 @@ def oracle_at(cfg_id, step, scale=None):
     return sim
 
 
-def gpu_context(sim, torch):
-    """A libivp Context loaded with the oracle simulation's particle state."""
+def gpu_context(sim, torch, gradient_cache=False):
+    """A libivp Context loaded with the oracle simulation's particle state.  gradient_cache
+    selects the neighbour-row format (cached float32 gradW or ids only)."""
     from implicitvelocityprojection_amd.context import Context
     cfg = sim.cfg
     dev = torch.device("cuda")
     ctx = Context(sim.h, cfg.domain_start, cfg.domain_size, max_particles=sim.x.shape[0], cache_size=sim.C)
+    ctx.gradient_cache = gradient_cache
     t = lambda a, dt=torch.float32: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)
     ctx.set_particles(t(sim.x), t(sim.v), t(sim.m), t(sim.m_inv), t(sim.density0), t(sim.is_dynamic, torch.int32),
                       t(sim.material, torch.int32))

@@ -30,11 +30,15 @@ def _export(ctx, n, C):
     return _np(count), _np(ids), _np(gradW).reshape(-1, 3)
 
 
+CACHE = pytest.mark.parametrize("cache", [False, True], ids=["ids_only", "gradW_cache"])
+
+
+@CACHE
 @pytest.mark.parametrize("step", [0, 12])
-def test_neighbors_density_k_bit_exact(step):
+def test_neighbors_density_k_bit_exact(step, cache):
     sim = oracle_at(0, step)
     n = sim.x.shape[0]
-    ctx = gpu_context(sim, torch)
+    ctx = gpu_context(sim, torch, cache)
     assert ctx.neighbors_build() == 0
     count, ids, gradW = _export(ctx, n, sim.C)
     np.testing.assert_array_equal(count, sim.count)
@@ -54,8 +58,9 @@ def test_neighbors_density_k_bit_exact(step):
     ctx.close()
 
 
+@CACHE
 @pytest.mark.parametrize("step", [0, 5, 20])
-def test_ip_solve_matches_oracle(step):
+def test_ip_solve_matches_oracle(step, cache):
     sim = oracle_at(0, step)
     # IP input: velocities after gravity (viscosity is off in this scene)
     v_in = O.gravity(sim.v, sim.is_dynamic, sim.g, sim.dt)
@@ -63,7 +68,7 @@ def test_ip_solve_matches_oracle(step):
     v_o, div_o, res_o = O.ip_solve(v_in, sim.m, sim.density, sim.density0, sim.is_dynamic, sim.C, sim.count,
                                    sim.ids, sim.gradW, sim.k, sim.dt, **kw)
     assert res_o.status == O.ORC_OK
-    ctx = gpu_context(sim, torch)
+    ctx = gpu_context(sim, torch, cache)
     assert ctx.neighbors_build() == 0
     n = sim.x.shape[0]
     ctx.write(1, torch.from_numpy(v_in).cuda())
@@ -83,9 +88,10 @@ def test_ip_solve_matches_oracle(step):
     ctx.close()
 
 
-def test_full_step_matches_oracle():
+@CACHE
+def test_full_step_matches_oracle(cache):
     sim = oracle_at(0, 8)
-    ctx = gpu_context(sim, torch)
+    ctx = gpu_context(sim, torch, cache)
     assert ctx.neighbors_build() == 0
     from implicitvelocityprojection_amd.context import IPParams
     kw = ip_kwargs(sim.cfg)
@@ -94,20 +100,25 @@ def test_full_step_matches_oracle():
     n = sim.x.shape[0]
     x_g = _np(ctx.read(0, torch.zeros((n, 3), device="cuda")))
     v_g = _np(ctx.read(1, torch.zeros((n, 3), device="cuda")))
+    div_g = _np(ctx.read(4, torch.zeros(n, device="cuda")))
     dyn = sim.is_dynamic == 1
     assert rel_inf(v_g[dyn], sim.v[dyn]) <= V_TOL
     assert rel_inf(x_g[dyn], sim.x[dyn]) <= V_TOL
     assert np.array_equal(x_g[~dyn], sim.x[~dyn])
+    # divergence of the corrected velocity, before advection (ip.py:402)
+    assert rel_inf(div_g, sim.divergence) <= 1e-4
+    assert np.all(div_g[~dyn] == 0.0)
     assert res_g.outer_iters == sim.last_ip.outer_iters
     ctx.close()
 
 
-def test_viscosity_explicit_jacobi_matches_oracle():
+@CACHE
+def test_viscosity_explicit_jacobi_matches_oracle(cache):
     sim = oracle_at(0, 10)
     coeff = 0.01
     v_o = O.viscosity(sim.x, sim.v, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h, sim.C,
                       sim.count, sim.ids, sim.gradW, coeff, sim.dt, mode=O.VISC_JACOBI)
-    ctx = gpu_context(sim, torch)
+    ctx = gpu_context(sim, torch, cache)
     assert ctx.neighbors_build() == 0
     ctx.viscosity_explicit(coeff, sim.dt)
     n = sim.x.shape[0]
@@ -118,12 +129,13 @@ def test_viscosity_explicit_jacobi_matches_oracle():
     ctx.close()
 
 
-def test_weiler18_matches_oracle():
+@CACHE
+def test_weiler18_matches_oracle(cache):
     sim = oracle_at(0, 10)
     coeff = 0.05
     v_o, it_o = O.weiler18(sim.x, sim.v, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h,
                            sim.C, sim.count, sim.ids, sim.gradW, coeff, sim.dt)
-    ctx = gpu_context(sim, torch)
+    ctx = gpu_context(sim, torch, cache)
     assert ctx.neighbors_build() == 0
     it_g = ctx.viscosity_weiler18(coeff, sim.dt)
     n = sim.x.shape[0]
@@ -134,13 +146,14 @@ def test_weiler18_matches_oracle():
     ctx.close()
 
 
-def test_deterministic_rerun():
+@CACHE
+def test_deterministic_rerun(cache):
     sim = oracle_at(0, 6)
     from implicitvelocityprojection_amd.context import IPParams
     kw = ip_kwargs(sim.cfg)
     outs = []
     for _ in range(2):
-        ctx = gpu_context(sim, torch)
+        ctx = gpu_context(sim, torch, cache)
         assert ctx.neighbors_build() == 0
         for _ in range(3):
             ctx.step(sim.dt, sim.g, 0, 0.0, IPParams(**kw))
