@@ -96,7 +96,7 @@ int alloc_all(ivp_ctx* c) {
 int ensure_rows(ivp_ctx* c) {
     size_t tiles = (size_t)blocks_for(std::max(c->nd, 1)) * WAVES_PER_BLOCK;
     static_assert(sizeof(uint32_t) == 4, "");
-    size_t slots = tiles * (size_t)c->C * 64u;
+    size_t slots = tiles * (size_t)c->Cs * 64u;
     if (c->nb) return IVP_OK;
     IVP_TRY(dalloc(&c->nb, slots * (size_t)(c->rec_words / 64)));
     return IVP_OK;
@@ -204,6 +204,7 @@ int ivp_create(const ivp_config* cfg, int max_particles, void* stream, ivp_ctx**
     c->stream = (hipStream_t)stream;
     c->h = cfg->support_radius;
     c->C = cfg->cache_size;
+    c->Cs = (c->C + ROW_PAD - 1) / ROW_PAD * ROW_PAD;
     c->cap = std::max(max_particles, 1);
     // uniform grid: cell width w = h (1 + 2^-10), origin two cells below the domain start,
     // ceil(L / w) + 4 cells per axis
@@ -258,6 +259,7 @@ int ivp_set_particles(ivp_ctx* c, int n, const float* x, const float* v, const f
                       const float* density0, const int* is_dynamic, const int* material) {
     if (!c) return IVP_ERR_ARG;
     free_rows(c);
+    c->built = false;   // the rows describe the previous particle set
     int rc = grid_load(c, n, x, v, m, m_inv, density0, is_dynamic, material);
     if (rc != IVP_OK) return rc;
     return ensure_rows(c);

@@ -9,6 +9,8 @@
 // resulting order -- (cell key, user id) -- depends on the positions only.  Cell keys are
 // x-fastest linear, so the three cells (cx-1..cx+1, cy, cz) of a stencil row are one contiguous
 // particle range: a query visits 9 ranges per population.
+#include <cmath>
+
 #include "ivp_internal.h"
 #include "ivp_launch.h"
 #include "ivp_reduce.h"
@@ -100,25 +102,62 @@ __global__ __launch_bounds__(BLOCK) void k_scan3(int n, int* __restrict__ start,
     if (blockIdx.x == 0 && threadIdx.x == 0) start[n] = bsum[nb];
 }
 
+// Runs of equal keys among the valid lanes of a wave (valid lanes form a prefix of the wave).
+// Particles arrive in the previous step's cell order, so runs are ~ one cell long and one
+// atomic per run replaces one per particle.
+struct KeyRun {
+    bool head;   // first lane of its run
+    int len;     // run length (head lanes)
+    int head_lane;
+};
+
+__device__ __forceinline__ KeyRun key_run(int k, bool valid) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    int prev = __shfl_up(k, 1, WAVE);
+    bool head = valid && (lane == 0 || prev != k);
+    unsigned long long heads = __ballot(head);
+    int nvalid = __popcll(__ballot(valid));
+    unsigned long long above = lane == WAVE - 1 ? 0ull : (heads & (~0ull << (lane + 1)));
+    int next = above ? __ffsll((long long)above) - 1 : WAVE;
+    KeyRun r;
+    r.head = head;
+    r.len = (next < nvalid ? next : nvalid) - lane;
+    unsigned long long upto = lane == WAVE - 1 ? heads : (heads & ((2ull << lane) - 1));
+    r.head_lane = upto ? WAVE - 1 - __clzll((long long)upto) : lane;
+    return r;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_keys(int count, const float4* __restrict__ pos, Grid g,
                                                 int* __restrict__ key, int* __restrict__ ccount) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= count) return;
-    float4 p = pos[i];
-    int k = cell_key(g, cell_coord(g, p.x, 0), cell_coord(g, p.y, 1), cell_coord(g, p.z, 2));
-    key[i] = k;
-    atomicAdd(&ccount[k], 1);
+    bool valid = i < count;
+    int k = -1;
+    if (valid) {
+        float4 p = pos[i];
+        k = cell_key(g, cell_coord(g, p.x, 0), cell_coord(g, p.y, 1), cell_coord(g, p.z, 2));
+        key[i] = k;
+    }
+    KeyRun r = key_run(k, valid);
+    if (r.head) atomicAdd(&ccount[k], r.len);
 }
 
-// slot = start[key] + (remaining count - 1); leaves ccount at zero for the next sort
+// slot = start[key] + (remaining count - 1); leaves ccount at zero for the next sort.  A run's
+// head takes the run's slots with one atomic and its lanes take them in order.
 __global__ __launch_bounds__(BLOCK) void k_scatter(int count, const int* __restrict__ key,
                                                    const int* __restrict__ start, int* __restrict__ ccount,
                                                    int* __restrict__ tmp_src) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= count) return;
-    int k = key[i];
-    int slot = start[k] + atomicSub(&ccount[k], 1) - 1;
-    tmp_src[slot] = i;
+    bool valid = i < count;
+    int k = valid ? key[i] : -1;
+    KeyRun r = key_run(k, valid);
+    int old = 0;
+    if (r.head) old = atomicSub(&ccount[k], r.len);
+    old = __shfl(old, r.head_lane, WAVE);
+    if (valid) {
+        int lane = threadIdx.x & (WAVE - 1);
+        int slot = start[k] + old - 1 - (lane - r.head_lane);
+        tmp_src[slot] = i;
+    }
 }
 
 // final position = cell start + number of cell members with a smaller user id
@@ -200,12 +239,13 @@ __device__ __forceinline__ void visit(const Grid& g, const int* __restrict__ sta
 }
 
 // Candidate scan of one population: calls f(j) for every j != self with
-// sqrtf(dx*dx + dy*dy + dz*dz) < h, in the order of `visit`.  d^2 >= h2_hi (a bound a few ulps
-// above h^2) already implies sqrtf(d^2) >= h, so the correctly rounded square root is only
-// evaluated for the candidates below it.
+// sqrtf(dx*dx + dy*dy + dz*dz) < h, in the order of `visit`.  The
+// predicate is decided without the square root outside a few-ulp band around h^2:
+// d^2 < h2_lo = h^2 (1 - 2^-20) implies sqrtf(d^2) < h, and d^2 >= h2_hi = h^2 (1 + 2^-20)
+// implies sqrtf(d^2) >= h, so the correctly rounded square root runs only inside the band.
 template <class F>
 __device__ __forceinline__ void scan_ids(const Grid& g, const int* __restrict__ start, const float4* __restrict__ ppos,
-                                         int self, float4 pi, float h, float h2_hi, F&& f) {
+                                         int self, float4 pi, float h, float h2_lo, float h2_hi, F&& f) {
     int cx = cell_coord(g, pi.x, 0), cy = cell_coord(g, pi.y, 1), cz = cell_coord(g, pi.z, 2);
     int x0 = cx > 0 ? cx - 1 : 0;
     int x1 = cx < g.dims[0] - 1 ? cx + 1 : g.dims[0] - 1;
@@ -221,17 +261,17 @@ __device__ __forceinline__ void scan_ids(const Grid& g, const int* __restrict__ 
                 float4 pj = ppos[j];
                 float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
                 float d2 = rx * rx + ry * ry + rz * rz;
-                if (d2 < h2_hi && j != self) {
-                    if (sqrtf(d2) < h) f(j);
-                }
+                bool in = d2 < h2_lo;
+                if (!in && d2 < h2_hi) in = sqrtf(d2) < h;
+                if (in && j != self) f(j);
             }
         }
     }
 }
 
 struct BuildArgs {
-    int nd, ns, C;
-    float h, h2_hi;
+    int nd, ns, C, Cs;
+    float h, h2_lo, h2_hi;
     Grid g;
     const float4* pos;
     const float* minv;
@@ -251,11 +291,13 @@ struct BuildArgs {
 
 // One thread per dynamic row, one wave per 64-row tile, two phases.
 //  A: candidate scan; the ids of dynamic neighbours go to slots 0, 1, ... of the row and the
-//     ids of static neighbours to slots C-1, C-2, ... (an overflowing row sets the error flag).
+//     ids of static neighbours to slots Cs-1, Cs-2, ... (a row with more than C neighbours sets
+//     the error flag).
 //  B: the wave walks its slots in lockstep, computing gradW (stored next to the id, coalesced),
 //     density (pressure_base.py:12-46), Aii (pressure_base.py:49-87) and S_i, accumulated in the
 //     reference row order: dynamic neighbours, then static neighbours.  Dynamic slots from the
-//     row's count up to the tile length are padded with (id = row, gradW = 0).
+//     row's count up to the tile length -- the wave's longest row rounded up to ROW_PAD slots
+//     -- are padded with (id = row, gradW = 0).
 template <bool RECOMP>
 __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
     constexpr unsigned REC = RECOMP ? 64u : 256u;   // words per (tile, slot) record
@@ -265,19 +307,19 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
     int tile = i >> 6;
     bool valid = i < a.nd;
     const float h = a.h;
-    const int C = a.C;
-    uint32_t* tbase = a.nb + (size_t)tile * (size_t)C * REC + lane;
+    const int C = a.C, Cs = a.Cs;
+    uint32_t* tbase = a.nb + (size_t)tile * (size_t)Cs * REC + lane;
     int cnt = 0, cs = 0, tot = 0;
     float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (valid) {
         pi = a.pos[i];
-        scan_ids(a.g, a.dstart, a.pos, i, pi, h, a.h2_hi, [&](int j) {
+        scan_ids(a.g, a.dstart, a.pos, i, pi, h, a.h2_lo, a.h2_hi, [&](int j) {
             if (tot < C) tbase[(size_t)cnt * REC] = (uint32_t)j;
             if (tot < C) cnt++;
             tot++;
         });
-        scan_ids(a.g, a.sstart, a.pos + a.nd, -1, pi, h, a.h2_hi, [&](int j) {
-            if (tot < C) tbase[(size_t)(C - 1 - cs) * REC] = (uint32_t)j;
+        scan_ids(a.g, a.sstart, a.pos + a.nd, -1, pi, h, a.h2_lo, a.h2_hi, [&](int j) {
+            if (tot < C) tbase[(size_t)(Cs - 1 - cs) * REC] = (uint32_t)j;
             if (tot < C) cs++;
             tot++;
         });
@@ -285,6 +327,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
     }
     int tl = wave_max(cnt);
     int ts = wave_max(cs);
+    int tlp = (tl + ROW_PAD - 1) / ROW_PAD * ROW_PAD;   // <= Cs: tl <= C <= Cs, Cs a multiple of ROW_PAD
     float rho = 0.0f;
     rho = rho + pi.w * cubic_kernel(0.0f, h);
     float aii = 0.0f;
@@ -314,7 +357,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
     }
     for (int k = 0; k < ts; ++k) {
         if (!valid || k >= cs) continue;
-        int j = (int)tbase[(size_t)(C - 1 - k) * REC];
+        int j = (int)tbase[(size_t)(Cs - 1 - k) * REC];
         if ((unsigned)j >= (unsigned)a.ns) {  // never taken: phase A stored a static id here
             atomicOr(a.err_flag, 4);
             continue;
@@ -328,9 +371,9 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
         jx -= Jx; jy -= Jy; jz -= Jz;
         sx += Jx; sy += Jy; sz += Jz;
     }
-    // Pad only after the static ids (slots C-1-cs .. C-1) have been consumed: the padded range
-    // [cnt, tl) follows the wave's longest row and can reach into them.
-    for (int k = 0; k < tl; ++k) {
+    // Pad only after the static ids (slots Cs-cs .. Cs-1) have been consumed: the padded range
+    // [cnt, tlp) follows the wave's longest row and can reach into them.
+    for (int k = 0; k < tlp; ++k) {
         if (!valid || k < cnt) continue;
         uint32_t* rec = tbase + (size_t)k * REC;
         rec[0] = (uint32_t)i;
@@ -352,7 +395,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
     }
     double v[1] = {(double)cnt};
     block_partials<1>(v, a.partials, a.stride);
-    if (lane == 0) a.tile_len[tile] = tl;
+    if (lane == 0) a.tile_len[tile] = tlp;
 }
 
 struct ExportArgs {
@@ -615,8 +658,11 @@ int grid_build(ivp_ctx* c) {
     }
     IVP_HIP(hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream));
     if (c->nd > 0) {
-        float h2_hi = (float)((double)c->h * (double)c->h * (1.0 + 1.0 / 1048576.0));
-        BuildArgs ba{c->nd, c->ns, c->C, c->h, h2_hi, c->grid, c->pos, c->minv, c->dstart, c->sstart, c->nb, c->tile_len,
+        // float h2_lo <= h^2 (1 - 2^-20) and h2_hi >= h^2 (1 + 2^-20): the predicate bounds of scan_ids
+        const double hh = (double)c->h * (double)c->h;
+        float h2_lo = std::nextafter((float)(hh * (1.0 - 1.0 / 1048576.0)), 0.0f);
+        float h2_hi = std::nextafter((float)(hh * (1.0 + 1.0 / 1048576.0)), 1.0f);
+        BuildArgs ba{c->nd, c->ns, c->C, c->Cs, c->h, h2_lo, h2_hi, c->grid, c->pos, c->minv, c->dstart, c->sstart, c->nb, c->tile_len,
                      c->rowcnt, c->rowtot, c->density, c->kk, c->Ss, c->err_flag, c->partials, c->max_blocks};
         // bytes (pair count of the previous build): the pair records written, the ids read back
         // once, x and m of every particle once, the per-row outputs (rho, k, S, counts)

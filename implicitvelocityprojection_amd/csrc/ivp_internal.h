@@ -11,11 +11,12 @@
 // density and Aii, all accumulated during the build.  Rows are stored in wave tiles:
 // tile t = rows [64t, 64t+64), slot k, component c (0 = neighbour id, 1..3 = gradW xyz when
 // gradients are cached):
-//     nb[((t * C + k) * R + c) * 64 + lane],   R = 4 (cached gradW) or 1 (ids only)
-// so every (tile, slot) is one contiguous record (1 KiB or 256 B) read by one wave.  Slots past a
-// row's count up to the tile length hold (id = the row itself, gradW = 0): r_ii = 0 gives
-// gradW = 0 either way, which adds exactly 0 to every operator, so the hot loops run a
-// wave-uniform trip count.
+//     nb[((t * Cs + k) * R + c) * 64 + lane],   R = 4 (cached gradW) or 1 (ids only)
+// (Cs = cache size rounded up to ROW_PAD) so every (tile, slot) is one contiguous record (1 KiB
+// or 256 B) read by one wave.  The tile length is the tile's longest row rounded up to ROW_PAD;
+// slots past a row's count hold (id = the row itself, gradW = 0): r_ii = 0 gives gradW = 0
+// either way, which adds exactly 0 to every operator, so the hot loops run a wave-uniform trip
+// count.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -59,6 +60,8 @@ struct SolverState {
     long long total_pairs;  // stored (dynamic-dynamic) neighbour pairs of the last build
 };
 
+constexpr int ROW_PAD = 8;        // tile lengths are padded to a multiple of this many slots
+
 enum KernelClass {
     KC_SORT = 0,
     KC_BUILD,
@@ -92,7 +95,8 @@ struct ivp_ctx {
     hipStream_t stream = nullptr;
     ivp_config cfg{};
     float h = 0.0f;
-    int C = 60;
+    int C = 60;             // neighborCacheSize: the row capacity (overflow beyond it)
+    int Cs = 64;            // slots per row in the tile layout: C rounded up to ROW_PAD
     ivp::Grid grid{};
     int ncell = 0;
     int cap = 0;

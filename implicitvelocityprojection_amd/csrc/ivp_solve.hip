@@ -41,62 +41,55 @@ namespace {
 
 struct Rows {
     const uint32_t* nb;
-    const int* tile_len;
-    int C;
+    const int* tile_len;   // per tile: slots to walk, a multiple of ROW_PAD, padded with the row itself
+    int Cs;                // slots per row in the tile layout
     int nd;
     const float4* pos;   // particle positions (x, y, z, m), for evaluated gradients
     float h;
     float ks;            // 45 / (pi h^6)
 };
 
-__device__ __forceinline__ int row_index() {
-    int blk = xcd_block(blockIdx.x, gridDim.x);
-    return blk * BLOCK + threadIdx.x;
+// gradW_ij = s * r_ij in the evaluated form; returns s
+__device__ __forceinline__ float grad_scale(float h, float ks, float rx, float ry, float rz) {
+    float d2 = fmaf(rz, rz, fmaf(ry, ry, rx * rx));
+    float inv = __builtin_amdgcn_rsqf(fmaxf(d2, 1e-12f));   // 1 / max(|r|, 1e-6)
+    float t = h - d2 * inv;
+    return (t * t) * (-ks * inv);
 }
 
-// One lane's view of its neighbour row.
-template <bool RECOMP>
-struct RowIter {
-    static constexpr unsigned REC = RECOMP ? 64u : 256u;
-    const uint32_t* e;
-    int tl;
-    float4 xi;
-    const float4* __restrict__ pos;
-    float h, ks;
+// Per-particle pass data of NB floats.
+template <int NB>
+struct Vec {
+    float v[NB > 0 ? NB : 1];
+};
 
-    __device__ __forceinline__ RowIter(const Rows& R, int i, float4 xi_)
-        : e(R.nb + (size_t)(i >> 6) * (size_t)R.C * REC + (threadIdx.x & (WAVE - 1))),
-          tl(R.tile_len[i >> 6]), xi(xi_), pos(R.pos), h(R.h), ks(R.ks) {}
-
-    // gradW_ij = s * r_ij for the evaluated form; returns s
-    __device__ __forceinline__ float grad_scale(float rx, float ry, float rz) const {
-        float d2 = rx * rx + ry * ry + rz * rz;
-        float inv = __builtin_amdgcn_rsqf(fmaxf(d2, 1e-12f));   // 1 / max(|r|, 1e-6)
-        float rn = d2 * inv;
-        float t = h - rn;
-        return -ks * (t * t) * inv;
-    }
-
-    // Pair k: neighbour id, its (x, y, z, m) record (loaded when NEEDPJ or when gradW is
-    // evaluated) and gradW_ij.
-    template <bool NEEDPJ>
-    __device__ __forceinline__ int get(int k, float4& pj, float& gx, float& gy, float& gz) const {
+// Walk the dynamic slots of row i in order, calling f(A_j, B_j, gx, gy, gz) with A_j = loadA(j) =
+// (x, y, z, a_j) and B_j = loadB(j).  The tile length is a multiple of ROW_PAD, and slots past a
+// row's count name the row itself, whose gradW is 0.
+template <bool RECOMP, int NB, class LA, class LB, class F>
+__device__ __forceinline__ void row_pairs(const Rows& R, int i, float4 xi, LA&& loadA, LB&& loadB, F&& f) {
+    constexpr unsigned REC = RECOMP ? 64u : 256u;
+    const uint32_t* e = R.nb + (size_t)(i >> 6) * (size_t)R.Cs * REC + (threadIdx.x & (WAVE - 1));
+    const int tl = __builtin_amdgcn_readfirstlane(R.tile_len[i >> 6]);   // one tile per wave
+#pragma unroll 4
+    for (int k = 0; k < tl; ++k) {
         const uint32_t* r = e + (size_t)k * REC;
         int j = (int)__builtin_nontemporal_load(r);
+        float4 Aj = loadA(j);
+        Vec<NB> Bj = loadB(j);
+        float gx, gy, gz;
         if (RECOMP) {
-            pj = pos[j];
-            float rx = xi.x - pj.x, ry = xi.y - pj.y, rz = xi.z - pj.z;
-            float s = grad_scale(rx, ry, rz);
-            gx = s * rx; gy = s * ry; gz = s * rz;
+            float rx = xi.x - Aj.x, ry = xi.y - Aj.y, rz = xi.z - Aj.z;
+            float sc = grad_scale(R.h, R.ks, rx, ry, rz);
+            gx = sc * rx; gy = sc * ry; gz = sc * rz;
         } else {
             gx = __uint_as_float(__builtin_nontemporal_load(r + 64));
             gy = __uint_as_float(__builtin_nontemporal_load(r + 128));
             gz = __uint_as_float(__builtin_nontemporal_load(r + 192));
-            if (NEEDPJ) pj = pos[j];
         }
-        return j;
+        f(Aj, Bj, gx, gy, gz);
     }
-};
+}
 
 // ---- IP ------------------------------------------------------------------------------------
 
@@ -111,40 +104,19 @@ __global__ __launch_bounds__(BLOCK) void k_ip_init(int nd, float dt, const float
     d4[i] = make_float4(dt * vel[3 * i + 0], dt * vel[3 * i + 1], dt * vel[3 * i + 2], pos[i].w);
 }
 
-// J pass, form of ip_pack_fused / compute_J_x_kernel (ip.py:150-163, pressure_base.py:111-127):
-// sum_j m_j dot(u_i - [dyn_j] u_j, gradW_ij), static part as dot(u_i, S_i).
+// Neighbour record (x, y, z, m) of the J passes: positions are needed only to evaluate gradW;
+// with cached gradients only m_j is read, from the w of the pass vector (u4[j].w = m_j).
 template <bool RECOMP>
-__device__ __forceinline__ float J_row_mdot(const Rows& R, int i, float4 ui, const float4* __restrict__ u4,
-                                            const float* __restrict__ Ss) {
-    RowIter<RECOMP> it(R, i, RECOMP ? R.pos[i] : make_float4(0, 0, 0, 0));
-    float acc = 0.0f;
-#pragma unroll 4
-    for (int k = 0; k < it.tl; ++k) {
-        float4 pj; float gx, gy, gz;
-        int j = it.template get<false>(k, pj, gx, gy, gz);
-        float4 uj = u4[j];
-        float dx = ui.x - uj.x, dy = ui.y - uj.y, dz = ui.z - uj.z;
-        acc += uj.w * (dx * gx + dy * gy + dz * gz);
-    }
-    return acc + (ui.x * Ss[3 * i + 0] + ui.y * Ss[3 * i + 1] + ui.z * Ss[3 * i + 2]);
+__device__ __forceinline__ float4 j_record(const float4* __restrict__ pos, const float4* __restrict__ u4, int j) {
+    return RECOMP ? pos[j] : make_float4(0.0f, 0.0f, 0.0f, u4[j].w);
 }
 
-// J pass, form of compute_Jx_pcg (ip.py:51-58): sum_j dot(p_i - [dyn_j] p_j, m_j gradW_ij)
-template <bool RECOMP>
-__device__ __forceinline__ float J_row_dotm(const Rows& R, int i, float4 pi, const float4* __restrict__ p4,
-                                            const float* __restrict__ Ss) {
-    RowIter<RECOMP> it(R, i, RECOMP ? R.pos[i] : make_float4(0, 0, 0, 0));
-    float acc = 0.0f;
-#pragma unroll 4
-    for (int k = 0; k < it.tl; ++k) {
-        float4 pj_pos; float gx, gy, gz;
-        int j = it.template get<false>(k, pj_pos, gx, gy, gz);
-        float4 pj = p4[j];
-        float wx = pj.w * gx, wy = pj.w * gy, wz = pj.w * gz;
-        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        acc += dx * wx + dy * wy + dz * wz;
-    }
-    return acc + (pi.x * Ss[3 * i + 0] + pi.y * Ss[3 * i + 1] + pi.z * Ss[3 * i + 2]);
+// xyz of a float4 array entry as pass data
+__device__ __forceinline__ Vec<3> vec3_of(const float4* __restrict__ u4, int j) {
+    float4 u = u4[j];
+    Vec<3> B;
+    B.v[0] = u.x; B.v[1] = u.y; B.v[2] = u.z;
+    return B;
 }
 
 struct PackArgs {
@@ -163,26 +135,37 @@ struct PackArgs {
 };
 
 // ip_pack_fused (ip.py:124-171) + coef_wise_div + compute_avg_density_error
-// (math_utils.py:76-91, pressure_base.py:187-203, 413-423).
+// (math_utils.py:76-91, pressure_base.py:187-203, 413-423).  J d in the form of
+// compute_J_x_kernel (pressure_base.py:111-127): sum_j m_j dot(d_i - d_j, gradW_ij) + dot(d_i, S_i).
 template <bool RECOMP>
 __global__ __launch_bounds__(BLOCK) void k_ip_pack(PackArgs a) {
-    int i = row_index();
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int i = blk * BLOCK + threadIdx.x;
+    // neighbour record: A = (x, y, z, m), B = d
+    auto loadA = [&](int j) { return j_record<RECOMP>(a.R.pos, a.d4, j); };
+    auto loadB = [&](int j) { return vec3_of(a.d4, j); };
     double err = 0.0, nact = 0.0, npairs = 0.0;
     if (i < a.R.nd) {
         float4 di = a.d4[i];
-        float jd = J_row_mdot<RECOMP>(a.R, i, di, a.d4, a.Ss);
+        float acc = 0.0f;
+        row_pairs<RECOMP, 3>(a.R, i, a.R.pos[i], loadA, loadB,
+                             [&](const float4& Aj, const Vec<3>& Bj, float gx, float gy, float gz) {
+                                 float dx = di.x - Bj.v[0], dy = di.y - Bj.v[1], dz = di.z - Bj.v[2];
+                                 acc = fmaf(Aj.w, fmaf(dz, gz, fmaf(dy, gy, dx * gx)), acc);
+                             });
+        float jd = acc + (di.x * a.Ss[3 * i + 0] + di.y * a.Ss[3 * i + 1] + di.z * a.Ss[3 * i + 2]);
         a.jd[i] = jd;
         float t = jd + a.cc[i];
         float f = fmaxf(t, 0.0f);
         float dfdt = t >= 1e-6f ? 1.0f : 0.0f;
         float k = a.kk[i];
-        float s = dfdt * k;
-        a.sfac[i] = s;
-        a.q4[i] = make_float4(f * k, di.w, s, 0.0f);
+        float sk = dfdt * k;
+        a.sfac[i] = sk;
+        a.q4[i] = make_float4(f * k, di.w, sk, 0.0f);
         float denom = a.rho0[i];
         if (denom < 1e-6f) denom = 1e-6f;
         err = (double)fmaxf(f / denom, 0.0f);
-        if (s != 0.0f) {
+        if (sk != 0.0f) {
             nact = 1.0;
             npairs = (double)a.rowcnt[i];
         }
@@ -207,23 +190,30 @@ struct RhsArgs {
 // non-zero (own dfdt > 0 or a dynamic neighbour with dfdt > 0).
 template <bool RECOMP>
 __global__ __launch_bounds__(BLOCK) void k_ip_rhs(RhsArgs a) {
-    int i = row_index();
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int i = blk * BLOCK + threadIdx.x;
+    // neighbour record: A = (x, y, z, m), B = (f k, dfdt k); q4 = (f k, m, dfdt k, 0)
+    auto loadA = [&](int j) {
+        return RECOMP ? a.R.pos[j] : make_float4(0.0f, 0.0f, 0.0f, a.q4[j].y);
+    };
+    auto loadB = [&](int j) {
+        float4 q = a.q4[j];
+        Vec<2> B;
+        B.v[0] = q.x; B.v[1] = q.z;
+        return B;
+    };
     double touched = 0.0, tpairs = 0.0;
     if (i < a.R.nd) {
         float4 qi = a.q4[i];
         float fki = qi.x, mi = qi.y;
         bool act = qi.z != 0.0f;
         float rx = 0.0f, ry = 0.0f, rz = 0.0f;
-        RowIter<RECOMP> it(a.R, i, RECOMP ? a.R.pos[i] : make_float4(0, 0, 0, 0));
-#pragma unroll 4
-        for (int k = 0; k < it.tl; ++k) {
-            float4 pj; float gx, gy, gz;
-            int j = it.template get<false>(k, pj, gx, gy, gz);
-            float4 qj = a.q4[j];
-            float s = qj.y * fki + mi * qj.x;
-            rx += s * gx; ry += s * gy; rz += s * gz;
-            act = act || (qj.z != 0.0f);
-        }
+        row_pairs<RECOMP, 2>(a.R, i, a.R.pos[i], loadA, loadB,
+                             [&](const float4& Aj, const Vec<2>& Bj, float gx, float gy, float gz) {
+                                 float sc = fmaf(Aj.w, fki, mi * Bj.v[0]);
+                                 rx = fmaf(sc, gx, rx); ry = fmaf(sc, gy, ry); rz = fmaf(sc, gz, rz);
+                                 act = act || (Bj.v[1] != 0.0f);
+                             });
         rx += fki * a.Ss[3 * i + 0];
         ry += fki * a.Ss[3 * i + 1];
         rz += fki * a.Ss[3 * i + 2];
@@ -278,17 +268,28 @@ struct CGJArgs {
     const SolverState* st;
 };
 
-// compute_Jx_pcg (ip.py:19-60): y_i = dfdt_i k_i (J p)_i on rows with dfdt > 0
+// compute_Jx_pcg (ip.py:19-60): y_i = dfdt_i k_i (J p)_i on rows with dfdt > 0, J p in the form
+// sum_j dot(p_i - p_j, m_j gradW_ij) + dot(p_i, S_i) (ip.py:51-58)
 template <bool RECOMP>
 __global__ __launch_bounds__(BLOCK) void k_cg_J(CGJArgs a) {
     if (a.st->active == 0) return;
-    int i = row_index();
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int i = blk * BLOCK + threadIdx.x;
     if (i >= a.R.nd) return;
-    float s = a.sfac[i];
-    if (s == 0.0f) return;
+    const float sk = a.sfac[i];
+    if (sk == 0.0f) return;
+    // neighbour record: A = (x, y, z, m), B = p
+    auto loadA = [&](int j) { return j_record<RECOMP>(a.R.pos, a.p4, j); };
+    auto loadB = [&](int j) { return vec3_of(a.p4, j); };
     float4 pi = a.p4[i];
-    float acc = J_row_dotm<RECOMP>(a.R, i, pi, a.p4, a.Ss);
-    a.y2[i] = make_float2(acc * s, pi.w);
+    float acc = 0.0f;
+    row_pairs<RECOMP, 3>(a.R, i, a.R.pos[i], loadA, loadB,
+                         [&](const float4& Aj, const Vec<3>& Bj, float gx, float gy, float gz) {
+                             float dx = pi.x - Bj.v[0], dy = pi.y - Bj.v[1], dz = pi.z - Bj.v[2];
+                             acc = fmaf(Aj.w, fmaf(dz, gz, fmaf(dy, gy, dx * gx)), acc);
+                         });
+    acc = acc + (pi.x * a.Ss[3 * i + 0] + pi.y * a.Ss[3 * i + 1] + pi.z * a.Ss[3 * i + 2]);
+    a.y2[i] = make_float2(acc * sk, pi.w);
 }
 
 struct CGJTArgs {
@@ -309,23 +310,30 @@ struct CGJTArgs {
 template <bool RECOMP>
 __global__ __launch_bounds__(BLOCK) void k_cg_JT(CGJTArgs a) {
     if (a.st->active == 0) return;
-    int i = row_index();
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int i = blk * BLOCK + threadIdx.x;
+    const bool touched = i < a.R.nd && a.nbact[i] != 0;
+    // neighbour record: A = (x, y, z, m), B = y; y2 = (y, m)
+    auto loadA = [&](int j) {
+        return RECOMP ? a.R.pos[j] : make_float4(0.0f, 0.0f, 0.0f, a.y2[j].y);
+    };
+    auto loadB = [&](int j) {
+        Vec<1> B;
+        B.v[0] = a.y2[j].x;
+        return B;
+    };
     double pap = 0.0;
     if (i < a.R.nd) {
         float4 pi = a.p4[i];
         float mi = pi.w;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        if (a.nbact[i] != 0) {
+        if (touched) {
             float yi = a.y2[i].x;
-            RowIter<RECOMP> it(a.R, i, RECOMP ? a.R.pos[i] : make_float4(0, 0, 0, 0));
-#pragma unroll 4
-            for (int k = 0; k < it.tl; ++k) {
-                float4 pj; float gx, gy, gz;
-                int j = it.template get<false>(k, pj, gx, gy, gz);
-                float2 yj = a.y2[j];
-                float s = yj.y * yi + mi * yj.x;
-                ax += s * gx; ay += s * gy; az += s * gz;
-            }
+            row_pairs<RECOMP, 1>(a.R, i, a.R.pos[i], loadA, loadB,
+                                 [&](const float4& Aj, const Vec<1>& Bj, float gx, float gy, float gz) {
+                                     float sc = fmaf(Aj.w, yi, mi * Bj.v[0]);
+                                     ax = fmaf(sc, gx, ax); ay = fmaf(sc, gy, ay); az = fmaf(sc, gz, az);
+                                 });
             ax += yi * a.Ss[3 * i + 0];
             ay += yi * a.Ss[3 * i + 1];
             az += yi * a.Ss[3 * i + 2];
@@ -420,11 +428,25 @@ __global__ __launch_bounds__(BLOCK) void k_v4(int nd, const float* __restrict__ 
     v4[i] = make_float4(vel[3 * i + 0], vel[3 * i + 1], vel[3 * i + 2], density[i]);
 }
 
+// viscosity.py:62-108 with Jacobi reads.  Neighbour record: A = (x, y, z, m_j), m_j = 0 for a
+// non-fluid neighbour (its term is then exactly 0, as the reference's skip), B = (v_j, rho_j).
 template <bool RECOMP, bool ALLFLUID>
 __global__ __launch_bounds__(BLOCK) void k_visc(Rows R, float coeff, float dt, int fluid,
                                                 const float4* __restrict__ v4, const float* __restrict__ rho0,
                                                 const int* __restrict__ mat, float* __restrict__ vel) {
-    int i = row_index();
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int i = blk * BLOCK + threadIdx.x;
+    auto loadA = [&](int j) {
+        float4 A = R.pos[j];
+        if (!ALLFLUID && mat[j] != fluid) A.w = 0.0f;
+        return A;
+    };
+    auto loadB = [&](int j) {
+        float4 u = v4[j];
+        Vec<4> B;
+        B.v[0] = u.x; B.v[1] = u.y; B.v[2] = u.z; B.v[3] = u.w;
+        return B;
+    };
     if (i >= R.nd) return;
     if (!ALLFLUID && mat[i] != fluid) return;
     float4 xi = R.pos[i];
@@ -433,24 +455,19 @@ __global__ __launch_bounds__(BLOCK) void k_visc(Rows R, float coeff, float dt, i
     float eps2 = (0.01f * R.h) * (0.01f * R.h);
     float k_visc = 2.0f * (3.0f + 2.0f);
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    RowIter<RECOMP> it(R, i, xi);
-#pragma unroll 2
-    for (int k = 0; k < it.tl; ++k) {
-        float4 xj; float gx, gy, gz;
-        int j = it.template get<true>(k, xj, gx, gy, gz);
-        if (!ALLFLUID && mat[j] != fluid) continue;
-        float4 vj = v4[j];
-        float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        float rn = len3(rx, ry, rz);
-        float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
-        float vxy = vx * rx + vy * ry + vz * rz;
-        float rho_j = vj.w;
-        float K = 2.0f * rho0_i / (rho_i + rho_j);
-        float s = k_visc * coeff * (xj.w / rho_j) * (vxy / (rn * rn + eps2));
-        ax = ax + K * (s * gx);
-        ay = ay + K * (s * gy);
-        az = az + K * (s * gz);
-    }
+    row_pairs<RECOMP, 4>(R, i, xi, loadA, loadB,
+                         [&](const float4& xj, const Vec<4>& vj, float gx, float gy, float gz) {
+                             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                             float rn = len3(rx, ry, rz);
+                             float vx = vi.x - vj.v[0], vy = vi.y - vj.v[1], vz = vi.z - vj.v[2];
+                             float vxy = vx * rx + vy * ry + vz * rz;
+                             float rho_j = vj.v[3];
+                             float K = 2.0f * rho0_i / (rho_i + rho_j);
+                             float sc = k_visc * coeff * (xj.w / rho_j) * (vxy / (rn * rn + eps2));
+                             ax = ax + K * (sc * gx);
+                             ay = ay + K * (sc * gy);
+                             az = az + K * (sc * gz);
+                         });
     vel[3 * i + 0] = vi.x + ax * dt;
     vel[3 * i + 1] = vi.y + ay * dt;
     vel[3 * i + 2] = vi.z + az * dt;
@@ -492,12 +509,25 @@ __device__ __forceinline__ void inverse33(const float a[9], float out[9]) {
     out[8] = (a[0] * a[4] - a[1] * a[3]) * id;
 }
 
+// Weiler18 neighbour record: A = (x, y, z, V_j), V_j = m_j / rho_j for a fluid neighbour and 0
+// otherwise (a zero term, as the reference's skip of non-fluid neighbours); B = the operator input.
+template <bool ALLFLUID>
+__device__ __forceinline__ float4 w18_record(const float4* __restrict__ pos, const float4* __restrict__ u4,
+                                             const int* __restrict__ mat, int fluid, int j) {
+    float4 x = pos[j];
+    float V = (ALLFLUID || mat[j] == fluid) ? u4[j].w : 0.0f;
+    return make_float4(x.x, x.y, x.z, V);
+}
+
 // compute_precond_weiler18 (weiler18.py:21-85): inverse of I - factor_i L_ii
 template <bool RECOMP, bool ALLFLUID>
 __global__ __launch_bounds__(BLOCK) void k_w18_pre(Rows R, int fluid, const float4* __restrict__ v4,
                                                    const float* __restrict__ fac, const int* __restrict__ mat,
                                                    float* __restrict__ pinv) {
-    int i = row_index();
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int i = blk * BLOCK + threadIdx.x;
+    auto loadA = [&](int j) { return w18_record<ALLFLUID>(R.pos, v4, mat, fluid, j); };
+    auto loadB = [&](int) { return Vec<0>{}; };
     if (i >= R.nd) return;
     float* P = pinv + 9 * (size_t)i;
     if (!ALLFLUID && mat[i] != fluid) {
@@ -509,19 +539,17 @@ __global__ __launch_bounds__(BLOCK) void k_w18_pre(Rows R, int fluid, const floa
     float dd = 10.0f;
     float L[9];
     for (int q = 0; q < 9; ++q) L[q] = 0.0f;
-    RowIter<RECOMP> it(R, i, xi);
-    for (int k = 0; k < it.tl; ++k) {
-        float4 xj; float g[3];
-        int j = it.template get<true>(k, xj, g[0], g[1], g[2]);
-        float x[3] = {xi.x - xj.x, xi.y - xj.y, xi.z - xj.z};
-        float r_sq = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
-        float c = dd * v4[j].w / (r_sq + eps2);
-        if (!ALLFLUID && mat[j] != fluid) continue;
+    row_pairs<RECOMP, 0>(R, i, xi, loadA, loadB,
+                         [&](const float4& xj, const Vec<0>&, float gx, float gy, float gz) {
+                             float x[3] = {xi.x - xj.x, xi.y - xj.y, xi.z - xj.z};
+                             float r_sq = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
+                             float c = dd * xj.w * __builtin_amdgcn_rcpf(r_sq + eps2);
+                             float cg[3] = {c * gx, c * gy, c * gz};
 #pragma unroll
-        for (int p = 0; p < 3; ++p)
+                             for (int p = 0; p < 3; ++p)
 #pragma unroll
-            for (int q = 0; q < 3; ++q) L[3 * p + q] = L[3 * p + q] + c * (g[p] * x[q]);
-    }
+                                 for (int q = 0; q < 3; ++q) L[3 * p + q] = fmaf(cg[p], x[q], L[3 * p + q]);
+                         });
     float f = fac[i];
     float M[9];
 #pragma unroll
@@ -530,35 +558,6 @@ __global__ __launch_bounds__(BLOCK) void k_w18_pre(Rows R, int fluid, const floa
     inverse33(M, out);
 #pragma unroll
     for (int q = 0; q < 9; ++q) P[q] = out[q];
-}
-
-// (A u)_i = u_i - factor_i * 10 sum_fluid (m_j/rho_j) (u_ij . x_ij) / (r^2 + 0.01 h^2) gradW_ij
-// (compute_Av_weiler18, weiler18.py:103-172)
-template <bool RECOMP, bool ALLFLUID>
-__device__ __forceinline__ float3 w18_Av_row(const Rows& R, int i, int fluid, const float4* __restrict__ u4,
-                                             const float* __restrict__ fac, const int* __restrict__ mat,
-                                             float4 ui) {
-    if (!ALLFLUID && mat[i] != fluid) return make_float3(ui.x, ui.y, ui.z);
-    float4 xi = R.pos[i];
-    float lx = 0.0f, ly = 0.0f, lz = 0.0f;
-    float eps2 = 0.01f * R.h * R.h;
-    float dd = 10.0f;
-    RowIter<RECOMP> it(R, i, xi);
-#pragma unroll 4
-    for (int k = 0; k < it.tl; ++k) {
-        float4 xj; float gx, gy, gz;
-        int j = it.template get<true>(k, xj, gx, gy, gz);
-        if (!ALLFLUID && mat[j] != fluid) continue;
-        float4 uj = u4[j];
-        float x0 = xi.x - xj.x, x1 = xi.y - xj.y, x2 = xi.z - xj.z;
-        float v0 = ui.x - uj.x, v1 = ui.y - uj.y, v2 = ui.z - uj.z;
-        float r_sq = x0 * x0 + x1 * x1 + x2 * x2;
-        float vx = v0 * x0 + v1 * x1 + v2 * x2;
-        float s = dd * uj.w * (vx / (r_sq + eps2));
-        lx += s * gx; ly += s * gy; lz += s * gz;
-    }
-    float f = fac[i];
-    return make_float3(ui.x - f * lx, ui.y - f * ly, ui.z - f * lz);
 }
 
 struct W18Args {
@@ -575,13 +574,38 @@ struct W18Args {
     const SolverState* st;
 };
 
+// (A u)_i = u_i - factor_i * 10 sum_fluid (m_j/rho_j) (u_ij . x_ij) / (r^2 + 0.01 h^2) gradW_ij
+// (compute_Av_weiler18, weiler18.py:103-172).  The caller stages the block's window.
+template <bool RECOMP, bool ALLFLUID, class LA, class LB>
+__device__ __forceinline__ float3 w18_Av_row(const W18Args& a, int i, float4 ui, LA&& loadA, LB&& loadB) {
+    if (!ALLFLUID && a.mat[i] != a.fluid) return make_float3(ui.x, ui.y, ui.z);
+    float4 xi = a.R.pos[i];
+    float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+    float eps2 = 0.01f * a.R.h * a.R.h;
+    float dd = 10.0f;
+    row_pairs<RECOMP, 3>(a.R, i, xi, loadA, loadB,
+                         [&](const float4& xj, const Vec<3>& uj, float gx, float gy, float gz) {
+                             float x0 = xi.x - xj.x, x1 = xi.y - xj.y, x2 = xi.z - xj.z;
+                             float v0 = ui.x - uj.v[0], v1 = ui.y - uj.v[1], v2 = ui.z - uj.v[2];
+                             float r_sq = fmaf(x2, x2, fmaf(x1, x1, x0 * x0));
+                             float vx = fmaf(v2, x2, fmaf(v1, x1, v0 * x0));
+                             float sc = dd * xj.w * (vx * __builtin_amdgcn_rcpf(r_sq + eps2));
+                             lx = fmaf(sc, gx, lx); ly = fmaf(sc, gy, ly); lz = fmaf(sc, gz, lz);
+                         });
+    float f = a.fac[i];
+    return make_float3(ui.x - f * lx, ui.y - f * ly, ui.z - f * lz);
+}
+
 // rhs = v - A v (weiler18.py:390-393)
 template <bool RECOMP, bool ALLFLUID>
 __global__ __launch_bounds__(BLOCK) void k_w18_rhs(W18Args a) {
-    int i = row_index();
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int i = blk * BLOCK + threadIdx.x;
+    auto loadA = [&](int j) { return w18_record<ALLFLUID>(a.R.pos, a.v4, a.mat, a.fluid, j); };
+    auto loadB = [&](int j) { return vec3_of(a.v4, j); };
     if (i >= a.R.nd) return;
     float4 vi = a.v4[i];
-    float3 Av = w18_Av_row<RECOMP, ALLFLUID>(a.R, i, a.fluid, a.v4, a.fac, a.mat, vi);
+    float3 Av = w18_Av_row<RECOMP, ALLFLUID>(a, i, vi, loadA, loadB);
     a.out[3 * i + 0] = vi.x + -1.0f * Av.x;
     a.out[3 * i + 1] = vi.y + -1.0f * Av.y;
     a.out[3 * i + 2] = vi.z + -1.0f * Av.z;
@@ -611,11 +635,14 @@ __global__ __launch_bounds__(BLOCK) void k_w18_init(W18Args a, float4* __restric
 template <bool RECOMP, bool ALLFLUID>
 __global__ __launch_bounds__(BLOCK) void k_w18_Av(W18Args a) {
     if (a.st->active == 0) return;
-    int i = row_index();
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int i = blk * BLOCK + threadIdx.x;
+    auto loadA = [&](int j) { return w18_record<ALLFLUID>(a.R.pos, a.v4, a.mat, a.fluid, j); };
+    auto loadB = [&](int j) { return vec3_of(a.v4, j); };
     double pap = 0.0;
     if (i < a.R.nd) {
         float4 pi = a.v4[i];
-        float3 Ap = w18_Av_row<RECOMP, ALLFLUID>(a.R, i, a.fluid, a.v4, a.fac, a.mat, pi);
+        float3 Ap = w18_Av_row<RECOMP, ALLFLUID>(a, i, pi, loadA, loadB);
         a.out[3 * i + 0] = Ap.x; a.out[3 * i + 1] = Ap.y; a.out[3 * i + 2] = Ap.z;
         pap = (double)(pi.x * Ap.x + pi.y * Ap.y + pi.z * Ap.z);
     }
@@ -709,7 +736,7 @@ __global__ __launch_bounds__(BLOCK) void k_advect_boundary(int nd, float dt, flo
     pos[i] = make_float4(x[0], x[1], x[2], p.w);
 }
 
-Rows rows_of(ivp_ctx* c) { return Rows{c->nb, c->tile_len, c->C, c->nd, c->pos, c->h, c->kspiky}; }
+Rows rows_of(ivp_ctx* c) { return Rows{c->nb, c->tile_len, c->Cs, c->nd, c->pos, c->h, c->kspiky}; }
 
 int read_state(ivp_ctx* c) {
     IVP_HIP(hipMemcpyAsync(c->state_host, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));

@@ -161,3 +161,4 @@ def test_deterministic_rerun(cache):
         outs.append(_np(ctx.read(1, torch.zeros((n, 3), device="cuda"))))
         ctx.close()
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+

@@ -77,3 +77,21 @@ def test_mesh_blocks_are_rejected():
     cfg = SimConfig.from_dict({"Configuration": {}, "RigidBodies": [{"objectId": 1, "geometryFile": "x.obj"}]})
     with pytest.raises(NotImplementedError):
         load_particles(cfg)
+
+
+@pytest.mark.parametrize("h", [0.02, 0.08, 0.1, 0.0625, 1.0])
+def test_neighbour_predicate_band(h):
+    """The build decides sqrtf(d2) < h without the square root outside [h2_lo, h2_hi), the
+    bounds computed as in grid_build (h^2 (1 -/+ 2^-20), stepped one float outward).  Check
+    every float32 d2 within h^2 (1 -/+ 2^-17) against the correctly rounded square root."""
+    h = np.float32(h)
+    hh = float(h) * float(h)
+    lo = np.nextafter(np.float32(hh * (1.0 - 2.0 ** -20)), np.float32(0.0))
+    hi = np.nextafter(np.float32(hh * (1.0 + 2.0 ** -20)), np.float32(1.0))
+    a = np.float32(hh * (1.0 - 2.0 ** -17)).view(np.uint32)
+    b = np.float32(hh * (1.0 + 2.0 ** -17)).view(np.uint32)
+    d2 = np.arange(a, b + 1, dtype=np.uint32).view(np.float32)
+    exact = np.sqrt(d2) < h
+    assert np.all(exact[d2 < lo])
+    assert not np.any(exact[d2 >= hi])
+    assert np.count_nonzero((d2 >= lo) & (d2 < hi)) < 64   # the band needing the root is a few ulps
