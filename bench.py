@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--gradient-cache", type=int, default=0,
                     help="1: neighbour rows cache float32 gradW (16 B per pair); 0: ids only, gradW from positions")
+    ap.add_argument("--event-scope", choices=["dominant", "all"], default="dominant",
+                    help="kernel classes timed with dispatch events during the timed steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device-to-device copies of MIB MiB (the reference kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes), then exit")
@@ -121,9 +123,9 @@ def main():
     x0 = ctx.read(0, torch.zeros((n, 3), device=dev))
     v0 = ctx.read(1, torch.zeros((n, 3), device=dev))
 
-    # timed region: events on the dominant kernel only
+    # timed region: events on the dominant kernel (or on every class)
     ctx.profile_reset()
-    ctx.profile_enable(kclass_mask(dominant))
+    ctx.profile_enable(kclass_mask(dominant) if args.event_scope == "dominant" else -1)
     pairs_sum = 0
     cg_iters = 0
     solve_ms = 0.0
@@ -189,7 +191,7 @@ def main():
         "max_final_density_error": max_err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant), "kernel": dominant,
-                     "avg_launch_us": avg_ms * 1e3, "launches": pd["launches"],
+                     "avg_launch_us": avg_ms * 1e3, "launches": pd["launches"], "event_scope": args.event_scope,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
     }
