@@ -212,7 +212,8 @@ int ivp_create(const ivp_config* cfg, int max_particles, void* stream, ivp_ctx**
     c->grid.inv_w = 1.0f / w;
     {
         float h = c->h;
-        c->kspiky = 45.0f / (3.14159265359f * h * h * h * h * h * h);
+        c->kspiky = spiky_constant(h);
+        c->kcubic = cubic_constant(h);
     }
     c->recomp = true;
     c->rec_words = 64;

@@ -21,11 +21,19 @@ struct Grid {
     int dims[3];
 };
 
-// cubic_kernel, scripts/sim/sph_kernel.py:8-26 (same operation order)
-__device__ __forceinline__ float cubic_kernel(float r_norm, float h) {
-    float res = 0.0f;
+// The kernel constants of sph_kernel.py:14-15 and :55, evaluated in float32 with the reference's
+// operation order (once per launch: a division inside a loop is not hoisted on this target).
+__host__ __device__ inline float cubic_constant(float h) {
     float k = 8.0f / 3.14159265359f;
-    k = k / (h * h * h);
+    return k / (h * h * h);
+}
+__host__ __device__ inline float spiky_constant(float h) {
+    return 45.0f / (3.14159265359f * h * h * h * h * h * h);
+}
+
+// cubic_kernel, scripts/sim/sph_kernel.py:8-26 (same operation order); k = cubic_constant(h)
+__device__ __forceinline__ float cubic_kernel(float r_norm, float h, float k) {
+    float res = 0.0f;
     float q = r_norm / h;
     if (q <= 1.0f) {
         if (q <= 0.5f) {
@@ -44,9 +52,9 @@ __device__ __forceinline__ float len3(float x, float y, float z) {
     return sqrtf(x * x + y * y + z * z);
 }
 
-// spiky_kernel_derivative, scripts/sim/sph_kernel.py:52-61 (same operation order)
-__device__ __forceinline__ float3 spiky_grad(float rx, float ry, float rz, float h) {
-    float k = 45.0f / (3.14159265359f * h * h * h * h * h * h);
+// spiky_kernel_derivative, scripts/sim/sph_kernel.py:52-61 (same operation order);
+// k = spiky_constant(h)
+__device__ __forceinline__ float3 spiky_grad(float rx, float ry, float rz, float h, float k) {
     float r_norm = len3(rx, ry, rz);
     if (r_norm < 1e-6f) r_norm = 1e-6f;
     float gx = rx / r_norm, gy = ry / r_norm, gz = rz / r_norm;

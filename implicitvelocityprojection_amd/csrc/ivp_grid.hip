@@ -239,16 +239,33 @@ __device__ __forceinline__ void visit(const Grid& g, const int* __restrict__ sta
 }
 
 // Candidate scan of one population: calls f(j) for every j != self with
-// sqrtf(dx*dx + dy*dy + dz*dz) < h, in the order of `visit`.  The
-// predicate is decided without the square root outside a few-ulp band around h^2:
-// d^2 < h2_lo = h^2 (1 - 2^-20) implies sqrtf(d^2) < h, and d^2 >= h2_hi = h^2 (1 + 2^-20)
-// implies sqrtf(d^2) >= h, so the correctly rounded square root runs only inside the band.
+// sqrtf(dx*dx + dy*dy + dz*dz) < h, in the order of `visit`.  The predicate is decided without
+// the square root outside a few-ulp band around h^2: d^2 < h2_lo = h^2 (1 - 2^-20) implies
+// sqrtf(d^2) < h, and d^2 >= h2_hi = h^2 (1 + 2^-20) implies sqrtf(d^2) >= h.  The correctly
+// rounded root runs only when some lane of the wave has a candidate inside the band (a uniform
+// branch, so the common case carries no root at all).  Candidates are loaded two at a time.
 template <class F>
 __device__ __forceinline__ void scan_ids(const Grid& g, const int* __restrict__ start, const float4* __restrict__ ppos,
                                          int self, float4 pi, float h, float h2_lo, float h2_hi, F&& f) {
     int cx = cell_coord(g, pi.x, 0), cy = cell_coord(g, pi.y, 1), cz = cell_coord(g, pi.z, 2);
     int x0 = cx > 0 ? cx - 1 : 0;
     int x1 = cx < g.dims[0] - 1 ? cx + 1 : g.dims[0] - 1;
+    auto test = [&](int j, float4 pj) {
+        float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        float d2 = rx * rx + ry * ry + rz * rz;
+        bool in = d2 < h2_lo;
+        bool band = !in && d2 < h2_hi;
+        if (__ballot(band)) {
+            if (band) {
+                // the empty volatile asm keeps the root inside this branch (it would otherwise be
+                // evaluated speculatively for every candidate)
+                float d = d2;
+                asm volatile("" : "+v"(d));
+                in = sqrtf(d) < h;
+            }
+        }
+        if (in && j != self) f(j);
+    };
     for (int dz = -1; dz <= 1; ++dz) {
         int zz = cz + dz;
         if (zz < 0 || zz >= g.dims[2]) continue;
@@ -257,13 +274,13 @@ __device__ __forceinline__ void scan_ids(const Grid& g, const int* __restrict__ 
             if (yy < 0 || yy >= g.dims[1]) continue;
             int s0 = start[cell_key(g, x0, yy, zz)];
             int s1 = start[cell_key(g, x1, yy, zz) + 1];
-            for (int j = s0; j < s1; ++j) {
-                float4 pj = ppos[j];
-                float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                float d2 = rx * rx + ry * ry + rz * rz;
-                bool in = d2 < h2_lo;
-                if (!in && d2 < h2_hi) in = sqrtf(d2) < h;
-                if (in && j != self) f(j);
+            for (int j = s0; j < s1; j += 2) {
+                int j1 = j + 1 < s1 ? j + 1 : j;
+                float4 p0 = ppos[j];
+                float4 p1 = ppos[j1];
+                __builtin_amdgcn_sched_barrier(0);   // both loads in flight before the first test
+                test(j, p0);
+                if (j + 1 < s1) test(j1, p1);
             }
         }
     }
@@ -272,6 +289,7 @@ __device__ __forceinline__ void scan_ids(const Grid& g, const int* __restrict__ 
 struct BuildArgs {
     int nd, ns, C, Cs;
     float h, h2_lo, h2_hi;
+    float kc, ks;   // cubic and spiky kernel constants
     Grid g;
     const float4* pos;
     const float* minv;
@@ -329,7 +347,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
     int ts = wave_max(cs);
     int tlp = (tl + ROW_PAD - 1) / ROW_PAD * ROW_PAD;   // <= Cs: tl <= C <= Cs, Cs a multiple of ROW_PAD
     float rho = 0.0f;
-    rho = rho + pi.w * cubic_kernel(0.0f, h);
+    rho = rho + pi.w * cubic_kernel(0.0f, h, a.kc);
     float aii = 0.0f;
     float jx = 0.0f, jy = 0.0f, jz = 0.0f;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
@@ -344,13 +362,13 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
         float4 pj = a.pos[j];
         float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         float rn = len3(rx, ry, rz);
-        float3 gw = spiky_grad(rx, ry, rz, h);
+        float3 gw = spiky_grad(rx, ry, rz, h, a.ks);
         if (!RECOMP) {
             rec[64] = __float_as_uint(gw.x);
             rec[128] = __float_as_uint(gw.y);
             rec[192] = __float_as_uint(gw.z);
         }
-        rho += pj.w * cubic_kernel(rn, h);
+        rho += pj.w * cubic_kernel(rn, h, a.kc);
         float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
         aii += (Jx * Jx + Jy * Jy + Jz * Jz) * a.minv[j];
         jx -= Jx; jy -= Jy; jz -= Jz;
@@ -365,8 +383,8 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
         float4 pj = a.pos[a.nd + j];
         float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         float rn = len3(rx, ry, rz);
-        float3 gw = spiky_grad(rx, ry, rz, h);
-        rho += pj.w * cubic_kernel(rn, h);
+        float3 gw = spiky_grad(rx, ry, rz, h, a.ks);
+        rho += pj.w * cubic_kernel(rn, h, a.kc);
         float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
         jx -= Jx; jy -= Jy; jz -= Jz;
         sx += Jx; sy += Jy; sz += Jz;
@@ -400,7 +418,7 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
 
 struct ExportArgs {
     int n, nd, C;
-    float h;
+    float h, ks;
     Grid g;
     const float4* pos;
     const int* dstart;
@@ -423,7 +441,7 @@ __global__ __launch_bounds__(BLOCK) void k_export(ExportArgs a) {
     bool is_dyn = s < a.nd;
     auto put = [&](int jint, float rx, float ry, float rz) {
         if (cnt < a.C) {
-            float3 gw = spiky_grad(rx, ry, rz, a.h);
+            float3 gw = spiky_grad(rx, ry, rz, a.h, a.ks);
             a.ids[base + cnt] = a.uid[jint];
             a.gradW[3 * (base + cnt) + 0] = gw.x;
             a.gradW[3 * (base + cnt) + 1] = gw.y;
@@ -662,7 +680,7 @@ int grid_build(ivp_ctx* c) {
         const double hh = (double)c->h * (double)c->h;
         float h2_lo = std::nextafter((float)(hh * (1.0 - 1.0 / 1048576.0)), 0.0f);
         float h2_hi = std::nextafter((float)(hh * (1.0 + 1.0 / 1048576.0)), 1.0f);
-        BuildArgs ba{c->nd, c->ns, c->C, c->Cs, c->h, h2_lo, h2_hi, c->grid, c->pos, c->minv, c->dstart, c->sstart, c->nb, c->tile_len,
+        BuildArgs ba{c->nd, c->ns, c->C, c->Cs, c->h, h2_lo, h2_hi, c->kcubic, c->kspiky, c->grid, c->pos, c->minv, c->dstart, c->sstart, c->nb, c->tile_len,
                      c->rowcnt, c->rowtot, c->density, c->kk, c->Ss, c->err_flag, c->partials, c->max_blocks};
         // bytes (pair count of the previous build): the pair records written, the ids read back
         // once, x and m of every particle once, the per-row outputs (rho, k, S, counts)
@@ -683,7 +701,7 @@ int grid_export(ivp_ctx* c, int cache_size, int* count, int* ids, float* gradW) 
     if (!c->built) return IVP_ERR_STATE;
     if (cache_size <= 0 || !count || !ids || !gradW) return IVP_ERR_ARG;
     if (c->n == 0) return IVP_OK;
-    ExportArgs ea{c->n, c->nd, cache_size, c->h, c->grid, c->pos, c->dstart, c->sstart, c->uid, count, ids, gradW};
+    ExportArgs ea{c->n, c->nd, cache_size, c->h, c->kspiky, c->grid, c->pos, c->dstart, c->sstart, c->uid, count, ids, gradW};
     hipLaunchKernelGGL(k_export, dim3(blocks_for(c->n)), dim3(BLOCK), 0, c->stream, ea);
     IVP_HIP(hipGetLastError());
     return IVP_OK;

@@ -110,6 +110,7 @@ struct ivp_ctx {
     bool recomp = true;
     int rec_words = 64;
     float kspiky = 0.0f;    // 45 / (pi h^6), the spiky gradient constant
+    float kcubic = 0.0f;    // 8 / pi / h^3, the cubic kernel constant
 
     // particle store (internal order) and reorder buffers
     float4* pos = nullptr;  float4* pos_b = nullptr;    // x, y, z, m
