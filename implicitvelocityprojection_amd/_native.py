@@ -87,6 +87,7 @@ SIGNATURES = {
     "ivp_profile_enable": (_I, [_P, _I]),
     "ivp_profile_read": (_I, [_P, ctypes.POINTER(KernelTiming), _I]),
     "ivp_profile_reset": (_I, [_P]),
+    "ivp_reset": (_I, [_P]),
     "ivp_counts": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_longlong)]),
 }
 

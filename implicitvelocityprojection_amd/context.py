@@ -120,6 +120,10 @@ class Context:
         N.check(self.lib.ivp_set_particles(self.h, n, _ptr(x), _ptr(v), _ptr(m), _ptr(m_inv), _ptr(density0),
                                            _ptr(is_dynamic), _ptr(material)), "ivp_set_particles")
 
+    def reset(self):
+        """Density back to density0, k and divergence zeroed, neighbour rows dropped."""
+        N.check(self.lib.ivp_reset(self.h), "ivp_reset")
+
     def write(self, field, t):
         N.check(self.lib.ivp_write(self.h, field, _ptr(t)), "ivp_write")
 

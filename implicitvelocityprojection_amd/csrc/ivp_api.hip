@@ -266,6 +266,21 @@ int ivp_set_particles(ivp_ctx* c, int n, const float* x, const float* v, const f
     return ensure_rows(c);
 }
 
+int ivp_reset(ivp_ctx* c) {
+    if (!c) return IVP_ERR_ARG;
+    if (!c->loaded) return IVP_ERR_STATE;
+    if (c->n > 0) {
+        const size_t n = (size_t)c->n;
+        IVP_HIP(hipMemcpyAsync(c->density, c->rho0, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        IVP_HIP(hipMemsetAsync(c->kk, 0, n * sizeof(float), c->stream));
+        IVP_HIP(hipMemsetAsync(c->divg, 0, n * sizeof(float), c->stream));
+        IVP_HIP(hipMemsetAsync(c->divg_b, 0, n * sizeof(float), c->stream));
+        IVP_HIP(hipStreamSynchronize(c->stream));
+    }
+    c->built = false;
+    return IVP_OK;
+}
+
 int ivp_set_option(ivp_ctx* c, int option, int value) {
     if (!c) return IVP_ERR_ARG;
     if (option == IVP_OPT_GRADIENT_CACHE) {

@@ -111,6 +111,13 @@ int ivp_destroy(ivp_ctx *ctx);
 int ivp_set_particles(ivp_ctx *ctx, int n, const float *x, const float *v, const float *m,
                       const float *m_inv, const float *density0, const int *is_dynamic,
                       const int *material);
+/* particle_num growth (ScheduledSpawns, scheduled_spawn.py:68-117) is a new ivp_set_particles
+ * call with the grown set, up to the capacity given to ivp_create. */
+
+/* The solver part of Framework.reset (framework.py:254-270): density back to density0, k
+ * (pressure.k) and divergence zeroed, neighbour rows dropped.  The caller restores x = x0 and
+ * v = v0 with ivp_write and calls ivp_neighbors_build, as Framework.reset re-initialises. */
+int ivp_reset(ivp_ctx *ctx);
 
 /* Options.  IVP_OPT_GRADIENT_CACHE selects the neighbour-row format: 1 caches float32 gradW
  * next to every neighbour id (the reference's neighbor_gradW cache, 16 B per pair and pass);

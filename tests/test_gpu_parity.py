@@ -162,3 +162,29 @@ def test_deterministic_rerun(cache):
         ctx.close()
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 
+
+
+def test_reset_restores_the_initial_build():
+    """Framework.reset's solver part (framework.py:254-270): after ivp_reset density is
+    density0 and k, divergence are 0; restoring x0, v0 and rebuilding gives the initial density
+    and k bitwise."""
+    sim = oracle_at(0, 0)
+    from implicitvelocityprojection_amd.context import IPParams
+    kw = ip_kwargs(sim.cfg)
+    n = sim.x.shape[0]
+    ctx = gpu_context(sim, torch)
+    assert ctx.neighbors_build() == 0
+    d0 = _np(ctx.read(2, torch.zeros(n, device="cuda")))
+    k0 = _np(ctx.read(3, torch.zeros(n, device="cuda")))
+    for _ in range(2):
+        ctx.step(sim.dt, sim.g, 0, 0.0, IPParams(**kw))
+    ctx.reset()
+    assert np.array_equal(_np(ctx.read(2, torch.zeros(n, device="cuda"))), sim.density0)
+    assert np.all(_np(ctx.read(3, torch.zeros(n, device="cuda"))) == 0.0)
+    assert np.all(_np(ctx.read(4, torch.zeros(n, device="cuda"))) == 0.0)
+    ctx.write(0, torch.from_numpy(sim.x).cuda())
+    ctx.write(1, torch.from_numpy(sim.v).cuda())
+    assert ctx.neighbors_build() == 0
+    assert np.array_equal(_np(ctx.read(2, torch.zeros(n, device="cuda"))).view(np.uint32), d0.view(np.uint32))
+    assert np.array_equal(_np(ctx.read(3, torch.zeros(n, device="cuda"))).view(np.uint32), k0.view(np.uint32))
+    ctx.close()
