@@ -59,6 +59,40 @@ def measured_traffic(kernel):
     return None if k is None else k.get("traffic_bytes")
 
 
+def precision_study(ctx, cfg, ps, m, m_inv, prm, steps, torch, dev, n):
+    """Per-step error of the float32 GPU step against the float64 restatement of the same step
+    (oracle/fp64.py), each step starting from the GPU's state; neighbour sets from the float32
+    oracle's build of that state."""
+    from oracle import fp64
+    from oracle import oracle as O
+    grid = O.grid_init(cfg.domain_start, cfg.domain_size, float(np.float32(cfg.support_radius)))
+    h = float(np.float32(cfg.support_radius))
+    C = int(cfg.get_cfg("neighborCacheSize", 60))
+    dyn = ps.is_dynamic == 1
+    ip_kw = dict(omega=prm.omega, tol_opt=prm.tol_opt, max_iteration_opt=prm.max_iteration_opt,
+                 use_fixed_iters=prm.use_fixed_iters, tol_pcg=prm.tol_pcg)
+    rows = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        x = ctx.read(0, torch.zeros((n, 3), device=dev)).cpu().numpy()
+        v = ctx.read(1, torch.zeros((n, 3), device=dev)).cpu().numpy()
+        count, ids, _, _ = O.build_neighbors(grid, x, ps.is_dynamic, h, C)
+        _, x64, v64, st = fp64.step(x, v, m, m_inv, ps.density0, ps.is_dynamic, ps.material, h, count, ids, C,
+                                    cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff,
+                                    cfg.domain_size, ip_kw)
+        res, _ = ctx.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, prm)
+        xg = ctx.read(0, torch.zeros((n, 3), device=dev)).cpu().numpy().astype(np.float64)
+        vg = ctx.read(1, torch.zeros((n, 3), device=dev)).cpu().numpy().astype(np.float64)
+        rows.append({"velocity_rel_l2": float(np.linalg.norm(vg[dyn] - v64[dyn]) / np.linalg.norm(v64[dyn])),
+                     "velocity_rel_max": float(np.abs(vg[dyn] - v64[dyn]).max() / np.abs(v64[dyn]).max()),
+                     "position_max_abs": float(np.abs(xg[dyn] - x64[dyn]).max()),
+                     "density_error_gpu": res.final_density_error,
+                     "density_error_fp64": st["final_density_error"],
+                     "outer_iters_gpu": res.outer_iters, "outer_iters_fp64": st["outer_iters"]})
+    return {"reference": "float64 restatement of the step (CPU), from the GPU state", "dtype_gpu": "f32",
+            "steps": rows, "seconds": time.perf_counter() - t0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +102,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--gradient-cache", type=int, default=0,
                     help="1: neighbour rows cache float32 gradW (16 B per pair); 0: ids only, gradW from positions")
+    ap.add_argument("--precision-steps", type=int, default=1,
+                    help="steps (from the state after the timed region) whose GPU result is compared with the "
+                         "float64 restatement of the step; 0 skips the comparison")
     ap.add_argument("--event-scope", choices=["dominant", "all"], default="dominant",
                     help="kernel classes timed with dispatch events during the timed steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -151,6 +188,10 @@ def main():
     pairs_end = ctx.pair_count()
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
 
+    precision = None
+    if args.precision_steps > 0:
+        precision = precision_study(ctx, cfg, ps, m, m_inv, prm, args.precision_steps, torch, dev, n)
+
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -194,6 +235,7 @@ def main():
                      "avg_launch_us": avg_ms * 1e3, "launches": pd["launches"], "event_scope": args.event_scope,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
+        "precision": precision,
     }
     print(json.dumps(out))
 
