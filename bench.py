@@ -89,8 +89,9 @@ def precision_study(ctx, cfg, ps, m, m_inv, prm, steps, torch, dev, n):
                      "density_error_gpu": res.final_density_error,
                      "density_error_fp64": st["final_density_error"],
                      "outer_iters_gpu": res.outer_iters, "outer_iters_fp64": st["outer_iters"]})
+    gw = ["evaluated in float32", "float32 cache", "float16 cache in the CG operator"][ctx.gradient_cache]
     return {"reference": "float64 restatement of the step (CPU), from the GPU state", "dtype_gpu": "f32",
-            "steps": rows, "seconds": time.perf_counter() - t0}
+            "gradW": gw, "steps": rows, "seconds": time.perf_counter() - t0}
 
 
 def main():
@@ -100,8 +101,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--gradient-cache", type=int, default=0,
-                    help="1: neighbour rows cache float32 gradW (16 B per pair); 0: ids only, gradW from positions")
+    ap.add_argument("--gradient-cache", type=int, default=0, choices=[0, 1, 2],
+                    help="neighbour-row format: 0 ids only, gradW from positions (4 B per pair); 1 float32 gradW "
+                         "cached (16 B per pair); 2 float16 gradW cached for the IP CG operator (12 B per pair) "
+                         "with a float32 true-residual refresh every 8 CG iterations")
     ap.add_argument("--precision-steps", type=int, default=1,
                     help="steps (from the state after the timed region) whose GPU result is compared with the "
                          "float64 restatement of the step; 0 skips the comparison")
@@ -132,7 +135,7 @@ def main():
     dev = torch.device("cuda:0")
     t = lambda a, dt=torch.float32: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)
     ctx = Context(h, cfg.domain_start, cfg.domain_size, ps.n, cache_size=int(cfg.get_cfg("neighborCacheSize", 60)))
-    ctx.gradient_cache = bool(args.gradient_cache)
+    ctx.gradient_cache = args.gradient_cache
     ctx.set_particles(t(ps.x), t(ps.v), t(m), t(m_inv), t(ps.density0), t(ps.is_dynamic, torch.int32),
                       t(ps.material, torch.int32))
     if ctx.neighbors_build() != 0:
@@ -221,7 +224,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "dambreak_1m", "fluid_particles": nd, "static_particles": ns,
-                   "gradient_cache": bool(args.gradient_cache),
+                   "gradient_cache": ["ids", "f32", "f16"][args.gradient_cache],
                    "dt": cfg.dt, "viscosity": "Weiler18 0.01", "densityErrorTol": prm.tol_opt,
                    "maxOuterIters": prm.max_iteration_opt, "neighbour_pairs": pairs_end},
         "cg_iters_per_s": cg_iters / (solve_ms * 1e-3) if solve_ms > 0 else 0.0,

@@ -23,10 +23,11 @@ IVP_ERR_INTERNAL = -6
 
 FIELD_X, FIELD_V, FIELD_DENSITY, FIELD_K, FIELD_DIVERGENCE = 0, 1, 2, 3, 4
 OPT_GRADIENT_CACHE = 1
+ROWS_IDS, ROWS_F32, ROWS_F16 = 0, 1, 2     # values of OPT_GRADIENT_CACHE
 
 # kernel classes of ivp_profile_read / ivp_profile_enable masks
 KCLASSES = ["sort", "build", "ip_pack", "ip_rhs", "cg_J", "cg_JT", "cg_update", "cg_p", "visc", "w18_Av",
-            "w18_update", "w18_p", "w18_pre", "glue"]
+            "w18_update", "w18_p", "w18_pre", "cg_refresh", "glue"]
 
 
 class Config(ctypes.Structure):

@@ -105,14 +105,15 @@ class Context:
 
     @property
     def gradient_cache(self):
-        """True when neighbour rows cache float32 gradW per pair (see include/ivp.h)."""
+        """Neighbour-row format: ROWS_IDS (0, gradW evaluated per pass), ROWS_F32 (1, float32
+        gradW cached per pair) or ROWS_F16 (2, float16 gradW cached for the IP CG operator)."""
         v = ctypes.c_int()
         N.check(self.lib.ivp_get_option(self.h, N.OPT_GRADIENT_CACHE, ctypes.byref(v)))
-        return bool(v.value)
+        return v.value
 
     @gradient_cache.setter
-    def gradient_cache(self, on):
-        N.check(self.lib.ivp_set_option(self.h, N.OPT_GRADIENT_CACHE, int(bool(on))), "ivp_set_option")
+    def gradient_cache(self, fmt):
+        N.check(self.lib.ivp_set_option(self.h, N.OPT_GRADIENT_CACHE, int(fmt)), "ivp_set_option")
 
     # -- particle store ----------------------------------------------------------------------
     def set_particles(self, x, v, m, m_inv, density0, is_dynamic, material):

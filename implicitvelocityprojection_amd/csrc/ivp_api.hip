@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "ivp_internal.h"
 #include "ivp_launch.h"
@@ -39,7 +40,7 @@ void free_all(ivp_ctx* c) {
     dfree(c->rowcnt); dfree(c->rowtot); dfree(c->nb); dfree(c->tile_len); dfree(c->dstart); dfree(c->sstart);
     dfree(c->ccount); dfree(c->key); dfree(c->tmp_src); dfree(c->order); dfree(c->scan_bsum);
     dfree(c->d4); dfree(c->q4); dfree(c->b); dfree(c->sol); dfree(c->r); dfree(c->Ap); dfree(c->p4); dfree(c->y2);
-    dfree(c->cc); dfree(c->sfac); dfree(c->jd); dfree(c->nbact);
+    dfree(c->cc); dfree(c->sfac); dfree(c->jd); dfree(c->xs4); dfree(c->nbact);
     dfree(c->w_pinv); dfree(c->w_p4); dfree(c->w_v4); dfree(c->w_dv); dfree(c->w_r); dfree(c->w_z);
     dfree(c->w_Ap); dfree(c->w_rhs); dfree(c->w_fac); dfree(c->v4);
     dfree(c->partials); dfree(c->state); dfree(c->err_flag);
@@ -74,7 +75,7 @@ int alloc_all(ivp_ctx* c) {
     c->nscan_blocks = (int)((scan_n + 2047) / 2048) + 2;
     A(scan_bsum, (size_t)c->nscan_blocks + 1);
     A(d4, cap); A(q4, cap); A(b, 3 * cap); A(sol, 3 * cap); A(r, 3 * cap); A(Ap, 3 * cap); A(p4, cap); A(y2, cap);
-    A(cc, cap); A(sfac, cap); A(jd, cap); A(nbact, cap);
+    A(cc, cap); A(sfac, cap); A(jd, cap); A(xs4, cap); A(nbact, cap);
     A(w_pinv, 9 * cap); A(w_p4, cap); A(w_v4, cap); A(w_dv, 3 * cap); A(w_r, 3 * cap); A(w_z, 3 * cap);
     A(w_Ap, 3 * cap); A(w_rhs, 3 * cap); A(w_fac, cap); A(v4, cap);
     c->max_blocks = blocks_for(c->cap) + 1;
@@ -171,7 +172,7 @@ int prof_collect(ivp_ctx* c, int cg_iters_done, int epoch) {
 
 static const char* kclass_name(int k) {
     static const char* names[KC_COUNT] = {"sort", "build", "ip_pack", "ip_rhs", "cg_J", "cg_JT", "cg_update", "cg_p",
-                                          "visc", "w18_Av", "w18_update", "w18_p", "w18_pre", "glue"};
+                                          "visc", "w18_Av", "w18_update", "w18_p", "w18_pre", "cg_refresh", "glue"};
     return (k >= 0 && k < KC_COUNT) ? names[k] : "?";
 }
 
@@ -215,8 +216,16 @@ int ivp_create(const ivp_config* cfg, int max_particles, void* stream, ivp_ctx**
         c->kspiky = spiky_constant(h);
         c->kcubic = cubic_constant(h);
     }
-    c->recomp = true;
+    c->row_fmt = IVP_ROWS_IDS;
     c->rec_words = 64;
+    {
+        // power-of-two scale at or above the largest |gradW| = 45/(pi h^6) h^2: float16 records
+        // then hold gradW / scale in [-1, 1] and decode by an exact multiplication
+        int e = 0;
+        double gmax = (double)c->kspiky * (double)c->h * (double)c->h;
+        double m = std::frexp(gmax, &e);   // gmax = m 2^e, m in [0.5, 1)
+        c->gscale = (float)std::ldexp(1.0, m == 0.5 ? e - 1 : e);
+    }
     long long ncell = 1;
     for (int a = 0; a < 3; ++a) {
         c->grid.origin[a] = cfg->domain_start[a] - 2.0f * w;
@@ -284,12 +293,12 @@ int ivp_reset(ivp_ctx* c) {
 int ivp_set_option(ivp_ctx* c, int option, int value) {
     if (!c) return IVP_ERR_ARG;
     if (option == IVP_OPT_GRADIENT_CACHE) {
-        bool recomp = value == 0;
-        if (recomp == c->recomp) return IVP_OK;
+        if (value != IVP_ROWS_IDS && value != IVP_ROWS_F32 && value != IVP_ROWS_F16) return IVP_ERR_ARG;
+        if (value == c->row_fmt) return IVP_OK;
         IVP_HIP(hipStreamSynchronize(c->stream));
         free_rows(c);
-        c->recomp = recomp;
-        c->rec_words = recomp ? 64 : 256;
+        c->row_fmt = value;
+        c->rec_words = value == IVP_ROWS_F32 ? 256 : value == IVP_ROWS_F16 ? 192 : 64;
         c->built = false;
         return c->loaded ? ensure_rows(c) : IVP_OK;
     }
@@ -299,7 +308,7 @@ int ivp_set_option(ivp_ctx* c, int option, int value) {
 int ivp_get_option(ivp_ctx* c, int option, int* value) {
     if (!c || !value) return IVP_ERR_ARG;
     if (option == IVP_OPT_GRADIENT_CACHE) {
-        *value = c->recomp ? 0 : 1;
+        *value = c->row_fmt;
         return IVP_OK;
     }
     return IVP_ERR_ARG;

@@ -63,6 +63,13 @@ __device__ __forceinline__ float3 spiky_grad(float rx, float ry, float rz, float
     return make_float3(s * gx, s * gy, s * gz);
 }
 
+// Two floats rounded to float16 (round to nearest even), packed low | high << 16
+__device__ __forceinline__ uint32_t pack_half2(float lo, float hi) {
+    uint32_t a = __builtin_bit_cast(uint16_t, (_Float16)lo);
+    uint32_t b = __builtin_bit_cast(uint16_t, (_Float16)hi);
+    return a | (b << 16);
+}
+
 // Cell coordinate along one axis: floor((x - origin) * inv_w), clamped into the grid.
 // The cell width w = h * (1 + 2^-10) keeps the 3x3x3 stencil complete under rounding.
 __device__ __forceinline__ int cell_coord(const Grid& g, float x, int a) {

@@ -290,6 +290,7 @@ struct BuildArgs {
     int nd, ns, C, Cs;
     float h, h2_lo, h2_hi;
     float kc, ks;   // cubic and spiky kernel constants
+    float inv_gs;   // 1 / scale of float16 gradW records (a power of two)
     Grid g;
     const float4* pos;
     const float* minv;
@@ -316,9 +317,11 @@ struct BuildArgs {
 //     reference row order: dynamic neighbours, then static neighbours.  Dynamic slots from the
 //     row's count up to the tile length -- the wave's longest row rounded up to ROW_PAD slots
 //     -- are padded with (id = row, gradW = 0).
-template <bool RECOMP>
+// FMT is the row format (include/ivp.h): ids only, ids + float32 gradW, or ids + float16 gradW
+// scaled by 1/inv_gs, packed as (gx, gy) and (gz, 0) half pairs.
+template <int FMT>
 __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
-    constexpr unsigned REC = RECOMP ? 64u : 256u;   // words per (tile, slot) record
+    constexpr unsigned REC = FMT == IVP_ROWS_F32 ? 256u : FMT == IVP_ROWS_F16 ? 192u : 64u;   // words per record
     int blk = xcd_block(blockIdx.x, gridDim.x);
     int i = blk * BLOCK + threadIdx.x;
     int lane = threadIdx.x & (WAVE - 1);
@@ -363,10 +366,13 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
         float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         float rn = len3(rx, ry, rz);
         float3 gw = spiky_grad(rx, ry, rz, h, a.ks);
-        if (!RECOMP) {
+        if (FMT == IVP_ROWS_F32) {
             rec[64] = __float_as_uint(gw.x);
             rec[128] = __float_as_uint(gw.y);
             rec[192] = __float_as_uint(gw.z);
+        } else if (FMT == IVP_ROWS_F16) {
+            rec[64] = pack_half2(gw.x * a.inv_gs, gw.y * a.inv_gs);
+            rec[128] = pack_half2(gw.z * a.inv_gs, 0.0f);
         }
         rho += pj.w * cubic_kernel(rn, h, a.kc);
         float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
@@ -395,10 +401,13 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
         if (!valid || k < cnt) continue;
         uint32_t* rec = tbase + (size_t)k * REC;
         rec[0] = (uint32_t)i;
-        if (!RECOMP) {
+        if (FMT == IVP_ROWS_F32) {
             rec[64] = 0u;
             rec[128] = 0u;
             rec[192] = 0u;
+        } else if (FMT == IVP_ROWS_F16) {
+            rec[64] = 0u;
+            rec[128] = 0u;
         }
     }
     if (valid) {
@@ -680,17 +689,19 @@ int grid_build(ivp_ctx* c) {
         const double hh = (double)c->h * (double)c->h;
         float h2_lo = std::nextafter((float)(hh * (1.0 - 1.0 / 1048576.0)), 0.0f);
         float h2_hi = std::nextafter((float)(hh * (1.0 + 1.0 / 1048576.0)), 1.0f);
-        BuildArgs ba{c->nd, c->ns, c->C, c->Cs, c->h, h2_lo, h2_hi, c->kcubic, c->kspiky, c->grid, c->pos, c->minv, c->dstart, c->sstart, c->nb, c->tile_len,
+        BuildArgs ba{c->nd, c->ns, c->C, c->Cs, c->h, h2_lo, h2_hi, c->kcubic, c->kspiky, 1.0f / c->gscale, c->grid, c->pos, c->minv, c->dstart, c->sstart, c->nb, c->tile_len,
                      c->rowcnt, c->rowtot, c->density, c->kk, c->Ss, c->err_flag, c->partials, c->max_blocks};
         // bytes (pair count of the previous build): the pair records written, the ids read back
         // once, x and m of every particle once, the per-row outputs (rho, k, S, counts)
         double pairs = (double)c->state_host->total_pairs;
-        double bytes = pairs * ((c->recomp ? 4.0 : 16.0) + 4.0) + 16.0 * c->n + 32.0 * c->nd;
+        double bytes = pairs * (4.0 * c->rec_words / 64 + 4.0) + 16.0 * c->n + 32.0 * c->nd;
         const dim3 nb(blocks_for(c->nd));
-        if (c->recomp)
-            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<true>, nb, ba));
+        if (c->row_fmt == IVP_ROWS_F32)
+            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_F32>, nb, ba));
+        else if (c->row_fmt == IVP_ROWS_F16)
+            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_F16>, nb, ba));
         else
-            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<false>, nb, ba));
+            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_IDS>, nb, ba));
         IVP_TRY(finalize<FIN_BUILD>(c, nb.x));
     }
     c->built = true;

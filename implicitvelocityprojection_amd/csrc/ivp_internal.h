@@ -49,6 +49,8 @@ struct SolverState {
     double tol;         // CG stop threshold on r.r
     double rr_last;     // r.r at the last completed iteration
     double rr0;         // r.r before the first iteration
+    double rr_best;     // smallest true r.r so far (r0.r0, then residual replacements)
+    int stalls;         // consecutive residual replacements that did not lower rr_best
     int iter;           // CG iterations completed
     int max_iter;
     int active;         // 1 while CG continues
@@ -76,6 +78,7 @@ enum KernelClass {
     KC_W18_UPDATE,
     KC_W18_P,
     KC_W18_PRE,
+    KC_CG_REFRESH,
     KC_GLUE,
     KC_COUNT
 };
@@ -105,10 +108,13 @@ struct ivp_ctx {
     bool loaded = false;
     bool built = false;
     bool all_dyn_fluid = true;
-    // neighbour-row format: recomp = rows hold ids only and the operator passes evaluate gradW
-    // from positions; otherwise rows cache float32 gradW next to each id (rec_words per slot)
-    bool recomp = true;
+    // neighbour-row format (IVP_ROWS_*): ids only, the operator passes evaluating gradW from
+    // positions; or ids with cached float32 / float16 gradW.  rec_words = words per (tile, slot)
+    // record; gscale = the float16 records' scale, the power of two at or above 45/(pi h^6) h^2,
+    // the largest |gradW|.
+    int row_fmt = IVP_ROWS_IDS;
     int rec_words = 64;
+    float gscale = 1.0f;
     float kspiky = 0.0f;    // 45 / (pi h^6), the spiky gradient constant
     float kcubic = 0.0f;    // 8 / pi / h^3, the cubic kernel constant
 
@@ -150,6 +156,7 @@ struct ivp_ctx {
     float* cc = nullptr;    // constraint c = rho - rho0
     float* sfac = nullptr;  // dfdt * k
     float* jd = nullptr;    // J d of the last outer iteration
+    float4* xs4 = nullptr;  // (x, m) of the CG iterate, input of the true-residual passes
     int* nbact = nullptr;   // row has an active particle in its closed neighbourhood
 
     // Weiler18 vectors

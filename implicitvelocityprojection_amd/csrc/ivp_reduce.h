@@ -19,7 +19,10 @@ __device__ __forceinline__ void block_partials(const double (&v)[NV], double* pa
     }
 }
 
-enum FinOp { FIN_BUILD, FIN_PACK, FIN_RHS, FIN_CG_INIT, FIN_CG_PAP, FIN_CG_UPDATE, FIN_W18_INIT, FIN_W18_PAP, FIN_W18_UPDATE };
+enum FinOp {
+    FIN_BUILD, FIN_PACK, FIN_RHS, FIN_CG_INIT, FIN_CG_PAP, FIN_CG_UPDATE, FIN_W18_INIT, FIN_W18_PAP, FIN_W18_UPDATE,
+    FIN_CG_REPLACE
+};
 
 struct FinArgs {
     const double* partials;
@@ -76,9 +79,9 @@ __device__ __forceinline__ void fold_partials(const double* __restrict__ partial
 // state (the scalar host logic of ip.py:296-327, pressure_base.py:405-423, weiler18.py:330-365).
 template <int OP>
 __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(FinArgs a) {
-    constexpr int NV = OP == FIN_PACK ? 3 : (OP == FIN_RHS || OP == FIN_W18_UPDATE) ? 2 : 1;
+    constexpr int NV = OP == FIN_PACK ? 3 : (OP == FIN_RHS || OP == FIN_W18_UPDATE || OP == FIN_CG_REPLACE) ? 2 : 1;
     SolverState* st = a.st;
-    if (OP == FIN_CG_PAP || OP == FIN_CG_UPDATE || OP == FIN_W18_PAP || OP == FIN_W18_UPDATE) {
+    if (OP == FIN_CG_PAP || OP == FIN_CG_UPDATE || OP == FIN_W18_PAP || OP == FIN_W18_UPDATE || OP == FIN_CG_REPLACE) {
         if (st->active == 0) return;
     }
     double t[3] = {0.0, 0.0, 0.0};
@@ -100,6 +103,8 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(FinArgs a) {
         st->rr_new = t[0];
         st->rr_last = t[0];
         st->rr0 = t[0];
+        st->rr_best = t[0];
+        st->stalls = 0;
         st->tol = a.tol;
         st->max_iter = a.max_iter;
         st->iter = 0;
@@ -135,6 +140,30 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(FinArgs a) {
             double ro = st->rr_old;
             st->beta = t[0] / (ro > 1e-30 ? ro : 1e-30);
             st->rr_old = t[0];
+        }
+    } else if (OP == FIN_CG_REPLACE) {
+        // CG iteration whose recursive residual (t[0] = its r.r) was replaced by the true
+        // residual b - A x (t[1] = its r.r).  The stop test reads the true r.r; beta keeps the
+        // recursion's ratio, so a true residual far above the recursive one (an operator error
+        // floor) turns p into about r instead of amplifying p.  Two replacements in a row that
+        // do not lower the smallest true r.r end the solve: that floor has been reached.
+        int it = st->iter + 1;
+        st->iter = it;
+        st->rr_new = t[1];
+        st->rr_last = t[1];
+        if (t[1] < st->rr_best) {
+            st->rr_best = t[1];
+            st->stalls = 0;
+        } else {
+            st->stalls = st->stalls + 1;
+        }
+        bool stop = (t[1] < st->tol && it >= 2) || it >= st->max_iter || st->stalls >= 2;
+        if (stop) {
+            st->active = 0;
+        } else {
+            double ro = st->rr_old;
+            st->beta = t[0] / (ro > 1e-30 ? ro : 1e-30);
+            st->rr_old = t[1];
         }
     } else if (OP == FIN_W18_UPDATE) {
         // weiler18.py:350-365 (t[0] = r.r, t[1] = r.z)

@@ -39,6 +39,28 @@ using namespace ivp;
 
 namespace {
 
+// Row format seen by one pass: REC words per (tile, slot) record and the source of gradW --
+// evaluated from positions (GS_EVAL), or read from the record as float32 (GS_F32) or as two
+// packed float16 words scaled by Rows::gs (GS_F16).
+enum { GS_EVAL = 0, GS_F32 = 1, GS_F16 = 2 };
+template <unsigned REC_, int GS_>
+struct Fmt {
+    static constexpr unsigned REC = REC_;
+    static constexpr int GS = GS_;
+    static constexpr bool EVAL = GS_ == GS_EVAL;
+};
+using FmtIds = Fmt<64u, GS_EVAL>;
+using FmtF32 = Fmt<256u, GS_F32>;
+using FmtF16 = Fmt<192u, GS_F16>;
+using FmtF16Eval = Fmt<192u, GS_EVAL>;   // FP16 rows, gradW evaluated (the exact operator)
+
+__device__ __forceinline__ float half_lo(uint32_t w) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+}
+__device__ __forceinline__ float half_hi(uint32_t w) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+}
+
 struct Rows {
     const uint32_t* nb;
     const int* tile_len;   // per tile: slots to walk, a multiple of ROW_PAD, padded with the row itself
@@ -47,6 +69,7 @@ struct Rows {
     const float4* pos;   // particle positions (x, y, z, m), for evaluated gradients
     float h;
     float ks;            // 45 / (pi h^6)
+    float gs;            // scale of float16 gradW records
 };
 
 // gradW_ij = s * r_ij in the evaluated form; returns s
@@ -66,9 +89,9 @@ struct Vec {
 // Walk the dynamic slots of row i in order, calling f(A_j, B_j, gx, gy, gz) with A_j = loadA(j) =
 // (x, y, z, a_j) and B_j = loadB(j).  The tile length is a multiple of ROW_PAD, and slots past a
 // row's count name the row itself, whose gradW is 0.
-template <bool RECOMP, int NB, class LA, class LB, class F>
+template <class FM, int NB, class LA, class LB, class F>
 __device__ __forceinline__ void row_pairs(const Rows& R, int i, float4 xi, LA&& loadA, LB&& loadB, F&& f) {
-    constexpr unsigned REC = RECOMP ? 64u : 256u;
+    constexpr unsigned REC = FM::REC;
     const uint32_t* e = R.nb + (size_t)(i >> 6) * (size_t)R.Cs * REC + (threadIdx.x & (WAVE - 1));
     const int tl = __builtin_amdgcn_readfirstlane(R.tile_len[i >> 6]);   // one tile per wave
 #pragma unroll 4
@@ -78,14 +101,17 @@ __device__ __forceinline__ void row_pairs(const Rows& R, int i, float4 xi, LA&& 
         float4 Aj = loadA(j);
         Vec<NB> Bj = loadB(j);
         float gx, gy, gz;
-        if (RECOMP) {
+        if (FM::GS == GS_EVAL) {
             float rx = xi.x - Aj.x, ry = xi.y - Aj.y, rz = xi.z - Aj.z;
             float sc = grad_scale(R.h, R.ks, rx, ry, rz);
             gx = sc * rx; gy = sc * ry; gz = sc * rz;
-        } else {
+        } else if (FM::GS == GS_F32) {
             gx = __uint_as_float(__builtin_nontemporal_load(r + 64));
             gy = __uint_as_float(__builtin_nontemporal_load(r + 128));
             gz = __uint_as_float(__builtin_nontemporal_load(r + 192));
+        } else {
+            uint32_t w1 = __builtin_nontemporal_load(r + 64), w2 = __builtin_nontemporal_load(r + 128);
+            gx = R.gs * half_lo(w1); gy = R.gs * half_hi(w1); gz = R.gs * half_lo(w2);
         }
         f(Aj, Bj, gx, gy, gz);
     }
@@ -106,9 +132,9 @@ __global__ __launch_bounds__(BLOCK) void k_ip_init(int nd, float dt, const float
 
 // Neighbour record (x, y, z, m) of the J passes: positions are needed only to evaluate gradW;
 // with cached gradients only m_j is read, from the w of the pass vector (u4[j].w = m_j).
-template <bool RECOMP>
+template <bool EVAL>
 __device__ __forceinline__ float4 j_record(const float4* __restrict__ pos, const float4* __restrict__ u4, int j) {
-    return RECOMP ? pos[j] : make_float4(0.0f, 0.0f, 0.0f, u4[j].w);
+    return EVAL ? pos[j] : make_float4(0.0f, 0.0f, 0.0f, u4[j].w);
 }
 
 // xyz of a float4 array entry as pass data
@@ -137,18 +163,18 @@ struct PackArgs {
 // ip_pack_fused (ip.py:124-171) + coef_wise_div + compute_avg_density_error
 // (math_utils.py:76-91, pressure_base.py:187-203, 413-423).  J d in the form of
 // compute_J_x_kernel (pressure_base.py:111-127): sum_j m_j dot(d_i - d_j, gradW_ij) + dot(d_i, S_i).
-template <bool RECOMP>
+template <class FM>
 __global__ __launch_bounds__(BLOCK) void k_ip_pack(PackArgs a) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = blk * BLOCK + threadIdx.x;
     // neighbour record: A = (x, y, z, m), B = d
-    auto loadA = [&](int j) { return j_record<RECOMP>(a.R.pos, a.d4, j); };
+    auto loadA = [&](int j) { return j_record<FM::EVAL>(a.R.pos, a.d4, j); };
     auto loadB = [&](int j) { return vec3_of(a.d4, j); };
     double err = 0.0, nact = 0.0, npairs = 0.0;
     if (i < a.R.nd) {
         float4 di = a.d4[i];
         float acc = 0.0f;
-        row_pairs<RECOMP, 3>(a.R, i, a.R.pos[i], loadA, loadB,
+        row_pairs<FM, 3>(a.R, i, a.R.pos[i], loadA, loadB,
                              [&](const float4& Aj, const Vec<3>& Bj, float gx, float gy, float gz) {
                                  float dx = di.x - Bj.v[0], dy = di.y - Bj.v[1], dz = di.z - Bj.v[2];
                                  acc = fmaf(Aj.w, fmaf(dz, gz, fmaf(dy, gy, dx * gx)), acc);
@@ -188,13 +214,13 @@ struct RhsArgs {
 
 // ip_rhs_fused (ip.py:174-215): b = omega J^T (f k); also marks rows whose CG J^T loop can be
 // non-zero (own dfdt > 0 or a dynamic neighbour with dfdt > 0).
-template <bool RECOMP>
+template <class FM>
 __global__ __launch_bounds__(BLOCK) void k_ip_rhs(RhsArgs a) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = blk * BLOCK + threadIdx.x;
     // neighbour record: A = (x, y, z, m), B = (f k, dfdt k); q4 = (f k, m, dfdt k, 0)
     auto loadA = [&](int j) {
-        return RECOMP ? a.R.pos[j] : make_float4(0.0f, 0.0f, 0.0f, a.q4[j].y);
+        return FM::EVAL ? a.R.pos[j] : make_float4(0.0f, 0.0f, 0.0f, a.q4[j].y);
     };
     auto loadB = [&](int j) {
         float4 q = a.q4[j];
@@ -208,7 +234,7 @@ __global__ __launch_bounds__(BLOCK) void k_ip_rhs(RhsArgs a) {
         float fki = qi.x, mi = qi.y;
         bool act = qi.z != 0.0f;
         float rx = 0.0f, ry = 0.0f, rz = 0.0f;
-        row_pairs<RECOMP, 2>(a.R, i, a.R.pos[i], loadA, loadB,
+        row_pairs<FM, 2>(a.R, i, a.R.pos[i], loadA, loadB,
                              [&](const float4& Aj, const Vec<2>& Bj, float gx, float gy, float gz) {
                                  float sc = fmaf(Aj.w, fki, mi * Bj.v[0]);
                                  rx = fmaf(sc, gx, rx); ry = fmaf(sc, gy, ry); rz = fmaf(sc, gz, rz);
@@ -270,7 +296,7 @@ struct CGJArgs {
 
 // compute_Jx_pcg (ip.py:19-60): y_i = dfdt_i k_i (J p)_i on rows with dfdt > 0, J p in the form
 // sum_j dot(p_i - p_j, m_j gradW_ij) + dot(p_i, S_i) (ip.py:51-58)
-template <bool RECOMP>
+template <class FM>
 __global__ __launch_bounds__(BLOCK) void k_cg_J(CGJArgs a) {
     if (a.st->active == 0) return;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -279,11 +305,11 @@ __global__ __launch_bounds__(BLOCK) void k_cg_J(CGJArgs a) {
     const float sk = a.sfac[i];
     if (sk == 0.0f) return;
     // neighbour record: A = (x, y, z, m), B = p
-    auto loadA = [&](int j) { return j_record<RECOMP>(a.R.pos, a.p4, j); };
+    auto loadA = [&](int j) { return j_record<FM::EVAL>(a.R.pos, a.p4, j); };
     auto loadB = [&](int j) { return vec3_of(a.p4, j); };
     float4 pi = a.p4[i];
     float acc = 0.0f;
-    row_pairs<RECOMP, 3>(a.R, i, a.R.pos[i], loadA, loadB,
+    row_pairs<FM, 3>(a.R, i, a.R.pos[i], loadA, loadB,
                          [&](const float4& Aj, const Vec<3>& Bj, float gx, float gy, float gz) {
                              float dx = pi.x - Bj.v[0], dy = pi.y - Bj.v[1], dz = pi.z - Bj.v[2];
                              acc = fmaf(Aj.w, fmaf(dz, gz, fmaf(dy, gy, dx * gx)), acc);
@@ -307,7 +333,7 @@ struct CGJTArgs {
 
 // compute_Ax_part1 + compute_Ax_part2 (ip.py:63-121) in gather form, fused with p.Ap
 // (ip.py:307)
-template <bool RECOMP>
+template <class FM>
 __global__ __launch_bounds__(BLOCK) void k_cg_JT(CGJTArgs a) {
     if (a.st->active == 0) return;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -315,7 +341,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_JT(CGJTArgs a) {
     const bool touched = i < a.R.nd && a.nbact[i] != 0;
     // neighbour record: A = (x, y, z, m), B = y; y2 = (y, m)
     auto loadA = [&](int j) {
-        return RECOMP ? a.R.pos[j] : make_float4(0.0f, 0.0f, 0.0f, a.y2[j].y);
+        return FM::EVAL ? a.R.pos[j] : make_float4(0.0f, 0.0f, 0.0f, a.y2[j].y);
     };
     auto loadB = [&](int j) {
         Vec<1> B;
@@ -329,7 +355,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_JT(CGJTArgs a) {
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         if (touched) {
             float yi = a.y2[i].x;
-            row_pairs<RECOMP, 1>(a.R, i, a.R.pos[i], loadA, loadB,
+            row_pairs<FM, 1>(a.R, i, a.R.pos[i], loadA, loadB,
                                  [&](const float4& Aj, const Vec<1>& Bj, float gx, float gy, float gz) {
                                      float sc = fmaf(Aj.w, yi, mi * Bj.v[0]);
                                      ax = fmaf(sc, gx, ax); ay = fmaf(sc, gy, ay); az = fmaf(sc, gz, az);
@@ -395,6 +421,34 @@ __global__ __launch_bounds__(BLOCK) void k_cg_p(int nd, const float* __restrict_
     p4[i] = p;
 }
 
+// (x, m) of the CG iterate, the input of the true-residual passes
+__global__ __launch_bounds__(BLOCK) void k_x4(int nd, const float* __restrict__ sol, const float4* __restrict__ pos,
+                                              float4* __restrict__ xs4, const SolverState* __restrict__ st) {
+    if (st->active == 0) return;
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nd) return;
+    xs4[i] = make_float4(sol[3 * i + 0], sol[3 * i + 1], sol[3 * i + 2], pos[i].w);
+}
+
+// Residual replacement: r = b - A x with A applied in float32 from positions, and r.r
+__global__ __launch_bounds__(BLOCK) void k_cg_refresh(int nd, const float* __restrict__ b,
+                                                      const float* __restrict__ Ax, float* __restrict__ r,
+                                                      double* __restrict__ partials, int stride,
+                                                      const SolverState* __restrict__ st) {
+    if (st->active == 0) return;
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    double rr = 0.0;
+    if (i < nd) {
+        float r0 = b[3 * i + 0] - Ax[3 * i + 0];
+        float r1 = b[3 * i + 1] - Ax[3 * i + 1];
+        float r2 = b[3 * i + 2] - Ax[3 * i + 2];
+        r[3 * i + 0] = r0; r[3 * i + 1] = r1; r[3 * i + 2] = r2;
+        rr = (double)(r0 * r0 + r1 * r1 + r2 * r2);
+    }
+    double v[1] = {rr};
+    block_partials<1>(v, partials, stride);
+}
+
 // d -= sol (add_vec3 with scale -1, ip.py:389)
 __global__ __launch_bounds__(BLOCK) void k_ip_dupdate(int nd, float4* __restrict__ d4, const float* __restrict__ sol) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
@@ -430,7 +484,7 @@ __global__ __launch_bounds__(BLOCK) void k_v4(int nd, const float* __restrict__ 
 
 // viscosity.py:62-108 with Jacobi reads.  Neighbour record: A = (x, y, z, m_j), m_j = 0 for a
 // non-fluid neighbour (its term is then exactly 0, as the reference's skip), B = (v_j, rho_j).
-template <bool RECOMP, bool ALLFLUID>
+template <class FM, bool ALLFLUID>
 __global__ __launch_bounds__(BLOCK) void k_visc(Rows R, float coeff, float dt, int fluid,
                                                 const float4* __restrict__ v4, const float* __restrict__ rho0,
                                                 const int* __restrict__ mat, float* __restrict__ vel) {
@@ -455,7 +509,7 @@ __global__ __launch_bounds__(BLOCK) void k_visc(Rows R, float coeff, float dt, i
     float eps2 = (0.01f * R.h) * (0.01f * R.h);
     float k_visc = 2.0f * (3.0f + 2.0f);
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    row_pairs<RECOMP, 4>(R, i, xi, loadA, loadB,
+    row_pairs<FM, 4>(R, i, xi, loadA, loadB,
                          [&](const float4& xj, const Vec<4>& vj, float gx, float gy, float gz) {
                              float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
                              float rn = len3(rx, ry, rz);
@@ -520,7 +574,7 @@ __device__ __forceinline__ float4 w18_record(const float4* __restrict__ pos, con
 }
 
 // compute_precond_weiler18 (weiler18.py:21-85): inverse of I - factor_i L_ii
-template <bool RECOMP, bool ALLFLUID>
+template <class FM, bool ALLFLUID>
 __global__ __launch_bounds__(BLOCK) void k_w18_pre(Rows R, int fluid, const float4* __restrict__ v4,
                                                    const float* __restrict__ fac, const int* __restrict__ mat,
                                                    float* __restrict__ pinv) {
@@ -539,7 +593,7 @@ __global__ __launch_bounds__(BLOCK) void k_w18_pre(Rows R, int fluid, const floa
     float dd = 10.0f;
     float L[9];
     for (int q = 0; q < 9; ++q) L[q] = 0.0f;
-    row_pairs<RECOMP, 0>(R, i, xi, loadA, loadB,
+    row_pairs<FM, 0>(R, i, xi, loadA, loadB,
                          [&](const float4& xj, const Vec<0>&, float gx, float gy, float gz) {
                              float x[3] = {xi.x - xj.x, xi.y - xj.y, xi.z - xj.z};
                              float r_sq = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
@@ -576,14 +630,14 @@ struct W18Args {
 
 // (A u)_i = u_i - factor_i * 10 sum_fluid (m_j/rho_j) (u_ij . x_ij) / (r^2 + 0.01 h^2) gradW_ij
 // (compute_Av_weiler18, weiler18.py:103-172).  The caller stages the block's window.
-template <bool RECOMP, bool ALLFLUID, class LA, class LB>
+template <class FM, bool ALLFLUID, class LA, class LB>
 __device__ __forceinline__ float3 w18_Av_row(const W18Args& a, int i, float4 ui, LA&& loadA, LB&& loadB) {
     if (!ALLFLUID && a.mat[i] != a.fluid) return make_float3(ui.x, ui.y, ui.z);
     float4 xi = a.R.pos[i];
     float lx = 0.0f, ly = 0.0f, lz = 0.0f;
     float eps2 = 0.01f * a.R.h * a.R.h;
     float dd = 10.0f;
-    row_pairs<RECOMP, 3>(a.R, i, xi, loadA, loadB,
+    row_pairs<FM, 3>(a.R, i, xi, loadA, loadB,
                          [&](const float4& xj, const Vec<3>& uj, float gx, float gy, float gz) {
                              float x0 = xi.x - xj.x, x1 = xi.y - xj.y, x2 = xi.z - xj.z;
                              float v0 = ui.x - uj.v[0], v1 = ui.y - uj.v[1], v2 = ui.z - uj.v[2];
@@ -597,7 +651,7 @@ __device__ __forceinline__ float3 w18_Av_row(const W18Args& a, int i, float4 ui,
 }
 
 // rhs = v - A v (weiler18.py:390-393)
-template <bool RECOMP, bool ALLFLUID>
+template <class FM, bool ALLFLUID>
 __global__ __launch_bounds__(BLOCK) void k_w18_rhs(W18Args a) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = blk * BLOCK + threadIdx.x;
@@ -605,7 +659,7 @@ __global__ __launch_bounds__(BLOCK) void k_w18_rhs(W18Args a) {
     auto loadB = [&](int j) { return vec3_of(a.v4, j); };
     if (i >= a.R.nd) return;
     float4 vi = a.v4[i];
-    float3 Av = w18_Av_row<RECOMP, ALLFLUID>(a, i, vi, loadA, loadB);
+    float3 Av = w18_Av_row<FM, ALLFLUID>(a, i, vi, loadA, loadB);
     a.out[3 * i + 0] = vi.x + -1.0f * Av.x;
     a.out[3 * i + 1] = vi.y + -1.0f * Av.y;
     a.out[3 * i + 2] = vi.z + -1.0f * Av.z;
@@ -632,7 +686,7 @@ __global__ __launch_bounds__(BLOCK) void k_w18_init(W18Args a, float4* __restric
 }
 
 // Ap = A p, p.Ap (weiler18.py:337-338)
-template <bool RECOMP, bool ALLFLUID>
+template <class FM, bool ALLFLUID>
 __global__ __launch_bounds__(BLOCK) void k_w18_Av(W18Args a) {
     if (a.st->active == 0) return;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -642,7 +696,7 @@ __global__ __launch_bounds__(BLOCK) void k_w18_Av(W18Args a) {
     double pap = 0.0;
     if (i < a.R.nd) {
         float4 pi = a.v4[i];
-        float3 Ap = w18_Av_row<RECOMP, ALLFLUID>(a, i, pi, loadA, loadB);
+        float3 Ap = w18_Av_row<FM, ALLFLUID>(a, i, pi, loadA, loadB);
         a.out[3 * i + 0] = Ap.x; a.out[3 * i + 1] = Ap.y; a.out[3 * i + 2] = Ap.z;
         pap = (double)(pi.x * Ap.x + pi.y * Ap.y + pi.z * Ap.z);
     }
@@ -736,7 +790,7 @@ __global__ __launch_bounds__(BLOCK) void k_advect_boundary(int nd, float dt, flo
     pos[i] = make_float4(x[0], x[1], x[2], p.w);
 }
 
-Rows rows_of(ivp_ctx* c) { return Rows{c->nb, c->tile_len, c->Cs, c->nd, c->pos, c->h, c->kspiky}; }
+Rows rows_of(ivp_ctx* c) { return Rows{c->nb, c->tile_len, c->Cs, c->nd, c->pos, c->h, c->kspiky, c->gscale}; }
 
 int read_state(ivp_ctx* c) {
     IVP_HIP(hipMemcpyAsync(c->state_host, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
@@ -771,18 +825,23 @@ int run_chunks(ivp_ctx* c, int max_iter, int guess, L&& launch_it, int* iters, i
     return IVP_OK;
 }
 
-// Bytes one pass over the rows reads from the pair stream: id (4 B), plus 12 B of gradW when
-// gradients are cached.
-double pair_bytes(const ivp_ctx* c) { return c->recomp ? 4.0 : 16.0; }
+// Bytes a pass of format FM reads from the pair stream: the id (4 B), plus 12 B of float32 or
+// 8 B of float16 gradW when it reads cached gradients.
+template <class FM>
+constexpr double pair_bytes() { return FM::GS == GS_EVAL ? 4.0 : FM::GS == GS_F32 ? 16.0 : 12.0; }
 
-template <bool RECOMP>
+// CG iterations between true-residual replacements when the CG operator reads float16 gradW
+constexpr int REFRESH_EVERY = 8;
+
+// FP: format of the outer loop's passes (pack, rhs); FC: format of the CG operator passes.
+template <class FP, class FC>
 int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats) {
     ivp_ip_stats& st = *stats;
     const int nd = c->nd;
     if (nd == 0) return IVP_OK;
     const dim3 nb(blocks_for(nd));
     const int stride = c->max_blocks;
-    const double pb = pair_bytes(c);
+    const double pb = pair_bytes<FP>(), pbc = pair_bytes<FC>();
     Rows R = rows_of(c);
     float4* d4 = c->d4;
     IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_init, nb, nd, dt, (const float*)c->density, (const float*)c->rho0,
@@ -793,7 +852,7 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
     for (;;) {
         PackArgs pa{R, d4, c->Ss, c->cc, c->kk, c->rho0, c->rowcnt, c->q4, c->sfac, c->jd, c->partials, stride};
         double total_pairs = (double)c->state_host->total_pairs;
-        IVP_TRY(launch(c, KC_IP_PACK, -1, pb * total_pairs + 68.0 * nd, k_ip_pack<RECOMP>, nb, pa));
+        IVP_TRY(launch(c, KC_IP_PACK, -1, pb * total_pairs + 68.0 * nd, k_ip_pack<FP>, nb, pa));
         IVP_TRY(finalize<FIN_PACK>(c, nb.x));
         IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         IVP_TRY(read_state(c));
@@ -808,7 +867,7 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
         int act_rows = c->state_host->n_active_rows;
 
         RhsArgs ra{R, c->q4, c->Ss, prm->omega, c->b, c->nbact, c->rowcnt, c->partials, stride};
-        IVP_TRY(launch(c, KC_IP_RHS, -1, pb * total_pairs + 48.0 * nd, k_ip_rhs<RECOMP>, nb, ra));
+        IVP_TRY(launch(c, KC_IP_RHS, -1, pb * total_pairs + 48.0 * nd, k_ip_rhs<FP>, nb, ra));
         IVP_TRY(finalize<FIN_RHS>(c, nb.x));
         CGInitArgs ia{nd, c->b, c->pos, c->sol, c->r, c->p4, c->y2, c->partials, stride};
         IVP_TRY(launch(c, -1, -1, 0.0, k_cg_init, nb, ia));
@@ -822,19 +881,38 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
             int epoch = c->prof_epoch;
             // algorithmic bytes per launch: the pair stream once per traversed pair, per-row
             // arrays once (gathered neighbour values counted once per particle)
-            double bJ = pb * (double)act_pairs + (double)act_rows * (16 + 12 + 8) + (double)nd * 4;
-            double bJT = pb * (double)touch_pairs + (double)touch_rows * (8 + 12) + (double)nd * (16 + 4 + 12);
+            double bJ = pbc * (double)act_pairs + (double)act_rows * (16 + 12 + 8) + (double)nd * 4;
+            double bJT = pbc * (double)touch_pairs + (double)touch_rows * (8 + 12) + (double)nd * (16 + 4 + 12);
             double bU = (double)nd * (12 + 12 + 16 + 12 + 12 + 12);
             double bP = (double)nd * (12 + 16 + 16);
             CGJArgs ja{R, c->p4, c->Ss, c->sfac, c->y2, c->state};
             CGJTArgs ta{R, c->p4, c->y2, c->Ss, c->nbact, prm->omega, c->Ap, c->partials, stride, c->state};
             CGUpdArgs ua{nd, c->sol, c->r, c->p4, c->Ap, c->partials, stride, c->state};
+            // true-residual passes: the same J / J^T kernels on (x, m), gradW evaluated
+            using FE = Fmt<FC::REC, GS_EVAL>;
+            CGJArgs jx{R, c->xs4, c->Ss, c->sfac, c->y2, c->state};
+            // (its x.Ax partials go to row 2, clear of the update's r.r in row 0)
+            CGJTArgs tx{R, c->xs4, c->y2, c->Ss, c->nbact, prm->omega, c->Ap, c->partials + 2 * stride, stride,
+                        c->state};
             auto one_iter = [&](int tag) -> int {
-                IVP_TRY(launch(c, KC_CG_J, tag, bJ, k_cg_J<RECOMP>, nb, ja));
-                IVP_TRY(launch(c, KC_CG_JT, tag, bJT, k_cg_JT<RECOMP>, nb, ta));
+                IVP_TRY(launch(c, KC_CG_J, tag, bJ, k_cg_J<FC>, nb, ja));
+                IVP_TRY(launch(c, KC_CG_JT, tag, bJT, k_cg_JT<FC>, nb, ta));
                 IVP_TRY(finalize<FIN_CG_PAP>(c, nb.x));
                 IVP_TRY(launch(c, KC_CG_UPDATE, tag, bU, k_cg_update, nb, ua));
-                IVP_TRY(finalize<FIN_CG_UPDATE>(c, nb.x));
+                if (FC::GS == GS_F16 && (tag + 1) % REFRESH_EVERY == 0) {
+                    // residual replacement: r = b - A x with A in float32 from the positions;
+                    // its r.r (row 1) and the recursive r.r (row 0) go to the stop test and beta
+                    IVP_TRY(launch(c, KC_CG_REFRESH, tag, 0.0, k_x4, nb, nd, (const float*)c->sol,
+                                   (const float4*)c->pos, c->xs4, (const SolverState*)c->state));
+                    IVP_TRY(launch(c, KC_CG_REFRESH, tag, 0.0, k_cg_J<FE>, nb, jx));
+                    IVP_TRY(launch(c, KC_CG_REFRESH, tag, 0.0, k_cg_JT<FE>, nb, tx));
+                    IVP_TRY(launch(c, KC_CG_REFRESH, tag, 0.0, k_cg_refresh, nb, nd, (const float*)c->b,
+                                   (const float*)c->Ap, c->r, c->partials + stride, stride,
+                                   (const SolverState*)c->state));
+                    IVP_TRY(finalize<FIN_CG_REPLACE>(c, nb.x));
+                } else {
+                    IVP_TRY(finalize<FIN_CG_UPDATE>(c, nb.x));
+                }
                 return launch(c, KC_CG_P, tag, bP, k_cg_p, nb, nd, (const float*)c->r, c->p4,
                               (const SolverState*)c->state);
             };
@@ -864,21 +942,21 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
                   c->divg);
 }
 
-template <bool RECOMP>
+template <class FM>
 int visc_explicit_body(ivp_ctx* c, float coeff, float dt) {
     const int nd = c->nd;
     const dim3 nb(blocks_for(nd));
     IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_v4, nb, nd, (const float*)c->vel, (const float*)c->density, c->v4));
-    double bytes = pair_bytes(c) * (double)c->state_host->total_pairs + 48.0 * nd;
+    double bytes = pair_bytes<FM>() * (double)c->state_host->total_pairs + 48.0 * nd;
     Rows R = rows_of(c);
     if (c->all_dyn_fluid)
-        return launch(c, KC_VISC, -1, bytes, k_visc<RECOMP, true>, nb, R, coeff, dt, c->cfg.material_fluid,
+        return launch(c, KC_VISC, -1, bytes, k_visc<FM, true>, nb, R, coeff, dt, c->cfg.material_fluid,
                       (const float4*)c->v4, (const float*)c->rho0, (const int*)c->mat, c->vel);
-    return launch(c, KC_VISC, -1, bytes, k_visc<RECOMP, false>, nb, R, coeff, dt, c->cfg.material_fluid,
+    return launch(c, KC_VISC, -1, bytes, k_visc<FM, false>, nb, R, coeff, dt, c->cfg.material_fluid,
                   (const float4*)c->v4, (const float*)c->rho0, (const int*)c->mat, c->vel);
 }
 
-template <bool RECOMP, bool ALLFLUID>
+template <class FM, bool ALLFLUID>
 int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
     const int nd = c->nd;
     const dim3 nb(blocks_for(nd));
@@ -887,13 +965,13 @@ int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
     IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_w18_prep, nb, nd, coeff, dt, (const float*)c->vel, (const float4*)c->pos,
                    (const float*)c->density, (const float*)c->rho0, c->w_v4, c->w_fac));
     const double pairs = (double)c->state_host->total_pairs;
-    const double pb = pair_bytes(c);
-    IVP_TRY(launch(c, KC_W18_PRE, -1, pb * pairs + 56.0 * nd, k_w18_pre<RECOMP, ALLFLUID>, nb, R,
+    const double pb = pair_bytes<FM>();
+    IVP_TRY(launch(c, KC_W18_PRE, -1, pb * pairs + 56.0 * nd, k_w18_pre<FM, ALLFLUID>, nb, R,
                    c->cfg.material_fluid, (const float4*)c->w_v4, (const float*)c->w_fac, (const int*)c->mat,
                    c->w_pinv));
     W18Args wa{R, c->cfg.material_fluid, c->w_fac, c->mat, c->w_pinv, c->w_v4, c->w_rhs,
                c->w_dv, c->w_r, c->w_z, c->partials, stride, c->state};
-    IVP_TRY(launch(c, KC_W18_AV, -1, pb * pairs + 48.0 * nd, k_w18_rhs<RECOMP, ALLFLUID>, nb, wa));
+    IVP_TRY(launch(c, KC_W18_AV, -1, pb * pairs + 48.0 * nd, k_w18_rhs<FM, ALLFLUID>, nb, wa));
     IVP_TRY(launch(c, -1, -1, 0.0, k_w18_init, nb, wa, c->w_p4));
     IVP_TRY(finalize<FIN_W18_INIT>(c, nb.x, 1e-5, 1000));
     IVP_TRY(read_state(c));
@@ -904,7 +982,7 @@ int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
         cg.out = c->w_Ap;
         const double bAv = pb * (double)c->state_host->total_pairs + 48.0 * nd;
         auto one_iter = [&](int tag) -> int {
-            IVP_TRY(launch(c, KC_W18_AV, tag, bAv, k_w18_Av<RECOMP, ALLFLUID>, nb, cg));
+            IVP_TRY(launch(c, KC_W18_AV, tag, bAv, k_w18_Av<FM, ALLFLUID>, nb, cg));
             IVP_TRY(finalize<FIN_W18_PAP>(c, nb.x));
             IVP_TRY(launch(c, KC_W18_UPDATE, tag, 124.0 * nd, k_w18_update, nb, cg));
             IVP_TRY(finalize<FIN_W18_UPDATE>(c, nb.x));
@@ -942,7 +1020,12 @@ int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats
     st.rows_traversed = 0;
     st.cg_launch_chunks = 0;
     auto t0 = std::chrono::steady_clock::now();
-    int rc = c->recomp ? ip_solve_body<true>(c, dt, prm, stats) : ip_solve_body<false>(c, dt, prm, stats);
+    int rc;
+    switch (c->row_fmt) {
+    case IVP_ROWS_F32: rc = ip_solve_body<FmtF32, FmtF32>(c, dt, prm, stats); break;
+    case IVP_ROWS_F16: rc = ip_solve_body<FmtF16Eval, FmtF16>(c, dt, prm, stats); break;
+    default: rc = ip_solve_body<FmtIds, FmtIds>(c, dt, prm, stats); break;
+    }
     if (rc >= 0) IVP_HIP(hipStreamSynchronize(c->stream));
     stats->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
@@ -951,7 +1034,11 @@ int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats
 int visc_explicit(ivp_ctx* c, float coeff, float dt) {
     if (!c->built) return IVP_ERR_STATE;
     if (c->nd == 0) return IVP_OK;
-    return c->recomp ? visc_explicit_body<true>(c, coeff, dt) : visc_explicit_body<false>(c, coeff, dt);
+    switch (c->row_fmt) {
+    case IVP_ROWS_F32: return visc_explicit_body<FmtF32>(c, coeff, dt);
+    case IVP_ROWS_F16: return visc_explicit_body<FmtF16Eval>(c, coeff, dt);
+    default: return visc_explicit_body<FmtIds>(c, coeff, dt);
+    }
 }
 
 int visc_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* out) {
@@ -959,12 +1046,20 @@ int visc_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* out) {
     if (out) { out->cg_iters = 0; out->residual_norm2 = 0.0; }
     if (c->nd == 0) return IVP_OK;
     int iters = 0, rc;
-    if (c->recomp)
-        rc = c->all_dyn_fluid ? weiler18_body<true, true>(c, coeff, dt, &iters)
-                              : weiler18_body<true, false>(c, coeff, dt, &iters);
-    else
-        rc = c->all_dyn_fluid ? weiler18_body<false, true>(c, coeff, dt, &iters)
-                              : weiler18_body<false, false>(c, coeff, dt, &iters);
+    switch (c->row_fmt) {
+    case IVP_ROWS_F32:
+        rc = c->all_dyn_fluid ? weiler18_body<FmtF32, true>(c, coeff, dt, &iters)
+                              : weiler18_body<FmtF32, false>(c, coeff, dt, &iters);
+        break;
+    case IVP_ROWS_F16:
+        rc = c->all_dyn_fluid ? weiler18_body<FmtF16Eval, true>(c, coeff, dt, &iters)
+                              : weiler18_body<FmtF16Eval, false>(c, coeff, dt, &iters);
+        break;
+    default:
+        rc = c->all_dyn_fluid ? weiler18_body<FmtIds, true>(c, coeff, dt, &iters)
+                              : weiler18_body<FmtIds, false>(c, coeff, dt, &iters);
+        break;
+    }
     if (rc != IVP_OK) return rc;
     if (out) {
         out->cg_iters = iters;

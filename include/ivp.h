@@ -119,12 +119,22 @@ int ivp_set_particles(ivp_ctx *ctx, int n, const float *x, const float *v, const
  * v = v0 with ivp_write and calls ivp_neighbors_build, as Framework.reset re-initialises. */
 int ivp_reset(ivp_ctx *ctx);
 
-/* Options.  IVP_OPT_GRADIENT_CACHE selects the neighbour-row format: 1 caches float32 gradW
- * next to every neighbour id (the reference's neighbor_gradW cache, 16 B per pair and pass);
- * 0 (default) stores ids only (4 B) and evaluates gradW from the positions in every operator
- * pass.  Density and Aii use the reference's exact expression either way.  Changing the option
+/* Options.  IVP_OPT_GRADIENT_CACHE selects the neighbour-row format:
+ *   IVP_ROWS_IDS (0, default): ids only (4 B per pair and pass); every operator pass evaluates
+ *     gradW from the positions.
+ *   IVP_ROWS_F32 (1): float32 gradW cached next to every id (the reference's neighbor_gradW
+ *     cache, 16 B per pair and pass) and read by every operator pass.
+ *   IVP_ROWS_F16 (2): float16 gradW cached next to every id (12 B per pair and pass) with one
+ *     power-of-two scale per context; the IP CG's J and J^T passes read it, and every 8th CG
+ *     iteration replaces the recursive residual by r = b - A x evaluated in float32 from the
+ *     positions.  The outer-loop passes (constraint, right-hand side) and viscosity evaluate
+ *     gradW from the positions as in IVP_ROWS_IDS.
+ * Density and Aii use the reference's exact expression in every format.  Changing the option
  * drops the neighbour rows: call ivp_neighbors_build again.  No counterpart in the reference. */
 #define IVP_OPT_GRADIENT_CACHE 1
+#define IVP_ROWS_IDS 0
+#define IVP_ROWS_F32 1
+#define IVP_ROWS_F16 2
 int ivp_set_option(ivp_ctx *ctx, int option, int value);
 int ivp_get_option(ivp_ctx *ctx, int option, int *value);
 

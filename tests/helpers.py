@@ -44,9 +44,9 @@ def oracle_at(cfg_id, step, scale=None):
     return sim
 
 
-def gpu_context(sim, torch, gradient_cache=False):
+def gpu_context(sim, torch, gradient_cache=0):
     """A libivp Context loaded with the oracle simulation's particle state.  gradient_cache
-    selects the neighbour-row format (cached float32 gradW or ids only)."""
+    selects the neighbour-row format: 0 ids only, 1 cached float32 gradW, 2 cached float16 gradW."""
     from implicitvelocityprojection_amd.context import Context
     cfg = sim.cfg
     dev = torch.device("cuda")

@@ -1,0 +1,126 @@
+"""The float16 gradient-cache row format (IVP_ROWS_F16) against the oracle and the float64
+restatement of the step.
+
+In this format the IP CG's J and J^T passes read gradW rounded to float16 (one power-of-two
+scale), and every 8th CG iteration replaces the recursive residual by b - A x evaluated in
+float32 from the positions.  Everything else (build, constraint, right-hand side, viscosity)
+is the exact float32 path, bit-for-bit the same as the ids-only format.  Bars:
+
+* same outer-iteration count and CG iteration counts within +-1 of the oracle, final density
+  error at or below densityErrorTol;
+* velocities within 5e-4 relative (max-norm) of the oracle and of the float64 step, the IP
+  velocity correction within 5e-3 relative: float16 rounds gradW to 2^-11 relative, and a
+  solve that ends before its first residual replacement keeps that operator error (measured:
+  1e-4 and 7e-4);
+* with a CG tolerance that needs residual replacements (tol_pcg 1e-16), the correction within
+  1e-4 of the float64 solve: replacement brings the float16 operator's solution to the float32
+  one (measured 2.5e-5), where a solve without it would stop at the 7e-4 operator error;
+* a CG tolerance below float32's attainable residual ends in finite values, through the
+  stall test, not in a blow-up of the search direction.
+"""
+
+import numpy as np
+import pytest
+
+from tests.helpers import gpu_context, ip_kwargs, oracle_at, rel_inf
+from oracle import fp64
+from oracle import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+F16 = 2
+V_TOL16 = 5e-4
+DV_TOL16 = 5e-3
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _ip_gpu(sim, v_in, kw):
+    from implicitvelocityprojection_amd.context import IPParams
+    ctx = gpu_context(sim, torch, F16)
+    assert ctx.gradient_cache == F16
+    assert ctx.neighbors_build() == 0
+    n = sim.x.shape[0]
+    ctx.write(1, torch.from_numpy(v_in).cuda())
+    res = ctx.ip_solve(sim.dt, IPParams(**kw))
+    v = _np(ctx.read(1, torch.zeros((n, 3), device="cuda")))
+    ctx.close()
+    return res, v
+
+
+@pytest.mark.parametrize("step", [0, 5, 20])
+def test_f16_ip_solve_against_oracle(step):
+    sim = oracle_at(0, step)
+    v_in = O.gravity(sim.v, sim.is_dynamic, sim.g, sim.dt)
+    kw = ip_kwargs(sim.cfg)
+    v_o, _, res_o = O.ip_solve(v_in, sim.m, sim.density, sim.density0, sim.is_dynamic, sim.C, sim.count,
+                               sim.ids, sim.gradW, sim.k, sim.dt, **kw)
+    res, v_g = _ip_gpu(sim, v_in, kw)
+    dyn = sim.is_dynamic == 1
+    assert res.status == 0
+    assert res.outer_iters == res_o.outer_iters
+    for a, b in zip(res.pcg_iters, res_o.pcg_iters):
+        assert abs(a - b) <= 1, (res.pcg_iters, res_o.pcg_iters)
+    assert res.final_density_error <= kw["tol_opt"]
+    assert rel_inf(v_g[dyn], v_o[dyn]) <= V_TOL16
+    assert rel_inf(v_g[dyn] - v_in[dyn], v_o[dyn] - v_in[dyn]) <= DV_TOL16
+
+
+@pytest.mark.parametrize("method,coeff", [(0, 0.0), (1, 0.05)], ids=["ip", "weiler18"])
+def test_f16_step_against_fp64(method, coeff):
+    from implicitvelocityprojection_amd.context import IPParams
+    sim = oracle_at(0, 6)
+    kw = ip_kwargs(sim.cfg)
+    _, x64, v64, st = fp64.step(sim.x, sim.v, sim.m, sim.m_inv, sim.density0, sim.is_dynamic, sim.material, sim.h,
+                                sim.count, sim.ids, sim.C, sim.dt, sim.g, method, coeff, sim.cfg.domain_size, kw)
+    ctx = gpu_context(sim, torch, F16)
+    assert ctx.neighbors_build() == 0
+    res, _ = ctx.step(sim.dt, sim.g, method, coeff, IPParams(**kw))
+    n = sim.x.shape[0]
+    x_g = _np(ctx.read(0, torch.zeros((n, 3), device="cuda")))
+    v_g = _np(ctx.read(1, torch.zeros((n, 3), device="cuda")))
+    ctx.close()
+    dyn = sim.is_dynamic == 1
+    assert res.status == 0 and res.outer_iters == st["outer_iters"]
+    assert rel_inf(v_g[dyn], v64[dyn]) <= V_TOL16
+    assert rel_inf(x_g[dyn], x64[dyn]) <= 1e-5
+    assert abs(res.final_density_error - st["final_density_error"]) <= 1e-2 * st["final_density_error"]
+    assert res.final_density_error <= kw["tol_opt"]
+
+
+def _fp64_ip(sim, v_in, kw):
+    s = fp64.Step64(sim.x, sim.m, sim.m_inv, sim.density0, sim.is_dynamic, sim.material, sim.h,
+                    fp64.Pairs(sim.count, sim.ids, sim.C, sim.is_dynamic))
+    return s.ip_solve(v_in.astype(np.float64), sim.dt, **kw)
+
+
+def test_f16_residual_replacement_reaches_float32_accuracy():
+    sim = oracle_at(0, 5)
+    v_in = O.gravity(sim.v, sim.is_dynamic, sim.g, sim.dt)
+    kw = ip_kwargs(sim.cfg)
+    kw.update(tol_pcg=1e-16, max_iteration_pcg=200)
+    v64, st = _fp64_ip(sim, v_in, kw)
+    res, v_g = _ip_gpu(sim, v_in, kw)
+    dyn = sim.is_dynamic == 1
+    assert res.status == 0
+    assert res.outer_iters == st["outer_iters"]
+    assert max(res.pcg_iters) > 8          # at least one replacement ran
+    assert rel_inf(v_g[dyn] - v_in[dyn], v64[dyn] - v_in[dyn]) <= 1e-4
+
+
+def test_f16_unattainable_tolerance_stays_finite():
+    sim = oracle_at(0, 0)
+    v_in = O.gravity(sim.v, sim.is_dynamic, sim.g, sim.dt)
+    kw = ip_kwargs(sim.cfg)
+    kw.update(tol_pcg=1e-16, max_iteration_pcg=200)
+    v64, st = _fp64_ip(sim, v_in, kw)
+    res, v_g = _ip_gpu(sim, v_in, kw)
+    dyn = sim.is_dynamic == 1
+    assert res.status == 0
+    assert np.all(np.isfinite(v_g))
+    assert all(it < 200 for it in res.pcg_iters), res.pcg_iters     # the stall test ended each solve
+    assert res.final_density_error <= kw["tol_opt"]
+    assert rel_inf(v_g[dyn], v64[dyn]) <= V_TOL16
