@@ -24,6 +24,9 @@ IVP_ERR_INTERNAL = -6
 FIELD_X, FIELD_V, FIELD_DENSITY, FIELD_K, FIELD_DIVERGENCE = 0, 1, 2, 3, 4
 OPT_GRADIENT_CACHE = 1
 ROWS_IDS, ROWS_F32, ROWS_F16 = 0, 1, 2     # values of OPT_GRADIENT_CACHE
+OPT_SCENE_CG_CHUNK = 2
+OPT_SCENE_GRAPH = 3
+MIN_SCENE_DYNAMIC = 256
 
 # kernel classes of ivp_profile_read / ivp_profile_enable masks
 KCLASSES = ["sort", "build", "ip_pack", "ip_rhs", "cg_J", "cg_JT", "cg_update", "cg_p", "visc", "w18_Av",
@@ -58,6 +61,16 @@ class StepParams(ctypes.Structure):
                 ("viscosity_coeff", ctypes.c_float), ("ip", IPParams)]
 
 
+class Scene(ctypes.Structure):
+    _fields_ = [("n_particles", ctypes.c_int), ("domain_start", ctypes.c_float * 3),
+                ("domain_size", ctypes.c_float * 3)]
+
+
+class SceneStats(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("outer_iters", ctypes.c_int), ("cg_iters", ctypes.c_int),
+                ("final_density_error", ctypes.c_double)]
+
+
 class KernelTiming(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_longlong), ("total_ms", ctypes.c_double),
                 ("bytes", ctypes.c_double)]
@@ -90,6 +103,10 @@ SIGNATURES = {
     "ivp_profile_reset": (_I, [_P]),
     "ivp_reset": (_I, [_P]),
     "ivp_counts": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_longlong)]),
+    "ivp_set_scenes": (_I, [_P, _I, ctypes.POINTER(Scene)]),
+    "ivp_step_scenes": (_I, [_P, ctypes.POINTER(StepParams), ctypes.POINTER(SceneStats)]),
+    "ivp_scene_history": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I), ctypes.POINTER(_I),
+                               ctypes.POINTER(_I)]),
 }
 
 _lib = None

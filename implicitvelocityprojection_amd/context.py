@@ -187,6 +187,66 @@ class Context:
         status = N.check(self.lib.ivp_step(self.h, ctypes.byref(sp), ctypes.byref(st), ctypes.byref(cs)), "ivp_step")
         return IPResult(status, st, errs, its), cs.cg_iters
 
+    # -- independent scenes (one context, many scenes) ----------------------------------------
+    def set_scenes(self, sizes, domain_starts, domain_sizes):
+        """Declare the next set_particles call's particles as consecutive scenes: sizes[s]
+        particles with domain (domain_starts[s], domain_sizes[s]) each (see include/ivp.h)."""
+        S = len(sizes)
+        arr = (N.Scene * S)()
+        for s in range(S):
+            arr[s].n_particles = int(sizes[s])
+            arr[s].domain_start[:] = [float(v) for v in np.asarray(domain_starts[s], np.float32)]
+            arr[s].domain_size[:] = [float(v) for v in np.asarray(domain_sizes[s], np.float32)]
+        N.check(self.lib.ivp_set_scenes(self.h, S, arr), "ivp_set_scenes")
+        self.n_scenes = S
+
+    def step_scenes(self, dt, gravity, params: IPParams):
+        """One step of every scene (viscosity off).  Returns (status, list of per-scene
+        (status, outer_iters, cg_iters, final_density_error))."""
+        sp = N.StepParams()
+        sp.dt = float(dt)
+        sp.gravity[:] = [float(v) for v in gravity]
+        sp.viscosity_method = 0
+        sp.viscosity_coeff = 0.0
+        sp.ip = params.c()
+        out = (N.SceneStats * self.n_scenes)()
+        status = N.check(self.lib.ivp_step_scenes(self.h, ctypes.byref(sp), out), "ivp_step_scenes")
+        return status, [(o.status, o.outer_iters, o.cg_iters, o.final_density_error) for o in out]
+
+    def scene_history(self, scene, capacity=1026):
+        """(density errors, CG iterations per update) of one scene in the last step_scenes."""
+        errs = np.zeros(capacity, np.float64)
+        its = np.zeros(capacity, np.int32)
+        ne, npcg = ctypes.c_int(), ctypes.c_int()
+        N.check(self.lib.ivp_scene_history(self.h, int(scene), capacity,
+                                           errs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                           its.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(ne),
+                                           ctypes.byref(npcg)), "ivp_scene_history")
+        return [float(e) for e in errs[:min(ne.value, capacity)]], [int(i) for i in its[:min(npcg.value, capacity)]]
+
+    def _opt(self, opt):
+        v = ctypes.c_int()
+        N.check(self.lib.ivp_get_option(self.h, opt, ctypes.byref(v)))
+        return v.value
+
+    @property
+    def scene_cg_chunk(self):
+        """CG iterations per replay of the scene solve's launch sequence."""
+        return self._opt(N.OPT_SCENE_CG_CHUNK)
+
+    @scene_cg_chunk.setter
+    def scene_cg_chunk(self, k):
+        N.check(self.lib.ivp_set_option(self.h, N.OPT_SCENE_CG_CHUNK, int(k)), "ivp_set_option")
+
+    @property
+    def scene_graph(self):
+        """True: the scene solve replays a captured hipGraph; False: it enqueues the launches."""
+        return bool(self._opt(N.OPT_SCENE_GRAPH))
+
+    @scene_graph.setter
+    def scene_graph(self, on):
+        N.check(self.lib.ivp_set_option(self.h, N.OPT_SCENE_GRAPH, int(bool(on))), "ivp_set_option")
+
     # -- instrumentation ----------------------------------------------------------------------
     def profile_enable(self, mask=-1):
         N.check(self.lib.ivp_profile_enable(self.h, int(mask)))

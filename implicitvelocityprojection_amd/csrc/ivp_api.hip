@@ -33,7 +33,29 @@ void dfree(T*& p) {
     p = nullptr;
 }
 
+template <class T>
+void hfree(T*& p) {
+    if (p) (void)hipHostFree((void*)p);
+    p = nullptr;
+}
+
+void scenes_free(ivp_ctx* c) {
+    dfree(c->grids); dfree(c->sid); dfree(c->sdomain); dfree(c->ss); dfree(c->sprm);
+    dfree(c->shist_err); dfree(c->shist_pcg); dfree(c->spartials); dfree(c->sactive);
+    hfree(c->ss_host); hfree(c->sprm_host); hfree(c->sactive_host);
+    for (auto& g : c->sgraphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = SceneGraph{};
+    }
+    if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
+    c->cap_stream = nullptr;
+    c->S = 0;
+    c->scenes.clear();
+    c->grids_host.clear();
+}
+
 void free_all(ivp_ctx* c) {
+    scenes_free(c);
     dfree(c->pos); dfree(c->pos_b); dfree(c->vel); dfree(c->vel_b); dfree(c->minv); dfree(c->minv_b);
     dfree(c->rho0); dfree(c->rho0_b); dfree(c->mat); dfree(c->mat_b); dfree(c->uid); dfree(c->uid_b);
     dfree(c->inv); dfree(c->density); dfree(c->kk); dfree(c->Ss); dfree(c->divg); dfree(c->divg_b);
@@ -105,6 +127,73 @@ int ensure_rows(ivp_ctx* c) {
 
 void free_rows(ivp_ctx* c) {
     dfree(c->nb);
+}
+
+// Independent scenes: one cell grid per scene (same cell width and margins as the single-scene
+// grid), keys consecutive in scene order; cell tables and scene buffers reallocated.
+int scenes_set(ivp_ctx* c, int S, const ivp_scene* sc) {
+    if (S <= 0 || !sc) return IVP_ERR_ARG;
+    long long total = 0;
+    for (int s = 0; s < S; ++s) {
+        if (sc[s].n_particles <= 0) return IVP_ERR_ARG;
+        total += sc[s].n_particles;
+    }
+    if (total > c->cap) return IVP_ERR_CAPACITY;
+    IVP_HIP(hipStreamSynchronize(c->stream));
+    const float w = c->h * 1.0009765625f;
+    std::vector<Grid> g((size_t)S);
+    long long base = 0;
+    for (int s = 0; s < S; ++s) {
+        g[s].inv_w = c->grid.inv_w;
+        g[s].base = (int)base;
+        long long cells_s = 1;
+        for (int a = 0; a < 3; ++a) {
+            g[s].origin[a] = sc[s].domain_start[a] - 2.0f * w;
+            float cells = ceilf(sc[s].domain_size[a] / w);
+            if (!(cells >= 0.0f) || cells > 2044.0f) return IVP_ERR_GRID;
+            g[s].dims[a] = (int)cells + 4;
+            cells_s *= g[s].dims[a];
+        }
+        base += cells_s;
+        if (base > (1ll << 28)) return IVP_ERR_GRID;
+    }
+    scenes_free(c);
+    free_rows(c);
+    c->loaded = false;
+    c->built = false;
+    const size_t cap = (size_t)c->cap;
+    dfree(c->dstart); dfree(c->sstart); dfree(c->ccount); dfree(c->scan_bsum);
+    c->ncell = (int)base;
+    size_t scan_n = std::max((size_t)c->ncell, cap) + 1;
+    c->nscan_blocks = (int)((scan_n + 2047) / 2048) + 2;
+    int rc = IVP_OK;
+#define A(p, n) if ((rc = dalloc(&c->p, (n))) != IVP_OK) return rc
+    A(dstart, (size_t)c->ncell + 1); A(sstart, (size_t)c->ncell + 1); A(ccount, (size_t)c->ncell);
+    A(scan_bsum, (size_t)c->nscan_blocks + 1);
+    A(grids, (size_t)S); A(sid, cap); A(sdomain, 3 * (size_t)S); A(ss, (size_t)S); A(sprm, 1);
+    A(shist_err, (size_t)S * SCENE_HIST); A(shist_pcg, (size_t)S * SCENE_HIST);
+    A(spartials, 8 * (size_t)c->max_blocks); A(sactive, 1);
+#undef A
+    IVP_HIP(hipHostMalloc((void**)&c->ss_host, sizeof(SceneState) * (size_t)S, hipHostMallocDefault));
+    IVP_HIP(hipHostMalloc((void**)&c->sprm_host, sizeof(SceneParams), hipHostMallocDefault));
+    IVP_HIP(hipHostMalloc((void**)&c->sactive_host, sizeof(int), hipHostMallocDefault));
+    memset(c->ss_host, 0, sizeof(SceneState) * (size_t)S);
+    std::vector<float> dom(3 * (size_t)S);
+    for (int s = 0; s < S; ++s)
+        for (int a = 0; a < 3; ++a) dom[3 * s + a] = sc[s].domain_size[a];
+    IVP_HIP(hipMemsetAsync(c->ccount, 0, sizeof(int) * (size_t)c->ncell, c->stream));
+    IVP_HIP(hipMemsetAsync(c->shist_err, 0, sizeof(double) * (size_t)S * SCENE_HIST, c->stream));
+    IVP_HIP(hipMemsetAsync(c->shist_pcg, 0, sizeof(int) * (size_t)S * SCENE_HIST, c->stream));
+    IVP_HIP(hipMemsetAsync(c->ss, 0, sizeof(SceneState) * (size_t)S, c->stream));
+    IVP_HIP(hipMemcpyAsync(c->grids, g.data(), sizeof(Grid) * (size_t)S, hipMemcpyHostToDevice, c->stream));
+    IVP_HIP(hipMemcpyAsync(c->sdomain, dom.data(), sizeof(float) * dom.size(), hipMemcpyHostToDevice, c->stream));
+    IVP_HIP(hipStreamSynchronize(c->stream));
+    c->S = S;
+    c->scenes.assign(sc, sc + S);
+    c->grids_host = g;
+    c->row_fmt = IVP_ROWS_IDS;
+    c->rec_words = 64;
+    return IVP_OK;
 }
 
 }  // namespace
@@ -272,6 +361,17 @@ int ivp_set_particles(ivp_ctx* c, int n, const float* x, const float* v, const f
     c->built = false;   // the rows describe the previous particle set
     int rc = grid_load(c, n, x, v, m, m_inv, density0, is_dynamic, material);
     if (rc != IVP_OK) return rc;
+    if (c->S > 0) {
+        // the dynamic rows of every scene, for the per-scene reductions
+        for (int s = 0; s < c->S; ++s) {
+            c->ss_host[s].row0 = c->scene_row0[s];
+            c->ss_host[s].nd = c->scene_row0[s + 1] - c->scene_row0[s];
+            c->ss_host[s].phase = PH_DONE;
+        }
+        IVP_HIP(hipMemcpyAsync(c->ss, c->ss_host, sizeof(SceneState) * (size_t)c->S, hipMemcpyHostToDevice,
+                               c->stream));
+        IVP_HIP(hipStreamSynchronize(c->stream));
+    }
     return ensure_rows(c);
 }
 
@@ -292,9 +392,19 @@ int ivp_reset(ivp_ctx* c) {
 
 int ivp_set_option(ivp_ctx* c, int option, int value) {
     if (!c) return IVP_ERR_ARG;
+    if (option == IVP_OPT_SCENE_CG_CHUNK) {
+        if (value < 1 || value > 64) return IVP_ERR_ARG;
+        c->scene_chunk = value;
+        return IVP_OK;
+    }
+    if (option == IVP_OPT_SCENE_GRAPH) {
+        c->scene_graph = value != 0;
+        return IVP_OK;
+    }
     if (option == IVP_OPT_GRADIENT_CACHE) {
         if (value != IVP_ROWS_IDS && value != IVP_ROWS_F32 && value != IVP_ROWS_F16) return IVP_ERR_ARG;
         if (value == c->row_fmt) return IVP_OK;
+        if (c->S > 0) return IVP_ERR_ARG;   // independent scenes use ids-only rows
         IVP_HIP(hipStreamSynchronize(c->stream));
         free_rows(c);
         c->row_fmt = value;
@@ -309,6 +419,14 @@ int ivp_get_option(ivp_ctx* c, int option, int* value) {
     if (!c || !value) return IVP_ERR_ARG;
     if (option == IVP_OPT_GRADIENT_CACHE) {
         *value = c->row_fmt;
+        return IVP_OK;
+    }
+    if (option == IVP_OPT_SCENE_CG_CHUNK) {
+        *value = c->scene_chunk;
+        return IVP_OK;
+    }
+    if (option == IVP_OPT_SCENE_GRAPH) {
+        *value = c->scene_graph ? 1 : 0;
         return IVP_OK;
     }
     return IVP_ERR_ARG;
@@ -341,16 +459,19 @@ int ivp_neighbors_export(ivp_ctx* c, int cache_size, int* count, int* ids, float
 
 int ivp_ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats) {
     if (!c) return IVP_ERR_ARG;
+    if (c->S > 0) return IVP_ERR_STATE;
     return ip_solve(c, dt, prm, stats);
 }
 
 int ivp_viscosity_explicit(ivp_ctx* c, float coeff, float dt) {
     if (!c) return IVP_ERR_ARG;
+    if (c->S > 0) return IVP_ERR_STATE;
     return visc_explicit(c, coeff, dt);
 }
 
 int ivp_viscosity_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* stats) {
     if (!c) return IVP_ERR_ARG;
+    if (c->S > 0) return IVP_ERR_STATE;
     return visc_weiler18(c, coeff, dt, stats);
 }
 
@@ -366,7 +487,7 @@ int ivp_advect_boundary(ivp_ctx* c, float dt) {
 
 int ivp_step(ivp_ctx* c, const ivp_step_params* prm, ivp_ip_stats* stats, ivp_cg_stats* visc) {
     if (!c || !prm || !stats) return IVP_ERR_ARG;
-    if (!c->built) return IVP_ERR_STATE;
+    if (!c->built || c->S > 0) return IVP_ERR_STATE;
     int rc = glue_gravity(c, prm->gravity, prm->dt);
     if (rc != IVP_OK) return rc;
     if (visc) { visc->cg_iters = 0; visc->residual_norm2 = 0.0; }
@@ -384,6 +505,55 @@ int ivp_step(ivp_ctx* c, const ivp_step_params* prm, ivp_ip_stats* stats, ivp_cg
     rc = glue_advect_boundary(c, prm->dt);
     if (rc != IVP_OK) return rc;
     return grid_build(c);
+}
+
+int ivp_set_scenes(ivp_ctx* c, int n_scenes, const ivp_scene* scenes) {
+    if (!c) return IVP_ERR_ARG;
+    return scenes_set(c, n_scenes, scenes);
+}
+
+int ivp_step_scenes(ivp_ctx* c, const ivp_step_params* prm, ivp_scene_stats* stats) {
+    if (!c || !prm || !stats) return IVP_ERR_ARG;
+    if (!c->built || c->S == 0) return IVP_ERR_STATE;
+    if (prm->viscosity_coeff != 0.0f) return IVP_ERR_ARG;   // viscosity is not part of the scene step
+    int rc = glue_gravity(c, prm->gravity, prm->dt);
+    if (rc != IVP_OK) return rc;
+    rc = ip_solve_scenes(c, prm->dt, &prm->ip);
+    if (rc != IVP_OK) return rc;
+    int status = IVP_OK;
+    for (int s = 0; s < c->S; ++s) {
+        const SceneState& q = c->ss_host[s];
+        stats[s].status = q.status;
+        stats[s].outer_iters = q.outer_iters;
+        stats[s].cg_iters = q.cg_total;
+        stats[s].final_density_error = q.final_err;
+        if (status == IVP_OK && q.status != IVP_OK) status = q.status;
+    }
+    rc = glue_advect_boundary(c, prm->dt);
+    if (rc != IVP_OK) return rc;
+    rc = grid_build(c);
+    if (rc != IVP_OK) return rc;
+    return status;
+}
+
+int ivp_scene_history(ivp_ctx* c, int scene, int capacity, double* density_errors, int* pcg_iters, int* n_errors,
+                      int* n_pcg) {
+    if (!c) return IVP_ERR_ARG;
+    if (c->S == 0) return IVP_ERR_STATE;
+    if (scene < 0 || scene >= c->S || capacity < 0) return IVP_ERR_ARG;
+    const SceneState& q = c->ss_host[scene];
+    int ne = q.n_err, np = q.outer_iters;
+    if (n_errors) *n_errors = ne;
+    if (n_pcg) *n_pcg = np;
+    int ce = std::min(std::min(ne, capacity), SCENE_HIST), cp = std::min(std::min(np, capacity), SCENE_HIST);
+    const size_t off = (size_t)scene * SCENE_HIST;
+    if (density_errors && ce > 0)
+        IVP_HIP(hipMemcpyAsync(density_errors, c->shist_err + off, sizeof(double) * ce, hipMemcpyDeviceToHost,
+                               c->stream));
+    if (pcg_iters && cp > 0)
+        IVP_HIP(hipMemcpyAsync(pcg_iters, c->shist_pcg + off, sizeof(int) * cp, hipMemcpyDeviceToHost, c->stream));
+    IVP_HIP(hipStreamSynchronize(c->stream));
+    return IVP_OK;
 }
 
 int ivp_profile_enable(ivp_ctx* c, int enable) {

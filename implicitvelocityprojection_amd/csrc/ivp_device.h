@@ -19,6 +19,7 @@ struct Grid {
     float origin[3];
     float inv_w;
     int dims[3];
+    int base;      // first key of this grid (independent scenes: one grid per scene, keys consecutive)
 };
 
 // The kernel constants of sph_kernel.py:14-15 and :55, evaluated in float32 with the reference's
@@ -82,7 +83,7 @@ __device__ __forceinline__ int cell_coord(const Grid& g, float x, int a) {
 }
 
 __device__ __forceinline__ int cell_key(const Grid& g, int cx, int cy, int cz) {
-    return (cz * g.dims[1] + cy) * g.dims[0] + cx;
+    return g.base + (cz * g.dims[1] + cy) * g.dims[0] + cx;
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8).  Remap so

@@ -127,12 +127,16 @@ __device__ __forceinline__ KeyRun key_run(int k, bool valid) {
     return r;
 }
 
+// SCENES: particle i uses the grid of its scene, grids[sid[i]]
+template <bool SCENES>
 __global__ __launch_bounds__(BLOCK) void k_keys(int count, const float4* __restrict__ pos, Grid g,
+                                                const Grid* __restrict__ grids, const int* __restrict__ sid,
                                                 int* __restrict__ key, int* __restrict__ ccount) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     bool valid = i < count;
     int k = -1;
     if (valid) {
+        if (SCENES) g = grids[sid[i]];
         float4 p = pos[i];
         k = cell_key(g, cell_coord(g, p.x, 0), cell_coord(g, p.y, 1), cell_coord(g, p.z, 2));
         key[i] = k;
@@ -292,6 +296,8 @@ struct BuildArgs {
     float kc, ks;   // cubic and spiky kernel constants
     float inv_gs;   // 1 / scale of float16 gradW records (a power of two)
     Grid g;
+    const Grid* grids;   // independent scenes: grid of each scene, indexed by sid
+    const int* sid;
     const float4* pos;
     const float* minv;
     const int* dstart;
@@ -319,7 +325,7 @@ struct BuildArgs {
 //     -- are padded with (id = row, gradW = 0).
 // FMT is the row format (include/ivp.h): ids only, ids + float32 gradW, or ids + float16 gradW
 // scaled by 1/inv_gs, packed as (gx, gy) and (gz, 0) half pairs.
-template <int FMT>
+template <int FMT, bool SCENES>
 __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
     constexpr unsigned REC = FMT == IVP_ROWS_F32 ? 256u : FMT == IVP_ROWS_F16 ? 192u : 64u;   // words per record
     int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -334,12 +340,13 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
     float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (valid) {
         pi = a.pos[i];
-        scan_ids(a.g, a.dstart, a.pos, i, pi, h, a.h2_lo, a.h2_hi, [&](int j) {
+        const Grid g = SCENES ? a.grids[a.sid[i]] : a.g;
+        scan_ids(g, a.dstart, a.pos, i, pi, h, a.h2_lo, a.h2_hi, [&](int j) {
             if (tot < C) tbase[(size_t)cnt * REC] = (uint32_t)j;
             if (tot < C) cnt++;
             tot++;
         });
-        scan_ids(a.g, a.sstart, a.pos + a.nd, -1, pi, h, a.h2_lo, a.h2_hi, [&](int j) {
+        scan_ids(g, a.sstart, a.pos + a.nd, -1, pi, h, a.h2_lo, a.h2_hi, [&](int j) {
             if (tot < C) tbase[(size_t)(Cs - 1 - cs) * REC] = (uint32_t)j;
             if (tot < C) cs++;
             tot++;
@@ -429,6 +436,8 @@ struct ExportArgs {
     int n, nd, C;
     float h, ks;
     Grid g;
+    const Grid* grids;
+    const int* sid;
     const float4* pos;
     const int* dstart;
     const int* sstart;
@@ -440,9 +449,11 @@ struct ExportArgs {
 
 // The reference cache layout (every row, user ids, count capped at C), same enumeration as
 // k_build.
+template <bool SCENES>
 __global__ __launch_bounds__(BLOCK) void k_export(ExportArgs a) {
     int s = blockIdx.x * BLOCK + threadIdx.x;
     if (s >= a.n) return;
+    const Grid g = SCENES ? a.grids[a.sid[s]] : a.g;
     float4 pi = a.pos[s];
     int u = a.uid[s];
     size_t base = (size_t)u * (size_t)a.C;
@@ -458,9 +469,9 @@ __global__ __launch_bounds__(BLOCK) void k_export(ExportArgs a) {
         }
         cnt++;
     };
-    visit(a.g, a.dstart, a.pos, is_dyn ? s : -1, pi, a.h,
+    visit(g, a.dstart, a.pos, is_dyn ? s : -1, pi, a.h,
           [&](int j, float4, float rx, float ry, float rz, float) { put(j, rx, ry, rz); });
-    visit(a.g, a.sstart, a.pos + a.nd, is_dyn ? -1 : s - a.nd, pi, a.h,
+    visit(g, a.sstart, a.pos + a.nd, is_dyn ? -1 : s - a.nd, pi, a.h,
           [&](int j, float4, float rx, float ry, float rz, float) { put(a.nd + j, rx, ry, rz); });
     a.count[u] = cnt < a.C ? cnt : a.C;
 }
@@ -479,6 +490,9 @@ __global__ __launch_bounds__(BLOCK) void k_dyn_flags(int n, const int* __restric
 
 struct StageArgs {
     int n, nd;
+    int S;                 // independent scenes (0: one scene)
+    const int* ustart;     // S + 1 user-order scene starts
+    int* sid;
     const int* dpos;
     const int* is_dynamic;
     const float* x; const float* v; const float* m; const float* m_inv;
@@ -501,6 +515,14 @@ __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs a) {
     a.mat[s] = a.material[u];
     a.uid[s] = u;
     a.inv[u] = s;
+    if (a.S > 0) {
+        int lo = 0, hi = a.S;   // scene q with ustart[q] <= u < ustart[q + 1]
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (a.ustart[mid] <= u) lo = mid; else hi = mid;
+        }
+        a.sid[s] = lo;
+    }
     a.density[s] = a.density0[u];
     a.kk[s] = 0.0f;
     a.divg[s] = 0.0f;
@@ -589,8 +611,12 @@ int grid_sort_population(ivp_ctx* c, int offset, int count, int* start_table, bo
     // bytes: keys (read x 16, write key 4), scatter (key 4 + slot 4), rank (key, uid, cell members
     // ~ 3 x 4), gather (index 4 + every per-particle array read and written)
     const double per = dynamic ? 4 + 2 * (16 + 12 + 4 + 4 + 4 + 4 + 4) + 4 : 4 + 2 * (16 + 4 + 4 + 4 + 4) + 4;
-    IVP_TRY(launch(c, KC_SORT, -1, 20.0 * count, k_keys, nb, count, (const float4*)(c->pos + offset), c->grid, c->key,
-                   c->ccount));
+    if (c->S > 0)
+        IVP_TRY(launch(c, KC_SORT, -1, 24.0 * count, k_keys<true>, nb, count, (const float4*)(c->pos + offset), c->grid,
+                       (const Grid*)c->grids, (const int*)(c->sid + offset), c->key, c->ccount));
+    else
+        IVP_TRY(launch(c, KC_SORT, -1, 20.0 * count, k_keys<false>, nb, count, (const float4*)(c->pos + offset),
+                       c->grid, (const Grid*)nullptr, (const int*)nullptr, c->key, c->ccount));
     IVP_TRY(scan(c, c->ccount, start_table, c->ncell));
     IVP_TRY(launch(c, KC_SORT, -1, 12.0 * count, k_scatter, nb, count, (const int*)c->key, (const int*)start_table,
                    c->ccount, c->tmp_src));
@@ -639,7 +665,28 @@ int grid_load(ivp_ctx* c, int n, const float* x, const float* v, const float* m,
     c->nd = *c->count_host;
     c->ns = n - c->nd;
     c->all_dyn_fluid = (*c->err_host & 2) == 0;
-    StageArgs sa{n, c->nd, c->tmp_src, is_dynamic, x, v, m, m_inv, density0, material,
+    int* ustart = nullptr;
+    if (c->S > 0) {
+        // scene boundaries in user order -> dynamic / static starts in internal order
+        int S = c->S;
+        std::vector<int> us(S + 1, 0);
+        for (int s = 0; s < S; ++s) us[s + 1] = us[s] + c->scenes[s].n_particles;
+        if (us[S] != n) return IVP_ERR_ARG;
+        std::vector<int> pre(S + 1, 0);
+        for (int s = 0; s <= S; ++s)
+            IVP_HIP(hipMemcpy(&pre[s], c->tmp_src + us[s], sizeof(int), hipMemcpyDeviceToHost));
+        c->scene_row0.assign(S + 1, 0);
+        c->scene_srow0.assign(S + 1, 0);
+        for (int s = 0; s <= S; ++s) {
+            c->scene_row0[s] = pre[s];
+            c->scene_srow0[s] = us[s] - pre[s];
+        }
+        for (int s = 0; s < S; ++s)
+            if (c->scene_row0[s + 1] - c->scene_row0[s] < MIN_SCENE_DYNAMIC) return IVP_ERR_ARG;
+        ustart = c->key;   // sort scratch, free until the static sort below
+        IVP_HIP(hipMemcpy(ustart, us.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice));
+    }
+    StageArgs sa{n, c->nd, c->S, ustart, c->sid, c->tmp_src, is_dynamic, x, v, m, m_inv, density0, material,
                  c->pos, c->vel, c->minv, c->rho0, c->mat, c->uid, c->inv, c->density, c->kk, c->divg};
     hipLaunchKernelGGL(k_stage, dim3(nb), dim3(BLOCK), 0, c->stream, sa);
     IVP_HIP(hipGetLastError());
@@ -689,19 +736,22 @@ int grid_build(ivp_ctx* c) {
         const double hh = (double)c->h * (double)c->h;
         float h2_lo = std::nextafter((float)(hh * (1.0 - 1.0 / 1048576.0)), 0.0f);
         float h2_hi = std::nextafter((float)(hh * (1.0 + 1.0 / 1048576.0)), 1.0f);
-        BuildArgs ba{c->nd, c->ns, c->C, c->Cs, c->h, h2_lo, h2_hi, c->kcubic, c->kspiky, 1.0f / c->gscale, c->grid, c->pos, c->minv, c->dstart, c->sstart, c->nb, c->tile_len,
+        BuildArgs ba{c->nd, c->ns, c->C, c->Cs, c->h, h2_lo, h2_hi, c->kcubic, c->kspiky, 1.0f / c->gscale, c->grid,
+                     c->grids, c->sid, c->pos, c->minv, c->dstart, c->sstart, c->nb, c->tile_len,
                      c->rowcnt, c->rowtot, c->density, c->kk, c->Ss, c->err_flag, c->partials, c->max_blocks};
         // bytes (pair count of the previous build): the pair records written, the ids read back
         // once, x and m of every particle once, the per-row outputs (rho, k, S, counts)
         double pairs = (double)c->state_host->total_pairs;
         double bytes = pairs * (4.0 * c->rec_words / 64 + 4.0) + 16.0 * c->n + 32.0 * c->nd;
         const dim3 nb(blocks_for(c->nd));
-        if (c->row_fmt == IVP_ROWS_F32)
-            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_F32>, nb, ba));
+        if (c->S > 0)   // independent scenes use ids-only rows (ivp_set_scenes fixes the format)
+            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_IDS, true>, nb, ba));
+        else if (c->row_fmt == IVP_ROWS_F32)
+            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_F32, false>, nb, ba));
         else if (c->row_fmt == IVP_ROWS_F16)
-            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_F16>, nb, ba));
+            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_F16, false>, nb, ba));
         else
-            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_IDS>, nb, ba));
+            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_IDS, false>, nb, ba));
         IVP_TRY(finalize<FIN_BUILD>(c, nb.x));
     }
     c->built = true;
@@ -712,8 +762,12 @@ int grid_export(ivp_ctx* c, int cache_size, int* count, int* ids, float* gradW) 
     if (!c->built) return IVP_ERR_STATE;
     if (cache_size <= 0 || !count || !ids || !gradW) return IVP_ERR_ARG;
     if (c->n == 0) return IVP_OK;
-    ExportArgs ea{c->n, c->nd, cache_size, c->h, c->kspiky, c->grid, c->pos, c->dstart, c->sstart, c->uid, count, ids, gradW};
-    hipLaunchKernelGGL(k_export, dim3(blocks_for(c->n)), dim3(BLOCK), 0, c->stream, ea);
+    ExportArgs ea{c->n, c->nd, cache_size, c->h, c->kspiky, c->grid, c->grids, c->sid, c->pos, c->dstart, c->sstart,
+                  c->uid, count, ids, gradW};
+    if (c->S > 0)
+        hipLaunchKernelGGL(k_export<true>, dim3(blocks_for(c->n)), dim3(BLOCK), 0, c->stream, ea);
+    else
+        hipLaunchKernelGGL(k_export<false>, dim3(blocks_for(c->n)), dim3(BLOCK), 0, c->stream, ea);
     IVP_HIP(hipGetLastError());
     return IVP_OK;
 }

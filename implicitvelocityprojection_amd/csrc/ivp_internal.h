@@ -62,6 +62,42 @@ struct SolverState {
     long long total_pairs;  // stored (dynamic-dynamic) neighbour pairs of the last build
 };
 
+// Independent scenes in one context.  Each scene has its own cell grid (keys of scene s follow
+// those of scene s-1), so the cell sort keeps every scene's particles contiguous: dynamic
+// particles of scene s at internal [row0, row0 + nd), static ones in the same scene order after
+// all dynamic particles.  Each scene runs its own IP outer loop and CG, advanced by a phase:
+enum ScenePhase { PH_PACK = 0, PH_RHS = 1, PH_CG = 2, PH_DUPDATE = 3, PH_DONE = 4 };
+
+constexpr int MIN_SCENE_DYNAMIC = 256;   // a 256-row block then spans at most two scenes
+constexpr int SCENE_HIST = 1026;         // per-scene outer-loop history entries kept
+
+struct SceneState {
+    SolverState cg;      // the scene's CG and error sums (single-scene semantics)
+    int phase;
+    int opt_iter;        // outer iterations done
+    int outer_iters;     // final count (stats_outer_iters)
+    int status;          // IVP_OK, IVP_NOT_CONVERGED, IVP_NOT_PD
+    int row0, nd;        // dynamic rows of the scene
+    int cg_total;        // CG iterations over the solve
+    int n_err;           // density errors recorded
+    double final_err;
+};
+
+// An instantiated replay graph and what its capture depended on
+struct SceneGraph {
+    hipGraphExec_t exec = nullptr;
+    const void* pos = nullptr;   // particle store buffer read by the captured launches
+    int chunk = 0;
+    float omega = 0.0f;
+    int nd = 0;
+};
+
+// IP parameters of the current batched solve (device copy, read by the finalize kernels)
+struct SceneParams {
+    double tol_opt, tol_pcg;
+    int max_opt, max_pcg, fixed;
+};
+
 constexpr int ROW_PAD = 8;        // tile lengths are padded to a multiple of this many slots
 
 enum KernelClass {
@@ -179,6 +215,31 @@ struct ivp_ctx {
     int* err_host = nullptr;                // pinned
     int* count_host = nullptr;              // pinned scratch
 
+    // independent scenes (S > 0 after ivp_set_scenes; S == 0: one scene, the grid above)
+    int S = 0;
+    std::vector<ivp_scene> scenes;
+    std::vector<ivp::Grid> grids_host;
+    std::vector<int> scene_row0;          // S + 1 dynamic row starts (internal order)
+    std::vector<int> scene_srow0;         // S + 1 static starts, relative to nd
+    ivp::Grid* grids = nullptr;           // device [S]
+    int* sid = nullptr;                   // device [cap]: scene of each internal particle
+    float* sdomain = nullptr;             // device [3 S]: domain size per scene (boundary clamp)
+    ivp::SceneState* ss = nullptr;        // device [S]
+    ivp::SceneState* ss_host = nullptr;   // pinned [S]
+    ivp::SceneParams* sprm = nullptr;     // device
+    ivp::SceneParams* sprm_host = nullptr;  // pinned
+    double* shist_err = nullptr;          // device [S * SCENE_HIST]
+    int* shist_pcg = nullptr;             // device [S * SCENE_HIST]
+    double* spartials = nullptr;          // device [8 * max_blocks]: per block, two scene segments
+    int* sactive = nullptr;               // device: scenes not done
+    int* sactive_host = nullptr;          // pinned
+    int scene_chunk = 4;                  // CG iterations per replay
+    bool scene_graph = true;              // replay a captured hipGraph
+    hipStream_t cap_stream = nullptr;     // private stream the replay graph is captured on
+    ivp::SceneGraph sgraphs[2];
+    int sgraph_next = 0;
+    long long scene_replays = 0;          // replays of the last batched solve
+
     // profiling
     bool prof = false;
     unsigned prof_mask = 0xffffffffu;
@@ -206,6 +267,9 @@ int grid_load(ivp_ctx* c, int n, const float* x, const float* v, const float* m,
               const float* density0, const int* is_dynamic, const int* material);
 int store_write(ivp_ctx* c, int field, const void* src);
 int store_read(ivp_ctx* c, int field, void* dst);
+
+// ivp_solve.hip, independent scenes
+int ip_solve_scenes(ivp_ctx* c, float dt, const ivp_ip_params* prm);
 
 // ivp_solve.hip
 int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* st);

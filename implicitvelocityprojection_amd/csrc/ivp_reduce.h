@@ -75,18 +75,10 @@ __device__ __forceinline__ void fold_partials(const double* __restrict__ partial
     }
 }
 
-// One block: fold the partials of the previous kernel in a fixed order and advance the solver
-// state (the scalar host logic of ip.py:296-327, pressure_base.py:405-423, weiler18.py:330-365).
+// Scalar state update after a reduction (the host logic of ip.py:296-327,
+// pressure_base.py:405-423, weiler18.py:330-365), applied to one solver state.
 template <int OP>
-__global__ __launch_bounds__(FIN_BLOCK) void k_finalize(FinArgs a) {
-    constexpr int NV = OP == FIN_PACK ? 3 : (OP == FIN_RHS || OP == FIN_W18_UPDATE || OP == FIN_CG_REPLACE) ? 2 : 1;
-    SolverState* st = a.st;
-    if (OP == FIN_CG_PAP || OP == FIN_CG_UPDATE || OP == FIN_W18_PAP || OP == FIN_W18_UPDATE || OP == FIN_CG_REPLACE) {
-        if (st->active == 0) return;
-    }
-    double t[3] = {0.0, 0.0, 0.0};
-    fold_partials<NV>(a.partials, a.stride, a.nblocks, t);
-    if (threadIdx.x != 0) return;
+__device__ __forceinline__ void fin_apply(SolverState* st, const double (&t)[3], double tol, int max_iter) {
     if (OP == FIN_BUILD) {
         st->total_pairs = (long long)t[0];
     } else if (OP == FIN_PACK) {
@@ -105,11 +97,11 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(FinArgs a) {
         st->rr0 = t[0];
         st->rr_best = t[0];
         st->stalls = 0;
-        st->tol = a.tol;
-        st->max_iter = a.max_iter;
+        st->tol = tol;
+        st->max_iter = max_iter;
         st->iter = 0;
         st->status = 0;
-        st->active = (t[0] > floor_ && a.max_iter > 0) ? 1 : 0;
+        st->active = (t[0] > floor_ && max_iter > 0) ? 1 : 0;
     } else if (OP == FIN_CG_PAP) {
         // ip.py:307-311
         st->pAp = t[0];
@@ -181,6 +173,26 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(FinArgs a) {
     }
 }
 
+template <int OP>
+constexpr int fin_nv() {
+    return OP == FIN_PACK ? 3 : (OP == FIN_RHS || OP == FIN_W18_UPDATE || OP == FIN_CG_REPLACE) ? 2 : 1;
+}
+
+// One block: fold the partials of the previous kernel in a fixed order and advance the solver
+// state.
+template <int OP>
+__global__ __launch_bounds__(FIN_BLOCK) void k_finalize(FinArgs a) {
+    constexpr int NV = fin_nv<OP>();
+    SolverState* st = a.st;
+    if (OP == FIN_CG_PAP || OP == FIN_CG_UPDATE || OP == FIN_W18_PAP || OP == FIN_W18_UPDATE || OP == FIN_CG_REPLACE) {
+        if (st->active == 0) return;
+    }
+    double t[3] = {0.0, 0.0, 0.0};
+    fold_partials<NV>(a.partials, a.stride, a.nblocks, t);
+    if (threadIdx.x != 0) return;
+    fin_apply<OP>(st, t, a.tol, a.max_iter);
+}
+
 
 template <int OP>
 inline int finalize(ivp_ctx* c, int nblocks, double tol = 0.0, int max_iter = 0) {
@@ -188,6 +200,106 @@ inline int finalize(ivp_ctx* c, int nblocks, double tol = 0.0, int max_iter = 0)
     hipLaunchKernelGGL(k_finalize<OP>, dim3(1), dim3(FIN_BLOCK), 0, c->stream, fa);
     IVP_HIP(hipGetLastError());
     return IVP_OK;
+}
+
+// ---- independent scenes ----------------------------------------------------------------------
+
+// Per-block partial sums split by scene.  A block's rows span at most two scenes (every scene
+// holds at least MIN_SCENE_DYNAMIC = BLOCK dynamic rows): seg = 0 for rows of the block's first
+// scene, 1 for the second.  Value q of segment g goes to row 2 q + g, column blk (the block's
+// row block, so the blocks of one scene are one contiguous range).  `two` is block-uniform.
+template <int NV>
+__device__ __forceinline__ void block_partials_seg(const double (&v)[NV], int seg, bool two, double* partials,
+                                                   int stride, int blk) {
+    __shared__ double lds[WAVES_PER_BLOCK];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double s0 = block_sum(seg == 0 ? v[q] : 0.0, lds);
+        if (threadIdx.x == 0) partials[(2 * q) * stride + blk] = s0;
+        if (two) {
+            double s1 = block_sum(seg == 1 ? v[q] : 0.0, lds);
+            if (threadIdx.x == 0) partials[(2 * q + 1) * stride + blk] = s1;
+        }
+    }
+}
+
+struct FinScenesArgs {
+    const double* partials;
+    int stride;
+    const int* sid;
+    SceneState* ss;
+    const SceneParams* prm;
+    double* hist_err;     // [S * SCENE_HIST]
+    int* hist_pcg;        // [S * SCENE_HIST]
+    int S;
+};
+
+// One wave per scene: fold the scene's block partials in a fixed order and advance its state
+// with the single-scene logic (fin_apply), then its outer-loop phase (the host logic of
+// ip.py:345-395 for one scene).  Scenes not in the phase this reduction belongs to are skipped.
+template <int OP>
+__global__ __launch_bounds__(BLOCK) void k_fin_scenes(FinScenesArgs a) {
+    constexpr int NV = fin_nv<OP>();
+    constexpr int want = OP == FIN_PACK ? PH_PACK : (OP == FIN_RHS || OP == FIN_CG_INIT) ? PH_RHS : PH_CG;
+    const int s = blockIdx.x * WAVES_PER_BLOCK + (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (s >= a.S) return;
+    SceneState* sc = a.ss + s;
+    if (sc->phase != want) return;
+    const int b0 = sc->row0 / BLOCK, b1 = (sc->row0 + sc->nd - 1) / BLOCK;
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int b = b0 + lane; b <= b1; b += WAVE) {
+        const int seg = a.sid[b * BLOCK] == s ? 0 : 1;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) t[q] += a.partials[(2 * q + seg) * a.stride + b];
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) t[q] = wave_sum(t[q]);
+    if (lane != 0) return;
+    const SceneParams& p = *a.prm;
+    double* herr = a.hist_err + (size_t)s * SCENE_HIST;
+    int* hpcg = a.hist_pcg + (size_t)s * SCENE_HIST;
+    if (OP == FIN_PACK) {
+        fin_apply<OP>(&sc->cg, t, 0.0, 0);
+        double err = sc->cg.err_sum / (double)sc->nd;
+        if (sc->n_err < SCENE_HIST) herr[sc->n_err] = err;
+        sc->n_err = sc->n_err + 1;
+        bool converged = err < p.tol_opt && !p.fixed;
+        if ((converged && sc->opt_iter >= 1) || sc->opt_iter >= p.max_opt) {
+            sc->phase = PH_DONE;
+            sc->final_err = err;
+            sc->outer_iters = sc->opt_iter;
+            if (!p.fixed && err >= p.tol_opt && sc->opt_iter >= p.max_opt) sc->status = IVP_NOT_CONVERGED;
+        } else {
+            sc->phase = PH_RHS;
+        }
+    } else if (OP == FIN_RHS) {
+        fin_apply<OP>(&sc->cg, t, 0.0, 0);
+    } else if (OP == FIN_CG_INIT) {
+        fin_apply<OP>(&sc->cg, t, p.tol_pcg, p.max_pcg);
+        if (sc->cg.active) {
+            sc->phase = PH_CG;
+        } else {
+            sc->phase = PH_DUPDATE;
+            if (sc->opt_iter < SCENE_HIST) hpcg[sc->opt_iter] = 0;
+        }
+    } else if (OP == FIN_CG_PAP) {
+        fin_apply<OP>(&sc->cg, t, 0.0, 0);
+        if (sc->cg.status == IVP_NOT_PD) {
+            sc->status = IVP_NOT_PD;
+            sc->phase = PH_DONE;
+            sc->outer_iters = sc->opt_iter;
+            sc->cg_total = sc->cg_total + sc->cg.iter;
+            if (sc->opt_iter < SCENE_HIST) hpcg[sc->opt_iter] = sc->cg.iter;
+        }
+    } else if (OP == FIN_CG_UPDATE) {
+        fin_apply<OP>(&sc->cg, t, 0.0, 0);
+        if (!sc->cg.active) {
+            sc->phase = PH_DUPDATE;
+            sc->cg_total = sc->cg_total + sc->cg.iter;
+            if (sc->opt_iter < SCENE_HIST) hpcg[sc->opt_iter] = sc->cg.iter;
+        }
+    }
 }
 
 }  // namespace ivp

@@ -80,6 +80,49 @@ __device__ __forceinline__ float grad_scale(float h, float ks, float rx, float r
     return (t * t) * (-ks * inv);
 }
 
+// Independent scenes: the scene of each internal particle and the scene states.  A pass over
+// scenes (template flag SC) runs on the rows whose scene is in the pass's phase.
+struct SceneRef {
+    const int* sid;
+    const SceneState* ss;
+};
+
+// Row gate of a pass.  One scene: every row i < nd is on.  Scenes: the block's rows span at
+// most two scenes (s0 = the first row's); gate_block returns false when neither is in `phase`,
+// and the whole block then skips (writing no partials, which the per-scene finalize of a scene
+// in the phase never reads).
+struct Gate {
+    int s0;      // scene of the block's first row
+    bool two;    // the block's rows span a second scene
+    bool on;     // this thread's row takes part
+    int seg;     // 0: row of scene s0, 1: row of the second scene
+    int s;       // scene of this thread's row
+};
+
+__device__ __forceinline__ Gate gate_all(int i, int nd) { return Gate{0, false, i < nd, 0, 0}; }
+
+__device__ __forceinline__ bool gate_block(const SceneRef& sc, int blk, int i, int nd, int phase, Gate& g) {
+    const int r0 = blk * BLOCK;
+    const int rl = (r0 + BLOCK < nd ? r0 + BLOCK : nd) - 1;
+    g.s0 = sc.sid[r0];
+    const int s1 = sc.sid[rl];
+    g.two = s1 != g.s0;
+    if (sc.ss[g.s0].phase != phase && sc.ss[s1].phase != phase) return false;
+    g.s = i < nd ? sc.sid[i] : g.s0;
+    g.seg = g.s != g.s0 ? 1 : 0;
+    g.on = i < nd && sc.ss[g.s].phase == phase;
+    return true;
+}
+
+template <bool SC, int NV>
+__device__ __forceinline__ void row_partials(const double (&v)[NV], const Gate& g, double* partials, int stride,
+                                             int blk) {
+    if (SC)
+        block_partials_seg<NV>(v, g.seg, g.two, partials, stride, blk);
+    else
+        block_partials<NV>(v, partials, stride);
+}
+
 // Per-particle pass data of NB floats.
 template <int NB>
 struct Vec {
@@ -158,20 +201,23 @@ struct PackArgs {
     float* jd;       // J d, kept for the divergence of the final d
     double* partials;
     int stride;
+    SceneRef sc;
 };
 
 // ip_pack_fused (ip.py:124-171) + coef_wise_div + compute_avg_density_error
 // (math_utils.py:76-91, pressure_base.py:187-203, 413-423).  J d in the form of
 // compute_J_x_kernel (pressure_base.py:111-127): sum_j m_j dot(d_i - d_j, gradW_ij) + dot(d_i, S_i).
-template <class FM>
+template <class FM, bool SC>
 __global__ __launch_bounds__(BLOCK) void k_ip_pack(PackArgs a) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = blk * BLOCK + threadIdx.x;
+    Gate gt = gate_all(i, a.R.nd);
+    if (SC && !gate_block(a.sc, blk, i, a.R.nd, PH_PACK, gt)) return;
     // neighbour record: A = (x, y, z, m), B = d
     auto loadA = [&](int j) { return j_record<FM::EVAL>(a.R.pos, a.d4, j); };
     auto loadB = [&](int j) { return vec3_of(a.d4, j); };
     double err = 0.0, nact = 0.0, npairs = 0.0;
-    if (i < a.R.nd) {
+    if (gt.on) {
         float4 di = a.d4[i];
         float acc = 0.0f;
         row_pairs<FM, 3>(a.R, i, a.R.pos[i], loadA, loadB,
@@ -197,7 +243,7 @@ __global__ __launch_bounds__(BLOCK) void k_ip_pack(PackArgs a) {
         }
     }
     double v[3] = {err, nact, npairs};
-    block_partials<3>(v, a.partials, a.stride);
+    row_partials<SC, 3>(v, gt, a.partials, a.stride, blk);
 }
 
 struct RhsArgs {
@@ -210,14 +256,17 @@ struct RhsArgs {
     const int* rowcnt;
     double* partials;
     int stride;
+    SceneRef sc;
 };
 
 // ip_rhs_fused (ip.py:174-215): b = omega J^T (f k); also marks rows whose CG J^T loop can be
 // non-zero (own dfdt > 0 or a dynamic neighbour with dfdt > 0).
-template <class FM>
+template <class FM, bool SC>
 __global__ __launch_bounds__(BLOCK) void k_ip_rhs(RhsArgs a) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = blk * BLOCK + threadIdx.x;
+    Gate gt = gate_all(i, a.R.nd);
+    if (SC && !gate_block(a.sc, blk, i, a.R.nd, PH_RHS, gt)) return;
     // neighbour record: A = (x, y, z, m), B = (f k, dfdt k); q4 = (f k, m, dfdt k, 0)
     auto loadA = [&](int j) {
         return FM::EVAL ? a.R.pos[j] : make_float4(0.0f, 0.0f, 0.0f, a.q4[j].y);
@@ -229,7 +278,7 @@ __global__ __launch_bounds__(BLOCK) void k_ip_rhs(RhsArgs a) {
         return B;
     };
     double touched = 0.0, tpairs = 0.0;
-    if (i < a.R.nd) {
+    if (gt.on) {
         float4 qi = a.q4[i];
         float fki = qi.x, mi = qi.y;
         bool act = qi.z != 0.0f;
@@ -253,7 +302,7 @@ __global__ __launch_bounds__(BLOCK) void k_ip_rhs(RhsArgs a) {
         }
     }
     double v[2] = {touched, tpairs};
-    block_partials<2>(v, a.partials, a.stride);
+    row_partials<SC, 2>(v, gt, a.partials, a.stride, blk);
 }
 
 struct CGInitArgs {
@@ -266,13 +315,18 @@ struct CGInitArgs {
     float2* y2;
     double* partials;
     int stride;
+    SceneRef sc;
 };
 
 // IP.PCG prologue (ip.py:292-304): x = 0, r = b, rr = r.r, p = r
+template <bool SC>
 __global__ __launch_bounds__(BLOCK) void k_cg_init(CGInitArgs a) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int blk = blockIdx.x;
+    int i = blk * BLOCK + threadIdx.x;
+    Gate gt = gate_all(i, a.nd);
+    if (SC && !gate_block(a.sc, blk, i, a.nd, PH_RHS, gt)) return;
     double rr = 0.0;
-    if (i < a.nd) {
+    if (gt.on) {
         float bx = a.b[3 * i + 0], by = a.b[3 * i + 1], bz = a.b[3 * i + 2];
         float m = a.pos[i].w;
         a.sol[3 * i + 0] = 0.0f; a.sol[3 * i + 1] = 0.0f; a.sol[3 * i + 2] = 0.0f;
@@ -282,7 +336,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_init(CGInitArgs a) {
         rr = (double)(bx * bx + by * by + bz * bz);
     }
     double v[1] = {rr};
-    block_partials<1>(v, a.partials, a.stride);
+    row_partials<SC, 1>(v, gt, a.partials, a.stride, blk);
 }
 
 struct CGJArgs {
@@ -292,16 +346,19 @@ struct CGJArgs {
     const float* sfac;
     float2* y2;
     const SolverState* st;
+    SceneRef sc;
 };
 
 // compute_Jx_pcg (ip.py:19-60): y_i = dfdt_i k_i (J p)_i on rows with dfdt > 0, J p in the form
 // sum_j dot(p_i - p_j, m_j gradW_ij) + dot(p_i, S_i) (ip.py:51-58)
-template <class FM>
+template <class FM, bool SC>
 __global__ __launch_bounds__(BLOCK) void k_cg_J(CGJArgs a) {
-    if (a.st->active == 0) return;
+    if (!SC && a.st->active == 0) return;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = blk * BLOCK + threadIdx.x;
-    if (i >= a.R.nd) return;
+    Gate gt = gate_all(i, a.R.nd);
+    if (SC && !gate_block(a.sc, blk, i, a.R.nd, PH_CG, gt)) return;
+    if (!gt.on) return;
     const float sk = a.sfac[i];
     if (sk == 0.0f) return;
     // neighbour record: A = (x, y, z, m), B = p
@@ -329,16 +386,19 @@ struct CGJTArgs {
     double* partials;
     int stride;
     const SolverState* st;
+    SceneRef sc;
 };
 
 // compute_Ax_part1 + compute_Ax_part2 (ip.py:63-121) in gather form, fused with p.Ap
 // (ip.py:307)
-template <class FM>
+template <class FM, bool SC>
 __global__ __launch_bounds__(BLOCK) void k_cg_JT(CGJTArgs a) {
-    if (a.st->active == 0) return;
+    if (!SC && a.st->active == 0) return;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = blk * BLOCK + threadIdx.x;
-    const bool touched = i < a.R.nd && a.nbact[i] != 0;
+    Gate gt = gate_all(i, a.R.nd);
+    if (SC && !gate_block(a.sc, blk, i, a.R.nd, PH_CG, gt)) return;
+    const bool touched = gt.on && a.nbact[i] != 0;
     // neighbour record: A = (x, y, z, m), B = y; y2 = (y, m)
     auto loadA = [&](int j) {
         return FM::EVAL ? a.R.pos[j] : make_float4(0.0f, 0.0f, 0.0f, a.y2[j].y);
@@ -349,7 +409,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_JT(CGJTArgs a) {
         return B;
     };
     double pap = 0.0;
-    if (i < a.R.nd) {
+    if (gt.on) {
         float4 pi = a.p4[i];
         float mi = pi.w;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
@@ -371,7 +431,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_JT(CGJTArgs a) {
         pap = (double)(pi.x * Apx + pi.y * Apy + pi.z * Apz);
     }
     double v[1] = {pap};
-    block_partials<1>(v, a.partials, a.stride);
+    row_partials<SC, 1>(v, gt, a.partials, a.stride, blk);
 }
 
 struct CGUpdArgs {
@@ -383,15 +443,20 @@ struct CGUpdArgs {
     double* partials;
     int stride;
     const SolverState* st;
+    SceneRef sc;
 };
 
 // pcg_update_x_r_scalar (math_utils.py:180-195) + r.r (ip.py:314-317)
+template <bool SC>
 __global__ __launch_bounds__(BLOCK) void k_cg_update(CGUpdArgs a) {
-    if (a.st->active == 0) return;
-    float alpha = (float)a.st->alpha;
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (!SC && a.st->active == 0) return;
+    const int blk = blockIdx.x;
+    int i = blk * BLOCK + threadIdx.x;
+    Gate gt = gate_all(i, a.nd);
+    if (SC && !gate_block(a.sc, blk, i, a.nd, PH_CG, gt)) return;
+    float alpha = (float)(SC ? a.sc.ss[gt.s].cg.alpha : a.st->alpha);
     double rr = 0.0;
-    if (i < a.nd) {
+    if (gt.on) {
         float4 p = a.p4[i];
         float x0 = a.sol[3 * i + 0] + alpha * p.x;
         float x1 = a.sol[3 * i + 1] + alpha * p.y;
@@ -404,16 +469,20 @@ __global__ __launch_bounds__(BLOCK) void k_cg_update(CGUpdArgs a) {
         rr = (double)(r0 * r0 + r1 * r1 + r2 * r2);
     }
     double v[1] = {rr};
-    block_partials<1>(v, a.partials, a.stride);
+    row_partials<SC, 1>(v, gt, a.partials, a.stride, blk);
 }
 
 // p = r + beta p (add_vec3, ip.py:323-326)
+template <bool SC>
 __global__ __launch_bounds__(BLOCK) void k_cg_p(int nd, const float* __restrict__ r, float4* __restrict__ p4,
-                                                const SolverState* __restrict__ st) {
-    if (st->active == 0) return;
-    float beta = (float)st->beta;
-    int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= nd) return;
+                                                const SolverState* __restrict__ st, SceneRef sc) {
+    if (!SC && st->active == 0) return;
+    const int blk = blockIdx.x;
+    int i = blk * BLOCK + threadIdx.x;
+    Gate gt = gate_all(i, nd);
+    if (SC && !gate_block(sc, blk, i, nd, PH_CG, gt)) return;
+    if (!gt.on) return;
+    float beta = (float)(SC ? sc.ss[gt.s].cg.beta : st->beta);
     float4 p = p4[i];
     p.x = r[3 * i + 0] + beta * p.x;
     p.y = r[3 * i + 1] + beta * p.y;
@@ -450,9 +519,14 @@ __global__ __launch_bounds__(BLOCK) void k_cg_refresh(int nd, const float* __res
 }
 
 // d -= sol (add_vec3 with scale -1, ip.py:389)
-__global__ __launch_bounds__(BLOCK) void k_ip_dupdate(int nd, float4* __restrict__ d4, const float* __restrict__ sol) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= nd) return;
+template <bool SC>
+__global__ __launch_bounds__(BLOCK) void k_ip_dupdate(int nd, float4* __restrict__ d4, const float* __restrict__ sol,
+                                                      SceneRef sc) {
+    const int blk = blockIdx.x;
+    int i = blk * BLOCK + threadIdx.x;
+    Gate gt = gate_all(i, nd);
+    if (SC && !gate_block(sc, blk, i, nd, PH_DUPDATE, gt)) return;
+    if (!gt.on) return;
     float4 d = d4[i];
     d.x = d.x + -1.0f * sol[3 * i + 0];
     d.y = d.y + -1.0f * sol[3 * i + 1];
@@ -764,15 +838,22 @@ __global__ __launch_bounds__(BLOCK) void k_gravity(int nd, float gx, float gy, f
     vel[3 * i + 2] = vel[3 * i + 2] + gz;
 }
 
-// advect_position (framework.py:117-136) + enforce_boundary (boundary.py:27-91)
+// advect_position (framework.py:117-136) + enforce_boundary (boundary.py:27-91); with SC each
+// particle is clamped to its scene's domain size sdom[3 sid[i] ..].
+template <bool SC>
 __global__ __launch_bounds__(BLOCK) void k_advect_boundary(int nd, float dt, float L0, float L1, float L2, float pad,
-                                                           float4* __restrict__ pos, float* __restrict__ vel) {
+                                                           float4* __restrict__ pos, float* __restrict__ vel,
+                                                           const float* __restrict__ sdom, const int* __restrict__ sid) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= nd) return;
     float4 p = pos[i];
     float v[3] = {vel[3 * i + 0], vel[3 * i + 1], vel[3 * i + 2]};
     float x[3] = {p.x + dt * v[0], p.y + dt * v[1], p.z + dt * v[2]};
     float L[3] = {L0, L1, L2};
+    if (SC) {
+        const int s = sid[i];
+        L[0] = sdom[3 * s + 0]; L[1] = sdom[3 * s + 1]; L[2] = sdom[3 * s + 2];
+    }
     float nrm[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -791,6 +872,118 @@ __global__ __launch_bounds__(BLOCK) void k_advect_boundary(int nd, float dt, flo
 }
 
 Rows rows_of(ivp_ctx* c) { return Rows{c->nb, c->tile_len, c->Cs, c->nd, c->pos, c->h, c->kspiky, c->gscale}; }
+
+// ---- independent scenes -------------------------------------------------------------------
+
+// Start of a batched solve: every scene begins its outer loop (ip.py:345)
+__global__ __launch_bounds__(BLOCK) void k_scene_init(int S, SceneState* __restrict__ ss) {
+    int s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= S) return;
+    SceneState& q = ss[s];
+    q.phase = PH_PACK;
+    q.opt_iter = 0;
+    q.outer_iters = 0;
+    q.status = IVP_OK;
+    q.cg_total = 0;
+    q.n_err = 0;
+    q.final_err = 0.0;
+    q.cg.active = 0;
+    q.cg.status = 0;
+    q.cg.iter = 0;
+}
+
+// End of a replay (one block): scenes whose displacement was just updated start their next
+// outer iteration (ip.py:389-391); *active = scenes not done.
+__global__ __launch_bounds__(BLOCK) void k_scene_advance(int S, SceneState* __restrict__ ss, int* __restrict__ active) {
+    __shared__ double lds[WAVES_PER_BLOCK];
+    int cnt = 0;
+    for (int s = threadIdx.x; s < S; s += BLOCK) {
+        SceneState& q = ss[s];
+        if (q.phase == PH_DUPDATE) {
+            q.phase = PH_PACK;
+            q.opt_iter = q.opt_iter + 1;
+        }
+        cnt += q.phase != PH_DONE ? 1 : 0;
+    }
+    double t = block_sum((double)cnt, lds);
+    if (threadIdx.x == 0) *active = (int)t;
+}
+
+// One replay of the batched outer iteration on stream `st`: constraint pass for scenes starting
+// an outer iteration, right-hand side and CG set-up for those that continue, K CG iterations
+// for scenes in their CG, the displacement update for scenes whose CG ended.  Every launch
+// covers all rows; rows of scenes in another phase skip (whole blocks when both of their
+// scenes do).
+int enqueue_scene_replay(ivp_ctx* c, hipStream_t st, float omega, int K) {
+    const int nd = c->nd;
+    const dim3 nb(blocks_for(nd)), blk(BLOCK);
+    const dim3 nf((c->S + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
+    const int stride = c->max_blocks;
+    Rows R = rows_of(c);
+    SceneRef sc{c->sid, c->ss};
+    double* P = c->spartials;
+    FinScenesArgs fa{P, stride, c->sid, c->ss, c->sprm, c->shist_err, c->shist_pcg, c->S};
+    PackArgs pa{R, c->d4, c->Ss, c->cc, c->kk, c->rho0, c->rowcnt, c->q4, c->sfac, c->jd, P, stride, sc};
+    hipLaunchKernelGGL((k_ip_pack<FmtIds, true>), nb, blk, 0, st, pa);
+    hipLaunchKernelGGL(k_fin_scenes<FIN_PACK>, nf, blk, 0, st, fa);
+    RhsArgs ra{R, c->q4, c->Ss, omega, c->b, c->nbact, c->rowcnt, P, stride, sc};
+    hipLaunchKernelGGL((k_ip_rhs<FmtIds, true>), nb, blk, 0, st, ra);
+    hipLaunchKernelGGL(k_fin_scenes<FIN_RHS>, nf, blk, 0, st, fa);
+    CGInitArgs ia{nd, c->b, c->pos, c->sol, c->r, c->p4, c->y2, P, stride, sc};
+    hipLaunchKernelGGL(k_cg_init<true>, nb, blk, 0, st, ia);
+    hipLaunchKernelGGL(k_fin_scenes<FIN_CG_INIT>, nf, blk, 0, st, fa);
+    CGJArgs ja{R, c->p4, c->Ss, c->sfac, c->y2, c->state, sc};
+    CGJTArgs ta{R, c->p4, c->y2, c->Ss, c->nbact, omega, c->Ap, P, stride, c->state, sc};
+    CGUpdArgs ua{nd, c->sol, c->r, c->p4, c->Ap, P, stride, c->state, sc};
+    for (int k = 0; k < K; ++k) {
+        hipLaunchKernelGGL((k_cg_J<FmtIds, true>), nb, blk, 0, st, ja);
+        hipLaunchKernelGGL((k_cg_JT<FmtIds, true>), nb, blk, 0, st, ta);
+        hipLaunchKernelGGL(k_fin_scenes<FIN_CG_PAP>, nf, blk, 0, st, fa);
+        hipLaunchKernelGGL(k_cg_update<true>, nb, blk, 0, st, ua);
+        hipLaunchKernelGGL(k_fin_scenes<FIN_CG_UPDATE>, nf, blk, 0, st, fa);
+        hipLaunchKernelGGL(k_cg_p<true>, nb, blk, 0, st, nd, (const float*)c->r, c->p4,
+                           (const SolverState*)c->state, sc);
+    }
+    hipLaunchKernelGGL(k_ip_dupdate<true>, nb, blk, 0, st, nd, c->d4, (const float*)c->sol, sc);
+    hipLaunchKernelGGL(k_scene_advance, dim3(1), blk, 0, st, c->S, c->ss, c->sactive);
+    IVP_HIP(hipGetLastError());
+    return IVP_OK;
+}
+
+// The instantiated graph of one replay for (K, omega) and the current store buffers (the sort
+// alternates two buffers, so at most two captures are live).
+int scene_graph(ivp_ctx* c, float omega, int K, hipGraphExec_t* out) {
+    for (auto& g : c->sgraphs) {
+        if (g.exec && g.pos == (const void*)c->pos && g.chunk == K && g.omega == omega && g.nd == c->nd) {
+            *out = g.exec;
+            return IVP_OK;
+        }
+    }
+    if (!c->cap_stream) IVP_HIP(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+    hipGraph_t graph = nullptr;
+    IVP_HIP(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeRelaxed));
+    int rc = enqueue_scene_replay(c, c->cap_stream, omega, K);
+    hipError_t e = hipStreamEndCapture(c->cap_stream, &graph);
+    if (rc != IVP_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    IVP_HIP(e);
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    IVP_HIP(e);
+    SceneGraph& slot = c->sgraphs[c->sgraph_next];
+    c->sgraph_next ^= 1;
+    if (slot.exec) (void)hipGraphExecDestroy(slot.exec);
+    slot.exec = exec;
+    slot.pos = (const void*)c->pos;
+    slot.chunk = K;
+    slot.omega = omega;
+    slot.nd = c->nd;
+    *out = exec;
+    return IVP_OK;
+}
 
 int read_state(ivp_ctx* c) {
     IVP_HIP(hipMemcpyAsync(c->state_host, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
@@ -852,7 +1045,7 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
     for (;;) {
         PackArgs pa{R, d4, c->Ss, c->cc, c->kk, c->rho0, c->rowcnt, c->q4, c->sfac, c->jd, c->partials, stride};
         double total_pairs = (double)c->state_host->total_pairs;
-        IVP_TRY(launch(c, KC_IP_PACK, -1, pb * total_pairs + 68.0 * nd, k_ip_pack<FP>, nb, pa));
+        IVP_TRY(launch(c, KC_IP_PACK, -1, pb * total_pairs + 68.0 * nd, k_ip_pack<FP, false>, nb, pa));
         IVP_TRY(finalize<FIN_PACK>(c, nb.x));
         IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         IVP_TRY(read_state(c));
@@ -867,10 +1060,10 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
         int act_rows = c->state_host->n_active_rows;
 
         RhsArgs ra{R, c->q4, c->Ss, prm->omega, c->b, c->nbact, c->rowcnt, c->partials, stride};
-        IVP_TRY(launch(c, KC_IP_RHS, -1, pb * total_pairs + 48.0 * nd, k_ip_rhs<FP>, nb, ra));
+        IVP_TRY(launch(c, KC_IP_RHS, -1, pb * total_pairs + 48.0 * nd, k_ip_rhs<FP, false>, nb, ra));
         IVP_TRY(finalize<FIN_RHS>(c, nb.x));
         CGInitArgs ia{nd, c->b, c->pos, c->sol, c->r, c->p4, c->y2, c->partials, stride};
-        IVP_TRY(launch(c, -1, -1, 0.0, k_cg_init, nb, ia));
+        IVP_TRY(launch(c, -1, -1, 0.0, k_cg_init<false>, nb, ia));
         IVP_TRY(finalize<FIN_CG_INIT>(c, nb.x, prm->tol_pcg, prm->max_iteration_pcg));
         IVP_TRY(read_state(c));
         long long touch_pairs = c->state_host->touched_pairs;
@@ -895,17 +1088,17 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
             CGJTArgs tx{R, c->xs4, c->y2, c->Ss, c->nbact, prm->omega, c->Ap, c->partials + 2 * stride, stride,
                         c->state};
             auto one_iter = [&](int tag) -> int {
-                IVP_TRY(launch(c, KC_CG_J, tag, bJ, k_cg_J<FC>, nb, ja));
-                IVP_TRY(launch(c, KC_CG_JT, tag, bJT, k_cg_JT<FC>, nb, ta));
+                IVP_TRY(launch(c, KC_CG_J, tag, bJ, k_cg_J<FC, false>, nb, ja));
+                IVP_TRY(launch(c, KC_CG_JT, tag, bJT, k_cg_JT<FC, false>, nb, ta));
                 IVP_TRY(finalize<FIN_CG_PAP>(c, nb.x));
-                IVP_TRY(launch(c, KC_CG_UPDATE, tag, bU, k_cg_update, nb, ua));
+                IVP_TRY(launch(c, KC_CG_UPDATE, tag, bU, k_cg_update<false>, nb, ua));
                 if (FC::GS == GS_F16 && (tag + 1) % REFRESH_EVERY == 0) {
                     // residual replacement: r = b - A x with A in float32 from the positions;
                     // its r.r (row 1) and the recursive r.r (row 0) go to the stop test and beta
                     IVP_TRY(launch(c, KC_CG_REFRESH, tag, 0.0, k_x4, nb, nd, (const float*)c->sol,
                                    (const float4*)c->pos, c->xs4, (const SolverState*)c->state));
-                    IVP_TRY(launch(c, KC_CG_REFRESH, tag, 0.0, k_cg_J<FE>, nb, jx));
-                    IVP_TRY(launch(c, KC_CG_REFRESH, tag, 0.0, k_cg_JT<FE>, nb, tx));
+                    IVP_TRY(launch(c, KC_CG_REFRESH, tag, 0.0, k_cg_J<FE, false>, nb, jx));
+                    IVP_TRY(launch(c, KC_CG_REFRESH, tag, 0.0, k_cg_JT<FE, false>, nb, tx));
                     IVP_TRY(launch(c, KC_CG_REFRESH, tag, 0.0, k_cg_refresh, nb, nd, (const float*)c->b,
                                    (const float*)c->Ap, c->r, c->partials + stride, stride,
                                    (const SolverState*)c->state));
@@ -913,8 +1106,8 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
                 } else {
                     IVP_TRY(finalize<FIN_CG_UPDATE>(c, nb.x));
                 }
-                return launch(c, KC_CG_P, tag, bP, k_cg_p, nb, nd, (const float*)c->r, c->p4,
-                              (const SolverState*)c->state);
+                return launch(c, KC_CG_P, tag, bP, k_cg_p<false>, nb, nd, (const float*)c->r, c->p4,
+                              (const SolverState*)c->state, SceneRef{});
             };
             IVP_TRY(run_chunks(c, prm->max_iteration_pcg, cg_guess, one_iter, &iters, &st.cg_launch_chunks));
             prof_collect(c, iters, epoch);
@@ -929,7 +1122,7 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
         }
         if (st.n_pcg < st.capacity && st.pcg_iters) st.pcg_iters[st.n_pcg] = iters;
         st.n_pcg++;
-        IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_dupdate, nb, nd, d4, (const float*)c->sol));
+        IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_dupdate<false>, nb, nd, d4, (const float*)c->sol, SceneRef{}));
         opt_iter++;
     }
     st.outer_iters = opt_iter;
@@ -1068,6 +1261,50 @@ int visc_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* out) {
     return IVP_OK;
 }
 
+// IP.solve (ip.py:331-404) for every scene of the context at once: per-scene outer loops and
+// CGs advanced by replays of one captured launch sequence, one host synchronisation per replay.
+int ip_solve_scenes(ivp_ctx* c, float dt, const ivp_ip_params* prm) {
+    if (!c->built || c->S == 0) return IVP_ERR_STATE;
+    if (!prm) return IVP_ERR_ARG;
+    const int nd = c->nd;
+    const dim3 nb(blocks_for(nd));
+    SceneParams& hp = *c->sprm_host;
+    hp.tol_opt = prm->tol_opt;
+    hp.tol_pcg = prm->tol_pcg;
+    hp.max_opt = prm->max_iteration_opt;
+    hp.max_pcg = prm->max_iteration_pcg;
+    hp.fixed = prm->use_fixed_iters ? 1 : 0;
+    IVP_HIP(hipMemcpyAsync(c->sprm, c->sprm_host, sizeof(SceneParams), hipMemcpyHostToDevice, c->stream));
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_init, nb, nd, dt, (const float*)c->density, (const float*)c->rho0,
+                   (const float*)c->vel, (const float4*)c->pos, c->cc, c->d4));
+    hipLaunchKernelGGL(k_scene_init, dim3((c->S + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, c->stream, c->S, c->ss);
+    IVP_HIP(hipGetLastError());
+    const int K = std::max(1, c->scene_chunk);
+    hipGraphExec_t exec = nullptr;
+    if (c->scene_graph) IVP_TRY(scene_graph(c, prm->omega, K, &exec));
+    // every replay advances each unfinished scene by a phase or by K CG iterations
+    const long long cap = (long long)(std::max(prm->max_iteration_opt, 0) + 1) *
+                          ((long long)std::max(prm->max_iteration_pcg, 0) / K + 3) + 2;
+    c->scene_replays = 0;
+    for (;;) {
+        if (exec)
+            IVP_HIP(hipGraphLaunch(exec, c->stream));
+        else
+            IVP_TRY(enqueue_scene_replay(c, c->stream, prm->omega, K));
+        c->scene_replays++;
+        IVP_HIP(hipMemcpyAsync(c->sactive_host, c->sactive, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        IVP_HIP(hipStreamSynchronize(c->stream));
+        if (*c->sactive_host == 0) break;
+        if (c->scene_replays >= cap) return IVP_ERR_INTERNAL;
+    }
+    float inv_dt = (float)(1.0 / (double)dt);
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_finish, nb, nd, inv_dt, (const float4*)c->d4, (const float*)c->jd, c->vel,
+                   c->divg));
+    IVP_HIP(hipMemcpyAsync(c->ss_host, c->ss, sizeof(SceneState) * (size_t)c->S, hipMemcpyDeviceToHost, c->stream));
+    IVP_HIP(hipStreamSynchronize(c->stream));
+    return IVP_OK;
+}
+
 int glue_gravity(ivp_ctx* c, const float g[3], float dt) {
     if (!c->loaded) return IVP_ERR_STATE;
     if (c->nd == 0) return IVP_OK;
@@ -1078,8 +1315,13 @@ int glue_gravity(ivp_ctx* c, const float g[3], float dt) {
 int glue_advect_boundary(ivp_ctx* c, float dt) {
     if (!c->loaded) return IVP_ERR_STATE;
     if (c->nd == 0) return IVP_OK;
-    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_advect_boundary, dim3(blocks_for(c->nd)), c->nd, dt, c->cfg.domain_size[0],
-                   c->cfg.domain_size[1], c->cfg.domain_size[2], c->h, c->pos, c->vel));
+    if (c->S > 0)
+        IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_advect_boundary<true>, dim3(blocks_for(c->nd)), c->nd, dt, 0.0f, 0.0f,
+                       0.0f, c->h, c->pos, c->vel, (const float*)c->sdomain, (const int*)c->sid));
+    else
+        IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_advect_boundary<false>, dim3(blocks_for(c->nd)), c->nd, dt,
+                       c->cfg.domain_size[0], c->cfg.domain_size[1], c->cfg.domain_size[2], c->h, c->pos, c->vel,
+                       (const float*)nullptr, (const int*)nullptr));
     c->built = false;
     return IVP_OK;
 }

@@ -173,14 +173,14 @@ _SYNTHETIC = {
 }
 
 
-def synthetic_scene(cfg_id: int, scale=None, box=None) -> dict:
+def synthetic_scene(cfg_id: int, scale=None, box=None, seed=None) -> dict:
     """Scene JSON dict of a synthetic benchmark scene.
 
     A fluid block of nx*ny*nz particles sits at voxel centres (k + 1/2) d with a uniform jitter of
     +-0.01 d, resting on the floor of a static two-layer particle box whose inner faces are at 0
     and L (shifted by the margin o inside the domain [0, L + 2o], so the domain clamp never
-    engages).  ``scale`` optionally replaces the fluid counts and ``box`` the box size L (for
-    reduced test sizes).
+    engages).  ``scale`` optionally replaces the fluid counts, ``box`` the box size L and ``seed``
+    the jitter seed (reduced test sizes, the scenes of ``synthetic_batch``).
       0: 16x32x16 = 8192 particles, box 0.64 x 1.0 x 0.64 m, r = 0.01, dt = 1 ms, no viscosity.
       1: 40x64x40 = 102400 particles, box 1.6 x 3.0 x 1.6 m, r = 0.01, dt = 1 ms,
          explicit viscosity 0.01.
@@ -230,7 +230,7 @@ def synthetic_scene(cfg_id: int, scale=None, box=None) -> dict:
             "velocity": [0.0, 0.0, 0.0],
             "density": 1000.0,
             "jitter": 0.01,
-            "jitterSeed": s["seed"],
+            "jitterSeed": s["seed"] if seed is None else int(seed),
         }],
         "GeneratedBodies": [{
             "objectId": 1,
@@ -242,6 +242,29 @@ def synthetic_scene(cfg_id: int, scale=None, box=None) -> dict:
             "isDynamic": False,
         }],
     }
+
+
+def synthetic_batch(n_scenes=256, seed=4, count_range=(2048, 32768), aspect_range=(0.5, 4.0)):
+    """Scene JSON dicts of the batched workload: independent fluid columns with the physics of
+    synthetic scene 0 (r = 0.01, dt = 1 ms, no viscosity).  With rng = default_rng(seed), the
+    particle counts n_s = rng.integers(lo, hi + 1, n_scenes) and then the aspect ratios
+    a_s = height / width = rng.uniform(a_lo, a_hi, n_scenes) are drawn; scene s has
+    nx = nz = round((n_s / a_s)^(1/3)) and ny = round(n_s / nx^2) fluid particles per axis, a box
+    (2 w, 1.5625 hgt, 2 w) with w = nx d and hgt = ny d (scene 0's proportions), and jitter seed
+    1000 + s."""
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(count_range[0], count_range[1] + 1, size=n_scenes)
+    aspects = rng.uniform(aspect_range[0], aspect_range[1], size=n_scenes)
+    d = 2.0 * _SYNTHETIC[0]["r"]
+    out = []
+    for s in range(n_scenes):
+        nx = max(1, int(round((float(counts[s]) / float(aspects[s])) ** (1.0 / 3.0))))
+        ny = max(1, int(round(float(counts[s]) / (nx * nx))))
+        box = (2.0 * nx * d, 1.5625 * ny * d, 2.0 * nx * d)
+        scene = synthetic_scene(0, scale=(nx, ny, nx), box=box, seed=1000 + s)
+        scene["Configuration"]["name"] = f"batch_scene_{s}"
+        out.append(scene)
+    return out
 
 
 def particle_masses(config: SimConfig, ps: ParticleSet):

@@ -176,6 +176,54 @@ int ivp_advect_boundary(ivp_ctx *ctx, float dt);
  * viscosity, IP solve, advect, boundary, neighbour rebuild, density, Aii. */
 int ivp_step(ivp_ctx *ctx, const ivp_step_params *prm, ivp_ip_stats *stats, ivp_cg_stats *visc);
 
+/* ---- independent scenes in one context ------------------------------------------------------
+ * No counterpart in the reference (one Framework per scene there).  For callers that simulate
+ * many small scenes at once: every scene keeps its own coordinates, cell grid, IP outer loop,
+ * CG and boundary clamp, exactly as if it ran alone in its own context, while each pass of the
+ * step runs once over all scenes. */
+typedef struct {
+    int n_particles;         /* particles of the scene; scenes are consecutive in user order */
+    float domain_start[3];   /* the scene's domainStart, in its own coordinates             */
+    float domain_size[3];    /* the scene's domainEnd - domainStart                         */
+} ivp_scene;
+
+/* Declare the particle set of the next ivp_set_particles as n_scenes consecutive scenes of the
+ * given sizes (their sum must equal that call's n).  support_radius, cache_size and
+ * material_fluid come from ivp_create; ivp_create's domain is not used afterwards.  Every scene
+ * needs at least 256 dynamic particles.  Entry points for one scene (ivp_ip_solve, ivp_step,
+ * the viscosity solvers) return IVP_ERR_STATE on such a context; ivp_neighbors_build,
+ * ivp_read / ivp_write, ivp_neighbors_export (ids are user indices over all scenes) and
+ * ivp_apply_gravity work on all scenes. */
+int ivp_set_scenes(ivp_ctx *ctx, int n_scenes, const ivp_scene *scenes);
+
+typedef struct {
+    int status;                  /* IVP_OK, IVP_NOT_CONVERGED or IVP_NOT_PD for this scene  */
+    int outer_iters;             /* stats_outer_iters                                       */
+    int cg_iters;                /* CG iterations summed over the outer iterations          */
+    double final_density_error;  /* stats_final_density_error                               */
+} ivp_scene_stats;
+
+/* Framework.forward (framework.py:423-544, velocity-based IP branch, viscosity off:
+ * viscosity_coeff must be 0) for every scene: gravity, IP solve, advect, per-scene boundary
+ * clamp.  The IP outer iterations of all scenes advance together: each replay of one captured
+ * launch sequence runs the constraint pass for scenes that start an outer iteration, the
+ * right-hand side and CG set-up, a fixed number of CG iterations for scenes in their CG, and
+ * the displacement update for scenes whose CG ended; each scene stops on its own density error
+ * (densityErrorTol / maxOuterIters semantics of ip.py:345-395).  stats has n_scenes entries.
+ * Returns IVP_OK or the first scene's non-OK status. */
+int ivp_step_scenes(ivp_ctx *ctx, const ivp_step_params *prm, ivp_scene_stats *stats);
+
+/* Outer-loop history of one scene in the last ivp_step_scenes: stats_density_errors and
+ * stats_pcg_iters (at most `capacity` entries each; *n_errors and *n_pcg give the counts). */
+int ivp_scene_history(ivp_ctx *ctx, int scene, int capacity, double *density_errors, int *pcg_iters,
+                      int *n_errors, int *n_pcg);
+
+/* Options of the scene solve: CG iterations per replay of the launch sequence (default 4), and
+ * whether the sequence is captured once into a hipGraph and replayed (1, default) or enqueued
+ * launch by launch (0). */
+#define IVP_OPT_SCENE_CG_CHUNK 2
+#define IVP_OPT_SCENE_GRAPH 3
+
 /* ---- instrumentation ---------------------------------------------------------------------- */
 /* Per-kernel-class event timing on the context's stream.  Only launches that did work are
  * counted (CG launches after convergence inside a launch chunk are excluded). */

@@ -1,0 +1,179 @@
+"""Independent scenes in one context (ivp_set_scenes / ivp_step_scenes) against the same scenes
+run one per context, and against the oracle.
+
+A scene in the batch has its own grid with the single-scene grid's origin, dims and key order,
+so its neighbour rows, density and k are bit-identical to a context holding only that scene.
+The solve's reductions group rows into blocks differently (a block can span two scenes), so the
+FP64 sums, and from them alpha and beta, can differ in the last bits: bars for the solve are the
+single-scene parity bars (velocities and positions within 1e-5 relative, CG iterations within
++-1, same outer iterations, final density error at or below densityErrorTol).  The hipGraph
+replay and the launch-by-launch form of the batched solve are bitwise equal over several steps.
+"""
+
+import numpy as np
+import pytest
+
+from implicitvelocityprojection_amd.config import SimConfig
+from implicitvelocityprojection_amd.scenes import load_particles, particle_masses, synthetic_batch
+from oracle import oracle as O
+from tests.helpers import ip_kwargs, rel_inf
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+V_TOL = 1e-5
+STEPS = 3
+
+
+def _scenes():
+    return synthetic_batch(n_scenes=5, seed=4, count_range=(2048, 6000))
+
+
+def _single(scene):
+    from implicitvelocityprojection_amd.context import Context
+    cfg = SimConfig.from_dict(scene)
+    ps = load_particles(cfg)
+    m, m_inv = particle_masses(cfg, ps)
+    dev = torch.device("cuda")
+    t = lambda a, dt=torch.float32: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)
+    ctx = Context(float(np.float32(cfg.support_radius)), cfg.domain_start, cfg.domain_size, ps.n,
+                  cache_size=int(cfg.get_cfg("neighborCacheSize", 60)))
+    ctx.set_particles(t(ps.x), t(ps.v), t(m), t(m_inv), t(ps.density0), t(ps.is_dynamic, torch.int32),
+                      t(ps.material, torch.int32))
+    assert ctx.neighbors_build() == 0
+    return cfg, ps, ctx
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def test_scene_build_bit_exact_against_single_contexts():
+    from implicitvelocityprojection_amd.scene_batch import SceneBatch
+    scenes = _scenes()
+    batch = SceneBatch(scenes)
+    dens_b = batch.read(2, (batch.n,))
+    k_b = batch.read(3, (batch.n,))
+    for s, scene in enumerate(scenes):
+        _, ps, ctx = _single(scene)
+        sl = batch.scene_slice(s)
+        dens = _np(ctx.read(2, torch.zeros(ps.n, device="cuda")))
+        k = _np(ctx.read(3, torch.zeros(ps.n, device="cuda")))
+        assert np.array_equal(dens_b[sl].view(np.uint32), dens.view(np.uint32)), s
+        assert np.array_equal(k_b[sl].view(np.uint32), k.view(np.uint32)), s
+        ctx.close()
+    batch.close()
+
+
+def test_scene_neighbour_export_matches_single_contexts():
+    from implicitvelocityprojection_amd.scene_batch import SceneBatch
+    scenes = _scenes()[:3]
+    batch = SceneBatch(scenes)
+    C = 60
+    cnt_b = torch.zeros(batch.n, dtype=torch.int32, device="cuda")
+    ids_b = torch.zeros(batch.n * C, dtype=torch.int32, device="cuda")
+    g_b = torch.zeros(batch.n * C * 3, dtype=torch.float32, device="cuda")
+    batch.ctx.neighbors_export(cnt_b, ids_b, g_b)
+    cnt_b, ids_b = _np(cnt_b), _np(ids_b).reshape(batch.n, C)
+    for s, scene in enumerate(scenes):
+        _, ps, ctx = _single(scene)
+        cnt = torch.zeros(ps.n, dtype=torch.int32, device="cuda")
+        ids = torch.zeros(ps.n * C, dtype=torch.int32, device="cuda")
+        g = torch.zeros(ps.n * C * 3, dtype=torch.float32, device="cuda")
+        ctx.neighbors_export(cnt, ids, g)
+        cnt, ids = _np(cnt), _np(ids).reshape(ps.n, C)
+        sl = batch.scene_slice(s)
+        off = int(batch.offsets[s])
+        assert np.array_equal(cnt_b[sl], cnt)
+        live = np.arange(C)[None, :] < cnt[:, None]
+        assert np.array_equal(ids_b[sl][live] - off, ids[live])
+        ctx.close()
+    batch.close()
+
+
+def test_scene_steps_match_single_contexts():
+    from implicitvelocityprojection_amd.context import IPParams
+    from implicitvelocityprojection_amd.scene_batch import SceneBatch
+    scenes = _scenes()
+    batch = SceneBatch(scenes, chunk=2)
+    hist_b = []
+    for _ in range(STEPS):
+        status, stats = batch.step()
+        assert status == 0
+        hist_b.append([batch.ctx.scene_history(s) for s in range(len(scenes))] + [stats])
+    x_b = batch.read(0, (batch.n, 3))
+    v_b = batch.read(1, (batch.n, 3))
+    for s, scene in enumerate(scenes):
+        cfg, ps, ctx = _single(scene)
+        kw = ip_kwargs(cfg)
+        dyn = ps.is_dynamic == 1
+        for step in range(STEPS):
+            res, _ = ctx.step(cfg.dt, cfg.gravity, 0, 0.0, IPParams(**kw))
+            errs_b, pcg_b = hist_b[step][s]
+            st_b = hist_b[step][-1][s]
+            assert st_b[0] == 0
+            assert st_b[1] == res.outer_iters, (s, step)
+            assert len(pcg_b) == len(res.pcg_iters)
+            for a, b in zip(pcg_b, res.pcg_iters):
+                assert abs(a - b) <= 1, (s, step, pcg_b, res.pcg_iters)
+            assert st_b[3] <= kw["tol_opt"]
+            assert len(errs_b) == len(res.density_errors)
+        x = _np(ctx.read(0, torch.zeros((ps.n, 3), device="cuda")))
+        v = _np(ctx.read(1, torch.zeros((ps.n, 3), device="cuda")))
+        sl = batch.scene_slice(s)
+        assert rel_inf(x_b[sl][dyn], x[dyn]) <= V_TOL, s
+        assert rel_inf(v_b[sl][dyn], v[dyn]) <= V_TOL, s
+        assert np.array_equal(x_b[sl][~dyn], x[~dyn])
+        ctx.close()
+    batch.close()
+
+
+def test_scene_step_matches_oracle():
+    from implicitvelocityprojection_amd.scene_batch import SceneBatch
+    scenes = _scenes()[:2]
+    batch = SceneBatch(scenes)
+    status, stats = batch.step()
+    assert status == 0
+    x_b = batch.read(0, (batch.n, 3))
+    v_b = batch.read(1, (batch.n, 3))
+    for s, scene in enumerate(scenes):
+        cfg = SimConfig.from_dict(scene)
+        sim = O.OracleSim(cfg, load_particles(cfg))
+        sim.rebuild()
+        sim.step()
+        dyn = sim.is_dynamic == 1
+        sl = batch.scene_slice(s)
+        assert stats[s][1] == sim.last_ip.outer_iters
+        assert rel_inf(v_b[sl][dyn], sim.v[dyn]) <= V_TOL
+        assert rel_inf(x_b[sl][dyn], sim.x[dyn]) <= V_TOL
+    batch.close()
+
+
+def test_scene_graph_replays_equal_direct_launches():
+    from implicitvelocityprojection_amd.scene_batch import SceneBatch
+    scenes = _scenes()[:3]
+    outs = []
+    for graph in (True, False):
+        batch = SceneBatch(scenes, graph=graph, chunk=2)
+        for _ in range(STEPS):
+            status, _ = batch.step()
+            assert status == 0
+            torch.cuda.synchronize()
+        outs.append((batch.read(0, (batch.n, 3)), batch.read(1, (batch.n, 3))))
+        batch.close()
+    assert np.array_equal(outs[0][0].view(np.uint32), outs[1][0].view(np.uint32))
+    assert np.array_equal(outs[0][1].view(np.uint32), outs[1][1].view(np.uint32))
+
+
+def test_scene_entry_points_of_one_scene_are_refused():
+    from implicitvelocityprojection_amd import _native as N
+    from implicitvelocityprojection_amd.context import IPParams
+    from implicitvelocityprojection_amd.scene_batch import SceneBatch
+    batch = SceneBatch(_scenes()[:2])
+    with pytest.raises(N.IVPError):
+        batch.ctx.ip_solve(1e-3, IPParams())
+    with pytest.raises(N.IVPError):
+        batch.ctx.step(1e-3, [0.0, -9.81, 0.0], 0, 0.0, IPParams())
+    with pytest.raises(N.IVPError):
+        batch.ctx.gradient_cache = 1
+    batch.close()

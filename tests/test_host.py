@@ -95,3 +95,29 @@ def test_neighbour_predicate_band(h):
     assert np.all(exact[d2 < lo])
     assert not np.any(exact[d2 >= hi])
     assert np.count_nonzero((d2 >= lo) & (d2 < hi)) < 64   # the band needing the root is a few ulps
+
+
+def test_synthetic_batch_scenes():
+    """The batched workload's scene list: seeded counts and aspect ratios, scene-0 proportions,
+    at least 256 fluid particles per scene (the library's per-scene minimum)."""
+    from implicitvelocityprojection_amd.config import SimConfig
+    from implicitvelocityprojection_amd.scenes import synthetic_batch
+    a = synthetic_batch(n_scenes=8)
+    b = synthetic_batch(n_scenes=8)
+    assert a == b
+    rng = np.random.default_rng(4)
+    counts = rng.integers(2048, 32769, size=8)
+    aspects = rng.uniform(0.5, 4.0, size=8)
+    for s, scene in enumerate(a):
+        cfg = SimConfig.from_dict(scene)
+        fb = scene["FluidBlocks"][0]
+        nx = int(round((counts[s] / aspects[s]) ** (1.0 / 3.0)))
+        ny = int(round(counts[s] / nx ** 2))
+        d = 0.02
+        n_axis = [int(round((fb["end"][q] - fb["start"][q]) / d + 0.5)) for q in range(3)]
+        assert n_axis == [nx, ny, nx]
+        assert nx * ny * nx >= 256
+        L = scene["GeneratedBodies"][0]["size"]
+        assert np.allclose(L, [2 * nx * d, 1.5625 * ny * d, 2 * nx * d])
+        assert fb["jitterSeed"] == 1000 + s
+        assert cfg.viscosity_coeff == 0.0
