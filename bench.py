@@ -94,6 +94,79 @@ def precision_study(ctx, cfg, ps, m, m_inv, prm, steps, torch, dev, n):
             "gradW": gw, "steps": rows, "seconds": time.perf_counter() - t0}
 
 
+def batched_study(n_scenes, steps, torch):
+    """The batched form: n_scenes independent columns (synthetic_batch, scene-0 physics) stepped
+    in one context, against the same scenes stepped one by one, each in its own context.
+    Inputs resident in HBM in both; the CG iterations per replay are autotuned first."""
+    from implicitvelocityprojection_amd.context import Context
+    from implicitvelocityprojection_amd.scene_batch import SceneBatch, _ip_params
+    from implicitvelocityprojection_amd.config import SimConfig
+    from implicitvelocityprojection_amd.scenes import load_particles, particle_masses, synthetic_batch
+    scenes = synthetic_batch(n_scenes=n_scenes)
+    batch = SceneBatch(scenes)
+    tune = batch.autotune(candidates=(1, 2, 4, 8), steps=2)
+    for _ in range(2):
+        batch.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cg = 0
+    max_err = 0.0
+    outer = 0
+    replays = 0
+    for _ in range(steps):
+        status, stats = batch.step()
+        if status != 0:
+            raise RuntimeError(f"scene step status {status}")
+        cg += sum(s[2] for s in stats)
+        outer = max(outer, max(s[1] for s in stats))
+        max_err = max(max_err, max(s[3] for s in stats))
+    torch.cuda.synchronize()
+    wall_b = time.perf_counter() - t0
+    nd = batch.n_dynamic
+    n_all = batch.n
+    batch.close()
+    # the same scenes one by one
+    dev = torch.device("cuda:0")
+    t = lambda a, dt=torch.float32: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)
+    ctxs = []
+    for sc in scenes:
+        cfg = SimConfig.from_dict(sc)
+        ps = load_particles(cfg)
+        m, mi = particle_masses(cfg, ps)
+        ctx = Context(float(np.float32(cfg.support_radius)), cfg.domain_start, cfg.domain_size, ps.n,
+                      cache_size=int(cfg.get_cfg("neighborCacheSize", 60)))
+        ctx.set_particles(t(ps.x), t(ps.v), t(m), t(mi), t(ps.density0), t(ps.is_dynamic, torch.int32),
+                          t(ps.material, torch.int32))
+        if ctx.neighbors_build() != 0:
+            raise RuntimeError("neighbour overflow at load")
+        ctxs.append((cfg, ctx))
+    prm = _ip_params(ctxs[0][0])
+    for cfg, ctx in ctxs:
+        ctx.step(cfg.dt, cfg.gravity, 0, 0.0, prm)
+    one_steps = min(steps, 3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for cfg, ctx in ctxs:
+        for _ in range(one_steps):
+            res, _ = ctx.step(cfg.dt, cfg.gravity, 0, 0.0, prm)
+            max_err = max(max_err, res.final_density_error)
+    torch.cuda.synchronize()
+    wall_o = time.perf_counter() - t0
+    for _, ctx in ctxs:
+        ctx.close()
+    batched = nd * steps / wall_b
+    one = nd * one_steps / wall_o
+    return {"workload": f"{n_scenes} independent columns, 2048-32768 fluid particles each (seed 4), "
+                        "r = 0.01, dt = 1 ms, no viscosity",
+            "fluid_particles": nd, "particles": n_all, "steps": steps,
+            "value": batched, "unit": "particle-steps/s", "ms_per_step": wall_b * 1e3 / steps,
+            "cg_iters_per_s": cg / wall_b, "max_outer_iters": outer, "max_final_density_error": max_err,
+            "scene_cg_chunk_ms": tune,
+            "one_by_one": {"value": one, "unit": "particle-steps/s", "steps": one_steps,
+                           "ms_per_step_of_all_scenes": wall_o * 1e3 / one_steps},
+            "speedup_vs_one_by_one": batched / one}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +181,9 @@ def main():
     ap.add_argument("--precision-steps", type=int, default=1,
                     help="steps (from the state after the timed region) whose GPU result is compared with the "
                          "float64 restatement of the step; 0 skips the comparison")
+    ap.add_argument("--batch-scenes", type=int, default=256,
+                    help="independent scenes of the batched-form comparison; 0 skips it")
+    ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--event-scope", choices=["dominant", "all"], default="dominant",
                     help="kernel classes timed with dispatch events during the timed steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -195,6 +271,10 @@ def main():
     if args.precision_steps > 0:
         precision = precision_study(ctx, cfg, ps, m, m_inv, prm, args.precision_steps, torch, dev, n)
 
+    batched = None
+    if args.batch_scenes > 0:
+        batched = batched_study(args.batch_scenes, args.batch_steps, torch)
+
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -239,6 +319,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
         "precision": precision,
+        "batched": batched,
     }
     print(json.dumps(out))
 
