@@ -100,7 +100,7 @@ int alloc_all(ivp_ctx* c) {
     A(cc, cap); A(sfac, cap); A(jd, cap); A(xs4, cap); A(nbact, cap);
     A(w_pinv, 9 * cap); A(w_p4, cap); A(w_v4, cap); A(w_dv, 3 * cap); A(w_r, 3 * cap); A(w_z, 3 * cap);
     A(w_Ap, 3 * cap); A(w_rhs, 3 * cap); A(w_fac, cap); A(v4, cap);
-    c->max_blocks = blocks_for(c->cap) + 1;
+    c->max_blocks = blocks_for(c->cap) * MAX_PASS_LANES + 1;   // blocks of a pair pass at most
     A(partials, 4 * (size_t)c->max_blocks);
     A(state, 1); A(err_flag, 1);
 #undef A

@@ -68,7 +68,8 @@ struct SolverState {
 // all dynamic particles.  Each scene runs its own IP outer loop and CG, advanced by a phase:
 enum ScenePhase { PH_PACK = 0, PH_RHS = 1, PH_CG = 2, PH_DUPDATE = 3, PH_DONE = 4 };
 
-constexpr int MIN_SCENE_DYNAMIC = 256;   // a 256-row block then spans at most two scenes
+constexpr int MIN_SCENE_DYNAMIC = 256;   // a block (at most 256 rows) then spans at most two scenes
+constexpr int MAX_PASS_LANES = 4;        // lanes per row of a pair pass, at most
 constexpr int SCENE_HIST = 1026;         // per-scene outer-loop history entries kept
 
 struct SceneState {

@@ -205,7 +205,7 @@ inline int finalize(ivp_ctx* c, int nblocks, double tol = 0.0, int max_iter = 0)
 // ---- independent scenes ----------------------------------------------------------------------
 
 // Per-block partial sums split by scene.  A block's rows span at most two scenes (every scene
-// holds at least MIN_SCENE_DYNAMIC = BLOCK dynamic rows): seg = 0 for rows of the block's first
+// holds at least MIN_SCENE_DYNAMIC >= rows of a block): seg = 0 for rows of the block's first
 // scene, 1 for the second.  Value q of segment g goes to row 2 q + g, column blk (the block's
 // row block, so the blocks of one scene are one contiguous range).  `two` is block-uniform.
 template <int NV>
@@ -232,6 +232,7 @@ struct FinScenesArgs {
     double* hist_err;     // [S * SCENE_HIST]
     int* hist_pcg;        // [S * SCENE_HIST]
     int S;
+    int rpb;              // rows per block of the kernel that wrote the partials
 };
 
 // One wave per scene: fold the scene's block partials in a fixed order and advance its state
@@ -246,10 +247,10 @@ __global__ __launch_bounds__(BLOCK) void k_fin_scenes(FinScenesArgs a) {
     if (s >= a.S) return;
     SceneState* sc = a.ss + s;
     if (sc->phase != want) return;
-    const int b0 = sc->row0 / BLOCK, b1 = (sc->row0 + sc->nd - 1) / BLOCK;
+    const int b0 = sc->row0 / a.rpb, b1 = (sc->row0 + sc->nd - 1) / a.rpb;
     double t[3] = {0.0, 0.0, 0.0};
     for (int b = b0 + lane; b <= b1; b += WAVE) {
-        const int seg = a.sid[b * BLOCK] == s ? 0 : 1;
+        const int seg = a.sid[b * a.rpb] == s ? 0 : 1;
 #pragma unroll
         for (int q = 0; q < NV; ++q) t[q] += a.partials[(2 * q + seg) * a.stride + b];
     }
