@@ -59,10 +59,11 @@ def measured_traffic(kernel):
     return None if k is None else k.get("traffic_bytes")
 
 
-def precision_study(ctx, cfg, ps, m, m_inv, prm, steps, torch, dev, n):
-    """Per-step error of the float32 GPU step against the float64 restatement of the same step
-    (oracle/fp64.py), each step starting from the GPU's state; neighbour sets from the float32
-    oracle's build of that state."""
+def precision_study(ctxs, cfg, ps, m, m_inv, prm, steps, torch, dev, n, timed_steps=20):
+    """Per-step error of each GPU form against the float64 restatement of the same step
+    (oracle/fp64.py).  ctxs maps a form name to a context; every step starts all of them from
+    the first context's state (neighbour sets for the restatement from the float32 oracle's
+    build of that state).  Then each form times `timed_steps` steps from one common state."""
     from oracle import fp64
     from oracle import oracle as O
     grid = O.grid_init(cfg.domain_start, cfg.domain_size, float(np.float32(cfg.support_radius)))
@@ -71,27 +72,65 @@ def precision_study(ctx, cfg, ps, m, m_inv, prm, steps, torch, dev, n):
     dyn = ps.is_dynamic == 1
     ip_kw = dict(omega=prm.omega, tol_opt=prm.tol_opt, max_iteration_opt=prm.max_iteration_opt,
                  use_fixed_iters=prm.use_fixed_iters, tol_pcg=prm.tol_pcg)
-    rows = []
+    names = list(ctxs)
+    first = ctxs[names[0]]
+    rd = lambda c, f: c.read(f, torch.zeros((n, 3), device=dev))
+
+    def load_state(x, v):
+        for nm in names[1:]:
+            c = ctxs[nm]
+            c.write(0, x)
+            c.write(1, v)
+            if c.neighbors_build() != 0:
+                raise RuntimeError("neighbour overflow")
+
+    rows = {nm: [] for nm in names}
     t0 = time.perf_counter()
     for _ in range(steps):
-        x = ctx.read(0, torch.zeros((n, 3), device=dev)).cpu().numpy()
-        v = ctx.read(1, torch.zeros((n, 3), device=dev)).cpu().numpy()
+        xd, vd = rd(first, 0), rd(first, 1)
+        load_state(xd, vd)
+        x, v = xd.cpu().numpy(), vd.cpu().numpy()
         count, ids, _, _ = O.build_neighbors(grid, x, ps.is_dynamic, h, C)
         _, x64, v64, st = fp64.step(x, v, m, m_inv, ps.density0, ps.is_dynamic, ps.material, h, count, ids, C,
                                     cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff,
                                     cfg.domain_size, ip_kw)
-        res, _ = ctx.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, prm)
-        xg = ctx.read(0, torch.zeros((n, 3), device=dev)).cpu().numpy().astype(np.float64)
-        vg = ctx.read(1, torch.zeros((n, 3), device=dev)).cpu().numpy().astype(np.float64)
-        rows.append({"velocity_rel_l2": float(np.linalg.norm(vg[dyn] - v64[dyn]) / np.linalg.norm(v64[dyn])),
-                     "velocity_rel_max": float(np.abs(vg[dyn] - v64[dyn]).max() / np.abs(v64[dyn]).max()),
-                     "position_max_abs": float(np.abs(xg[dyn] - x64[dyn]).max()),
-                     "density_error_gpu": res.final_density_error,
-                     "density_error_fp64": st["final_density_error"],
-                     "outer_iters_gpu": res.outer_iters, "outer_iters_fp64": st["outer_iters"]})
-    gw = ["evaluated in float32", "float32 cache", "float16 cache in the CG operator"][ctx.gradient_cache]
+        for nm in names:
+            c = ctxs[nm]
+            res, _ = c.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, prm)
+            xg = rd(c, 0).cpu().numpy().astype(np.float64)
+            vg = rd(c, 1).cpu().numpy().astype(np.float64)
+            rows[nm].append({"velocity_rel_l2": float(np.linalg.norm(vg[dyn] - v64[dyn]) / np.linalg.norm(v64[dyn])),
+                             "velocity_rel_max": float(np.abs(vg[dyn] - v64[dyn]).max() / np.abs(v64[dyn]).max()),
+                             "position_max_abs": float(np.abs(xg[dyn] - x64[dyn]).max()),
+                             "density_error_gpu": res.final_density_error,
+                             "density_error_fp64": st["final_density_error"],
+                             "outer_iters_gpu": res.outer_iters, "outer_iters_fp64": st["outer_iters"]})
+    seconds = time.perf_counter() - t0
+    # speed of each form over the same steps from one state
+    timing = {"steps": timed_steps}
+    xd, vd = rd(first, 0), rd(first, 1)
+    for nm in names:
+        c = ctxs[nm]
+        c.write(0, xd)
+        c.write(1, vd)
+        if c.neighbors_build() != 0:
+            raise RuntimeError("neighbour overflow")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        errs = 0.0
+        for _ in range(timed_steps):
+            res, _ = c.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, prm)
+            errs = max(errs, res.final_density_error)
+        torch.cuda.synchronize()
+        dt_ = time.perf_counter() - t1
+        timing[nm] = {"ms_per_step": dt_ * 1e3 / timed_steps,
+                      "particle_steps_per_s": int(np.count_nonzero(dyn)) * timed_steps / dt_,
+                      "max_final_density_error": errs}
+    gw = {0: "evaluated in float32", 1: "float32 cache", 2: "float16 cache in the CG operator, float32 "
+                                                             "residual replacement every 8 CG iterations"}
     return {"reference": "float64 restatement of the step (CPU), from the GPU state", "dtype_gpu": "f32",
-            "gradW": gw, "steps": rows, "seconds": time.perf_counter() - t0}
+            "forms": {nm: {"gradW": gw[ctxs[nm].gradient_cache], "steps": rows[nm]} for nm in names},
+            "timing": timing, "seconds": seconds}
 
 
 def batched_study(n_scenes, steps, torch):
@@ -179,8 +218,9 @@ def main():
                          "cached (16 B per pair); 2 float16 gradW cached for the IP CG operator (12 B per pair) "
                          "with a float32 true-residual refresh every 8 CG iterations")
     ap.add_argument("--precision-steps", type=int, default=1,
-                    help="steps (from the state after the timed region) whose GPU result is compared with the "
-                         "float64 restatement of the step; 0 skips the comparison")
+                    help="steps (from the state after the timed region) whose GPU result, in float32 and in the "
+                         "float16 precision form, is compared with the float64 restatement of the step; then 20 "
+                         "steps of each form are timed from one state; 0 skips both")
     ap.add_argument("--batch-scenes", type=int, default=256,
                     help="independent scenes of the batched-form comparison; 0 skips it")
     ap.add_argument("--batch-steps", type=int, default=10)
@@ -269,7 +309,18 @@ def main():
 
     precision = None
     if args.precision_steps > 0:
-        precision = precision_study(ctx, cfg, ps, m, m_inv, prm, args.precision_steps, torch, dev, n)
+        forms = {"f32": ctx}
+        if args.gradient_cache != 2:
+            # the precision form: float16 gradW rows in the IP CG operator
+            c16 = Context(h, cfg.domain_start, cfg.domain_size, ps.n,
+                          cache_size=int(cfg.get_cfg("neighborCacheSize", 60)))
+            c16.gradient_cache = 2
+            c16.set_particles(t(ps.x), t(ps.v), t(m), t(m_inv), t(ps.density0), t(ps.is_dynamic, torch.int32),
+                              t(ps.material, torch.int32))
+            forms["f16"] = c16
+        precision = precision_study(forms, cfg, ps, m, m_inv, prm, args.precision_steps, torch, dev, n)
+        if "f16" in forms:
+            forms["f16"].close()
 
     batched = None
     if args.batch_scenes > 0:
