@@ -143,7 +143,8 @@ def batched_study(n_scenes, steps, torch):
     from implicitvelocityprojection_amd.scenes import load_particles, particle_masses, synthetic_batch
     scenes = synthetic_batch(n_scenes=n_scenes)
     batch = SceneBatch(scenes)
-    tune = batch.autotune(candidates=(1, 2, 4, 8), steps=2)
+    tune = batch.autotune(chunks=(1, 2, 4, 8), lanes=(1, 2, 4), steps=2)
+    chosen = {"cg_chunk": batch.ctx.scene_cg_chunk, "lanes_per_row": batch.ctx.scene_lanes}
     for _ in range(2):
         batch.step()
     torch.cuda.synchronize()
@@ -200,7 +201,7 @@ def batched_study(n_scenes, steps, torch):
             "fluid_particles": nd, "particles": n_all, "steps": steps,
             "value": batched, "unit": "particle-steps/s", "ms_per_step": wall_b * 1e3 / steps,
             "cg_iters_per_s": cg / wall_b, "max_outer_iters": outer, "max_final_density_error": max_err,
-            "scene_cg_chunk_ms": tune,
+            "autotune_ms_per_step": tune, "autotuned": chosen,
             "one_by_one": {"value": one, "unit": "particle-steps/s", "steps": one_steps,
                            "ms_per_step_of_all_scenes": wall_o * 1e3 / one_steps},
             "speedup_vs_one_by_one": batched / one}

@@ -239,6 +239,15 @@ class Context:
         N.check(self.lib.ivp_set_option(self.h, N.OPT_SCENE_CG_CHUNK, int(k)), "ivp_set_option")
 
     @property
+    def scene_lanes(self):
+        """Lanes per neighbour row in the scene solve's pair passes (1, 2 or 4)."""
+        return self._opt(N.OPT_SCENE_LANES)
+
+    @scene_lanes.setter
+    def scene_lanes(self, k):
+        N.check(self.lib.ivp_set_option(self.h, N.OPT_SCENE_LANES, int(k)), "ivp_set_option")
+
+    @property
     def scene_graph(self):
         """True: the scene solve replays a captured hipGraph; False: it enqueues the launches."""
         return bool(self._opt(N.OPT_SCENE_GRAPH))

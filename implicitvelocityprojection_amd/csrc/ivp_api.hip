@@ -401,6 +401,11 @@ int ivp_set_option(ivp_ctx* c, int option, int value) {
         c->scene_graph = value != 0;
         return IVP_OK;
     }
+    if (option == IVP_OPT_SCENE_LANES) {
+        if (value != 1 && value != 2 && value != 4) return IVP_ERR_ARG;
+        c->scene_lanes = value;
+        return IVP_OK;
+    }
     if (option == IVP_OPT_GRADIENT_CACHE) {
         if (value != IVP_ROWS_IDS && value != IVP_ROWS_F32 && value != IVP_ROWS_F16) return IVP_ERR_ARG;
         if (value == c->row_fmt) return IVP_OK;
@@ -427,6 +432,10 @@ int ivp_get_option(ivp_ctx* c, int option, int* value) {
     }
     if (option == IVP_OPT_SCENE_GRAPH) {
         *value = c->scene_graph ? 1 : 0;
+        return IVP_OK;
+    }
+    if (option == IVP_OPT_SCENE_LANES) {
+        *value = c->scene_lanes;
         return IVP_OK;
     }
     return IVP_ERR_ARG;

@@ -91,6 +91,7 @@ struct SceneGraph {
     int chunk = 0;
     float omega = 0.0f;
     int nd = 0;
+    int lanes = 0;
 };
 
 // IP parameters of the current batched solve (device copy, read by the finalize kernels)
@@ -235,6 +236,7 @@ struct ivp_ctx {
     int* sactive = nullptr;               // device: scenes not done
     int* sactive_host = nullptr;          // pinned
     int scene_chunk = 4;                  // CG iterations per replay
+    int scene_lanes = 1;                  // lanes per row in the scene passes (1, 2 or 4)
     bool scene_graph = true;              // replay a captured hipGraph
     hipStream_t cap_stream = nullptr;     // private stream the replay graph is captured on
     ivp::SceneGraph sgraphs[2];

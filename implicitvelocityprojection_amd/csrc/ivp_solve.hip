@@ -162,9 +162,9 @@ __device__ __forceinline__ int pass_row(int blk) { return blk * (BLOCK / G) + (i
 template <int G = PASS_LANES>
 __device__ __forceinline__ int pass_lane() { return (int)(threadIdx.x % G); }
 
-// Passes over independent scenes use one lane per row: late in a solve most blocks belong to
-// scenes in another phase and return at once, and four times as many blocks would make that
-// exit the larger cost.
+// Passes over independent scenes take their lanes per row from the context (1 by default: late
+// in a solve most blocks belong to scenes in another phase and return at once, and four times
+// as many blocks can make that exit the larger cost); the autotuner chooses.
 template <bool SC>
 constexpr int lanes_for() { return SC ? 1 : PASS_LANES; }
 
@@ -253,10 +253,9 @@ struct PackArgs {
 // ip_pack_fused (ip.py:124-171) + coef_wise_div + compute_avg_density_error
 // (math_utils.py:76-91, pressure_base.py:187-203, 413-423).  J d in the form of
 // compute_J_x_kernel (pressure_base.py:111-127): sum_j m_j dot(d_i - d_j, gradW_ij) + dot(d_i, S_i).
-template <class FM, bool SC>
+template <class FM, bool SC, int G = lanes_for<SC>()>
 __global__ __launch_bounds__(BLOCK) void k_ip_pack(PackArgs a) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    constexpr int G = lanes_for<SC>();
     const int i = pass_row<G>(blk), g = pass_lane<G>();
     Gate gt = gate_all(i, a.R.nd);
     if (SC && !gate_block<BLOCK / G>(a.sc, blk, i, a.R.nd, PH_PACK, gt)) return;
@@ -312,10 +311,9 @@ struct RhsArgs {
 
 // ip_rhs_fused (ip.py:174-215): b = omega J^T (f k); also marks rows whose CG J^T loop can be
 // non-zero (own dfdt > 0 or a dynamic neighbour with dfdt > 0).
-template <class FM, bool SC>
+template <class FM, bool SC, int G = lanes_for<SC>()>
 __global__ __launch_bounds__(BLOCK) void k_ip_rhs(RhsArgs a) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    constexpr int G = lanes_for<SC>();
     const int i = pass_row<G>(blk), g = pass_lane<G>();
     Gate gt = gate_all(i, a.R.nd);
     if (SC && !gate_block<BLOCK / G>(a.sc, blk, i, a.R.nd, PH_RHS, gt)) return;
@@ -409,11 +407,10 @@ struct CGJArgs {
 
 // compute_Jx_pcg (ip.py:19-60): y_i = dfdt_i k_i (J p)_i on rows with dfdt > 0, J p in the form
 // sum_j dot(p_i - p_j, m_j gradW_ij) + dot(p_i, S_i) (ip.py:51-58)
-template <class FM, bool SC>
+template <class FM, bool SC, int G = lanes_for<SC>()>
 __global__ __launch_bounds__(BLOCK) void k_cg_J(CGJArgs a) {
     if (!SC && a.st->active == 0) return;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    constexpr int G = lanes_for<SC>();
     const int i = pass_row<G>(blk), g = pass_lane<G>();
     Gate gt = gate_all(i, a.R.nd);
     if (SC && !gate_block<BLOCK / G>(a.sc, blk, i, a.R.nd, PH_CG, gt)) return;
@@ -452,11 +449,10 @@ struct CGJTArgs {
 
 // compute_Ax_part1 + compute_Ax_part2 (ip.py:63-121) in gather form, fused with p.Ap
 // (ip.py:307)
-template <class FM, bool SC>
+template <class FM, bool SC, int G = lanes_for<SC>()>
 __global__ __launch_bounds__(BLOCK) void k_cg_JT(CGJTArgs a) {
     if (!SC && a.st->active == 0) return;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    constexpr int G = lanes_for<SC>();
     const int i = pass_row<G>(blk), g = pass_lane<G>();
     Gate gt = gate_all(i, a.R.nd);
     if (SC && !gate_block<BLOCK / G>(a.sc, blk, i, a.R.nd, PH_CG, gt)) return;
@@ -995,21 +991,24 @@ __global__ __launch_bounds__(BLOCK) void k_scene_advance(int S, SceneState* __re
 // for scenes in their CG, the displacement update for scenes whose CG ended.  Every launch
 // covers all rows; rows of scenes in another phase skip (whole blocks when both of their
 // scenes do).
-int enqueue_scene_replay(ivp_ctx* c, hipStream_t st, float omega, int K) {
+template <int G>
+int enqueue_scene_replay_g(ivp_ctx* c, hipStream_t st, float omega, int K) {
     const int nd = c->nd;
-    const dim3 nb(blocks_for(nd)), blk(BLOCK);
+    const dim3 nb(blocks_for(nd)), np(blocks_for(nd * G)), blk(BLOCK);
     const dim3 nf((c->S + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
     const int stride = c->max_blocks;
     Rows R = rows_of(c);
     SceneRef sc{c->sid, c->ss};
     double* P = c->spartials;
     FinScenesArgs fa{P, stride, c->sid, c->ss, c->sprm, c->shist_err, c->shist_pcg, c->S, BLOCK};
+    FinScenesArgs fp = fa;   // partials of the pair passes: BLOCK / G rows per block
+    fp.rpb = BLOCK / G;
     PackArgs pa{R, c->d4, c->Ss, c->cc, c->kk, c->rho0, c->rowcnt, c->q4, c->sfac, c->jd, P, stride, sc};
-    hipLaunchKernelGGL((k_ip_pack<FmtIds, true>), nb, blk, 0, st, pa);
-    hipLaunchKernelGGL(k_fin_scenes<FIN_PACK>, nf, blk, 0, st, fa);
+    hipLaunchKernelGGL((k_ip_pack<FmtIds, true, G>), np, blk, 0, st, pa);
+    hipLaunchKernelGGL(k_fin_scenes<FIN_PACK>, nf, blk, 0, st, fp);
     RhsArgs ra{R, c->q4, c->Ss, omega, c->b, c->nbact, c->rowcnt, P, stride, sc};
-    hipLaunchKernelGGL((k_ip_rhs<FmtIds, true>), nb, blk, 0, st, ra);
-    hipLaunchKernelGGL(k_fin_scenes<FIN_RHS>, nf, blk, 0, st, fa);
+    hipLaunchKernelGGL((k_ip_rhs<FmtIds, true, G>), np, blk, 0, st, ra);
+    hipLaunchKernelGGL(k_fin_scenes<FIN_RHS>, nf, blk, 0, st, fp);
     CGInitArgs ia{nd, c->b, c->pos, c->sol, c->r, c->p4, c->y2, P, stride, sc};
     hipLaunchKernelGGL(k_cg_init<true>, nb, blk, 0, st, ia);
     hipLaunchKernelGGL(k_fin_scenes<FIN_CG_INIT>, nf, blk, 0, st, fa);
@@ -1017,9 +1016,9 @@ int enqueue_scene_replay(ivp_ctx* c, hipStream_t st, float omega, int K) {
     CGJTArgs ta{R, c->p4, c->y2, c->Ss, c->nbact, omega, c->Ap, P, stride, c->state, sc};
     CGUpdArgs ua{nd, c->sol, c->r, c->p4, c->Ap, P, stride, c->state, sc};
     for (int k = 0; k < K; ++k) {
-        hipLaunchKernelGGL((k_cg_J<FmtIds, true>), nb, blk, 0, st, ja);
-        hipLaunchKernelGGL((k_cg_JT<FmtIds, true>), nb, blk, 0, st, ta);
-        hipLaunchKernelGGL(k_fin_scenes<FIN_CG_PAP>, nf, blk, 0, st, fa);
+        hipLaunchKernelGGL((k_cg_J<FmtIds, true, G>), np, blk, 0, st, ja);
+        hipLaunchKernelGGL((k_cg_JT<FmtIds, true, G>), np, blk, 0, st, ta);
+        hipLaunchKernelGGL(k_fin_scenes<FIN_CG_PAP>, nf, blk, 0, st, fp);
         hipLaunchKernelGGL(k_cg_update<true>, nb, blk, 0, st, ua);
         hipLaunchKernelGGL(k_fin_scenes<FIN_CG_UPDATE>, nf, blk, 0, st, fa);
         hipLaunchKernelGGL(k_cg_p<true>, nb, blk, 0, st, nd, (const float*)c->r, c->p4,
@@ -1031,11 +1030,20 @@ int enqueue_scene_replay(ivp_ctx* c, hipStream_t st, float omega, int K) {
     return IVP_OK;
 }
 
+int enqueue_scene_replay(ivp_ctx* c, hipStream_t st, float omega, int K) {
+    switch (c->scene_lanes) {
+    case 2: return enqueue_scene_replay_g<2>(c, st, omega, K);
+    case 4: return enqueue_scene_replay_g<4>(c, st, omega, K);
+    default: return enqueue_scene_replay_g<1>(c, st, omega, K);
+    }
+}
+
 // The instantiated graph of one replay for (K, omega) and the current store buffers (the sort
 // alternates two buffers, so at most two captures are live).
 int scene_graph(ivp_ctx* c, float omega, int K, hipGraphExec_t* out) {
     for (auto& g : c->sgraphs) {
-        if (g.exec && g.pos == (const void*)c->pos && g.chunk == K && g.omega == omega && g.nd == c->nd) {
+        if (g.exec && g.pos == (const void*)c->pos && g.chunk == K && g.omega == omega && g.nd == c->nd &&
+            g.lanes == c->scene_lanes) {
             *out = g.exec;
             return IVP_OK;
         }
@@ -1062,6 +1070,7 @@ int scene_graph(ivp_ctx* c, float omega, int K, hipGraphExec_t* out) {
     slot.chunk = K;
     slot.omega = omega;
     slot.nd = c->nd;
+    slot.lanes = c->scene_lanes;
     *out = exec;
     return IVP_OK;
 }

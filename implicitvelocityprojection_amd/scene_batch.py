@@ -27,7 +27,7 @@ def _ip_params(cfg):
 class SceneBatch:
     """Scene JSON dicts -> one context holding all of them, scene-major in user order."""
 
-    def __init__(self, scene_dicts, device="cuda", graph=True, chunk=4):
+    def __init__(self, scene_dicts, device="cuda", graph=True, chunk=4, lanes=1):
         self.cfgs = [SimConfig.from_dict(d) for d in scene_dicts]
         c0 = self.cfgs[0]
         for c in self.cfgs[1:]:
@@ -63,6 +63,7 @@ class SceneBatch:
         self.ctx.set_scenes(self.sizes, [c.domain_start for c in self.cfgs], [c.domain_size for c in self.cfgs])
         self.ctx.scene_graph = graph
         self.ctx.scene_cg_chunk = chunk
+        self.ctx.scene_lanes = lanes
         t = self._t
         self.ctx.set_particles(t(self.x0), t(self.v0), t(self.m), t(self.m_inv), t(self.density0),
                                t(self.is_dynamic, torch.int32), t(self.material, torch.int32))
@@ -92,26 +93,29 @@ class SceneBatch:
         if self.ctx.neighbors_build() != 0:
             raise RuntimeError("neighbour overflow at restore")
 
-    def autotune(self, candidates=(1, 2, 4, 8), steps=2):
-        """Choose the CG iterations per replay: time `steps` steps from the same state for each
-        candidate, keep the fastest, restore the state.  Returns {chunk: ms per step}."""
+    def autotune(self, chunks=(1, 2, 4, 8), lanes=(1, 2, 4), steps=2):
+        """Choose the CG iterations per replay and the lanes per neighbour row of the pair
+        passes: time `steps` steps from the same saved state for every combination, keep the
+        fastest, restore the state.  Returns {"chunk K, lanes G": ms per step}."""
         snap = self.snapshot()
         times = {}
-        for k in candidates:
-            self.restore(snap)
-            self.ctx.scene_cg_chunk = k
-            self.step()                          # captures the replay graphs for this chunk
-            self.restore(snap)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                self.step()
-            torch.cuda.synchronize()
-            times[k] = (time.perf_counter() - t0) * 1e3 / steps
+        for g in lanes:
+            for k in chunks:
+                self.restore(snap)
+                self.ctx.scene_cg_chunk = k
+                self.ctx.scene_lanes = g
+                self.step()                      # captures the replay graphs for this setting
+                self.restore(snap)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    self.step()
+                torch.cuda.synchronize()
+                times[(k, g)] = (time.perf_counter() - t0) * 1e3 / steps
         best = min(times, key=times.get)
-        self.ctx.scene_cg_chunk = best
+        self.ctx.scene_cg_chunk, self.ctx.scene_lanes = best
         self.restore(snap)
-        return times
+        return {f"chunk {k}, lanes {g}": ms for (k, g), ms in times.items()}
 
     def close(self):
         self.ctx.close()

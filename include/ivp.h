@@ -218,11 +218,12 @@ int ivp_step_scenes(ivp_ctx *ctx, const ivp_step_params *prm, ivp_scene_stats *s
 int ivp_scene_history(ivp_ctx *ctx, int scene, int capacity, double *density_errors, int *pcg_iters,
                       int *n_errors, int *n_pcg);
 
-/* Options of the scene solve: CG iterations per replay of the launch sequence (default 4), and
+/* Options of the scene solve: CG iterations per replay of the launch sequence (default 4);
  * whether the sequence is captured once into a hipGraph and replayed (1, default) or enqueued
- * launch by launch (0). */
+ * launch by launch (0); lanes per neighbour row in its pair passes (1 default, 2 or 4). */
 #define IVP_OPT_SCENE_CG_CHUNK 2
 #define IVP_OPT_SCENE_GRAPH 3
+#define IVP_OPT_SCENE_LANES 4
 
 /* ---- instrumentation ---------------------------------------------------------------------- */
 /* Per-kernel-class event timing on the context's stream.  Only launches that did work are

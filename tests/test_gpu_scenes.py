@@ -91,11 +91,12 @@ def test_scene_neighbour_export_matches_single_contexts():
     batch.close()
 
 
-def test_scene_steps_match_single_contexts():
+@pytest.mark.parametrize("lanes", [1, 2, 4])
+def test_scene_steps_match_single_contexts(lanes):
     from implicitvelocityprojection_amd.context import IPParams
     from implicitvelocityprojection_amd.scene_batch import SceneBatch
     scenes = _scenes()
-    batch = SceneBatch(scenes, chunk=2)
+    batch = SceneBatch(scenes, chunk=2, lanes=lanes)
     hist_b = []
     for _ in range(STEPS):
         status, stats = batch.step()
@@ -149,12 +150,13 @@ def test_scene_step_matches_oracle():
     batch.close()
 
 
-def test_scene_graph_replays_equal_direct_launches():
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_scene_graph_replays_equal_direct_launches(lanes):
     from implicitvelocityprojection_amd.scene_batch import SceneBatch
     scenes = _scenes()[:3]
     outs = []
     for graph in (True, False):
-        batch = SceneBatch(scenes, graph=graph, chunk=2)
+        batch = SceneBatch(scenes, graph=graph, chunk=2, lanes=lanes)
         for _ in range(STEPS):
             status, _ = batch.step()
             assert status == 0
