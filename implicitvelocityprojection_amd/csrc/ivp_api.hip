@@ -39,14 +39,23 @@ void hfree(T*& p) {
     p = nullptr;
 }
 
-void scenes_free(ivp_ctx* c) {
-    dfree(c->grids); dfree(c->sid); dfree(c->sdomain); dfree(c->ss); dfree(c->sprm);
-    dfree(c->shist_err); dfree(c->shist_pcg); dfree(c->spartials); dfree(c->sactive);
-    hfree(c->ss_host); hfree(c->sprm_host); hfree(c->sactive_host);
+// Captured replay graphs hold the device addresses of the rows and scene buffers: drop them
+// whenever those are freed.
+void scene_graphs_clear(ivp_ctx* c) {
+    bool any = false;
+    for (auto& g : c->sgraphs) any = any || g.exec != nullptr;
+    if (any) (void)hipStreamSynchronize(c->stream);
     for (auto& g : c->sgraphs) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         g = SceneGraph{};
     }
+}
+
+void scenes_free(ivp_ctx* c) {
+    scene_graphs_clear(c);
+    dfree(c->grids); dfree(c->sid); dfree(c->sdomain); dfree(c->ss); dfree(c->sprm);
+    dfree(c->shist_err); dfree(c->shist_pcg); dfree(c->spartials); dfree(c->sactive);
+    hfree(c->ss_host); hfree(c->sprm_host); hfree(c->sactive_host);
     if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
     c->cap_stream = nullptr;
     c->S = 0;
@@ -126,6 +135,7 @@ int ensure_rows(ivp_ctx* c) {
 }
 
 void free_rows(ivp_ctx* c) {
+    scene_graphs_clear(c);
     dfree(c->nb);
 }
 

@@ -179,3 +179,26 @@ def test_scene_entry_points_of_one_scene_are_refused():
     with pytest.raises(N.IVPError):
         batch.ctx.gradient_cache = 1
     batch.close()
+
+
+def test_scene_reload_recaptures_graphs():
+    """Loading the particle set again frees and reallocates the neighbour rows; the replay
+    graphs captured before must not be reused (they hold the old rows' addresses)."""
+    from implicitvelocityprojection_amd.scene_batch import SceneBatch
+    scenes = _scenes()[:2]
+    batch = SceneBatch(scenes, chunk=2)
+    for _ in range(2):
+        assert batch.step()[0] == 0
+    ref = SceneBatch(scenes, chunk=2)
+    t = batch._t
+    batch.ctx.set_particles(t(batch.x0), t(batch.v0), t(batch.m), t(batch.m_inv), t(batch.density0),
+                            t(batch.is_dynamic, torch.int32), t(batch.material, torch.int32))
+    assert batch.ctx.neighbors_build() == 0
+    for _ in range(2):
+        assert batch.step()[0] == 0
+        assert ref.step()[0] == 0
+    a = batch.read(1, (batch.n, 3))
+    b = ref.read(1, (ref.n, 3))
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    batch.close()
+    ref.close()
