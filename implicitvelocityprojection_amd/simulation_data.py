@@ -83,6 +83,9 @@ class SimulationData:
         cfg = self.cfg
         self.ctx = Context(np.float32(self.support_radius), cfg.domain_start, cfg.domain_size, max(M, 1),
                            cache_size=self.cache_size, material_fluid=self.material_fluid)
+        # extension key (no counterpart in the reference): neighbour-row format, 0 ids only
+        # (default), 1 cached float32 gradW, 2 the float16 precision form (include/ivp.h)
+        self.ctx.gradient_cache = int(cfg.get_cfg("gradientCache", 0))
         self.allocated = True
 
     # -- particles (simulation_data.py:214-356) --------------------------------------------------
