@@ -1,6 +1,10 @@
 """Benchmark: the reference's velocity-based IP step on the 1M-particle dam break, one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--workload NAME]
+
+`--workload` also accepts "column_102k" (40 x 64 x 40 fluid column, explicit viscosity 0.01,
+dt = 1 ms) and "column_8k" (16 x 32 x 16 column, no viscosity, dt = 1 ms), the parity-test
+scenes, for profiling; the default and the bench line's workload is "dambreak_1m".
 
 Workload ("dambreak_1m"): a 100^3 fluid block (r = 0.005 m, jitter +-0.01 d, seed 2) in the corner
 of a 4.0 x 1.5 x 1.0 m static two-layer particle box, implicit (Weiler 2018) viscosity 0.01,
@@ -46,10 +50,13 @@ def copy_calibration(torch, mib):
     print(json.dumps({"copy_calibration_bytes": n * 4, "dispatches": 5}))
 
 
-def measured_traffic(kernel):
+WORKLOADS = {"column_8k": 0, "column_102k": 1, "dambreak_1m": 2}
+
+
+def measured_traffic(kernel, workload="dambreak_1m"):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     measurement (scaled by the copy-kernel calibration), or None when none was recorded."""
-    path = os.path.join(ROOT, "profiles", "traffic_dambreak_1m.json")
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     try:
         with open(path) as f:
             rec = json.load(f)
@@ -222,6 +229,7 @@ def main():
                     help="steps (from the state after the timed region) whose GPU result, in float32 and in the "
                          "float16 precision form, is compared with the float64 restatement of the step; then 20 "
                          "steps of each form are timed from one state; 0 skips both")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="dambreak_1m")
     ap.add_argument("--batch-scenes", type=int, default=256,
                     help="independent scenes of the batched-form comparison; 0 skips it")
     ap.add_argument("--batch-steps", type=int, default=10)
@@ -244,7 +252,7 @@ def main():
     from implicitvelocityprojection_amd.context import Context, IPParams, kclass_mask
     from implicitvelocityprojection_amd.scenes import load_particles, particle_masses, synthetic_scene
 
-    scene = synthetic_scene(2)
+    scene = synthetic_scene(WORKLOADS[args.workload])
     cfg = SimConfig.from_dict(scene)
     ps = load_particles(cfg)
     m, m_inv = particle_masses(cfg, ps)
@@ -339,7 +347,7 @@ def main():
             sim.step()
         tcpu = time.perf_counter() - tc
         cpu = {"value": nd * args.cpu_steps / tcpu, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of dambreak_1m from the GPU state at the start of the timed "
+               "sample": f"{args.cpu_steps} steps of {args.workload} from the GPU state at the start of the timed "
                          f"region, serial float32 C oracle ({tcpu:.1f} s)"}
 
     out = {
@@ -355,9 +363,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "dambreak_1m", "fluid_particles": nd, "static_particles": ns,
+        "config": {"workload": args.workload, "fluid_particles": nd, "static_particles": ns,
                    "gradient_cache": ["ids", "f32", "f16"][args.gradient_cache],
-                   "dt": cfg.dt, "viscosity": "Weiler18 0.01", "densityErrorTol": prm.tol_opt,
+                   "dt": cfg.dt, "viscosity": (["explicit", "Weiler18"][cfg.viscosity_method] + f" {cfg.viscosity_coeff:g}"
+                                               if cfg.viscosity_coeff != 0.0 else "off"),
+                   "densityErrorTol": prm.tol_opt,
                    "maxOuterIters": prm.max_iteration_opt, "neighbour_pairs": pairs_end},
         "cg_iters_per_s": cg_iters / (solve_ms * 1e-3) if solve_ms > 0 else 0.0,
         "cg_iters": cg_iters,
@@ -366,7 +376,7 @@ def main():
         "ip_solve_ms_per_step": solve_ms / args.steps,
         "max_final_density_error": max_err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant), "kernel": dominant,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant, args.workload), "kernel": dominant,
                      "avg_launch_us": avg_ms * 1e3, "launches": pd["launches"], "event_scope": args.event_scope,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
