@@ -235,6 +235,9 @@ struct ivp_ctx {
     double* spartials = nullptr;          // device [8 * max_blocks]: per block, two scene segments
     int* sactive = nullptr;               // device: scenes not done
     int* sactive_host = nullptr;          // pinned
+    // first CG launch chunk of the next IP / Weiler18 solve: the last solve's iteration count
+    int cg_guess = 4;
+    int w18_guess = 4;
     int scene_chunk = 4;                  // CG iterations per replay
     int scene_lanes = 1;                  // lanes per row in the scene passes (1, 2 or 4)
     bool scene_graph = true;              // replay a captured hipGraph

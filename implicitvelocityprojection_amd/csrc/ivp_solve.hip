@@ -1131,7 +1131,6 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
                    (const float*)c->vel, (const float4*)c->pos, c->cc, d4));
     int opt_iter = 0;
     double err = 0.0;
-    static int cg_guess = 4;
     for (;;) {
         PackArgs pa{R, d4, c->Ss, c->cc, c->kk, c->rho0, c->rowcnt, c->q4, c->sfac, c->jd, c->partials, stride};
         double total_pairs = (double)c->state_host->total_pairs;
@@ -1199,9 +1198,9 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
                 return launch(c, KC_CG_P, tag, bP, k_cg_p<false>, nb, nd, (const float*)c->r, c->p4,
                               (const SolverState*)c->state, SceneRef{});
             };
-            IVP_TRY(run_chunks(c, prm->max_iteration_pcg, cg_guess, one_iter, &iters, &st.cg_launch_chunks));
+            IVP_TRY(run_chunks(c, prm->max_iteration_pcg, c->cg_guess, one_iter, &iters, &st.cg_launch_chunks));
             prof_collect(c, iters, epoch);
-            cg_guess = std::max(2, iters);
+            c->cg_guess = std::max(2, iters);
             st.pairs_traversed += (long long)iters * (act_pairs + touch_pairs);
             st.rows_traversed += (long long)iters * (act_rows + touch_rows);
             if (c->state_host->status == IVP_NOT_PD) {
@@ -1274,10 +1273,9 @@ int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
         };
         c->prof_epoch++;
         int epoch = c->prof_epoch;
-        static int w_guess = 4;
-        IVP_TRY(run_chunks(c, 1000, w_guess, one_iter, &iters, &chunks));
+        IVP_TRY(run_chunks(c, 1000, c->w18_guess, one_iter, &iters, &chunks));
         prof_collect(c, iters, epoch);
-        w_guess = std::max(2, iters);
+        c->w18_guess = std::max(2, iters);
     }
     *iters_out = iters;
     return launch(c, KC_GLUE, -1, 0.0, k_w18_apply, dim3(blocks_for(3 * nd)), 3 * nd, (const float*)c->w_dv, c->vel);
