@@ -13,6 +13,7 @@
  *   - scripts/sim/math_utils.py:10-195                 vector kernels, tiled dot (block 128)
  *   - scripts/sim/forces/non_pressure/viscosity/viscosity.py:9-108   explicit viscosity
  *   - scripts/sim/forces/non_pressure/viscosity/weiler18.py:21-401   implicit viscosity PCG
+ *   - scripts/sim/forces/non_pressure/surface_tension/jeske23.py:23-655  implicit surface tension
  *   - scripts/framework.py:26-41, 117-136, 400-544     gravity, advect, step order
  *   - scripts/sim/boundary.py:10-91                    domain clamp + reflection
  *
@@ -830,6 +831,258 @@ int orc_weiler18(int n, const float *x, float *v, const float *m, const float *d
     }
     add_vec3(n, v, v, 1.0f, dv, is_dynamic);
     free(Pinv); free(dv); free(r); free(z); free(p); free(Ap); free(rhs);
+    return it;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Jeske23 implicit surface tension, jeske23.py                                                */
+/* ------------------------------------------------------------------------------------------ */
+
+/* compute_Wst, jeske23.py:23-43: theta * W_cubic(max(r, d); h_st), theta = 10/7 */
+float orc_wst(float r_ij, float d, float h_st)
+{
+    float theta = 10.0f / 7.0f;
+    float r_c = r_ij > d ? r_ij : d;
+    float q = r_c / h_st;
+    float norm = 8.0f / (3.14159265358979f * h_st * h_st * h_st);
+    float fq = 0.0f;
+    if (q <= 0.5f) {
+        fq = 1.0f - 6.0f * q * q + 6.0f * q * q * q;
+    } else if (q <= 1.0f) {
+        float t = 1.0f - q;
+        fq = 2.0f * t * t * t;
+    }
+    return theta * norm * fq;
+}
+
+/* compute_grad_Wst, jeske23.py:46-66: 0 for r <= d (the kernel is flat there) and r > h_st */
+void orc_grad_wst(const float x_ij[3], float r_ij, float d, float h_st, float out[3])
+{
+    float theta = 10.0f / 7.0f;
+    out[0] = out[1] = out[2] = 0.0f;
+    if (r_ij <= d || r_ij < 1.0e-12f) return;
+    float q = r_ij / h_st;
+    if (q > 1.0f) return;
+    float norm = 8.0f / (3.14159265358979f * h_st * h_st * h_st);
+    float fpq;
+    if (q <= 0.5f) {
+        fpq = -12.0f * q + 18.0f * q * q;
+    } else {
+        float t = 1.0f - q;
+        fpq = -6.0f * t * t;
+    }
+    float s = theta * norm * (1.0f / h_st) * fpq;
+    out[0] = s * (x_ij[0] / r_ij);
+    out[1] = s * (x_ij[1] / r_ij);
+    out[2] = s * (x_ij[2] / r_ij);
+}
+
+typedef struct {
+    int n, C;
+    const float *m;
+    const int *material, *is_dynamic, *count, *ids;
+    const float *g_tilde;
+    float sigma, dt;
+    int fluid;
+} st_op;
+
+/* compute_Av_jeske23, jeske23.py:245-295: (M v)_i = v_i + dt^2 (sigma / m_i) sum_j m_bar g~ v_ij */
+static void st_Av(const st_op *o, const float *vin, float *Av)
+{
+    for (int i = 0; i < o->n; ++i) {
+        float *out = Av + 3 * (size_t)i;
+        const float *vi = vin + 3 * i;
+        if (o->is_dynamic[i] == 0 || o->material[i] != o->fluid) {
+            out[0] = vi[0]; out[1] = vi[1]; out[2] = vi[2];
+            continue;
+        }
+        float m_i = o->m[i];
+        float dt2 = o->dt * o->dt;
+        float factor = dt2 * o->sigma / m_i;
+        float sum[3] = {0.0f, 0.0f, 0.0f};
+        size_t base = (size_t)i * (size_t)o->C;
+        int cnt = capped(o->count, i, o->C);
+        for (int kk = 0; kk < cnt; ++kk) {
+            int j = o->ids[base + (size_t)kk];
+            if (o->material[j] != o->fluid) continue;
+            float g_t = o->g_tilde[base + (size_t)kk];
+            if (g_t == 0.0f) continue;
+            float m_bar = 0.5f * (m_i + o->m[j]);
+            float w = m_bar * g_t;
+            for (int a = 0; a < 3; ++a) sum[a] = sum[a] + w * (vi[a] - vin[3 * j + a]);
+        }
+        for (int a = 0; a < 3; ++a) out[a] = vi[a] + factor * sum[a];
+    }
+}
+
+/* compute_precond_jeske23, jeske23.py:298-350: diagonal, 1 / max(1 + dt^2 sigma/m_i sum m_bar g~, 1e-12) */
+static void st_precond(const st_op *o, float *Pinv)
+{
+    for (int i = 0; i < o->n; ++i) {
+        float *P = Pinv + 9 * (size_t)i;
+        memset(P, 0, sizeof(float) * 9);
+        if (o->is_dynamic[i] == 0 || o->material[i] != o->fluid) {
+            P[0] = P[4] = P[8] = 1.0f;
+            continue;
+        }
+        float m_i = o->m[i];
+        float dt2 = o->dt * o->dt;
+        float factor = dt2 * o->sigma / m_i;
+        float diag_sum = 0.0f;
+        size_t base = (size_t)i * (size_t)o->C;
+        int cnt = capped(o->count, i, o->C);
+        for (int kk = 0; kk < cnt; ++kk) {
+            int j = o->ids[base + (size_t)kk];
+            if (o->material[j] != o->fluid) continue;
+            float m_bar = 0.5f * (m_i + o->m[j]);
+            diag_sum = diag_sum + m_bar * o->g_tilde[base + (size_t)kk];
+        }
+        float diag_val = 1.0f + factor * diag_sum;
+        if (diag_val < 1.0e-12f) diag_val = 1.0e-12f;
+        float inv_val = 1.0f / diag_val;
+        P[0] = P[4] = P[8] = inv_val;
+    }
+}
+
+/* compute_divergence_jeske23 + compute_g_tilde_jeske23 + compute_rhs_jeske23
+ * (jeske23.py:73-242): div (cached spiky gradient), g~ per pair, b */
+void orc_jeske23_setup(int n, const float *x, const float *v, const float *m, const float *density,
+                       const int *material, const int *is_dynamic, int C, const int *count, const int *ids,
+                       const float *gradW, float sigma, float dt, float d, float h_st, int fluid,
+                       float *div, float *g_tilde, float *b)
+{
+    for (int i = 0; i < n; ++i) {
+        if (is_dynamic[i] == 0 || material[i] != fluid) { div[i] = 0.0f; continue; }
+        const float *vi = v + 3 * i;
+        float div_val = 0.0f;
+        size_t base = (size_t)i * (size_t)C;
+        int cnt = capped(count, i, C);
+        for (int kk = 0; kk < cnt; ++kk) {
+            int j = ids[base + (size_t)kk];
+            if (material[j] != fluid) continue;
+            float v_ij[3] = {vi[0] - v[3 * j], vi[1] - v[3 * j + 1], vi[2] - v[3 * j + 2]};
+            div_val = div_val + m[j] * dot3(v_ij, gradW + 3 * (base + (size_t)kk));
+        }
+        div[i] = div_val;
+    }
+    for (int i = 0; i < n; ++i) {
+        size_t base = (size_t)i * (size_t)C;
+        int cnt = capped(count, i, C);
+        if (is_dynamic[i] == 0 || material[i] != fluid) {
+            for (int kk = 0; kk < cnt; ++kk) g_tilde[base + (size_t)kk] = 0.0f;
+            continue;
+        }
+        const float *xi = x + 3 * i, *vi = v + 3 * i;
+        float rho_i = density[i], div_i = div[i];
+        for (int kk = 0; kk < cnt; ++kk) {
+            int j = ids[base + (size_t)kk];
+            float *gt = g_tilde + base + (size_t)kk;
+            if (material[j] != fluid) { *gt = 0.0f; continue; }
+            float x_ij[3] = {xi[0] - x[3 * j], xi[1] - x[3 * j + 1], xi[2] - x[3 * j + 2]};
+            float r_ij = len3(x_ij);
+            if (r_ij >= h_st) { *gt = 0.0f; continue; }
+            float v_ij[3] = {vi[0] - v[3 * j], vi[1] - v[3 * j + 1], vi[2] - v[3 * j + 2]};
+            float rho_bar = 0.5f * (rho_i + density[j]);
+            float wst = orc_wst(r_ij, d, h_st);
+            float gwst[3];
+            orc_grad_wst(x_ij, r_ij, d, h_st, gwst);
+            float g_ij = wst / rho_bar;
+            float term_kernel = dot3(gwst, v_ij) / rho_bar;
+            float term_density = wst / (2.0f * rho_bar * rho_bar) * (div_i + div[j]);
+            *gt = g_ij + dt * (term_kernel - term_density);
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        float *bi = b + 3 * (size_t)i;
+        bi[0] = bi[1] = bi[2] = 0.0f;
+        if (is_dynamic[i] == 0 || material[i] != fluid) continue;
+        const float *xi = x + 3 * i;
+        float m_i = m[i];
+        float factor = -dt * sigma / m_i;
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+        size_t base = (size_t)i * (size_t)C;
+        int cnt = capped(count, i, C);
+        for (int kk = 0; kk < cnt; ++kk) {
+            int j = ids[base + (size_t)kk];
+            if (material[j] != fluid) continue;
+            float g_t = g_tilde[base + (size_t)kk];
+            if (g_t == 0.0f) continue;
+            float m_bar = 0.5f * (m_i + m[j]);
+            for (int a = 0; a < 3; ++a) acc[a] = acc[a] + m_bar * (xi[a] - x[3 * j + a]) * g_t;
+        }
+        for (int a = 0; a < 3; ++a) bi[a] = factor * acc[a];
+    }
+}
+
+void orc_jeske23_apply_M(int n, const float *m, const int *material, const int *is_dynamic, int C,
+                         const int *count, const int *ids, const float *g_tilde, float sigma, float dt,
+                         int fluid, const float *vin, float *Mv, float *Pinv)
+{
+    st_op o = {n, C, m, material, is_dynamic, count, ids, g_tilde, sigma, dt, fluid};
+    if (vin && Mv) st_Av(&o, vin, Mv);
+    if (Pinv) st_precond(&o, Pinv);
+}
+
+/* Jeske23.solve + Jeske23.PCG (jeske23.py:458-655): correction form M dv = v - M v + b from
+ * dv = 0, diagonal preconditioner, tol 1e-5 on r.r, at most 100 iterations; v += dv.
+ * Returns the CG iteration count. */
+int orc_jeske23(int n, const float *x, float *v, const float *m, const float *density, const int *material,
+                const int *is_dynamic, int C, const int *count, const int *ids, const float *gradW,
+                float sigma, float dt, float d, float h_st, int fluid)
+{
+    size_t n1 = (size_t)(n > 0 ? n : 1), n3 = 3 * n1;
+    float *div = (float *)calloc(n1, sizeof(float));
+    float *gt = (float *)calloc(n1 * (size_t)(C > 0 ? C : 1), sizeof(float));
+    float *bst = (float *)calloc(n3, sizeof(float));
+    float *Pinv = (float *)calloc(9 * n1, sizeof(float));
+    float *dv = (float *)calloc(n3, sizeof(float));
+    float *r = (float *)calloc(n3, sizeof(float));
+    float *z = (float *)calloc(n3, sizeof(float));
+    float *p = (float *)calloc(n3, sizeof(float));
+    float *Ap = (float *)calloc(n3, sizeof(float));
+    float *rhs = (float *)calloc(n3, sizeof(float));
+    orc_jeske23_setup(n, x, v, m, density, material, is_dynamic, C, count, ids, gradW, sigma, dt, d, h_st, fluid,
+                      div, gt, bst);
+    st_op o = {n, C, m, material, is_dynamic, count, ids, gt, sigma, dt, fluid};
+    st_precond(&o, Pinv);
+    st_Av(&o, v, Ap);
+    add_vec3(n, rhs, v, -1.0f, Ap, is_dynamic);
+    add_vec3(n, rhs, rhs, 1.0f, bst, is_dynamic);
+    memcpy(r, rhs, sizeof(float) * 3 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        if (is_dynamic[i] == 0) continue;
+        const float *P = Pinv + 9 * (size_t)i;
+        const float *ri = r + 3 * i;
+        for (int a = 0; a < 3; ++a) z[3 * i + a] = P[3 * a] * ri[0] + P[3 * a + 1] * ri[1] + P[3 * a + 2] * ri[2];
+    }
+    memcpy(p, z, sizeof(float) * 3 * (size_t)n);
+    double rz_old = dot_atomic(n, r, z, is_dynamic);
+    int it = 0;
+    if (rz_old > 1e-12) {
+        for (int s = 0; s < 100; ++s) {
+            st_Av(&o, p, Ap);
+            double pAp = dot_atomic(n, p, Ap, is_dynamic);
+            if (pAp <= 0.0) break;
+            double alpha = rz_old / (pAp > 1e-12 ? pAp : 1e-12);
+            add_vec3(n, dv, dv, (float)alpha, p, is_dynamic);
+            add_vec3(n, r, r, (float)(-alpha), Ap, is_dynamic);
+            double rn = dot_atomic(n, r, r, is_dynamic);
+            it++;
+            if ((rn < 1e-5 && it >= 2) || it >= 100) break;
+            for (int i = 0; i < n; ++i) {
+                if (is_dynamic[i] == 0) continue;
+                const float *P = Pinv + 9 * (size_t)i;
+                const float *ri = r + 3 * i;
+                for (int a = 0; a < 3; ++a) z[3 * i + a] = P[3 * a] * ri[0] + P[3 * a + 1] * ri[1] + P[3 * a + 2] * ri[2];
+            }
+            double rz_new = dot_atomic(n, r, z, is_dynamic);
+            double beta = rz_new / (rz_old > 1e-12 ? rz_old : 1e-12);
+            add_vec3(n, p, z, (float)beta, p, is_dynamic);
+            rz_old = rz_new;
+        }
+    }
+    add_vec3(n, v, v, 1.0f, dv, is_dynamic);
+    free(div); free(gt); free(bst); free(Pinv); free(dv); free(r); free(z); free(p); free(Ap); free(rhs);
     return it;
 }
 

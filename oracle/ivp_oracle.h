@@ -76,6 +76,18 @@ int orc_weiler18(int n, const float *x, float *v, const float *m, const float *d
                  const float *density0, const int *material, const int *is_dynamic, float h, int C,
                  const int *count, const int *ids, const float *gradW, float coeff, float dt,
                  int material_fluid);
+float orc_wst(float r_ij, float d, float h_st);
+void orc_grad_wst(const float x_ij[3], float r_ij, float d, float h_st, float out[3]);
+void orc_jeske23_setup(int n, const float *x, const float *v, const float *m, const float *density,
+                       const int *material, const int *is_dynamic, int C, const int *count, const int *ids,
+                       const float *gradW, float sigma, float dt, float d, float h_st, int fluid,
+                       float *div, float *g_tilde, float *b);
+void orc_jeske23_apply_M(int n, const float *m, const int *material, const int *is_dynamic, int C,
+                         const int *count, const int *ids, const float *g_tilde, float sigma, float dt,
+                         int fluid, const float *vin, float *Mv, float *Pinv);
+int orc_jeske23(int n, const float *x, float *v, const float *m, const float *density, const int *material,
+                const int *is_dynamic, int C, const int *count, const int *ids, const float *gradW,
+                float sigma, float dt, float d, float h_st, int fluid);
 void orc_gravity(int n, float *v, const int *is_dynamic, const float g[3], float dt);
 void orc_advect(int n, float *x, float *x_old, float *v, const int *is_dynamic, float dt);
 void orc_enforce_boundary(int n, float *x, float *v, const int *is_dynamic,

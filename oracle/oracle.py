@@ -237,6 +237,54 @@ def weiler18(x, v, m, dens, density0, material, is_dynamic, h, C, count, ids, gr
     return v, int(it)
 
 
+def wst(r, d, h_st):
+    lib().orc_wst.restype = ctypes.c_float
+    return float(lib().orc_wst(ctypes.c_float(r), ctypes.c_float(d), ctypes.c_float(h_st)))
+
+
+def grad_wst(x_ij, r, d, h_st):
+    out = np.zeros(3, np.float32)
+    lib().orc_grad_wst(_p(_f32(x_ij)), ctypes.c_float(r), ctypes.c_float(d), ctypes.c_float(h_st), _p(out))
+    return out
+
+
+def jeske23_setup(x, v, m, dens, material, is_dynamic, C, count, ids, gradW, sigma, dt, d, h_st,
+                  material_fluid=1):
+    """compute_divergence / compute_g_tilde / compute_rhs (jeske23.py:73-242): (div, g~[n*C], b)."""
+    n = x.shape[0]
+    div = np.zeros(n, np.float32)
+    gt = np.zeros(n * C, np.float32)
+    b = np.zeros((n, 3), np.float32)
+    lib().orc_jeske23_setup(ctypes.c_int(n), _p(_f32(x)), _p(_f32(v)), _p(_f32(m)), _p(_f32(dens)),
+                            _p(_i32(material)), _p(_i32(is_dynamic)), ctypes.c_int(C), _p(_i32(count)),
+                            _p(_i32(ids)), _p(_f32(gradW)), ctypes.c_float(sigma), ctypes.c_float(dt),
+                            ctypes.c_float(d), ctypes.c_float(h_st), ctypes.c_int(material_fluid),
+                            _p(div), _p(gt), _p(b))
+    return div, gt, b
+
+
+def jeske23_apply_M(m, material, is_dynamic, C, count, ids, g_tilde, sigma, dt, vin, material_fluid=1):
+    """(M v, block-diagonal preconditioner inverse) of jeske23.py:245-350."""
+    n = vin.shape[0]
+    Mv = np.zeros((n, 3), np.float32)
+    P = np.zeros((n, 9), np.float32)
+    lib().orc_jeske23_apply_M(ctypes.c_int(n), _p(_f32(m)), _p(_i32(material)), _p(_i32(is_dynamic)),
+                              ctypes.c_int(C), _p(_i32(count)), _p(_i32(ids)), _p(_f32(g_tilde)),
+                              ctypes.c_float(sigma), ctypes.c_float(dt), ctypes.c_int(material_fluid),
+                              _p(_f32(vin)), _p(Mv), _p(P))
+    return Mv, P
+
+
+def jeske23(x, v, m, dens, material, is_dynamic, C, count, ids, gradW, sigma, dt, d, h_st, material_fluid=1):
+    """Jeske23.solve (jeske23.py:590-655); returns (v, CG iterations)."""
+    v = _f32(v).copy()
+    it = lib().orc_jeske23(ctypes.c_int(v.shape[0]), _p(_f32(x)), _p(v), _p(_f32(m)), _p(_f32(dens)),
+                           _p(_i32(material)), _p(_i32(is_dynamic)), ctypes.c_int(C), _p(_i32(count)),
+                           _p(_i32(ids)), _p(_f32(gradW)), ctypes.c_float(sigma), ctypes.c_float(dt),
+                           ctypes.c_float(d), ctypes.c_float(h_st), ctypes.c_int(material_fluid))
+    return v, int(it)
+
+
 def gravity(v, is_dynamic, g, dt):
     v = _f32(v).copy()
     lib().orc_gravity(ctypes.c_int(v.shape[0]), _p(v), _p(_i32(is_dynamic)), _p(_f32(g)), ctypes.c_float(dt))
@@ -297,6 +345,7 @@ class OracleSim:
         )
         self.last_ip = None
         self.last_visc_iters = 0
+        self.last_st_iters = 0
         self.max_total = 0
 
     def rebuild(self):
@@ -307,9 +356,18 @@ class OracleSim:
         self.k = aii(self.m, self.m_inv, self.is_dynamic, self.C, self.count, self.ids, self.gradW)
 
     def non_pressure(self):
-        """compute_non_pressure_forces (framework.py:400-421); surface tension at coefficient 0
-        is an exact no-op and is skipped."""
+        """compute_non_pressure_forces (framework.py:400-421): gravity, surface tension, viscosity.
+        Surface tension at coefficient 0 is an exact no-op and is skipped; the explicit (Akinci)
+        model of surfaceTensionMethod 0 is not restated."""
         self.v = gravity(self.v, self.is_dynamic, self.g, self.dt)
+        st_coeff = float(self.cfg.surface_tension_coeff)
+        if st_coeff != 0.0:
+            if int(self.cfg.surface_tension_method) != 1:
+                raise NotImplementedError("explicit surface tension (surfaceTensionMethod 0) is not restated")
+            d = 2.0 * float(self.cfg.particle_radius)
+            self.v, self.last_st_iters = jeske23(self.x, self.v, self.m, self.density, self.material,
+                                                 self.is_dynamic, self.C, self.count, self.ids, self.gradW,
+                                                 st_coeff, self.dt, d, 2.0 * d)
         method = int(self.cfg.viscosity_method)
         coeff = float(self.cfg.viscosity_coeff)
         if method == 1:

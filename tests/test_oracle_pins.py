@@ -484,6 +484,120 @@ def test_weiler18_operator_preconditioner_and_solve():
 
 
 # ---------------------------------------------------------------------------------------------
+# Jeske23 implicit surface tension (jeske23.py:23-655)
+# ---------------------------------------------------------------------------------------------
+
+D = 0.02          # particle diameter d = 2r
+HST = 0.04        # h_st = 2d
+THETA = 10.0 / 7.0
+NORM = 8.0 / (math.pi * HST ** 3)
+
+
+def _wst64(r):
+    q = max(r, D) / HST
+    if q <= 0.5:
+        return THETA * NORM * (1.0 - 6.0 * q * q + 6.0 * q ** 3)
+    if q <= 1.0:
+        return THETA * NORM * 2.0 * (1.0 - q) ** 3
+    return 0.0
+
+
+def test_wst_closed_form_flat_core_and_support():
+    """W^st = theta W_cubic(max(r, d); h_st): flat below d, the cubic spline above, 0 past h_st."""
+    for r in np.linspace(0.0, 0.05, 101):
+        np.testing.assert_allclose(O.wst(float(r), D, HST), _wst64(float(r)), rtol=2e-6, atol=1e-3)
+    assert O.wst(0.0, D, HST) == O.wst(D, D, HST)
+    assert O.wst(0.041, D, HST) == 0.0
+
+
+def test_grad_wst_is_the_radial_derivative():
+    """grad_i W^st = dW^st/dr x_ij / r for d < r <= h_st (central differences in float64), 0 below d."""
+    rng = np.random.default_rng(3)
+    for r in np.linspace(0.021, 0.0395, 40):
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        x = (r * u).astype(np.float32)
+        rr = float(np.linalg.norm(x.astype(np.float64)))
+        g = O.grad_wst(x, np.float32(rr), D, HST).astype(np.float64)
+        e = 1e-7
+        dwdr = (_wst64(rr + e) - _wst64(rr - e)) / (2 * e)
+        np.testing.assert_allclose(g, dwdr * x / rr, rtol=2e-4, atol=1e-2)
+    np.testing.assert_array_equal(O.grad_wst(np.array([0.015, 0, 0], np.float32), 0.015, D, HST), 0.0)
+
+
+def _st_setup(sim, v, sigma, dt):
+    return O.jeske23_setup(sim.x, v, sim.m, sim.density, sim.material, sim.is_dynamic, sim.C, sim.count,
+                           sim.ids, sim.gradW, sigma, dt, D, HST)
+
+
+def test_jeske23_two_particles_at_rest():
+    """Two fluid particles at rest, d < r < h_st: div = 0, g~ = W^st / rho, and the extra
+    right-hand side pulls each towards the other with equal and opposite momenta."""
+    x = np.array([[0.1, 0.1, 0.1], [0.127, 0.104, 0.098]], np.float32)
+    v = np.zeros((2, 3), np.float32)
+    m = np.array([0.0064, 0.0064], np.float32)
+    rho = np.array([1000.0, 1000.0], np.float32)
+    mat = np.ones(2, np.int32)
+    dyn = np.ones(2, np.int32)
+    g = O.grid_init((0.0, 0.0, 0.0), (0.3, 0.3, 0.3), H)
+    count, ids, gradW, _ = O.build_neighbors(g, x, dyn, H, 60)
+    sigma, dt = 0.1, 1e-3
+    div, gt, b = O.jeske23_setup(x, v, m, rho, mat, dyn, 60, count, ids, gradW, sigma, dt, D, HST)
+    np.testing.assert_array_equal(div, 0.0)
+    r = x[0].astype(np.float64) - x[1]
+    rn = np.linalg.norm(r)
+    np.testing.assert_allclose(gt[0], _wst64(rn) / 1000.0, rtol=1e-5)
+    assert gt[0] == gt[60]                       # g~_ij == g~_ji
+    bi = -dt * sigma / 0.0064 * 0.0064 * r * (_wst64(rn) / 1000.0)
+    np.testing.assert_allclose(b[0], bi, rtol=1e-5)
+    np.testing.assert_allclose(b[0] + b[1], 0.0, atol=1e-7 * np.abs(bi).max())
+    assert np.dot(b[0], r) < 0.0                 # cohesion: towards the neighbour
+
+
+def _dense_st(sim, gt, sigma, dt):
+    n = sim.x.shape[0]
+    dyn = np.nonzero(sim.is_dynamic == 1)[0]
+    cols = []
+    for i in dyn:
+        for a in range(3):
+            e = np.zeros((n, 3), np.float32)
+            e[i, a] = 1.0
+            Mv, _ = O.jeske23_apply_M(sim.m, sim.material, sim.is_dynamic, sim.C, sim.count, sim.ids, gt, sigma, dt, e)
+            cols.append(Mv[dyn].reshape(-1).astype(np.float64))
+    return np.stack(cols, axis=1), dyn
+
+
+def test_jeske23_operator_structure_and_solve():
+    """M = I + dt^2 sigma diag(1/m) L with L symmetric (m_bar and g~ are symmetric in i, j) and
+    L 1 = 0 (a uniform velocity is left unchanged); the preconditioner is 1/diag(M); the solve
+    returns v + dv with M dv = v - M v + b to within the CG tolerance."""
+    sim = tiny_sim(n=(4, 4, 4))
+    rng = np.random.default_rng(5)
+    n = sim.x.shape[0]
+    v = rng.normal(scale=0.2, size=(n, 3)).astype(np.float32)
+    v[sim.is_dynamic == 0] = 0.0
+    sigma, dt = 50.0, 2e-3
+    div, gt, b = _st_setup(sim, v, sigma, dt)
+    M, dyn = _dense_st(sim, gt, sigma, dt)
+    mrep = np.repeat(sim.m[dyn].astype(np.float64), 3)
+    L = (M - np.eye(M.shape[0])) * mrep[:, None]
+    np.testing.assert_allclose(L, L.T, rtol=1e-4, atol=1e-6 * np.abs(L).max())
+    np.testing.assert_allclose(L.sum(axis=1).reshape(-1, 3)[:, 0], 0.0, atol=1e-5 * np.abs(L).max())
+    _, P = O.jeske23_apply_M(sim.m, sim.material, sim.is_dynamic, sim.C, sim.count, sim.ids, gt, sigma, dt, v)
+    np.testing.assert_allclose(P[dyn][:, 0], 1.0 / np.diag(M)[0::3], rtol=1e-5)
+    assert np.all(P[dyn][:, [1, 2, 3, 5, 6, 7]] == 0.0)
+    v2, it = O.jeske23(sim.x, v, sim.m, sim.density, sim.material, sim.is_dynamic, sim.C, sim.count, sim.ids,
+                       sim.gradW, sigma, dt, D, HST)
+    assert it >= 2
+    rhs = v[dyn].reshape(-1).astype(np.float64) - M @ v[dyn].reshape(-1) + b[dyn].reshape(-1)
+    dv = (v2[dyn] - v[dyn]).reshape(-1).astype(np.float64)
+    res = M @ dv - rhs
+    assert float(res @ res) < 2e-5                      # the CG stop test, r.r < 1e-5
+    expect = np.linalg.solve(M, rhs)
+    np.testing.assert_allclose(dv, expect, rtol=0, atol=5e-3 * max(np.abs(expect).max(), 1e-3))
+
+
+# ---------------------------------------------------------------------------------------------
 # Whole step (framework.py:423-544)
 # ---------------------------------------------------------------------------------------------
 
