@@ -31,7 +31,7 @@ MIN_SCENE_DYNAMIC = 256
 
 # kernel classes of ivp_profile_read / ivp_profile_enable masks
 KCLASSES = ["sort", "build", "ip_pack", "ip_rhs", "cg_J", "cg_JT", "cg_update", "cg_p", "visc", "w18_Av",
-            "w18_update", "w18_p", "w18_pre", "cg_refresh", "glue"]
+            "w18_update", "w18_p", "w18_pre", "cg_refresh", "st_setup", "st_Av", "glue"]
 
 
 class Config(ctypes.Structure):
@@ -59,7 +59,8 @@ class CGStats(ctypes.Structure):
 
 class StepParams(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_float), ("gravity", ctypes.c_float * 3), ("viscosity_method", ctypes.c_int),
-                ("viscosity_coeff", ctypes.c_float), ("ip", IPParams)]
+                ("viscosity_coeff", ctypes.c_float), ("ip", IPParams), ("surface_tension_method", ctypes.c_int),
+                ("surface_tension_coeff", ctypes.c_float)]
 
 
 class Scene(ctypes.Structure):
@@ -98,7 +99,9 @@ SIGNATURES = {
     "ivp_viscosity_weiler18": (_I, [_P, _F, _F, ctypes.POINTER(CGStats)]),
     "ivp_apply_gravity": (_I, [_P, ctypes.POINTER(_F), _F]),
     "ivp_advect_boundary": (_I, [_P, _F]),
-    "ivp_step": (_I, [_P, ctypes.POINTER(StepParams), ctypes.POINTER(IPStats), ctypes.POINTER(CGStats)]),
+    "ivp_step": (_I, [_P, ctypes.POINTER(StepParams), ctypes.POINTER(IPStats), ctypes.POINTER(CGStats),
+                      ctypes.POINTER(CGStats)]),
+    "ivp_surface_tension_jeske23": (_I, [_P, _F, _F, ctypes.POINTER(CGStats)]),
     "ivp_profile_enable": (_I, [_P, _I]),
     "ivp_profile_read": (_I, [_P, ctypes.POINTER(KernelTiming), _I]),
     "ivp_profile_reset": (_I, [_P]),

@@ -175,17 +175,32 @@ class Context:
     def advect_boundary(self, dt):
         N.check(self.lib.ivp_advect_boundary(self.h, ctypes.c_float(dt)), "ivp_advect_boundary")
 
-    def step(self, dt, gravity, viscosity_method, viscosity_coeff, params: IPParams):
+    def step(self, dt, gravity, viscosity_method, viscosity_coeff, params: IPParams, surface_tension_method=0,
+             surface_tension_coeff=0.0):
+        """One step; returns (IP result, viscosity CG iterations).  The surface-tension CG
+        iterations of the step are left in ``last_st_iters``."""
         sp = N.StepParams()
         sp.dt = float(dt)
         sp.gravity[:] = [float(v) for v in gravity]
         sp.viscosity_method = int(viscosity_method)
         sp.viscosity_coeff = float(viscosity_coeff)
         sp.ip = params.c()
+        sp.surface_tension_method = int(surface_tension_method)
+        sp.surface_tension_coeff = float(surface_tension_coeff)
         st, errs, its = self._stats(params.max_iteration_opt + 2)
         cs = N.CGStats()
-        status = N.check(self.lib.ivp_step(self.h, ctypes.byref(sp), ctypes.byref(st), ctypes.byref(cs)), "ivp_step")
+        cst = N.CGStats()
+        status = N.check(self.lib.ivp_step(self.h, ctypes.byref(sp), ctypes.byref(st), ctypes.byref(cs),
+                                           ctypes.byref(cst)), "ivp_step")
+        self.last_st_iters = cst.cg_iters
         return IPResult(status, st, errs, its), cs.cg_iters
+
+    def surface_tension_jeske23(self, sigma, dt):
+        """Jeske23.solve; returns the CG iteration count."""
+        cs = N.CGStats()
+        N.check(self.lib.ivp_surface_tension_jeske23(self.h, ctypes.c_float(sigma), ctypes.c_float(dt),
+                                                     ctypes.byref(cs)), "ivp_surface_tension_jeske23")
+        return cs.cg_iters
 
     # -- independent scenes (one context, many scenes) ----------------------------------------
     def set_scenes(self, sizes, domain_starts, domain_sizes):

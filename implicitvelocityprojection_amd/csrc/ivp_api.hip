@@ -74,6 +74,7 @@ void free_all(ivp_ctx* c) {
     dfree(c->cc); dfree(c->sfac); dfree(c->jd); dfree(c->xs4); dfree(c->nbact);
     dfree(c->w_pinv); dfree(c->w_p4); dfree(c->w_v4); dfree(c->w_dv); dfree(c->w_r); dfree(c->w_z);
     dfree(c->w_Ap); dfree(c->w_rhs); dfree(c->w_fac); dfree(c->v4);
+    dfree(c->st_gt); dfree(c->st_div); dfree(c->st_b);
     dfree(c->partials); dfree(c->state); dfree(c->err_flag);
     if (c->state_host) (void)hipHostFree(c->state_host);
     if (c->err_host) (void)hipHostFree(c->err_host);
@@ -137,7 +138,28 @@ int ensure_rows(ivp_ctx* c) {
 void free_rows(ivp_ctx* c) {
     scene_graphs_clear(c);
     dfree(c->nb);
+    dfree(c->st_gt);
 }
+
+}  // namespace
+
+namespace ivp {
+
+// Jeske23 buffers: g~ per row slot (sized like the ids of the current rows), divergence and the
+// extra right-hand side per particle.
+int st_buffers(ivp_ctx* c) {
+    if (!c->st_div) IVP_TRY(dalloc(&c->st_div, (size_t)c->cap));
+    if (!c->st_b) IVP_TRY(dalloc(&c->st_b, 3 * (size_t)c->cap));
+    if (!c->st_gt) {
+        size_t tiles = (size_t)blocks_for(std::max(c->nd, 1)) * WAVES_PER_BLOCK;
+        IVP_TRY(dalloc(&c->st_gt, tiles * (size_t)c->Cs * 64u));
+    }
+    return IVP_OK;
+}
+
+}  // namespace ivp
+
+namespace {
 
 // Independent scenes: one cell grid per scene (same cell width and margins as the single-scene
 // grid), keys consecutive in scene order; cell tables and scene buffers reallocated.
@@ -271,7 +293,7 @@ int prof_collect(ivp_ctx* c, int cg_iters_done, int epoch) {
 
 static const char* kclass_name(int k) {
     static const char* names[KC_COUNT] = {"sort", "build", "ip_pack", "ip_rhs", "cg_J", "cg_JT", "cg_update", "cg_p",
-                                          "visc", "w18_Av", "w18_update", "w18_p", "w18_pre", "cg_refresh", "glue"};
+                                          "visc", "w18_Av", "w18_update", "w18_p", "w18_pre", "cg_refresh", "st_setup", "st_Av", "glue"};
     return (k >= 0 && k < KC_COUNT) ? names[k] : "?";
 }
 
@@ -494,6 +516,12 @@ int ivp_viscosity_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* stat
     return visc_weiler18(c, coeff, dt, stats);
 }
 
+int ivp_surface_tension_jeske23(ivp_ctx* c, float sigma, float dt, ivp_cg_stats* stats) {
+    if (!c) return IVP_ERR_ARG;
+    if (c->S > 0) return IVP_ERR_STATE;
+    return st_jeske23(c, sigma, dt, stats);
+}
+
 int ivp_apply_gravity(ivp_ctx* c, const float gravity[3], float dt) {
     if (!c || !gravity) return IVP_ERR_ARG;
     return glue_gravity(c, gravity, dt);
@@ -504,12 +532,19 @@ int ivp_advect_boundary(ivp_ctx* c, float dt) {
     return glue_advect_boundary(c, dt);
 }
 
-int ivp_step(ivp_ctx* c, const ivp_step_params* prm, ivp_ip_stats* stats, ivp_cg_stats* visc) {
+int ivp_step(ivp_ctx* c, const ivp_step_params* prm, ivp_ip_stats* stats, ivp_cg_stats* visc, ivp_cg_stats* st) {
     if (!c || !prm || !stats) return IVP_ERR_ARG;
     if (!c->built || c->S > 0) return IVP_ERR_STATE;
+    if (prm->surface_tension_coeff != 0.0f && prm->surface_tension_method != 1) return IVP_ERR_ARG;
     int rc = glue_gravity(c, prm->gravity, prm->dt);
     if (rc != IVP_OK) return rc;
     if (visc) { visc->cg_iters = 0; visc->residual_norm2 = 0.0; }
+    if (st) { st->cg_iters = 0; st->residual_norm2 = 0.0; }
+    // compute_non_pressure_forces order (framework.py:400-421): gravity, surface tension, viscosity
+    if (prm->surface_tension_coeff != 0.0f) {
+        rc = st_jeske23(c, prm->surface_tension_coeff, prm->dt, st);
+        if (rc != IVP_OK) return rc;
+    }
     // coefficient 0 makes either viscosity model an exact no-op (the rhs v - A v is 0, and the
     // explicit force is scaled by the coefficient)
     if (prm->viscosity_coeff != 0.0f) {
@@ -534,7 +569,8 @@ int ivp_set_scenes(ivp_ctx* c, int n_scenes, const ivp_scene* scenes) {
 int ivp_step_scenes(ivp_ctx* c, const ivp_step_params* prm, ivp_scene_stats* stats) {
     if (!c || !prm || !stats) return IVP_ERR_ARG;
     if (!c->built || c->S == 0) return IVP_ERR_STATE;
-    if (prm->viscosity_coeff != 0.0f) return IVP_ERR_ARG;   // viscosity is not part of the scene step
+    // viscosity and surface tension are not part of the scene step
+    if (prm->viscosity_coeff != 0.0f || prm->surface_tension_coeff != 0.0f) return IVP_ERR_ARG;
     int rc = glue_gravity(c, prm->gravity, prm->dt);
     if (rc != IVP_OK) return rc;
     rc = ip_solve_scenes(c, prm->dt, &prm->ip);

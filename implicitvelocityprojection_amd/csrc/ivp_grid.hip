@@ -486,6 +486,7 @@ __global__ __launch_bounds__(BLOCK) void k_dyn_flags(int n, const int* __restric
     int dyn = is_dynamic[u] != 0 ? 1 : 0;
     flag[u] = dyn;
     if (dyn && material[u] != fluid) atomicOr(err, 2);
+    if (!dyn && material[u] == fluid) atomicOr(err, 8);
 }
 
 struct StageArgs {
@@ -665,6 +666,7 @@ int grid_load(ivp_ctx* c, int n, const float* x, const float* v, const float* m,
     c->nd = *c->count_host;
     c->ns = n - c->nd;
     c->all_dyn_fluid = (*c->err_host & 2) == 0;
+    c->static_fluid = (*c->err_host & 8) != 0;
     int* ustart = nullptr;
     if (c->S > 0) {
         // scene boundaries in user order -> dynamic / static starts in internal order

@@ -117,6 +117,8 @@ enum KernelClass {
     KC_W18_P,
     KC_W18_PRE,
     KC_CG_REFRESH,
+    KC_ST_SETUP,
+    KC_ST_AV,
     KC_GLUE,
     KC_COUNT
 };
@@ -146,6 +148,7 @@ struct ivp_ctx {
     bool loaded = false;
     bool built = false;
     bool all_dyn_fluid = true;
+    bool static_fluid = false;   // some static particle has the fluid material
     // neighbour-row format (IVP_ROWS_*): ids only, the operator passes evaluating gradW from
     // positions; or ids with cached float32 / float16 gradW.  rec_words = words per (tile, slot)
     // record; gscale = the float16 records' scale, the power of two at or above 45/(pi h^6) h^2,
@@ -208,6 +211,11 @@ struct ivp_ctx {
     // explicit viscosity
     float4* v4 = nullptr;     // (v, rho)
 
+    // Jeske23 surface tension (allocated at first use)
+    float* st_gt = nullptr;   // g~ per stored pair, tile layout of the ids: [(t*Cs + k)*64 + lane]
+    float* st_div = nullptr;  // velocity divergence per row
+    float* st_b = nullptr;    // extra right-hand side, 3 * cap
+
     // reductions
     double* partials = nullptr;   // 4 * max_blocks
     int max_blocks = 0;
@@ -238,6 +246,7 @@ struct ivp_ctx {
     // first CG launch chunk of the next IP / Weiler18 solve: the last solve's iteration count
     int cg_guess = 4;
     int w18_guess = 4;
+    int st_guess = 4;
     int scene_chunk = 4;                  // CG iterations per replay
     int scene_lanes = 1;                  // lanes per row in the scene passes (1, 2 or 4)
     bool scene_graph = true;              // replay a captured hipGraph
@@ -281,6 +290,8 @@ int ip_solve_scenes(ivp_ctx* c, float dt, const ivp_ip_params* prm);
 int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* st);
 int visc_explicit(ivp_ctx* c, float coeff, float dt);
 int visc_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* st);
+int st_jeske23(ivp_ctx* c, float sigma, float dt, ivp_cg_stats* st);
+int st_buffers(ivp_ctx* c);   // ivp_api.hip: allocate the Jeske23 buffers on first use
 int glue_gravity(ivp_ctx* c, const float g[3], float dt);
 int glue_advect_boundary(ivp_ctx* c, float dt);
 

@@ -904,6 +904,217 @@ __global__ __launch_bounds__(BLOCK) void k_w18_apply(int n3, const float* __rest
     vel[i] = vel[i] + 1.0f * dv[i];
 }
 
+// ---- Jeske23 implicit surface tension (jeske23.py) -----------------------------------------
+
+// compute_Wst, jeske23.py:23-43: theta W_cubic(max(r, d); h_st), theta = 10/7 (same operation
+// order as the reference; norm = 8 / (pi h_st^3) is passed in)
+__device__ __forceinline__ float st_wst(float r, float d, float hst, float tn) {
+    float rc = r > d ? r : d;
+    float q = rc / hst;
+    float fq = 0.0f;
+    if (q <= 0.5f) {
+        fq = 1.0f - 6.0f * q * q + 6.0f * q * q * q;
+    } else if (q <= 1.0f) {
+        float t = 1.0f - q;
+        fq = 2.0f * t * t * t;
+    }
+    return tn * fq;
+}
+
+// compute_grad_Wst, jeske23.py:46-66: returns s with grad = s * (x_ij / r); 0 for r <= d
+__device__ __forceinline__ float st_grad_scale(float r, float d, float hst, float tnh) {
+    if (r <= d || r < 1.0e-12f) return 0.0f;
+    float q = r / hst;
+    if (q > 1.0f) return 0.0f;
+    float fpq;
+    if (q <= 0.5f) {
+        fpq = -12.0f * q + 18.0f * q * q;
+    } else {
+        float t = 1.0f - q;
+        fpq = -6.0f * t * t;
+    }
+    return tnh * fpq;
+}
+
+struct StArgs {
+    Rows R;
+    int fluid;
+    const int* mat;
+    const float* density;
+    float4* u4;        // (v, m) of the pass input
+    float* div;
+    float* gt;         // g~ per slot, tile layout of the ids
+    float* b;          // extra right-hand side
+    float* pinv;       // 3x3 per row (diagonal)
+    float* out;        // M u, or v - M v + b
+    float sigma, dt, d, hst;
+    float tn, tnh;     // theta * 8/(pi h_st^3), and that times 1/h_st
+    double* partials;
+    int stride;
+    const SolverState* st;
+};
+
+// compute_divergence_jeske23 (jeske23.py:73-112): div_i = sum_fluid m_j (v_i - v_j) . gradW_ij
+template <class FM, bool ALLFLUID>
+__global__ __launch_bounds__(BLOCK) void k_st_div(StArgs a) {
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int i = pass_row(blk), g = pass_lane();
+    if (i >= a.R.nd) return;
+    if (!ALLFLUID && a.mat[i] != a.fluid) {
+        if (g == 0) a.div[i] = 0.0f;
+        return;
+    }
+    auto loadA = [&](int j) {
+        float4 A = a.R.pos[j];
+        if (!ALLFLUID && a.mat[j] != a.fluid) A.w = 0.0f;
+        return A;
+    };
+    auto loadB = [&](int j) { return vec3_of(a.u4, j); };
+    float4 ui = a.u4[i];
+    float acc = 0.0f;
+    row_pairs<FM, 3, PASS_LANES, PASS_UNROLL>(a.R, i, a.R.pos[i], loadA, loadB,
+                         [&](const float4& Aj, const Vec<3>& Bj, float gx, float gy, float gz) {
+                             float vx = ui.x - Bj.v[0], vy = ui.y - Bj.v[1], vz = ui.z - Bj.v[2];
+                             acc = acc + Aj.w * (vx * gx + vy * gy + vz * gz);
+                         }, g);
+    acc = group_sum<PASS_LANES>(acc);
+    if (g == 0) a.div[i] = acc;
+}
+
+// compute_g_tilde_jeske23 + compute_rhs_jeske23 + compute_precond_jeske23 (jeske23.py:115-350)
+// in one pass over each row's slots: g~_ij (stored per slot), b_i and the diagonal
+// preconditioner.  Padding slots (id = the row) and non-fluid neighbours get g~ = 0.
+template <class FM, bool ALLFLUID>
+__global__ __launch_bounds__(BLOCK) void k_st_setup(StArgs a) {
+    constexpr unsigned REC = FM::REC;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int i = pass_row(blk), g = pass_lane();
+    if (i >= a.R.nd) return;
+    float* P = a.pinv + 9 * (size_t)i;
+    const bool fluid_i = ALLFLUID || a.mat[i] == a.fluid;
+    if (!fluid_i) {
+        if (g == 0) {
+            for (int q = 0; q < 9; ++q) P[q] = (q % 4) == 0 ? 1.0f : 0.0f;
+            a.b[3 * i + 0] = 0.0f; a.b[3 * i + 1] = 0.0f; a.b[3 * i + 2] = 0.0f;
+        }
+        return;
+    }
+    const uint32_t* e = a.R.nb + (size_t)(i >> 6) * (size_t)a.R.Cs * REC + (i & (WAVE - 1));
+    float* eg = a.gt + (size_t)(i >> 6) * (size_t)a.R.Cs * 64u + (i & (WAVE - 1));
+    const int tl = __builtin_amdgcn_readfirstlane(a.R.tile_len[i >> 6]);
+    const float4 xi = a.R.pos[i];
+    const float4 ui = a.u4[i];
+    const float m_i = ui.w, rho_i = a.density[i], div_i = a.div[i];
+    float bx = 0.0f, by = 0.0f, bz = 0.0f, diag = 0.0f;
+    for (int k = g; k < tl; k += PASS_LANES) {
+        const int j = (int)__builtin_nontemporal_load(e + (size_t)k * REC);
+        float gt = 0.0f;
+        float4 xj = a.R.pos[j];
+        float4 uj = a.u4[j];
+        bool use = j != i && (ALLFLUID || a.mat[j] == a.fluid);
+        float x0 = xi.x - xj.x, x1 = xi.y - xj.y, x2 = xi.z - xj.z;
+        float r = len3(x0, x1, x2);
+        if (use && r < a.hst) {
+            float v0 = ui.x - uj.x, v1 = ui.y - uj.y, v2 = ui.z - uj.z;
+            float rho_bar = 0.5f * (rho_i + a.density[j]);
+            float wst = st_wst(r, a.d, a.hst, a.tn);
+            float s = st_grad_scale(r, a.d, a.hst, a.tnh);
+            float g0 = s * (x0 / r), g1 = s * (x1 / r), g2 = s * (x2 / r);
+            if (s == 0.0f) { g0 = 0.0f; g1 = 0.0f; g2 = 0.0f; }
+            float g_ij = wst / rho_bar;
+            float term_kernel = (g0 * v0 + g1 * v1 + g2 * v2) / rho_bar;
+            float term_density = wst / (2.0f * rho_bar * rho_bar) * (div_i + a.div[j]);
+            gt = g_ij + a.dt * (term_kernel - term_density);
+        }
+        eg[(size_t)k * 64u] = gt;
+        if (use) {
+            float m_bar = 0.5f * (m_i + uj.w);
+            diag = diag + m_bar * gt;
+            if (gt != 0.0f) {
+                bx = bx + m_bar * x0 * gt;
+                by = by + m_bar * x1 * gt;
+                bz = bz + m_bar * x2 * gt;
+            }
+        }
+    }
+    bx = group_sum<PASS_LANES>(bx);
+    by = group_sum<PASS_LANES>(by);
+    bz = group_sum<PASS_LANES>(bz);
+    diag = group_sum<PASS_LANES>(diag);
+    if (g != 0) return;
+    float fb = -a.dt * a.sigma / m_i;
+    a.b[3 * i + 0] = fb * bx; a.b[3 * i + 1] = fb * by; a.b[3 * i + 2] = fb * bz;
+    float dt2 = a.dt * a.dt;
+    float fm = dt2 * a.sigma / m_i;
+    float dv = 1.0f + fm * diag;
+    if (dv < 1.0e-12f) dv = 1.0e-12f;
+    float inv = 1.0f / dv;
+    for (int q = 0; q < 9; ++q) P[q] = (q % 4) == 0 ? inv : 0.0f;
+}
+
+// compute_Av_jeske23 (jeske23.py:245-295): (M u)_i = u_i + dt^2 (sigma/m_i) sum m_bar g~ (u_i - u_j).
+// RHS: out = u - M u + b (jeske23.py:637-647); otherwise out = M u and the block's p.Mp partial.
+// Only the ids and g~ are streamed and (u_j, m_j) gathered: positions are not needed.
+template <class FM, bool ALLFLUID, bool RHS>
+__global__ __launch_bounds__(BLOCK) void k_st_Av(StArgs a) {
+    if (!RHS && a.st->active == 0) return;
+    constexpr unsigned REC = FM::REC;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int i = pass_row(blk), g = pass_lane();
+    double pap = 0.0;
+    if (i < a.R.nd) {
+        const float4 ui = a.u4[i];
+        float Mx = ui.x, My = ui.y, Mz = ui.z;
+        if (ALLFLUID || a.mat[i] == a.fluid) {
+            const uint32_t* e = a.R.nb + (size_t)(i >> 6) * (size_t)a.R.Cs * REC + (i & (WAVE - 1));
+            const float* eg = a.gt + (size_t)(i >> 6) * (size_t)a.R.Cs * 64u + (i & (WAVE - 1));
+            const int tl = __builtin_amdgcn_readfirstlane(a.R.tile_len[i >> 6]);
+            const float m_i = ui.w;
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+#pragma unroll(PASS_UNROLL)
+            for (int k = g; k < tl; k += PASS_LANES) {
+                const int j = (int)__builtin_nontemporal_load(e + (size_t)k * REC);
+                const float gt = __builtin_nontemporal_load(eg + (size_t)k * 64u);
+                float4 uj = a.u4[j];
+                float w = (0.5f * (m_i + uj.w)) * gt;
+                sx = sx + w * (ui.x - uj.x);
+                sy = sy + w * (ui.y - uj.y);
+                sz = sz + w * (ui.z - uj.z);
+            }
+            sx = group_sum<PASS_LANES>(sx);
+            sy = group_sum<PASS_LANES>(sy);
+            sz = group_sum<PASS_LANES>(sz);
+            float dt2 = a.dt * a.dt;
+            float factor = dt2 * a.sigma / m_i;
+            Mx = ui.x + factor * sx;
+            My = ui.y + factor * sy;
+            Mz = ui.z + factor * sz;
+        }
+        if (g == 0) {
+            if (RHS) {
+                a.out[3 * i + 0] = (ui.x + -1.0f * Mx) + 1.0f * a.b[3 * i + 0];
+                a.out[3 * i + 1] = (ui.y + -1.0f * My) + 1.0f * a.b[3 * i + 1];
+                a.out[3 * i + 2] = (ui.z + -1.0f * Mz) + 1.0f * a.b[3 * i + 2];
+            } else {
+                a.out[3 * i + 0] = Mx; a.out[3 * i + 1] = My; a.out[3 * i + 2] = Mz;
+                pap = (double)(ui.x * Mx + ui.y * My + ui.z * Mz);
+            }
+        }
+    }
+    if (!RHS) {
+        double v[1] = {pap};
+        block_partials<1>(v, a.partials, a.stride);
+    }
+}
+
+// (v, m) of the particles for the surface-tension passes
+__global__ __launch_bounds__(BLOCK) void k_st_prep(int nd, const float* __restrict__ vel, const float4* __restrict__ pos,
+                                                   float4* __restrict__ u4) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nd) return;
+    u4[i] = make_float4(vel[3 * i + 0], vel[3 * i + 1], vel[3 * i + 2], pos[i].w);
+}
+
 // ---- glue ----------------------------------------------------------------------------------
 
 // apply_gravity (framework.py:26-41)
@@ -1281,9 +1492,94 @@ int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
     return launch(c, KC_GLUE, -1, 0.0, k_w18_apply, dim3(blocks_for(3 * nd)), 3 * nd, (const float*)c->w_dv, c->vel);
 }
 
+template <class FM, bool ALLFLUID>
+int jeske23_body(ivp_ctx* c, float sigma, float dt, int* iters_out) {
+    const int nd = c->nd;
+    const dim3 nb(blocks_for(nd)), np = pass_grid(nd);
+    const int stride = c->max_blocks;
+    IVP_TRY(st_buffers(c));
+    Rows R = rows_of(c);
+    // d = 2 r and h_st = 2 d = 4 r = h (jeske23.py:391-392 with h = 4 r): exact in float32
+    const float hst = c->h, d = 0.5f * c->h;
+    const float theta = 10.0f / 7.0f;
+    const float norm = 8.0f / (3.14159265358979f * hst * hst * hst);
+    const float tn = theta * norm;
+    const float tnh = tn * (1.0f / hst);
+    StArgs sa{R, c->cfg.material_fluid, c->mat, c->density, c->w_v4, c->st_div, c->st_gt, c->st_b, c->w_pinv,
+              c->w_rhs, sigma, dt, d, hst, tn, tnh, c->partials, stride, c->state};
+    const double pairs = (double)c->state_host->total_pairs;
+    const double pb = pair_bytes<FM>();
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_st_prep, nb, nd, (const float*)c->vel, (const float4*)c->pos, c->w_v4));
+    IVP_TRY(launch(c, KC_ST_SETUP, -1, pb * pairs + 36.0 * nd, k_st_div<FM, ALLFLUID>, np, sa));
+    IVP_TRY(launch(c, KC_ST_SETUP, -1, (pb + 4.0) * pairs + 72.0 * nd, k_st_setup<FM, ALLFLUID>, np, sa));
+    IVP_TRY(launch(c, KC_ST_AV, -1, (pb + 4.0) * pairs + 40.0 * nd, k_st_Av<FM, ALLFLUID, true>, np, sa));
+    W18Args wa{R, c->cfg.material_fluid, c->w_fac, c->mat, c->w_pinv, c->w_v4, c->w_rhs,
+               c->w_dv, c->w_r, c->w_z, c->partials, stride, c->state};
+    IVP_TRY(launch(c, -1, -1, 0.0, k_w18_init, nb, wa, c->w_p4));
+    // Jeske23.PCG (jeske23.py:499-570): tol 1e-5 on r.r, at most 100 iterations
+    IVP_TRY(finalize<FIN_W18_INIT>(c, nb.x, 1e-5, 100));
+    IVP_TRY(read_state(c));
+    int iters = 0, chunks = 0;
+    if (c->state_host->active) {
+        StArgs cga = sa;
+        cga.u4 = c->w_p4;
+        cga.out = c->w_Ap;
+        W18Args cg = wa;
+        cg.v4 = c->w_p4;
+        cg.out = c->w_Ap;
+        const double bAv = (pb + 4.0) * pairs + 28.0 * nd;
+        auto one_iter = [&](int tag) -> int {
+            IVP_TRY(launch(c, KC_ST_AV, tag, bAv, k_st_Av<FM, ALLFLUID, false>, np, cga));
+            IVP_TRY(finalize<FIN_W18_PAP>(c, np.x));
+            IVP_TRY(launch(c, KC_W18_UPDATE, tag, 124.0 * nd, k_w18_update, nb, cg));
+            IVP_TRY(finalize<FIN_W18_UPDATE>(c, nb.x));
+            return launch(c, KC_W18_P, tag, 44.0 * nd, k_w18_p, nb, nd, (const float*)c->w_z, c->w_p4,
+                          (const SolverState*)c->state);
+        };
+        c->prof_epoch++;
+        int epoch = c->prof_epoch;
+        IVP_TRY(run_chunks(c, 100, c->st_guess, one_iter, &iters, &chunks));
+        prof_collect(c, iters, epoch);
+        c->st_guess = std::max(2, iters);
+    }
+    *iters_out = iters;
+    return launch(c, KC_GLUE, -1, 0.0, k_w18_apply, dim3(blocks_for(3 * nd)), 3 * nd, (const float*)c->w_dv, c->vel);
+}
+
 }  // namespace
 
 namespace ivp {
+
+// Jeske23.solve (jeske23.py:590-655).  sigma = 0 leaves v unchanged (M = I, zero right-hand
+// side) and is skipped.  Static neighbours never enter: the reference skips non-fluid
+// neighbours, and a static particle with the fluid material is refused.
+int st_jeske23(ivp_ctx* c, float sigma, float dt, ivp_cg_stats* out) {
+    if (!c->built) return IVP_ERR_STATE;
+    if (out) { out->cg_iters = 0; out->residual_norm2 = 0.0; }
+    if (c->static_fluid) return IVP_ERR_ARG;
+    if (c->nd == 0 || sigma == 0.0f) return IVP_OK;
+    int iters = 0, rc;
+    switch (c->row_fmt) {
+    case IVP_ROWS_F32:
+        rc = c->all_dyn_fluid ? jeske23_body<FmtF32, true>(c, sigma, dt, &iters)
+                              : jeske23_body<FmtF32, false>(c, sigma, dt, &iters);
+        break;
+    case IVP_ROWS_F16:
+        rc = c->all_dyn_fluid ? jeske23_body<FmtF16Eval, true>(c, sigma, dt, &iters)
+                              : jeske23_body<FmtF16Eval, false>(c, sigma, dt, &iters);
+        break;
+    default:
+        rc = c->all_dyn_fluid ? jeske23_body<FmtIds, true>(c, sigma, dt, &iters)
+                              : jeske23_body<FmtIds, false>(c, sigma, dt, &iters);
+        break;
+    }
+    if (rc != IVP_OK) return rc;
+    if (out) {
+        out->cg_iters = iters;
+        out->residual_norm2 = c->state_host->rr_last;
+    }
+    return IVP_OK;
+}
 
 // IP.solve with its wall-clock cost (stats_total_cost_ms, ip.py:335, 403-404: the reference
 // synchronises before reading the clock, and so does this).

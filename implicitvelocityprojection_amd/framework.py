@@ -44,9 +44,14 @@ class Framework:
         self.viscosity = (Weiler18 if config.viscosity_method == 1 else Viscosity)(simulation_data, neighbour_search,
                                                                                    config)
         self.viscosity_coeff = config.viscosity_coeff
+        # framework.py:181-187: surfaceTensionMethod 1 is Jeske23 (implicit); the explicit model
+        # (method 0) is provided only at coefficient 0, where it does nothing
+        self.surface_tension_method = int(config.surface_tension_method)
         self.surface_tension_coeff = config.surface_tension_coeff
-        if self.surface_tension_coeff != 0.0:
-            raise NotImplementedError("surface tension is not part of the IP hot path (set surfaceTensionCoeff 0)")
+        if self.surface_tension_coeff != 0.0 and self.surface_tension_method != 1:
+            raise NotImplementedError("explicit surface tension (surfaceTensionMethod 0) is not part of this build; "
+                                      "use surfaceTensionMethod 1 (Jeske23) or surfaceTensionCoeff 0")
+        self.last_st_iters = 0
         self.pressure_enabled = bool(config.get_cfg("pressureEnabled", True))
         if not self.pressure_enabled:
             raise NotImplementedError("pressureEnabled false is not supported")
@@ -81,9 +86,10 @@ class Framework:
         ps.sync_to_library(positions=False, velocities=True)
         t0 = time.perf_counter()
         res, vit = ps.ctx.step(self.dt, self.g, self.cfg.viscosity_method, self.viscosity_coeff,
-                               self.pressure.params())
+                               self.pressure.params(), self.surface_tension_method, self.surface_tension_coeff)
         self.stats_pressure_ms = res.solve_ms
         self.last_visc_iters = vit
+        self.last_st_iters = ps.ctx.last_st_iters
         self.pressure.record(res)
         ps._cache_stale = True
         if self.sync_arrays:

@@ -85,6 +85,9 @@ typedef struct {
     int viscosity_method;    /* 0 explicit (Jacobi), 1 Weiler18 implicit           */
     float viscosity_coeff;   /* viscosityCoeff; 0 with method 0 skips the pass     */
     ivp_ip_params ip;
+    int surface_tension_method;   /* surfaceTensionMethod: 1 Jeske23 implicit; 0 (the explicit
+                                     model) only with coefficient 0                 */
+    float surface_tension_coeff;  /* surfaceTensionCoeff; 0 skips the pass          */
 } ivp_step_params;
 
 /* Fields readable with ivp_read / writable with ivp_write */
@@ -166,15 +169,23 @@ int ivp_viscosity_explicit(ivp_ctx *ctx, float coeff, float dt);
 /* Weiler18.solve (weiler18.py:371-401), block-Jacobi PCG, cold start. */
 int ivp_viscosity_weiler18(ivp_ctx *ctx, float coeff, float dt, ivp_cg_stats *stats);
 
+/* Jeske23.solve (jeske23.py:590-655): implicit surface tension, diagonal-preconditioned CG on
+ * M dv = v - M v + b (tol 1e-5 on r.r, at most 100 iterations), v += dv.  h_st = 2 d = h.
+ * IVP_ERR_ARG when a static particle has the fluid material (static neighbours are assumed to
+ * be boundary, which the reference skips as non-fluid). */
+int ivp_surface_tension_jeske23(ivp_ctx *ctx, float sigma, float dt, ivp_cg_stats *stats);
+
 /* apply_gravity (framework.py:26-41) */
 int ivp_apply_gravity(ivp_ctx *ctx, const float gravity[3], float dt);
 
 /* advect_position + enforce_boundary (framework.py:117-136, boundary.py:27-91) */
 int ivp_advect_boundary(ivp_ctx *ctx, float dt);
 
-/* Framework.forward (framework.py:423-544), velocity-based branch with IP: gravity,
- * viscosity, IP solve, advect, boundary, neighbour rebuild, density, Aii. */
-int ivp_step(ivp_ctx *ctx, const ivp_step_params *prm, ivp_ip_stats *stats, ivp_cg_stats *visc);
+/* Framework.forward (framework.py:423-544), velocity-based branch with IP: gravity, surface
+ * tension, viscosity, IP solve, advect, boundary, neighbour rebuild, density, Aii.  visc and st
+ * (either may be NULL) receive the viscosity and surface-tension CG statistics. */
+int ivp_step(ivp_ctx *ctx, const ivp_step_params *prm, ivp_ip_stats *stats, ivp_cg_stats *visc,
+             ivp_cg_stats *st);
 
 /* ---- independent scenes in one context ------------------------------------------------------
  * No counterpart in the reference (one Framework per scene there).  For callers that simulate

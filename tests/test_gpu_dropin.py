@@ -104,4 +104,9 @@ def test_unsupported_solvers_are_explicit():
     sc["Configuration"]["pressureMethod"] = 7
     with pytest.raises(ValueError):
         _setup(sc)
+    sc["Configuration"]["pressureMethod"] = 1
+    sc["Configuration"]["surfaceTensionMethod"] = 0
+    sc["Configuration"]["surfaceTensionCoeff"] = 0.1
+    with pytest.raises(NotImplementedError):
+        _setup(sc)
     assert Framework is not None
