@@ -230,6 +230,9 @@ def main():
                          "float16 precision form, is compared with the float64 restatement of the step; then 20 "
                          "steps of each form are timed from one state; 0 skips both")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="dambreak_1m")
+    ap.add_argument("--surface-tension", type=float, default=0.0, metavar="SIGMA",
+                    help="add Jeske23 implicit surface tension (surfaceTensionMethod 1) with this coefficient "
+                         "to the workload; 0 (default) leaves it off, as in the workload's scene")
     ap.add_argument("--batch-scenes", type=int, default=256,
                     help="independent scenes of the batched-form comparison; 0 skips it")
     ap.add_argument("--batch-steps", type=int, default=10)
@@ -253,6 +256,9 @@ def main():
     from implicitvelocityprojection_amd.scenes import load_particles, particle_masses, synthetic_scene
 
     scene = synthetic_scene(WORKLOADS[args.workload])
+    if args.surface_tension != 0.0:
+        scene["Configuration"]["surfaceTensionMethod"] = 1
+        scene["Configuration"]["surfaceTensionCoeff"] = args.surface_tension
     cfg = SimConfig.from_dict(scene)
     ps = load_particles(cfg)
     m, m_inv = particle_masses(cfg, ps)
@@ -270,8 +276,12 @@ def main():
     prm = IPParams(omega=cfg.get_cfg("omega", 10.0), tol_opt=cfg.get_cfg("densityErrorTol", 1e-3),
                    max_iteration_opt=cfg.get_cfg("maxOuterIters", 100), tol_pcg=cfg.get_cfg("tol_pcg", 1e-6))
 
+    st_method, st_coeff = int(cfg.surface_tension_method), float(cfg.surface_tension_coeff)
+    if st_method != 1:
+        st_coeff = 0.0     # the synthetic scenes set coefficient 0: the explicit model is a no-op
+
     def step():
-        res, vit = ctx.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, prm)
+        res, vit = ctx.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, prm, st_method, st_coeff)
         if res.status != 0:
             raise RuntimeError(f"IP solve status {res.status}")
         return res, vit
@@ -295,6 +305,7 @@ def main():
     cg_iters = 0
     solve_ms = 0.0
     visc_iters = 0
+    st_iters = 0
     outer = 0
     max_err = 0.0
     torch.cuda.synchronize()
@@ -304,6 +315,7 @@ def main():
         cg_iters += sum(res.pcg_iters)
         solve_ms += res.solve_ms
         visc_iters += vit
+        st_iters += ctx.last_st_iters
         outer += res.outer_iters
         max_err = max(max_err, res.final_density_error)
     torch.cuda.synchronize()
@@ -317,7 +329,7 @@ def main():
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
 
     precision = None
-    if args.precision_steps > 0:
+    if args.precision_steps > 0 and st_coeff == 0.0:   # the float64 restatement has no surface tension
         forms = {"f32": ctx}
         if args.gradient_cache != 2:
             # the precision form: float16 gradW rows in the IP CG operator
@@ -367,12 +379,14 @@ def main():
                    "gradient_cache": ["ids", "f32", "f16"][args.gradient_cache],
                    "dt": cfg.dt, "viscosity": (["explicit", "Weiler18"][cfg.viscosity_method] + f" {cfg.viscosity_coeff:g}"
                                                if cfg.viscosity_coeff != 0.0 else "off"),
+                   "surface_tension": f"Jeske23 {st_coeff:g}" if st_coeff != 0.0 else "off",
                    "densityErrorTol": prm.tol_opt,
                    "maxOuterIters": prm.max_iteration_opt, "neighbour_pairs": pairs_end},
         "cg_iters_per_s": cg_iters / (solve_ms * 1e-3) if solve_ms > 0 else 0.0,
         "cg_iters": cg_iters,
         "ip_outer_iters": outer,
         "weiler18_cg_iters": visc_iters,
+        "surface_tension_cg_iters": st_iters,
         "ip_solve_ms_per_step": solve_ms / args.steps,
         "max_final_density_error": max_err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
