@@ -608,6 +608,182 @@ done:
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* PRJ solver (projected Jacobi, constant density, optional divergence-free pass): prj.py     */
+/* ------------------------------------------------------------------------------------------ */
+
+/* pj_head_fused, prj.py:17-59: dt_v_i = dt_v_0_i - sum_j (m_j lam_i + [dyn j] m_i lam_j) gradW_ij
+ * / max(m_i, 1e-6).  Also the head of the divergence-free pass (prj.py:243-253:
+ * compute_J_tr_x, div_vec3_by_scalar, add_vec3 with scale -1 compute the same values). */
+static void prj_head(int n, const float *lam, const float *m, const float *dt_v_0, const int *is_dynamic,
+                     int C, const int *count, const int *ids, const float *gradW, float *dt_v)
+{
+    for (int i = 0; i < n; ++i) {
+        if (is_dynamic[i] == 0) continue;
+        float li = lam[i], mi = m[i];
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+        size_t base = (size_t)i * (size_t)C;
+        int cnt = capped(count, i, C);
+        for (int kk = 0; kk < cnt; ++kk) {
+            int j = ids[base + (size_t)kk];
+            const float *g = gradW + 3 * (base + (size_t)kk);
+            float s = is_dynamic[j] == 1 ? (m[j] * li + mi * lam[j]) : (m[j] * li);
+            acc[0] += s * g[0]; acc[1] += s * g[1]; acc[2] += s * g[2];
+        }
+        float denom = mi < 1e-6f ? 1e-6f : mi;
+        for (int a = 0; a < 3; ++a) dt_v[3 * i + a] = dt_v_0[3 * i + a] - acc[a] / denom;
+    }
+}
+
+/* One PRJ solve_PJ pass loop, prj.py:162-211.  Returns the pass count (stats_outer_iters). */
+static int prj_cd(int n, const float *m, const float *c, const float *D_inv, const float *density0,
+                  const int *is_dynamic, int C, const int *count, const int *ids, const float *gradW,
+                  const orc_prj_params *prm, const float *dt_v_0, float *dt_v, float *lam, float *Jx,
+                  orc_prj_stats *st, double *err_out)
+{
+    memcpy(dt_v, dt_v_0, sizeof(float) * 3 * (size_t)n);
+    st->n_density_errors = 0;
+    double err = 0.0;
+    int n_pass = 0;
+    for (n_pass = 0; n_pass <= prm->max_iteration_opt; ++n_pass) {
+        prj_head(n, lam, m, dt_v_0, is_dynamic, C, count, ids, gradW, dt_v);
+        /* pj_tail_fused, prj.py:62-115 (float32 atomics, here in index order) */
+        orc_J(n, dt_v, m, is_dynamic, C, count, ids, gradW, Jx);
+        float total = 0.0f, cnt = 0.0f;
+        for (int i = 0; i < n; ++i) {
+            if (is_dynamic[i] == 0) continue;
+            float r = Jx[i] + c[i];
+            lam[i] = fmaxf(lam[i] + prm->omega * r * D_inv[i], 0.0f);
+            float rho0 = density0[i] < 1e-6f ? 1e-6f : density0[i];
+            total += fmaxf(r / rho0, 0.0f);
+            cnt += 1.0f;
+        }
+        err = cnt > 0.0f ? (double)total / (double)cnt : 0.0;   /* read_avg_error, pressure_base.py:394-400 */
+        if (st->n_density_errors < st->capacity) st->density_errors[st->n_density_errors] = err;
+        st->n_density_errors++;
+        int converged = err < prm->tol_opt && !prm->use_fixed_iters;
+        if ((converged && n_pass >= 1) || n_pass >= prm->max_iteration_opt) break;
+    }
+    if (n_pass > prm->max_iteration_opt) n_pass = prm->max_iteration_opt;   /* max_iteration_opt < 0 */
+    *err_out = err;
+    return n_pass;
+}
+
+/* PRJ.solve_df, prj.py:213-290: projected Jacobi on J v <= 0 over rows with more than
+ * df_num_threshold neighbours (compute_active_mask, pressure_base.py:227-240), tol df_tol_opt. */
+static int prj_df(int n, float *v, const float *m, const float *D_inv, const float *density0,
+                  const int *is_dynamic, int C, const int *count, const int *ids, const float *gradW,
+                  float dt, const orc_prj_params *prm, float *lam_df_prev, float *dt_v_0, float *dt_v,
+                  float *lam, float *Jx, float *t, double *err_out)
+{
+    float *active = (float *)calloc((size_t)(n > 0 ? n : 1), sizeof(float));
+    for (int i = 0; i < n; ++i)
+        active[i] = (is_dynamic[i] == 1 && count[i] > prm->df_num_threshold) ? 1.0f : 0.0f;
+    for (int i = 0; i < n; ++i) {
+        if (is_dynamic[i] == 0) continue;
+        for (int a = 0; a < 3; ++a) dt_v_0[3 * i + a] = dt * v[3 * i + a];
+        lam[i] = 0.0f;
+        if (prm->warm_start) lam[i] = lam[i] + 0.5f * lam_df_prev[i];
+    }
+    memcpy(dt_v, dt_v_0, sizeof(float) * 3 * (size_t)n);
+    double err = 0.0;
+    int df_iter;
+    for (df_iter = 0; df_iter <= prm->max_iteration_opt; ++df_iter) {
+        prj_head(n, lam, m, dt_v_0, is_dynamic, C, count, ids, gradW, dt_v);
+        orc_J(n, dt_v, m, is_dynamic, C, count, ids, gradW, Jx);
+        for (int i = 0; i < n; ++i) {
+            if (is_dynamic[i] == 0) continue;
+            float dJ = Jx[i] * active[i];                    /* coef_wise_mul */
+            float Dr = dJ * D_inv[i];                        /* coef_wise_mul */
+            lam[i] = fmaxf(lam[i] + prm->omega * Dr, 0.0f);  /* add_scalar, max_zero */
+            float rho0 = density0[i] < 1e-6f ? 1e-6f : density0[i];
+            t[i] = dJ / rho0;                                /* coef_wise_div */
+        }
+        float total = 0.0f, cnt = 0.0f;
+        for (int i = 0; i < n; ++i) {
+            if (is_dynamic[i] == 0) continue;
+            total += fmaxf(t[i], 0.0f);
+            cnt += 1.0f;
+        }
+        err = cnt > 0.0f ? (double)total / (double)cnt : 0.0;
+        int converged = err < prm->df_tol_opt && !prm->use_fixed_iters;
+        if ((converged && df_iter >= 1) || df_iter >= prm->max_iteration_opt) {
+            memcpy(lam_df_prev, lam, sizeof(float) * (size_t)n);
+            break;
+        }
+    }
+    if (df_iter > prm->max_iteration_opt) df_iter = prm->max_iteration_opt;
+    float inv_dt = (float)(1.0 / (double)dt);
+    for (int i = 0; i < n; ++i) {
+        if (is_dynamic[i] == 0) continue;
+        for (int a = 0; a < 3; ++a) v[3 * i + a] = inv_dt * dt_v[3 * i + a];
+    }
+    free(active);
+    *err_out = err;
+    return df_iter;
+}
+
+/* PRJ.solve, prj.py:292-340 (+ solve_df when enabled, framework.py:483-487).  Writes v (dynamic
+ * rows), divergence, lam_prev (p_previous) and lam_df_prev.  Returns ORC_OK or
+ * ORC_NOT_CONVERGED (check_converged, pressure_base.py:487-495: v is then left unchanged). */
+int orc_prj_solve(int n, float *v, const float *m, const float *density, const float *density0,
+                  const int *is_dynamic, int C, const int *count, const int *ids, const float *gradW,
+                  const float *k, float dt, const orc_prj_params *prm, float *lam_prev,
+                  float *lam_df_prev, float *divergence, orc_prj_stats *st)
+{
+    size_t n1 = (size_t)(n > 0 ? n : 1), n3 = 3 * n1;
+    float *c = (float *)calloc(n1, sizeof(float));
+    float *t = (float *)calloc(n1, sizeof(float));
+    float *Jx = (float *)calloc(n1, sizeof(float));
+    float *lam = (float *)calloc(n1, sizeof(float));
+    float *dt_v_0 = (float *)calloc(n3, sizeof(float));
+    float *dt_v = (float *)calloc(n3, sizeof(float));
+    int status = ORC_OK;
+
+    constraint(n, density, density0, is_dynamic, c);
+    for (int i = 0; i < n; ++i) {
+        if (is_dynamic[i] == 0) continue;
+        for (int a = 0; a < 3; ++a) dt_v_0[3 * i + a] = dt * v[3 * i + a];   /* scale_vec3 */
+        lam[i] = 0.0f;
+        if (prm->warm_start) lam[i] = lam[i] + 0.5f * lam_prev[i];          /* add_scalar */
+    }
+    double err = 0.0;
+    st->outer_iters = prj_cd(n, m, c, k, density0, is_dynamic, C, count, ids, gradW, prm, dt_v_0, dt_v, lam,
+                             Jx, st, &err);
+    memcpy(lam_prev, lam, sizeof(float) * n1);
+    st->final_err_loop = err;
+    st->final_density_error = 0.0;
+    st->df_outer_iters = 0;
+    st->df_final_err = 0.0;
+    if (!prm->use_fixed_iters && err >= prm->tol_opt && st->outer_iters >= prm->max_iteration_opt) {
+        status = ORC_NOT_CONVERGED;
+        goto done;
+    }
+    float inv_dt = (float)(1.0 / (double)dt);
+    for (int i = 0; i < n; ++i) {
+        if (is_dynamic[i] == 0) continue;
+        for (int a = 0; a < 3; ++a) v[3 * i + a] = inv_dt * dt_v[3 * i + a];
+    }
+    /* final_density_error, pressure_base.py:438-447 */
+    orc_J(n, dt_v, m, is_dynamic, C, count, ids, gradW, Jx);
+    for (int i = 0; i < n; ++i)
+        if (is_dynamic[i] != 0) t[i] = Jx[i] + 1.0f * c[i];
+    st->final_density_error = avg_density_error(n, t, t, density0, is_dynamic);
+    if (prm->enable_df) {
+        double derr = 0.0;
+        st->df_outer_iters = prj_df(n, v, m, k, density0, is_dynamic, C, count, ids, gradW, dt, prm, lam_df_prev,
+                                    dt_v_0, dt_v, lam, Jx, t, &derr);
+        st->df_final_err = derr;
+    }
+    /* compute_divergence, pressure_base.py:449-457 */
+    orc_J(n, v, m, is_dynamic, C, count, ids, gradW, Jx);
+    if (divergence)
+        for (int i = 0; i < n; ++i) divergence[i] = is_dynamic[i] == 1 ? fmaxf(Jx[i], 0.0f) : 0.0f;
+done:
+    free(c); free(t); free(Jx); free(lam); free(dt_v_0); free(dt_v);
+    return status;
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* Explicit viscosity: viscosity.py:9-108                                                      */
 /* ------------------------------------------------------------------------------------------ */
 

@@ -41,6 +41,28 @@ typedef struct {
     int pcg_iter_bad;
 } orc_ip_stats;
 
+typedef struct {
+    double tol_opt;         /* densityErrorTol         (pressure_base.py:316)  */
+    int max_iteration_opt;  /* maxOuterIters           (pressure_base.py:314)  */
+    int use_fixed_iters;    /* fixedOuterIters         (pressure_base.py:315)  */
+    float omega;            /* PRJ.omega = 0.5         (prj.py:155)            */
+    int warm_start;         /* PRJ.warm_start          (prj.py:154)            */
+    int enable_df;          /* PRJ.enable_df           (prj.py:158)            */
+    int df_num_threshold;   /* PRJ.df_num_threshold    (prj.py:159)            */
+    double df_tol_opt;      /* PRJ.df_tol_opt          (prj.py:160)            */
+} orc_prj_params;
+
+typedef struct {
+    int capacity;              /* length of density_errors */
+    double *density_errors;    /* stats_density_errors, one per pass */
+    int n_density_errors;
+    int outer_iters;           /* stats_outer_iters */
+    double final_density_error;/* stats_final_density_error */
+    double final_err_loop;     /* err of the last pass */
+    int df_outer_iters;        /* stats_df_outer_iters */
+    double df_final_err;       /* err of the last divergence-free pass */
+} orc_prj_stats;
+
 float orc_cubic_kernel(float r_norm, float h);
 void orc_spiky_grad(const float r[3], float h, float out[3]);
 void orc_grid_init(const float domain_start[3], const float domain_size[3], float h, orc_grid *g);
@@ -63,6 +85,10 @@ int orc_ip_solve(int n, float *v, const float *m, const float *density, const fl
                  const int *is_dynamic, int C, const int *count, const int *ids, const float *gradW,
                  const float *k, float dt, const orc_ip_params *prm, float *divergence,
                  orc_ip_stats *st);
+int orc_prj_solve(int n, float *v, const float *m, const float *density, const float *density0,
+                  const int *is_dynamic, int C, const int *count, const int *ids, const float *gradW,
+                  const float *k, float dt, const orc_prj_params *prm, float *lam_prev,
+                  float *lam_df_prev, float *divergence, orc_prj_stats *st);
 void orc_viscosity(int n, const float *x, float *v, const float *m, const float *density,
                    const float *density0, const int *material, const int *is_dynamic, float h,
                    int C, const int *count, const int *ids, const float *gradW, float coeff,

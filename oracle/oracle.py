@@ -39,6 +39,19 @@ class OrcIPStats(ctypes.Structure):
                 ("pAp_bad", ctypes.c_double), ("pcg_iter_bad", ctypes.c_int)]
 
 
+class OrcPRJParams(ctypes.Structure):
+    _fields_ = [("tol_opt", ctypes.c_double), ("max_iteration_opt", ctypes.c_int),
+                ("use_fixed_iters", ctypes.c_int), ("omega", ctypes.c_float), ("warm_start", ctypes.c_int),
+                ("enable_df", ctypes.c_int), ("df_num_threshold", ctypes.c_int), ("df_tol_opt", ctypes.c_double)]
+
+
+class OrcPRJStats(ctypes.Structure):
+    _fields_ = [("capacity", ctypes.c_int), ("density_errors", ctypes.POINTER(ctypes.c_double)),
+                ("n_density_errors", ctypes.c_int), ("outer_iters", ctypes.c_int),
+                ("final_density_error", ctypes.c_double), ("final_err_loop", ctypes.c_double),
+                ("df_outer_iters", ctypes.c_int), ("df_final_err", ctypes.c_double)]
+
+
 def build(force=False):
     """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
     if force or not os.path.exists(_LIB_PATH):
@@ -194,6 +207,43 @@ def ip_solve(v, m, dens, density0, is_dynamic, C, count, ids, gradW, k, dt, omeg
     return v, div, res
 
 
+class PRJResult:
+    def __init__(self, status, stats, density_errors):
+        self.status = status
+        self.outer_iters = stats.outer_iters
+        self.final_density_error = stats.final_density_error
+        self.density_errors = density_errors
+        self.last_err = stats.final_err_loop
+        self.df_outer_iters = stats.df_outer_iters
+        self.df_last_err = stats.df_final_err
+
+
+def prj_solve(v, m, dens, density0, is_dynamic, C, count, ids, gradW, k, dt, tol_opt=1e-3, max_iteration_opt=1000,
+              use_fixed_iters=False, omega=0.5, warm_start=False, enable_df=False, df_num_threshold=25,
+              df_tol_opt=1e-2, lam_prev=None, lam_df_prev=None):
+    """PRJ.solve (prj.py:292-340, + solve_df when enable_df) on copies.  lam_prev / lam_df_prev
+    are the previous solve's multipliers (p_previous, dt2_lam_df_previous), used with warm_start.
+    Returns (v_new, divergence, PRJResult, lam_prev_new, lam_df_prev_new)."""
+    v = _f32(v).copy()
+    n = v.shape[0]
+    div = np.zeros(n, np.float32)
+    lp = np.zeros(n, np.float32) if lam_prev is None else _f32(lam_prev).copy()
+    ldp = np.zeros(n, np.float32) if lam_df_prev is None else _f32(lam_df_prev).copy()
+    prm = OrcPRJParams(tol_opt, max_iteration_opt, int(bool(use_fixed_iters)), omega, int(bool(warm_start)),
+                       int(bool(enable_df)), df_num_threshold, df_tol_opt)
+    cap = max_iteration_opt + 2
+    errs = np.zeros(cap, np.float64)
+    st = OrcPRJStats()
+    st.capacity = cap
+    st.density_errors = errs.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    status = lib().orc_prj_solve(ctypes.c_int(n), _p(v), _p(_f32(m)), _p(_f32(dens)), _p(_f32(density0)),
+                                 _p(_i32(is_dynamic)), ctypes.c_int(C), _p(_i32(count)), _p(_i32(ids)),
+                                 _p(_f32(gradW)), _p(_f32(k)), ctypes.c_float(dt), ctypes.byref(prm), _p(lp),
+                                 _p(ldp), _p(div), ctypes.byref(st))
+    res = PRJResult(status, st, list(errs[:min(st.n_density_errors, cap)]))
+    return v, div, res, lp, ldp
+
+
 def viscosity(x, v, m, dens, density0, material, is_dynamic, h, C, count, ids, gradW, coeff, dt,
               mode=VISC_JACOBI, material_fluid=1):
     v = _f32(v).copy()
@@ -343,7 +393,15 @@ class OracleSim:
             use_fixed_iters=bool(config.get_cfg("fixedOuterIters", False)),
             tol_pcg=float(config.get_cfg("tol_pcg", 1e-6)),
         )
+        self.pressure_method = int(config.get_cfg("pressureMethod", 1))
+        self.prj_kw = dict(
+            tol_opt=self.ip_kw["tol_opt"], max_iteration_opt=self.ip_kw["max_iteration_opt"],
+            use_fixed_iters=self.ip_kw["use_fixed_iters"],
+        )
+        self.lam_prev = np.zeros(particles.n, np.float32)
+        self.lam_df_prev = np.zeros(particles.n, np.float32)
         self.last_ip = None
+        self.last_prj = None
         self.last_visc_iters = 0
         self.last_st_iters = 0
         self.max_total = 0
@@ -380,6 +438,10 @@ class OracleSim:
                                self.dt, mode=self.visc_mode)
 
     def pressure(self):
+        if self.pressure_method == 0:
+            return self.pressure_prj()
+        if self.pressure_method != 1:
+            raise NotImplementedError(f"pressureMethod {self.pressure_method} is not restated")
         v, div, res = ip_solve(self.v, self.m, self.density, self.density0, self.is_dynamic, self.C,
                                self.count, self.ids, self.gradW, self.k, self.dt, **self.ip_kw)
         self.last_ip = res
@@ -389,6 +451,17 @@ class OracleSim:
         if res.status == ORC_NOT_CONVERGED:
             raise self.Diverged(f"[IP] does not converge! density err={res.last_err:.2e} "
                                 f"(tol={self.ip_kw['tol_opt']:.2e}) after {res.outer_iters} iters")
+        self.v, self.divergence = v, div
+
+    def pressure_prj(self):
+        """PRJ.solve (prj.py:292-340) and, with prj_kw enable_df, solve_df (framework.py:483-487)."""
+        v, div, res, self.lam_prev, self.lam_df_prev = prj_solve(
+            self.v, self.m, self.density, self.density0, self.is_dynamic, self.C, self.count, self.ids,
+            self.gradW, self.k, self.dt, lam_prev=self.lam_prev, lam_df_prev=self.lam_df_prev, **self.prj_kw)
+        self.last_prj = res
+        if res.status == ORC_NOT_CONVERGED:
+            raise self.Diverged(f"[PRJ] does not converge! density err={res.last_err:.2e} "
+                                f"(tol={self.prj_kw['tol_opt']:.2e}) after {res.outer_iters} iters")
         self.v, self.divergence = v, div
 
     def step(self):

@@ -373,6 +373,162 @@ def test_ip_converges_below_tolerance_and_leaves_static_untouched():
 
 
 # ---------------------------------------------------------------------------------------------
+# PRJ.solve / solve_df (prj.py:17-340)
+# ---------------------------------------------------------------------------------------------
+
+def _pair(rho0, v0, v1):
+    """Two dynamic fluid particles, r < h, with the oracle's neighbours, density and k."""
+    x = np.array([[0.1, 0.1, 0.1], [0.127, 0.104, 0.098]], np.float32)
+    v = np.array([v0, v1], np.float32)
+    m = np.array([0.0064, 0.0064], np.float32)
+    dyn = np.ones(2, np.int32)
+    g = O.grid_init((0.0, 0.0, 0.0), (0.3, 0.3, 0.3), H)
+    count, ids, gradW, _ = O.build_neighbors(g, x, dyn, H, 60)
+    dens = O.density(x, m, dyn, H, 60, count, ids)
+    k = O.aii(m, 1.0 / m, dyn, 60, count, ids, gradW)
+    rho0 = np.full(2, rho0, np.float32)
+    return dict(v=v, m=m, dens=dens, density0=rho0, is_dynamic=dyn, C=60, count=count, ids=ids, gradW=gradW, k=k)
+
+
+def _prj(p, dt, **kw):
+    return O.prj_solve(p["v"], p["m"], p["dens"], p["density0"], p["is_dynamic"], p["C"], p["count"], p["ids"],
+                       p["gradW"], p["k"], dt, **kw)
+
+
+def test_prj_two_particles_closed_form():
+    """Symmetric pair: J^T e_i = m g (i), -m g (j) and J u = m (u_i - u_j).g on both rows, so with
+    D = 1/(2 m |g|^2 + 1e-3) the first update lambda = omega D q (q = c + J dt v0) brings the
+    constraint to q 1e-3 D: converged after one update, v_i = v_i - 2 lambda g / dt, v_j mirrored
+    (momentum conserved)."""
+    dt = 1e-3
+    p = _pair(100.0, (0.5, 0.0, 0.0), (-0.5, 0.0, 0.0))
+    g = p["gradW"][0].astype(np.float64)
+    m = 0.0064
+    D = 1.0 / (2.0 * m * g.dot(g) + 1e-3)
+    assert p["k"][0] == pytest.approx(D, rel=1e-5) and p["k"][1] == pytest.approx(D, rel=1e-5)
+    c = float(p["dens"][0]) - 100.0
+    q = c + m * dt * (p["v"][0].astype(np.float64) - p["v"][1]).dot(g)
+    assert q > 0.0
+    v, div, res, lam, _ = _prj(p, dt)
+    assert res.status == O.ORC_OK and res.outer_iters == 1 and len(res.density_errors) == 2
+    assert res.density_errors[0] == pytest.approx(q / 100.0, rel=1e-5)
+    assert res.density_errors[1] < 1e-3 and res.final_density_error == pytest.approx(res.density_errors[1], abs=1e-6)
+    lam1 = 0.5 * D * q
+    np.testing.assert_allclose(lam, [lam1, lam1], rtol=1e-5)
+    expect0 = p["v"][0] - 2.0 * lam1 * g / dt
+    np.testing.assert_allclose(v[0], expect0, rtol=1e-5, atol=1e-5 * np.abs(expect0).max())
+    np.testing.assert_allclose(v[0] + v[1], p["v"][0] + p["v"][1], atol=1e-5 * np.abs(expect0).max())
+    assert (v[0] - v[1]).dot(g) < (p["v"][0] - p["v"][1]).dot(g)     # the approach is slowed
+
+
+def test_prj_separating_pair_is_left_alone():
+    """c + J dt v0 < 0 on both rows: lambda stays 0, two passes (one update) with zero error."""
+    p = _pair(1000.0, (0.5, 0.0, 0.0), (-0.5, 0.0, 0.0))
+    v, _, res, lam, _ = _prj(p, 1e-3)
+    assert res.status == O.ORC_OK and res.outer_iters == 1 and res.density_errors == [0.0, 0.0]
+    np.testing.assert_array_equal(lam, 0.0)
+    np.testing.assert_allclose(v, p["v"], rtol=1e-6)
+
+
+def _lcp_velocity(sim, v, dt):
+    """The constrained projection PRJ iterates towards, solved densely in float64:
+    min 1/2 |dt v' - dt v|_M^2  s.t.  c + J dt v' <= 0,  i.e. dt v' = dt v - M^-1 J^T lam with
+    lam >= 0, r = q - A lam <= 0, lam r = 0 (A = J M^-1 J^T, q = c + J dt v)."""
+    from scipy.optimize import minimize
+    J, dyn = _dense_J(sim)
+    Minv = np.repeat(sim.m_inv[dyn].astype(np.float64), 3)
+    A = J @ (Minv[:, None] * J.T)
+    d0 = (np.float32(dt) * v[dyn]).reshape(-1).astype(np.float64)
+    q = (sim.density[dyn] - sim.density0[dyn]).astype(np.float64) + J @ d0
+    res = minimize(lambda l: 0.5 * l @ A @ l - q @ l, np.zeros(len(dyn)), jac=lambda l: A @ l - q,
+                   method="L-BFGS-B", bounds=[(0.0, None)] * len(dyn),
+                   options=dict(maxiter=20000, ftol=1e-30, gtol=1e-14))
+    lam = res.x
+    d = d0 - Minv * (J.T @ lam)
+    return (d / dt).reshape(-1, 3), q - A @ lam, lam, dyn
+
+
+def test_prj_converges_to_the_constrained_projection():
+    """Projected Jacobi's fixed point is the KKT point of the density-constrained projection:
+    after many passes the velocities match the dense float64 solution and the constraint holds."""
+    sim = _compressed(tiny_sim(n=(4, 4, 4)), 0.85)
+    n = sim.x.shape[0]
+    dt = 1e-3
+    rng = np.random.default_rng(2)
+    v = rng.normal(scale=0.05, size=(n, 3)).astype(np.float32)
+    v[sim.is_dynamic == 0] = 0.0
+    v_qp, r_qp, lam_qp, dyn = _lcp_velocity(sim, v, dt)
+    assert r_qp.max() < 1e-6 * sim.density0[dyn].max()
+    assert np.count_nonzero(lam_qp > 0) > 0
+    v_new, _, res, lam, _ = O.prj_solve(v, sim.m, sim.density, sim.density0, sim.is_dynamic, sim.C, sim.count,
+                                        sim.ids, sim.gradW, sim.k, dt, tol_opt=0.0, max_iteration_opt=4000,
+                                        use_fixed_iters=True)
+    assert res.status == O.ORC_OK and res.outer_iters == 4000
+    scale = np.abs(v_qp - v[dyn]).max()
+    np.testing.assert_allclose(v_new[dyn], v_qp, atol=2e-3 * scale)
+    assert res.density_errors[-1] < 1e-6
+    assert np.all(lam >= 0.0) and np.all(lam[sim.is_dynamic == 0] == 0.0)
+
+
+def test_prj_stop_rules():
+    sim = _compressed(tiny_sim(n=(6, 6, 6)), 0.85)
+    n = sim.x.shape[0]
+    v = np.zeros((n, 3), np.float32)
+    args = (v, sim.m, sim.density, sim.density0, sim.is_dynamic, sim.C, sim.count, sim.ids, sim.gradW, sim.k, 1e-3)
+    # the loop stops at the first pass below tolerance, never before one update
+    _, _, res, _, _ = O.prj_solve(*args, tol_opt=1e-3, max_iteration_opt=1000)
+    e = res.density_errors
+    assert res.status == O.ORC_OK and len(e) == res.outer_iters + 1 and res.outer_iters >= 1
+    assert e[-1] < 1e-3 and all(x >= 1e-3 for x in e[1:-1])
+    # no update allowed, error above tolerance: check_converged raises and v is left unchanged
+    v2, _, res, _, _ = O.prj_solve(*args, tol_opt=1e-3, max_iteration_opt=0)
+    assert res.status == O.ORC_NOT_CONVERGED and res.outer_iters == 0 and len(res.density_errors) == 1
+    assert np.array_equal(v2, v)
+    # fixed iterations accept the result
+    _, _, res, _, _ = O.prj_solve(*args, tol_opt=1e-12, max_iteration_opt=5, use_fixed_iters=True)
+    assert res.status == O.ORC_OK and res.outer_iters == 5 and len(res.density_errors) == 6
+
+
+def test_prj_warm_start_starts_closer():
+    """Starting from half the converged multipliers of the same problem lowers the first pass's
+    error and does not lengthen the solve."""
+    sim = _compressed(tiny_sim(n=(6, 6, 6)), 0.85)
+    n = sim.x.shape[0]
+    v = np.zeros((n, 3), np.float32)
+    args = (v, sim.m, sim.density, sim.density0, sim.is_dynamic, sim.C, sim.count, sim.ids, sim.gradW, sim.k, 1e-3)
+    _, _, cold, lam, _ = O.prj_solve(*args, tol_opt=1e-4, max_iteration_opt=1000)
+    _, _, warm, _, _ = O.prj_solve(*args, tol_opt=1e-4, max_iteration_opt=1000, warm_start=True, lam_prev=lam)
+    assert warm.density_errors[0] < cold.density_errors[0]
+    assert warm.outer_iters <= cold.outer_iters
+
+
+def test_prj_divergence_free_pass_two_particles():
+    """Under-dense approaching pair: the constant-density loop leaves lambda = 0, and the
+    divergence-free loop removes the approach, J v = 0: v_i = v_i - 2 lambda g / dt with
+    lambda = omega D m dt (v_i - v_j).g.  Rows with no more than df_num_threshold neighbours are
+    left alone."""
+    dt = 1e-3
+    p = _pair(1000.0, (0.5, 0.0, 0.0), (-0.5, 0.0, 0.0))
+    g = p["gradW"][0].astype(np.float64)
+    m = 0.0064
+    D = float(p["k"][0])
+    qdf = m * dt * (p["v"][0].astype(np.float64) - p["v"][1]).dot(g)
+    assert qdf > 0.0
+    v, div, res, _, ldf = _prj(p, dt, enable_df=True, df_num_threshold=0)
+    assert res.status == O.ORC_OK and res.outer_iters == 1 and res.density_errors == [0.0, 0.0]
+    assert res.df_outer_iters == 1 and res.df_last_err < 1e-2
+    lam1 = 0.5 * D * qdf
+    np.testing.assert_allclose(ldf, [lam1, lam1], rtol=1e-5)
+    expect0 = p["v"][0] - 2.0 * lam1 * g / dt
+    np.testing.assert_allclose(v[0], expect0, rtol=1e-5, atol=1e-5 * np.abs(expect0).max())
+    assert abs(m * (v[0] - v[1]).dot(g)) < 1e-3 * m * abs((p["v"][0] - p["v"][1]).dot(g))
+    v, _, res, _, ldf = _prj(p, dt, enable_df=True, df_num_threshold=1)
+    assert res.df_outer_iters == 1 and res.df_last_err == 0.0
+    np.testing.assert_array_equal(ldf, 0.0)
+    np.testing.assert_allclose(v, p["v"], rtol=1e-6)
+
+
+# ---------------------------------------------------------------------------------------------
 # Step glue (framework.py:26-41, 117-136; boundary.py:27-91)
 # ---------------------------------------------------------------------------------------------
 
