@@ -31,7 +31,7 @@ MIN_SCENE_DYNAMIC = 256
 
 # kernel classes of ivp_profile_read / ivp_profile_enable masks
 KCLASSES = ["sort", "build", "ip_pack", "ip_rhs", "cg_J", "cg_JT", "cg_update", "cg_p", "visc", "w18_Av",
-            "w18_update", "w18_p", "w18_pre", "cg_refresh", "st_setup", "st_Av", "glue"]
+            "w18_update", "w18_p", "w18_pre", "cg_refresh", "st_setup", "st_Av", "prj_head", "prj_tail", "glue"]
 
 
 class Config(ctypes.Structure):
@@ -50,7 +50,13 @@ class IPStats(ctypes.Structure):
                 ("n_pcg", ctypes.c_int), ("outer_iters", ctypes.c_int), ("final_density_error", ctypes.c_double),
                 ("pAp_bad", ctypes.c_double), ("pcg_iter_bad", ctypes.c_int), ("pairs_traversed", ctypes.c_longlong),
                 ("rows_traversed", ctypes.c_longlong), ("cg_launch_chunks", ctypes.c_int),
-                ("solve_ms", ctypes.c_double)]
+                ("solve_ms", ctypes.c_double), ("df_outer_iters", ctypes.c_int), ("df_ms", ctypes.c_double)]
+
+
+class PRJParams(ctypes.Structure):
+    _fields_ = [("tol_opt", ctypes.c_double), ("max_iteration_opt", ctypes.c_int), ("use_fixed_iters", ctypes.c_int),
+                ("omega", ctypes.c_float), ("warm_start", ctypes.c_int), ("enable_df", ctypes.c_int),
+                ("df_num_threshold", ctypes.c_int), ("df_tol_opt", ctypes.c_double)]
 
 
 class CGStats(ctypes.Structure):
@@ -60,7 +66,7 @@ class CGStats(ctypes.Structure):
 class StepParams(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_float), ("gravity", ctypes.c_float * 3), ("viscosity_method", ctypes.c_int),
                 ("viscosity_coeff", ctypes.c_float), ("ip", IPParams), ("surface_tension_method", ctypes.c_int),
-                ("surface_tension_coeff", ctypes.c_float)]
+                ("surface_tension_coeff", ctypes.c_float), ("pressure_method", ctypes.c_int), ("prj", PRJParams)]
 
 
 class Scene(ctypes.Structure):
@@ -95,6 +101,8 @@ SIGNATURES = {
     "ivp_neighbors_build": (_I, [_P]),
     "ivp_neighbors_export": (_I, [_P, _I, _P, _P, _P]),
     "ivp_ip_solve": (_I, [_P, _F, ctypes.POINTER(IPParams), ctypes.POINTER(IPStats)]),
+    "ivp_prj_solve": (_I, [_P, _F, ctypes.POINTER(PRJParams), ctypes.POINTER(IPStats)]),
+    "ivp_prj_solve_df": (_I, [_P, _F, ctypes.POINTER(PRJParams), ctypes.POINTER(IPStats)]),
     "ivp_viscosity_explicit": (_I, [_P, _F, _F]),
     "ivp_viscosity_weiler18": (_I, [_P, _F, _F, ctypes.POINTER(CGStats)]),
     "ivp_apply_gravity": (_I, [_P, ctypes.POINTER(_F), _F]),

@@ -39,8 +39,27 @@ class IPParams:
                           self.max_iteration_pcg)
 
 
+class PRJParams:
+    """PRJ solver knobs (pressure_base.py:314-317, prj.py:150-161)."""
+
+    def __init__(self, tol_opt=1e-3, max_iteration_opt=1000, use_fixed_iters=False, omega=0.5, warm_start=False,
+                 enable_df=False, df_num_threshold=25, df_tol_opt=1e-2):
+        self.tol_opt = float(tol_opt)
+        self.max_iteration_opt = int(max_iteration_opt)
+        self.use_fixed_iters = bool(use_fixed_iters)
+        self.omega = float(omega)
+        self.warm_start = bool(warm_start)
+        self.enable_df = bool(enable_df)
+        self.df_num_threshold = int(df_num_threshold)
+        self.df_tol_opt = float(df_tol_opt)
+
+    def c(self):
+        return N.PRJParams(self.tol_opt, self.max_iteration_opt, int(self.use_fixed_iters), self.omega,
+                           int(self.warm_start), int(self.enable_df), self.df_num_threshold, self.df_tol_opt)
+
+
 class IPResult:
-    """Statistics of one IP solve."""
+    """Statistics of one pressure solve (IP, or PRJ: no CG iterations, df_ fields filled)."""
 
     def __init__(self, status, st, errs, its):
         self.status = status
@@ -54,6 +73,8 @@ class IPResult:
         self.rows_traversed = st.rows_traversed
         self.cg_launch_chunks = st.cg_launch_chunks
         self.solve_ms = st.solve_ms
+        self.df_outer_iters = st.df_outer_iters
+        self.df_ms = st.df_ms
 
     @property
     def last_err(self):
@@ -79,6 +100,7 @@ class Context:
                                     ctypes.byref(h)), "ivp_create")
         self.h = h
         self.max_particles = int(max_particles)
+        self.loaded = False          # a particle set has been loaded (ivp_set_particles)
 
     # -- lifecycle ---------------------------------------------------------------------------
     def close(self):
@@ -120,9 +142,11 @@ class Context:
         n = int(x.shape[0])
         N.check(self.lib.ivp_set_particles(self.h, n, _ptr(x), _ptr(v), _ptr(m), _ptr(m_inv), _ptr(density0),
                                            _ptr(is_dynamic), _ptr(material)), "ivp_set_particles")
+        self.loaded = True
 
     def reset(self):
-        """Density back to density0, k and divergence zeroed, neighbour rows dropped."""
+        """Density back to density0; k, divergence and PRJ's warm-start multipliers zeroed;
+        neighbour rows dropped."""
         N.check(self.lib.ivp_reset(self.h), "ivp_reset")
 
     def write(self, field, t):
@@ -158,6 +182,20 @@ class Context:
                          "ivp_ip_solve")
         return IPResult(status, st, errs, its)
 
+    def prj_solve(self, dt, params: PRJParams):
+        st, errs, its = self._stats(params.max_iteration_opt + 2)
+        prm = params.c()
+        status = N.check(self.lib.ivp_prj_solve(self.h, ctypes.c_float(dt), ctypes.byref(prm), ctypes.byref(st)),
+                         "ivp_prj_solve")
+        return IPResult(status, st, errs, its)
+
+    def prj_solve_df(self, dt, params: PRJParams):
+        st, errs, its = self._stats(2)
+        prm = params.c()
+        status = N.check(self.lib.ivp_prj_solve_df(self.h, ctypes.c_float(dt), ctypes.byref(prm),
+                                                   ctypes.byref(st)), "ivp_prj_solve_df")
+        return IPResult(status, st, errs, its)
+
     def viscosity_explicit(self, coeff, dt):
         N.check(self.lib.ivp_viscosity_explicit(self.h, ctypes.c_float(coeff), ctypes.c_float(dt)),
                 "ivp_viscosity_explicit")
@@ -175,16 +213,22 @@ class Context:
     def advect_boundary(self, dt):
         N.check(self.lib.ivp_advect_boundary(self.h, ctypes.c_float(dt)), "ivp_advect_boundary")
 
-    def step(self, dt, gravity, viscosity_method, viscosity_coeff, params: IPParams, surface_tension_method=0,
+    def step(self, dt, gravity, viscosity_method, viscosity_coeff, params, surface_tension_method=0,
              surface_tension_coeff=0.0):
-        """One step; returns (IP result, viscosity CG iterations).  The surface-tension CG
-        iterations of the step are left in ``last_st_iters``."""
+        """One step with the pressure solver of ``params`` (IPParams: IP, PRJParams: PRJ);
+        returns (pressure result, viscosity CG iterations).  The surface-tension CG iterations of
+        the step are left in ``last_st_iters``."""
         sp = N.StepParams()
         sp.dt = float(dt)
         sp.gravity[:] = [float(v) for v in gravity]
         sp.viscosity_method = int(viscosity_method)
         sp.viscosity_coeff = float(viscosity_coeff)
-        sp.ip = params.c()
+        if isinstance(params, PRJParams):
+            sp.pressure_method = 0
+            sp.prj = params.c()
+        else:
+            sp.pressure_method = 1
+            sp.ip = params.c()
         sp.surface_tension_method = int(surface_tension_method)
         sp.surface_tension_coeff = float(surface_tension_coeff)
         st, errs, its = self._stats(params.max_iteration_opt + 2)
@@ -223,6 +267,7 @@ class Context:
         sp.gravity[:] = [float(v) for v in gravity]
         sp.viscosity_method = 0
         sp.viscosity_coeff = 0.0
+        sp.pressure_method = 1
         sp.ip = params.c()
         out = (N.SceneStats * self.n_scenes)()
         status = N.check(self.lib.ivp_step_scenes(self.h, ctypes.byref(sp), out), "ivp_step_scenes")

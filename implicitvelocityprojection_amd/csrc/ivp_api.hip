@@ -75,6 +75,8 @@ void free_all(ivp_ctx* c) {
     dfree(c->w_pinv); dfree(c->w_p4); dfree(c->w_v4); dfree(c->w_dv); dfree(c->w_r); dfree(c->w_z);
     dfree(c->w_Ap); dfree(c->w_rhs); dfree(c->w_fac); dfree(c->v4);
     dfree(c->st_gt); dfree(c->st_div); dfree(c->st_b);
+    dfree(c->prj_v0); dfree(c->prj_lam); dfree(c->prj_prev); dfree(c->prj_df_prev); dfree(c->prj_hist);
+    c->prj_hist_cap = 0;
     dfree(c->partials); dfree(c->state); dfree(c->err_flag);
     if (c->state_host) (void)hipHostFree(c->state_host);
     if (c->err_host) (void)hipHostFree(c->err_host);
@@ -153,6 +155,30 @@ int st_buffers(ivp_ctx* c) {
     if (!c->st_gt) {
         size_t tiles = (size_t)blocks_for(std::max(c->nd, 1)) * WAVES_PER_BLOCK;
         IVP_TRY(dalloc(&c->st_gt, tiles * (size_t)c->Cs * 64u));
+    }
+    return IVP_OK;
+}
+
+// PRJ buffers.  The multipliers kept for warm starts start at zero (the reference allocates
+// p_previous and dt2_lam_df_previous with wp.zeros, prj.py:163-164, pressure_base.py:302).
+int prj_buffers(ivp_ctx* c, int passes) {
+    const size_t cap = (size_t)c->cap;
+    if (!c->prj_v0) IVP_TRY(dalloc(&c->prj_v0, 3 * cap));
+    if (!c->prj_lam) IVP_TRY(dalloc(&c->prj_lam, cap));
+    if (!c->prj_prev) {
+        IVP_TRY(dalloc(&c->prj_prev, cap));
+        IVP_HIP(hipMemsetAsync(c->prj_prev, 0, cap * sizeof(float), c->stream));
+    }
+    if (!c->prj_df_prev) {
+        IVP_TRY(dalloc(&c->prj_df_prev, cap));
+        IVP_HIP(hipMemsetAsync(c->prj_df_prev, 0, cap * sizeof(float), c->stream));
+    }
+    if (passes > c->prj_hist_cap) {
+        IVP_HIP(hipStreamSynchronize(c->stream));
+        dfree(c->prj_hist);
+        c->prj_hist_cap = 0;
+        IVP_TRY(dalloc(&c->prj_hist, (size_t)passes));
+        c->prj_hist_cap = passes;
     }
     return IVP_OK;
 }
@@ -293,7 +319,8 @@ int prof_collect(ivp_ctx* c, int cg_iters_done, int epoch) {
 
 static const char* kclass_name(int k) {
     static const char* names[KC_COUNT] = {"sort", "build", "ip_pack", "ip_rhs", "cg_J", "cg_JT", "cg_update", "cg_p",
-                                          "visc", "w18_Av", "w18_update", "w18_p", "w18_pre", "cg_refresh", "st_setup", "st_Av", "glue"};
+                                          "visc", "w18_Av", "w18_update", "w18_p", "w18_pre", "cg_refresh", "st_setup", "st_Av", "prj_head", "prj_tail",
+                                          "glue"};
     return (k >= 0 && k < KC_COUNT) ? names[k] : "?";
 }
 
@@ -416,8 +443,11 @@ int ivp_reset(ivp_ctx* c) {
         IVP_HIP(hipMemsetAsync(c->kk, 0, n * sizeof(float), c->stream));
         IVP_HIP(hipMemsetAsync(c->divg, 0, n * sizeof(float), c->stream));
         IVP_HIP(hipMemsetAsync(c->divg_b, 0, n * sizeof(float), c->stream));
-        IVP_HIP(hipStreamSynchronize(c->stream));
     }
+    const size_t cap = (size_t)c->cap;
+    if (c->prj_prev) IVP_HIP(hipMemsetAsync(c->prj_prev, 0, cap * sizeof(float), c->stream));
+    if (c->prj_df_prev) IVP_HIP(hipMemsetAsync(c->prj_df_prev, 0, cap * sizeof(float), c->stream));
+    IVP_HIP(hipStreamSynchronize(c->stream));
     c->built = false;
     return IVP_OK;
 }
@@ -516,6 +546,18 @@ int ivp_viscosity_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* stat
     return visc_weiler18(c, coeff, dt, stats);
 }
 
+int ivp_prj_solve(ivp_ctx* c, float dt, const ivp_prj_params* prm, ivp_ip_stats* stats) {
+    if (!c) return IVP_ERR_ARG;
+    if (c->S > 0) return IVP_ERR_STATE;
+    return prj_solve(c, dt, prm, stats, PRJ_CD);
+}
+
+int ivp_prj_solve_df(ivp_ctx* c, float dt, const ivp_prj_params* prm, ivp_ip_stats* stats) {
+    if (!c) return IVP_ERR_ARG;
+    if (c->S > 0) return IVP_ERR_STATE;
+    return prj_solve(c, dt, prm, stats, PRJ_DF);
+}
+
 int ivp_surface_tension_jeske23(ivp_ctx* c, float sigma, float dt, ivp_cg_stats* stats) {
     if (!c) return IVP_ERR_ARG;
     if (c->S > 0) return IVP_ERR_STATE;
@@ -536,6 +578,7 @@ int ivp_step(ivp_ctx* c, const ivp_step_params* prm, ivp_ip_stats* stats, ivp_cg
     if (!c || !prm || !stats) return IVP_ERR_ARG;
     if (!c->built || c->S > 0) return IVP_ERR_STATE;
     if (prm->surface_tension_coeff != 0.0f && prm->surface_tension_method != 1) return IVP_ERR_ARG;
+    if (prm->pressure_method != 0 && prm->pressure_method != 1) return IVP_ERR_ARG;
     int rc = glue_gravity(c, prm->gravity, prm->dt);
     if (rc != IVP_OK) return rc;
     if (visc) { visc->cg_iters = 0; visc->residual_norm2 = 0.0; }
@@ -554,7 +597,8 @@ int ivp_step(ivp_ctx* c, const ivp_step_params* prm, ivp_ip_stats* stats, ivp_cg
             rc = visc_explicit(c, prm->viscosity_coeff, prm->dt);
         if (rc != IVP_OK) return rc;
     }
-    rc = ip_solve(c, prm->dt, &prm->ip, stats);
+    // pressure solve (framework.py:481-487; PRJ's divergence-free pass runs inside prj_solve)
+    rc = prm->pressure_method == 0 ? prj_solve(c, prm->dt, &prm->prj, stats, PRJ_BOTH) : ip_solve(c, prm->dt, &prm->ip, stats);
     if (rc != IVP_OK) return rc;
     rc = glue_advect_boundary(c, prm->dt);
     if (rc != IVP_OK) return rc;
@@ -569,8 +613,9 @@ int ivp_set_scenes(ivp_ctx* c, int n_scenes, const ivp_scene* scenes) {
 int ivp_step_scenes(ivp_ctx* c, const ivp_step_params* prm, ivp_scene_stats* stats) {
     if (!c || !prm || !stats) return IVP_ERR_ARG;
     if (!c->built || c->S == 0) return IVP_ERR_STATE;
-    // viscosity and surface tension are not part of the scene step
+    // viscosity and surface tension are not part of the scene step, which solves with IP
     if (prm->viscosity_coeff != 0.0f || prm->surface_tension_coeff != 0.0f) return IVP_ERR_ARG;
+    if (prm->pressure_method != 1) return IVP_ERR_ARG;
     int rc = glue_gravity(c, prm->gravity, prm->dt);
     if (rc != IVP_OK) return rc;
     rc = ip_solve_scenes(c, prm->dt, &prm->ip);

@@ -119,6 +119,8 @@ enum KernelClass {
     KC_CG_REFRESH,
     KC_ST_SETUP,
     KC_ST_AV,
+    KC_PRJ_HEAD,
+    KC_PRJ_TAIL,
     KC_GLUE,
     KC_COUNT
 };
@@ -216,6 +218,16 @@ struct ivp_ctx {
     float* st_div = nullptr;  // velocity divergence per row
     float* st_b = nullptr;    // extra right-hand side, 3 * cap
 
+    // PRJ (allocated at first use).  prj_prev / prj_df_prev hold the last solve's multipliers
+    // of both loops in USER order, so that they follow the particles through every re-sort.
+    float* prj_v0 = nullptr;      // dt v at the start of the pass loop, 3 * cap
+    float2* prj_lam = nullptr;    // (dt^2 lambda, m) per dynamic row
+    float* prj_prev = nullptr;    // p_previous, user order
+    float* prj_df_prev = nullptr; // dt2_lam_df_previous, user order
+    double* prj_hist = nullptr;   // density error per pass
+    int prj_hist_cap = 0;
+    int prj_guess = 8;            // first launch chunk of the next pass loop
+
     // reductions
     double* partials = nullptr;   // 4 * max_blocks
     int max_blocks = 0;
@@ -292,6 +304,9 @@ int visc_explicit(ivp_ctx* c, float coeff, float dt);
 int visc_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* st);
 int st_jeske23(ivp_ctx* c, float sigma, float dt, ivp_cg_stats* st);
 int st_buffers(ivp_ctx* c);   // ivp_api.hip: allocate the Jeske23 buffers on first use
+enum PrjPart { PRJ_BOTH = 0, PRJ_CD = 1, PRJ_DF = 2 };   // solve (+ solve_df if enabled), solve, solve_df
+int prj_solve(ivp_ctx* c, float dt, const ivp_prj_params* prm, ivp_ip_stats* st, int part);
+int prj_buffers(ivp_ctx* c, int passes);   // ivp_api.hip: PRJ buffers, history for `passes`
 int glue_gravity(ivp_ctx* c, const float g[3], float dt);
 int glue_advect_boundary(ivp_ctx* c, float dt);
 

@@ -610,6 +610,186 @@ __global__ __launch_bounds__(BLOCK) void k_ip_finish(int nd, float inv_dt, const
     divg[i] = fmaxf(inv_dt * jd[i], 0.0f);
 }
 
+// ---- PRJ (prj.py) --------------------------------------------------------------------------
+//
+// Projected Jacobi on the constant-density constraint.  A pass is two gathers over the rows:
+// the head dt v = dt v0 - J^T lambda / m (the J^T form of the right-hand side pass) and the tail
+// r = c + J (dt v) with lambda = max(lambda + omega D r, 0) and the density-error partial.  The
+// tail reads only its own row's lambda, so lambda is updated in place.  The stop test runs on the
+// device after every pass; the host enqueues passes in chunks and reads the state once per chunk.
+
+struct PrjArgs {
+    Rows R;
+    const float* Ss;
+    const float* v0;      // dt v at the start of the loop, 3 per row
+    float2* lam;          // (dt^2 lambda, m)
+    float4* d4;           // dt v of the current pass, w = m
+    const float* cc;      // constraint c = rho - rho0 (constant-density loop)
+    const float* kk;      // D^-1 = k = 1 / (Aii + 1e-3)
+    const float* rho0;
+    const int* rowtot;    // neighbours of the row (dynamic and static)
+    float* jd;            // J (dt v), kept for the divergence
+    float omega;
+    int C;                // neighborCacheSize: the reference's neighbour count is capped at it
+    int df_threshold;     // divergence-free loop: rows with more neighbours than this take part
+    double* partials;
+    int stride;
+    const SolverState* st;
+};
+
+// PRJ.solve prologue (prj.py:307-324): c = rho - rho0 (constant-density loop only),
+// dt v0 = dt v, lambda = 0 or 0.5 x the previous solve's (warm start, add_scalar); prev is in
+// user order.
+template <bool DF>
+__global__ __launch_bounds__(BLOCK) void k_prj_init(int nd, float dt, const float* __restrict__ density,
+                                                    const float* __restrict__ rho0, const float* __restrict__ vel,
+                                                    const float4* __restrict__ pos, const int* __restrict__ uid,
+                                                    const float* __restrict__ prev, int warm, float* __restrict__ cc,
+                                                    float* __restrict__ v0, float2* __restrict__ lam) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nd) return;
+    if (!DF) cc[i] = density[i] - rho0[i];
+    v0[3 * i + 0] = dt * vel[3 * i + 0];
+    v0[3 * i + 1] = dt * vel[3 * i + 1];
+    v0[3 * i + 2] = dt * vel[3 * i + 2];
+    float l = 0.0f;
+    if (warm) l = l + 0.5f * prev[uid[i]];
+    lam[i] = make_float2(l, pos[i].w);
+}
+
+// pj_head_fused (prj.py:17-59); also the head of solve_df (prj.py:243-253)
+template <class FM>
+__global__ __launch_bounds__(BLOCK) void k_prj_head(PrjArgs a) {
+    if (a.st->active == 0) return;
+    constexpr int G = PASS_LANES;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int i = pass_row<G>(blk), g = pass_lane<G>();
+    if (i >= a.R.nd) return;   // the G lanes of a row leave together
+    // neighbour record: A = (x, y, z, m), B = lambda; lam = (lambda, m)
+    auto loadA = [&](int j) {
+        return FM::EVAL ? a.R.pos[j] : make_float4(0.0f, 0.0f, 0.0f, a.lam[j].y);
+    };
+    auto loadB = [&](int j) {
+        Vec<1> B;
+        B.v[0] = a.lam[j].x;
+        return B;
+    };
+    const float2 li = a.lam[i];
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    row_pairs<FM, 1, G, PASS_UNROLL>(a.R, i, a.R.pos[i], loadA, loadB,
+                                     [&](const float4& Aj, const Vec<1>& Bj, float gx, float gy, float gz) {
+                                         float s = fmaf(Aj.w, li.x, li.y * Bj.v[0]);
+                                         ax = fmaf(s, gx, ax); ay = fmaf(s, gy, ay); az = fmaf(s, gz, az);
+                                     }, g);
+    ax = group_sum<G>(ax);
+    ay = group_sum<G>(ay);
+    az = group_sum<G>(az);
+    if (g != 0) return;
+    ax += li.x * a.Ss[3 * i + 0];
+    ay += li.x * a.Ss[3 * i + 1];
+    az += li.x * a.Ss[3 * i + 2];
+    const float denom = li.y < 1e-6f ? 1e-6f : li.y;
+    a.d4[i] = make_float4(a.v0[3 * i + 0] - ax / denom, a.v0[3 * i + 1] - ay / denom, a.v0[3 * i + 2] - az / denom,
+                          li.y);
+}
+
+// pj_tail_fused (prj.py:62-115); DF: the pass of solve_df (prj.py:255-283: J v masked by the
+// active rows, D^-1 J v, lambda + omega D^-1 J v, max(., 0), error of J v / rho0)
+template <class FM, bool DF>
+__global__ __launch_bounds__(BLOCK) void k_prj_tail(PrjArgs a) {
+    if (a.st->active == 0) return;
+    constexpr int G = PASS_LANES;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int i = pass_row<G>(blk), g = pass_lane<G>();
+    const bool on = i < a.R.nd;
+    // neighbour record: A = (x, y, z, m), B = dt v
+    auto loadA = [&](int j) { return j_record<FM::EVAL>(a.R.pos, a.d4, j); };
+    auto loadB = [&](int j) { return vec3_of(a.d4, j); };
+    double err = 0.0;
+    if (on) {
+        const float4 di = a.d4[i];
+        float acc = 0.0f;
+        row_pairs<FM, 3, G, PASS_UNROLL>(a.R, i, a.R.pos[i], loadA, loadB,
+                                         [&](const float4& Aj, const Vec<3>& Bj, float gx, float gy, float gz) {
+                                             float dx = di.x - Bj.v[0], dy = di.y - Bj.v[1], dz = di.z - Bj.v[2];
+                                             acc = fmaf(Aj.w, fmaf(dz, gz, fmaf(dy, gy, dx * gx)), acc);
+                                         }, g);
+        acc = group_sum<G>(acc);
+        if (g == 0) {
+            const float jd = acc + (di.x * a.Ss[3 * i + 0] + di.y * a.Ss[3 * i + 1] + di.z * a.Ss[3 * i + 2]);
+            a.jd[i] = jd;
+            float l = a.lam[i].x, e;
+            if (!DF) {
+                const float r = jd + a.cc[i];
+                l = fmaxf(l + a.omega * r * a.kk[i], 0.0f);
+                e = r;
+            } else {
+                const int cnt = a.rowtot[i] < a.C ? a.rowtot[i] : a.C;
+                const float dJ = jd * (cnt > a.df_threshold ? 1.0f : 0.0f);
+                const float Dr = dJ * a.kk[i];
+                l = fmaxf(l + a.omega * Dr, 0.0f);
+                e = dJ;
+            }
+            a.lam[i] = make_float2(l, di.w);
+            float rho0 = a.rho0[i];
+            if (rho0 < 1e-6f) rho0 = 1e-6f;
+            err = (double)fmaxf(e / rho0, 0.0f);
+        }
+    }
+    double v[1] = {err};
+    block_partials<1>(v, a.partials, a.stride);
+}
+
+struct PrjFinArgs {
+    const double* partials;
+    int stride;
+    int nblocks;
+    SolverState* st;
+    double tol;
+    int max_pass;
+    int fixed;
+    int nd;
+    double* hist;      // density error per pass (nullptr: not recorded)
+    int hist_cap;
+};
+
+// The stop test of the pass loop (prj.py:197-206, 277-281): err = avg(max(r / rho0, 0)); stop when
+// err < tol after at least one update, or at pass max_iteration_opt.  st->iter is the pass index
+// (stats_outer_iters once the loop has stopped); rr0 / rr_last hold the first and the last err.
+__global__ __launch_bounds__(FIN_BLOCK) void k_fin_prj(PrjFinArgs a) {
+    SolverState* st = a.st;
+    if (st->active == 0) return;
+    double t[3] = {0.0, 0.0, 0.0};
+    fold_partials<1>(a.partials, a.stride, a.nblocks, t);
+    if (threadIdx.x != 0) return;
+    const int pass = st->iter;
+    const double err = t[0] / (double)a.nd;
+    if (a.hist && pass < a.hist_cap) a.hist[pass] = err;
+    if (pass == 0) st->rr0 = err;
+    st->rr_last = err;
+    const bool converged = err < a.tol && !a.fixed;
+    if ((converged && pass >= 1) || pass >= a.max_pass)
+        st->active = 0;
+    else
+        st->iter = pass + 1;
+}
+
+__global__ void k_prj_state_init(SolverState* st) {
+    st->iter = 0;
+    st->active = 1;
+    st->status = 0;
+    st->rr0 = 0.0;
+    st->rr_last = 0.0;
+}
+
+// wp.copy(p_previous, dt2_lam) (prj.py:204, 280), into user order
+__global__ __launch_bounds__(BLOCK) void k_prj_save(int nd, const float2* __restrict__ lam, const int* __restrict__ uid,
+                                                    float* __restrict__ prev) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nd) return;
+    prev[uid[i]] = lam[i].x;
+}
+
 // ---- explicit viscosity (viscosity.py:9-108), Jacobi reads ---------------------------------
 
 __global__ __launch_bounds__(BLOCK) void k_v4(int nd, const float* __restrict__ vel, const float* __restrict__ density,
@@ -1546,6 +1726,128 @@ int jeske23_body(ivp_ctx* c, float sigma, float dt, int* iters_out) {
     return launch(c, KC_GLUE, -1, 0.0, k_w18_apply, dim3(blocks_for(3 * nd)), 3 * nd, (const float*)c->w_dv, c->vel);
 }
 
+// One pass loop of PRJ (solve_PJ, prj.py:162-211; DF: solve_df's loop, prj.py:238-281) on the
+// multipliers and dt v0 set up by k_prj_init.  Passes are enqueued in chunks; each pass ends with
+// the device-side stop test, after which the remaining passes of a chunk return at once.
+// Returns the pass index the loop stopped at (stats_outer_iters) and its density error.
+template <class FM, bool DF>
+int prj_loop(ivp_ctx* c, const ivp_prj_params* prm, double tol, int* passes, double* err, int* chunks) {
+    const int nd = c->nd;
+    const dim3 np = pass_grid(nd);
+    const double pb = pair_bytes<FM>(), pairs = (double)c->state_host->total_pairs;
+    hipLaunchKernelGGL(k_prj_state_init, dim3(1), dim3(1), 0, c->stream, c->state);
+    IVP_HIP(hipGetLastError());
+    PrjArgs pa{rows_of(c), c->Ss, c->prj_v0, c->prj_lam, c->d4, c->cc, c->kk, c->rho0, c->rowtot, c->jd,
+               prm->omega, c->C, prm->df_num_threshold, c->partials, c->max_blocks, c->state};
+    PrjFinArgs fa{c->partials, c->max_blocks, (int)np.x, c->state, tol, prm->max_iteration_opt,
+                  prm->use_fixed_iters ? 1 : 0, nd, DF ? nullptr : c->prj_hist, DF ? 0 : c->prj_hist_cap};
+    // algorithmic bytes: the pair stream once, the gathered per-particle data once per row
+    const double bH = pb * pairs + (double)nd * (8 + 12 + 12 + 16);
+    const double bT = pb * pairs + (double)nd * (16 + 12 + 4 + 4 + 4 + 8 + 4 + 8 + (DF ? 4 : 0));
+    c->prof_epoch++;
+    const int epoch = c->prof_epoch;
+    const int total = prm->max_iteration_opt + 1;
+    int issued = 0;
+    int chunk = std::max(2, std::min(c->prj_guess, 128));
+    for (;;) {
+        const int todo = std::min(chunk, total - issued);
+        if (todo <= 0) break;
+        for (int q = 0; q < todo; ++q) {
+            const int tag = issued + q;
+            IVP_TRY(launch(c, KC_PRJ_HEAD, tag, bH, k_prj_head<FM>, np, pa));
+            IVP_TRY(launch(c, KC_PRJ_TAIL, tag, bT, k_prj_tail<FM, DF>, np, pa));
+            hipLaunchKernelGGL(k_fin_prj, dim3(1), dim3(FIN_BLOCK), 0, c->stream, fa);
+            IVP_HIP(hipGetLastError());
+        }
+        issued += todo;
+        IVP_TRY(read_state(c));
+        (*chunks)++;
+        const SolverState& s = *c->state_host;
+        if (s.active == 0) break;
+        // geometric extrapolation of the density error to the tolerance
+        const int done = s.iter;   // passes completed
+        double rate = done > 1 && s.rr0 > 0.0 && s.rr_last > 0.0 ? pow(s.rr_last / s.rr0, 1.0 / (done - 1)) : 0.0;
+        int est = 16;
+        if (rate > 0.0 && rate < 0.999 && s.rr_last > tol) est = (int)ceil(log(tol / s.rr_last) / log(rate)) + 1;
+        chunk = std::max(1, std::min(est, 128));
+    }
+    *passes = c->state_host->iter;
+    *err = c->state_host->rr_last;
+    prof_collect(c, *passes + 1, epoch);
+    return IVP_OK;
+}
+
+// PRJ.solve (prj.py:292-340)
+template <class FM>
+int prj_solve_body(ivp_ctx* c, float dt, const ivp_prj_params* prm, ivp_ip_stats* stats) {
+    ivp_ip_stats& st = *stats;
+    const int nd = c->nd;
+    if (nd == 0) return IVP_OK;
+    const dim3 nb(blocks_for(nd));
+    const float inv_dt = (float)(1.0 / (double)dt);
+    auto t0 = std::chrono::steady_clock::now();
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_prj_init<false>, nb, nd, dt, (const float*)c->density,
+                   (const float*)c->rho0, (const float*)c->vel, (const float4*)c->pos, (const int*)c->uid,
+                   (const float*)c->prj_prev, prm->warm_start ? 1 : 0, c->cc, c->prj_v0, c->prj_lam));
+    int passes = 0;
+    double err = 0.0;
+    IVP_TRY((prj_loop<FM, false>(c, prm, prm->tol_opt, &passes, &err, &st.cg_launch_chunks)));
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_prj_save, nb, nd, (const float2*)c->prj_lam, (const int*)c->uid,
+                   c->prj_prev));
+    const int n_err = passes + 1;
+    const int n_copy = std::min(std::min(n_err, c->prj_hist_cap), st.density_errors ? st.capacity : 0);
+    if (n_copy > 0)
+        IVP_HIP(hipMemcpyAsync(st.density_errors, c->prj_hist, sizeof(double) * (size_t)n_copy,
+                               hipMemcpyDeviceToHost, c->stream));
+    st.n_density_errors = n_err;
+    st.outer_iters = passes;
+    const long long tp = c->state_host->total_pairs;
+    st.pairs_traversed += 2ll * (passes + 1) * tp;
+    st.rows_traversed += 2ll * (passes + 1) * nd;
+    // final_density_error (pressure_base.py:438-447) evaluates avg(max((J dt v + c) / rho0, 0)) at
+    // the dt v of the last pass with the tail's per-row arithmetic: it equals that pass's err.
+    st.final_density_error = err;
+    c->prj_guess = std::max(2, passes + 1);
+    if (!prm->use_fixed_iters && err >= prm->tol_opt && passes >= prm->max_iteration_opt) {
+        IVP_HIP(hipStreamSynchronize(c->stream));
+        return IVP_NOT_CONVERGED;
+    }
+    // v = dt v / dt and the divergence max(J v, 0) from J (dt v) of the last pass
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_finish, nb, nd, inv_dt, (const float4*)c->d4, (const float*)c->jd,
+                   c->vel, c->divg));
+    IVP_HIP(hipStreamSynchronize(c->stream));
+    st.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return IVP_OK;
+}
+
+// PRJ.solve_df (prj.py:213-290) on the stored v
+template <class FM>
+int prj_df_body(ivp_ctx* c, float dt, const ivp_prj_params* prm, ivp_ip_stats* stats) {
+    ivp_ip_stats& st = *stats;
+    const int nd = c->nd;
+    if (nd == 0) return IVP_OK;
+    const dim3 nb(blocks_for(nd));
+    const float inv_dt = (float)(1.0 / (double)dt);
+    const long long tp = c->state_host->total_pairs;
+    auto t1 = std::chrono::steady_clock::now();
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_prj_init<true>, nb, nd, dt, (const float*)c->density,
+                   (const float*)c->rho0, (const float*)c->vel, (const float4*)c->pos, (const int*)c->uid,
+                   (const float*)c->prj_df_prev, prm->warm_start ? 1 : 0, c->cc, c->prj_v0, c->prj_lam));
+    int df_passes = 0;
+    double df_err = 0.0;
+    IVP_TRY((prj_loop<FM, true>(c, prm, prm->df_tol_opt, &df_passes, &df_err, &st.cg_launch_chunks)));
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_prj_save, nb, nd, (const float2*)c->prj_lam, (const int*)c->uid,
+                   c->prj_df_prev));
+    st.df_outer_iters = df_passes;
+    st.pairs_traversed += 2ll * (df_passes + 1) * tp;
+    st.rows_traversed += 2ll * (df_passes + 1) * nd;
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_finish, nb, nd, inv_dt, (const float4*)c->d4, (const float*)c->jd,
+                   c->vel, c->divg));
+    IVP_HIP(hipStreamSynchronize(c->stream));
+    st.df_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+    return IVP_OK;
+}
+
 }  // namespace
 
 namespace ivp {
@@ -1596,6 +1898,8 @@ int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats
     st.pairs_traversed = 0;
     st.rows_traversed = 0;
     st.cg_launch_chunks = 0;
+    st.df_outer_iters = 0;
+    st.df_ms = 0.0;
     auto t0 = std::chrono::steady_clock::now();
     int rc;
     switch (c->row_fmt) {
@@ -1606,6 +1910,46 @@ int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats
     if (rc >= 0) IVP_HIP(hipStreamSynchronize(c->stream));
     stats->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
+}
+
+// PRJ.solve (prj.py:292-340), followed by solve_df (prj.py:213-290) when part == PRJ_BOTH and
+// enable_df is set, or solve_df alone (part == PRJ_DF).  Rows are complete and symmetric, so the
+// head's J^T and the tail's J are the gathers of the IP passes; the operator arithmetic follows
+// the row format (FP16 rows evaluate gradW from the positions, as the IP outer-loop passes do).
+int prj_solve(ivp_ctx* c, float dt, const ivp_prj_params* prm, ivp_ip_stats* stats, int part) {
+    if (!c->built) return IVP_ERR_STATE;
+    if (!prm || !stats) return IVP_ERR_ARG;
+    if (prm->max_iteration_opt < 0 || prm->df_num_threshold < 0) return IVP_ERR_ARG;
+    ivp_ip_stats& st = *stats;
+    st.n_density_errors = 0;
+    st.n_pcg = 0;
+    st.outer_iters = 0;
+    st.final_density_error = 0.0;
+    st.pAp_bad = 0.0;
+    st.pcg_iter_bad = 0;
+    st.pairs_traversed = 0;
+    st.rows_traversed = 0;
+    st.cg_launch_chunks = 0;
+    st.solve_ms = 0.0;
+    st.df_outer_iters = 0;
+    st.df_ms = 0.0;
+    IVP_TRY(prj_buffers(c, prm->max_iteration_opt + 1));
+    // a row that overflowed the neighbour capacity at the last build (the IP solve checks the
+    // same flag in its first constraint pass)
+    IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    IVP_HIP(hipStreamSynchronize(c->stream));
+    if (*c->err_host & 4) return IVP_ERR_INTERNAL;
+    if (*c->err_host & 1) return IVP_NEIGHBOR_OVERFLOW;
+    const int fmt = c->row_fmt;
+    if (part != PRJ_DF) {
+        int rc = fmt == IVP_ROWS_F32   ? prj_solve_body<FmtF32>(c, dt, prm, stats)
+                 : fmt == IVP_ROWS_F16 ? prj_solve_body<FmtF16Eval>(c, dt, prm, stats)
+                                       : prj_solve_body<FmtIds>(c, dt, prm, stats);
+        if (rc != IVP_OK || part == PRJ_CD || !prm->enable_df) return rc;
+    }
+    return fmt == IVP_ROWS_F32   ? prj_df_body<FmtF32>(c, dt, prm, stats)
+           : fmt == IVP_ROWS_F16 ? prj_df_body<FmtF16Eval>(c, dt, prm, stats)
+                                 : prj_df_body<FmtIds>(c, dt, prm, stats);
 }
 
 int visc_explicit(ivp_ctx* c, float coeff, float dt) {

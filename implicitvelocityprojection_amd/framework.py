@@ -1,7 +1,7 @@
 """Step orchestrator, mirroring the reference's ``Framework`` (scripts/framework.py:143-544) for
-the velocity-based IP solver.
+the velocity-based solvers (IP, and PRJ with its optional divergence-free pass).
 
-``forward()`` runs one reference step -- gravity, viscosity, IP solve, advection, domain clamp,
+``forward()`` runs one reference step -- gravity, viscosity, pressure solve, advection, domain clamp,
 neighbour rebuild, density, Aii -- as a single library call (ivp_step) on the library's particle
 store, then copies x, v, density and divergence back into the SimulationData arrays (skip that
 with ``sync_arrays = False`` when only the final state is read).  ``ps.v`` is read at the start of
@@ -13,10 +13,11 @@ import time
 
 from . import _native as N
 from .neighbour_search import NeighbourSearch
-from .pressure import IP
+from .pressure import IP, PRJ
 from .viscosity import Viscosity, Weiler18
 
-_UNSUPPORTED_PRESSURE = {0: "PRJ", 3: "SISPH", 5: "PBF"}
+_UNSUPPORTED_PRESSURE = {3: "SISPH", 5: "PBF"}
+_PRESSURE_CLASSES = {0: PRJ, 1: IP}       # framework.py:168-172
 
 
 class Framework:
@@ -37,10 +38,10 @@ class Framework:
         pressure_method = int(config.get_cfg("pressureMethod", 1))
         if pressure_method in _UNSUPPORTED_PRESSURE:
             raise NotImplementedError(f"pressureMethod {pressure_method} ({_UNSUPPORTED_PRESSURE[pressure_method]}) "
-                                      "is not part of this build; use pressureMethod 1 (IP)")
-        if pressure_method != 1:
+                                      "is not part of this build; use pressureMethod 1 (IP) or 0 (PRJ)")
+        if pressure_method not in _PRESSURE_CLASSES:
             raise ValueError(f"Unknown pressureMethod {pressure_method} (valid: [0, 1, 3, 5]).")
-        self.pressure = IP(simulation_data, neighbour_search, config)
+        self.pressure = _PRESSURE_CLASSES[pressure_method](simulation_data, neighbour_search, config)
         self.viscosity = (Weiler18 if config.viscosity_method == 1 else Viscosity)(simulation_data, neighbour_search,
                                                                                    config)
         self.viscosity_coeff = config.viscosity_coeff
@@ -76,6 +77,10 @@ class Framework:
         ps.v[:n] = ps.v0[:n]
         ps.density.copy_(ps.density0)
         self.pressure.k.zero_()
+        # framework.py:265-270: the solvers' persistent multipliers (PRJ's warm-start lambdas) are
+        # zeroed with k
+        if ps.ctx is not None and ps.ctx.loaded:
+            ps.ctx.reset()
         self.time = 0.0
         ps._dirty = True
         self.initialize()

@@ -68,9 +68,26 @@ typedef struct {
     int pcg_iter_bad;            /* CG iteration at which IVP_NOT_PD was detected          */
     long long pairs_traversed;   /* neighbour pairs read by the CG operator passes         */
     long long rows_traversed;    /* rows whose neighbour loop ran in the CG operator passes */
-    int cg_launch_chunks;        /* host synchronisations spent on CG termination          */
+    int cg_launch_chunks;        /* host synchronisations spent on CG termination (IP) or
+                                    on the pass loop's stop test (PRJ)                       */
     double solve_ms;             /* host wall time of the solve, synchronisations included */
+    int df_outer_iters;          /* PRJ: stats_df_outer_iters of the divergence-free pass   */
+    double df_ms;                /* PRJ: stats_df_total_cost_ms                             */
 } ivp_ip_stats;
+/* ivp_ip_stats holds PressureBase's statistics (pressure_base.py:332-339), shared by both
+ * solvers: PRJ leaves pcg_iters empty (n_pcg = 0) and fills the df_ fields. */
+/* PRJ solver parameters (prj.py:150-161, pressure_base.py:314-317), read at every call */
+typedef struct {
+    double tol_opt;          /* densityErrorTol                                          */
+    int max_iteration_opt;   /* maxOuterIters: passes 0 .. max_iteration_opt              */
+    int use_fixed_iters;     /* fixedOuterIters                                          */
+    float omega;             /* PRJ.omega, the Jacobi relaxation (0.5)                    */
+    int warm_start;          /* PRJ.warm_start: start from 0.5 x the last solve's lambda  */
+    int enable_df;           /* PRJ.enable_df: divergence-free pass after the solve       */
+    int df_num_threshold;    /* PRJ.df_num_threshold (25): rows of the divergence-free
+                                pass have more neighbours than this                       */
+    double df_tol_opt;       /* PRJ.df_tol_opt (1e-2)                                     */
+} ivp_prj_params;
 
 /* Weiler18 solve statistics */
 typedef struct {
@@ -88,6 +105,9 @@ typedef struct {
     int surface_tension_method;   /* surfaceTensionMethod: 1 Jeske23 implicit; 0 (the explicit
                                      model) only with coefficient 0                 */
     float surface_tension_coeff;  /* surfaceTensionCoeff; 0 skips the pass          */
+    int pressure_method;     /* pressureMethod, the reference's numbering: 1 IP (ip above),
+                                0 PRJ (prj below).  A zero-filled struct selects PRJ.  */
+    ivp_prj_params prj;
 } ivp_step_params;
 
 /* Fields readable with ivp_read / writable with ivp_write */
@@ -118,7 +138,8 @@ int ivp_set_particles(ivp_ctx *ctx, int n, const float *x, const float *v, const
  * call with the grown set, up to the capacity given to ivp_create. */
 
 /* The solver part of Framework.reset (framework.py:254-270): density back to density0, k
- * (pressure.k) and divergence zeroed, neighbour rows dropped.  The caller restores x = x0 and
+ * (pressure.k), divergence and the PRJ multipliers kept for warm starts (p_previous,
+ * dt2_lam_df_previous) zeroed, neighbour rows dropped.  The caller restores x = x0 and
  * v = v0 with ivp_write and calls ivp_neighbors_build, as Framework.reset re-initialises. */
 int ivp_reset(ivp_ctx *ctx);
 
@@ -163,6 +184,17 @@ int ivp_neighbors_export(ivp_ctx *ctx, int cache_size, int *count, int *ids, flo
 /* IP.solve (ip.py:331-404): reads the stored v, density, k; writes v and divergence. */
 int ivp_ip_solve(ivp_ctx *ctx, float dt, const ivp_ip_params *prm, ivp_ip_stats *stats);
 
+/* PRJ.solve (prj.py:292-340): projected Jacobi on the constant-density constraint,
+ *   pass l:  dt v = dt v0 - J^T lambda / m;  r = c + J (dt v);  lambda = max(lambda + omega D r, 0)
+ * with D = k = 1/(Aii + 1e-3), stopping when avg(max(r / rho0, 0)) < densityErrorTol after at
+ * least one update, or after maxOuterIters.  Reads the stored v, density, k; writes v and
+ * divergence; keeps lambda (per particle) for warm starts.  IVP_NOT_CONVERGED leaves v
+ * unchanged, as check_converged raises before the velocity update.  enable_df is not read. */
+int ivp_prj_solve(ivp_ctx *ctx, float dt, const ivp_prj_params *prm, ivp_ip_stats *stats);
+/* PRJ.solve_df (prj.py:213-290): the same loop on J v <= 0 over rows with more than
+ * df_num_threshold neighbours (tolerance df_tol_opt), on the stored v; fills the df_ fields of
+ * stats.  ivp_step runs it after ivp_prj_solve's loop when enable_df is set. */
+int ivp_prj_solve_df(ivp_ctx *ctx, float dt, const ivp_prj_params *prm, ivp_ip_stats *stats);
 /* Viscosity.solve (viscosity.py:134-164) with out-of-place (Jacobi) reads. */
 int ivp_viscosity_explicit(ivp_ctx *ctx, float coeff, float dt);
 
@@ -181,9 +213,11 @@ int ivp_apply_gravity(ivp_ctx *ctx, const float gravity[3], float dt);
 /* advect_position + enforce_boundary (framework.py:117-136, boundary.py:27-91) */
 int ivp_advect_boundary(ivp_ctx *ctx, float dt);
 
-/* Framework.forward (framework.py:423-544), velocity-based branch with IP: gravity, surface
- * tension, viscosity, IP solve, advect, boundary, neighbour rebuild, density, Aii.  visc and st
- * (either may be NULL) receive the viscosity and surface-tension CG statistics. */
+/* Framework.forward (framework.py:423-544), velocity-based branch: gravity, surface tension,
+ * viscosity, the pressure solve of prm->pressure_method (IP, or PRJ with its optional
+ * divergence-free pass), advect, boundary, neighbour rebuild, density, Aii.  stats receives the
+ * pressure solver's statistics; visc and st (either may be NULL) the viscosity and
+ * surface-tension CG statistics. */
 int ivp_step(ivp_ctx *ctx, const ivp_step_params *prm, ivp_ip_stats *stats, ivp_cg_stats *visc,
              ivp_cg_stats *st);
 

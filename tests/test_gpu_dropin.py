@@ -95,10 +95,41 @@ def test_ip_solve_dropin_and_exceptions():
         ip.solve(cfg.dt)
 
 
+def test_framework_prj_matches_oracle():
+    """pressureMethod 0 selects PRJ (framework.py:168-172) with the reference's knobs; forward()
+    matches the oracle's PRJ step, and the divergence-free pass runs when enable_df is set."""
+    from implicitvelocityprojection_amd.pressure import PRJ
+    sc = synthetic_scene(0)
+    sc["Configuration"]["pressureMethod"] = 0
+    cfg, ps, ns, fw, particles = _setup(sc)
+    assert isinstance(fw.pressure, PRJ)
+    assert fw.pressure.omega == 0.5 and fw.pressure.enable_df is False and fw.pressure.warm_start is False
+    assert fw.pressure.df_num_threshold == 25 and fw.pressure.df_tol_opt == 1e-2
+    sim = O.OracleSim(cfg, particles)
+    sim.rebuild()
+    assert sim.pressure_method == 0
+    n = ps.particle_num
+    dyn = sim.is_dynamic == 1
+    for step in range(4):
+        if step == 3:
+            fw.pressure.enable_df = True
+            sim.prj_kw["enable_df"] = True
+        ps.x[:n] = torch.from_numpy(sim.x).cuda()
+        ps.v[:n] = torch.from_numpy(sim.v).cuda()
+        fw.initialize()
+        fw.forward()
+        sim.step()
+        assert rel_inf(ps.v[:n].cpu().numpy()[dyn], sim.v[dyn]) <= 1e-5
+        assert fw.pressure.stats_outer_iters == sim.last_prj.outer_iters
+        assert len(fw.pressure.stats_density_errors) == len(sim.last_prj.density_errors)
+        assert fw.pressure.stats_final_density_error <= fw.pressure.tol_opt
+    assert fw.pressure.stats_df_outer_iters == sim.last_prj.df_outer_iters >= 1
+
+
 def test_unsupported_solvers_are_explicit():
     from implicitvelocityprojection_amd.framework import Framework
     sc = synthetic_scene(0, scale=(2, 2, 2))
-    sc["Configuration"]["pressureMethod"] = 0
+    sc["Configuration"]["pressureMethod"] = 3
     with pytest.raises(NotImplementedError):
         _setup(sc)
     sc["Configuration"]["pressureMethod"] = 7
