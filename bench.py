@@ -5,6 +5,9 @@
 `--workload` also accepts "column_102k" (40 x 64 x 40 fluid column, explicit viscosity 0.01,
 dt = 1 ms) and "column_8k" (16 x 32 x 16 column, no viscosity, dt = 1 ms), the parity-test
 scenes, for profiling; the default and the bench line's workload is "dambreak_1m".
+`--pressure prj` times the steps with the PRJ solver (pressureMethod 0) instead of IP; the
+"solvers" object of the line compares IP, PRJ and PRJ with its divergence-free pass on a few
+steps from the same state either way.
 
 Workload ("dambreak_1m"): a 100^3 fluid block (r = 0.005 m, jitter +-0.01 d, seed 2) in the corner
 of a 4.0 x 1.5 x 1.0 m static two-layer particle box, implicit (Weiler 2018) viscosity 0.01,
@@ -140,6 +143,66 @@ def precision_study(ctxs, cfg, ps, m, m_inv, prm, steps, torch, dev, n, timed_st
             "timing": timing, "seconds": seconds}
 
 
+def solver_study(src, cfg, ps, m, m_inv, prm_ip, steps, torch, dev, n, gradient_cache):
+    """The reference's solver choice side by side (framework.py:168-172): from the state of
+    context `src`, `steps` steps of the workload with IP (pressureMethod 1), with PRJ
+    (pressureMethod 0, the reference's knobs: omega 0.5, cold start) and with PRJ plus its
+    divergence-free pass (enable_df), each from the same state in a fresh context, with the
+    solve's own pair passes timed by dispatch events."""
+    from implicitvelocityprojection_amd.context import Context, PRJParams, kclass_mask
+    t = lambda a, dt=torch.float32: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)
+    x = src.read(0, torch.zeros((n, 3), device=dev))
+    v = src.read(1, torch.zeros((n, 3), device=dev))
+    nd = int(np.count_nonzero(ps.is_dynamic == 1))
+    prm_prj = PRJParams(tol_opt=prm_ip.tol_opt, max_iteration_opt=prm_ip.max_iteration_opt)
+    prm_df = PRJParams(tol_opt=prm_ip.tol_opt, max_iteration_opt=prm_ip.max_iteration_opt, enable_df=True)
+    forms = {"IP": (prm_ip, ("ip_pack", "ip_rhs", "cg_J", "cg_JT")), "PRJ": (prm_prj, ("prj_head", "prj_tail")),
+             "PRJ+DF": (prm_df, ("prj_head", "prj_tail"))}
+    out = {"steps": steps, "densityErrorTol": prm_ip.tol_opt, "maxOuterIters": prm_ip.max_iteration_opt}
+    for name, (prm, kernels) in forms.items():
+        c = Context(float(np.float32(cfg.support_radius)), cfg.domain_start, cfg.domain_size, ps.n,
+                    cache_size=int(cfg.get_cfg("neighborCacheSize", 60)))
+        c.gradient_cache = gradient_cache
+        c.set_particles(x, v, t(m), t(m_inv), t(ps.density0), t(ps.is_dynamic, torch.int32),
+                        t(ps.material, torch.int32))
+        if c.neighbors_build() != 0:
+            raise RuntimeError("neighbour overflow")
+        c.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, prm)     # warm-up
+        c.write(0, x)
+        c.write(1, v)
+        if c.neighbors_build() != 0:
+            raise RuntimeError("neighbour overflow")
+        c.profile_reset()
+        c.profile_enable(kclass_mask(*kernels))
+        outer, df_outer, errs, solve_ms, status = [], [], [], 0.0, 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            res, _ = c.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, prm)
+            status = status or res.status
+            outer.append(res.outer_iters)
+            df_outer.append(res.df_outer_iters)
+            errs.append(res.final_density_error)
+            solve_ms += res.solve_ms + res.df_ms
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        prof = c.profile_read()
+        c.close()
+        rec = {"status": status, "ms_per_step": wall * 1e3 / steps, "particle_steps_per_s": nd * steps / wall,
+               "pressure_solve_ms_per_step": solve_ms / steps, "outer_iters": outer,
+               "max_final_density_error": max(errs), "kernels": {}}
+        if name == "PRJ+DF":
+            rec["df_outer_iters"] = df_outer
+        for k in kernels:
+            p = prof[k]
+            if p["launches"]:
+                us = p["total_ms"] * 1e3 / p["launches"]
+                rec["kernels"][k] = {"launches": p["launches"], "avg_launch_us": us,
+                                     "achieved_GBs": p["bytes"] / p["launches"] / (us * 1e-6) / 1e9}
+        out[name] = rec
+    return out
+
+
 def batched_study(n_scenes, steps, torch):
     """The batched form: n_scenes independent columns (synthetic_batch, scene-0 physics) stepped
     in one context, against the same scenes stepped one by one, each in its own context.
@@ -233,6 +296,11 @@ def main():
     ap.add_argument("--surface-tension", type=float, default=0.0, metavar="SIGMA",
                     help="add Jeske23 implicit surface tension (surfaceTensionMethod 1) with this coefficient "
                          "to the workload; 0 (default) leaves it off, as in the workload's scene")
+    ap.add_argument("--solver-steps", type=int, default=5,
+                    help="steps of the IP / PRJ / PRJ+divergence-free comparison from the state after the timed "
+                         "region; 0 skips it")
+    ap.add_argument("--pressure", choices=["ip", "prj"], default="ip",
+                    help="pressure solver of the timed steps: IP (the workload's, default) or PRJ (pressureMethod 0)")
     ap.add_argument("--batch-scenes", type=int, default=256,
                     help="independent scenes of the batched-form comparison; 0 skips it")
     ap.add_argument("--batch-steps", type=int, default=10)
@@ -252,7 +320,7 @@ def main():
         copy_calibration(torch, args.copy_calibration)
         return
     from implicitvelocityprojection_amd.config import SimConfig
-    from implicitvelocityprojection_amd.context import Context, IPParams, kclass_mask
+    from implicitvelocityprojection_amd.context import Context, IPParams, PRJParams, kclass_mask
     from implicitvelocityprojection_amd.scenes import load_particles, particle_masses, synthetic_scene
 
     scene = synthetic_scene(WORKLOADS[args.workload])
@@ -276,14 +344,16 @@ def main():
     prm = IPParams(omega=cfg.get_cfg("omega", 10.0), tol_opt=cfg.get_cfg("densityErrorTol", 1e-3),
                    max_iteration_opt=cfg.get_cfg("maxOuterIters", 100), tol_pcg=cfg.get_cfg("tol_pcg", 1e-6))
 
+    step_prm = prm if args.pressure == "ip" else PRJParams(tol_opt=prm.tol_opt, max_iteration_opt=prm.max_iteration_opt)
     st_method, st_coeff = int(cfg.surface_tension_method), float(cfg.surface_tension_coeff)
     if st_method != 1:
         st_coeff = 0.0     # the synthetic scenes set coefficient 0: the explicit model is a no-op
 
     def step():
-        res, vit = ctx.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, prm, st_method, st_coeff)
+        res, vit = ctx.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, step_prm, st_method,
+                            st_coeff)
         if res.status != 0:
-            raise RuntimeError(f"IP solve status {res.status}")
+            raise RuntimeError(f"{args.pressure.upper()} solve status {res.status}")
         return res, vit
 
     # warm-up, with every kernel class timed, to find the dominant kernel
@@ -329,7 +399,8 @@ def main():
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
 
     precision = None
-    if args.precision_steps > 0 and st_coeff == 0.0:   # the float64 restatement has no surface tension
+    if args.precision_steps > 0 and st_coeff == 0.0 and args.pressure == "ip":
+        # the float64 restatement has the IP solve and no surface tension
         forms = {"f32": ctx}
         if args.gradient_cache != 2:
             # the precision form: float16 gradW rows in the IP CG operator
@@ -343,6 +414,10 @@ def main():
         if "f16" in forms:
             forms["f16"].close()
 
+    solvers = None
+    if args.solver_steps > 0 and st_coeff == 0.0:
+        solvers = solver_study(ctx, cfg, ps, m, m_inv, prm, args.solver_steps, torch, dev, n, args.gradient_cache)
+
     batched = None
     if args.batch_scenes > 0:
         batched = batched_study(args.batch_scenes, args.batch_steps, torch)
@@ -351,6 +426,8 @@ def main():
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         sim = O.OracleSim(cfg, ps)
+        if args.pressure == "prj":
+            sim.pressure_method = 0
         sim.x = x0.cpu().numpy().astype(np.float32)
         sim.v = v0.cpu().numpy().astype(np.float32)
         sim.rebuild()
@@ -380,6 +457,7 @@ def main():
                    "dt": cfg.dt, "viscosity": (["explicit", "Weiler18"][cfg.viscosity_method] + f" {cfg.viscosity_coeff:g}"
                                                if cfg.viscosity_coeff != 0.0 else "off"),
                    "surface_tension": f"Jeske23 {st_coeff:g}" if st_coeff != 0.0 else "off",
+                   "pressure": {"ip": "IP", "prj": "PRJ"}[args.pressure],
                    "densityErrorTol": prm.tol_opt,
                    "maxOuterIters": prm.max_iteration_opt, "neighbour_pairs": pairs_end},
         "cg_iters_per_s": cg_iters / (solve_ms * 1e-3) if solve_ms > 0 else 0.0,
@@ -395,6 +473,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
         "precision": precision,
+        "solvers": solvers,
         "batched": batched,
     }
     print(json.dumps(out))
