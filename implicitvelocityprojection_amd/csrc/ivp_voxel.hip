@@ -1,0 +1,327 @@
+// ivp_voxel.hip -- mesh voxelisation for scene loading (scene_loader.py:547-580: the reference's
+// trimesh call mesh.voxelized(pitch, max_iter).fill().points).
+//
+// 1. Subdivision, level-synchronous: one thread per triangle of the level.  A triangle whose
+//    three edges are at most pitch / 2 (sqrt((dx^2 + dy^2) + dz^2), float64) marks the voxels of
+//    its corners, round(v / pitch) half to even; otherwise it appends its four midpoint children
+//    ((a + b) / 2) to the next level through one atomic per triangle.  Marking is a set
+//    operation, so the order in which triangles land in a level does not matter.
+// 2. Hole fill: every empty voxel 6-connected to the grid's border is outside; the rest is
+//    filled.  Outside status spreads along lines: each sweep kernel gives every line of one axis
+//    to one thread, which walks it forwards and backwards; rounds of x, y and z sweeps repeat
+//    until one changes nothing.
+// 3. Points in x-major order of the grid (z fastest): per-chunk counts, an exclusive scan of the
+//    chunk counts, and a second pass that writes (index - min) * pitch + min * pitch (float64,
+//    rounded to float32).
+//
+// The grid spans the rounded corners of the input triangles: every sample of the subdivision is
+// a convex combination of its triangle's corners, and rounding is monotonic.
+#include <climits>
+#include <math.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "ivp_internal.h"
+#include "ivp_launch.h"
+
+using namespace ivp;
+
+namespace {
+
+struct Tri {
+    double v[9];   // corners a, b, c
+};
+
+struct VoxGrid {
+    long long lo[3];
+    long long dims[3];
+    double pitch;
+};
+
+__device__ __forceinline__ double edge_len(const Tri& t, int p, int q) {
+    double dx = t.v[3 * q + 0] - t.v[3 * p + 0];
+    double dy = t.v[3 * q + 1] - t.v[3 * p + 1];
+    double dz = t.v[3 * q + 2] - t.v[3 * p + 2];
+    return sqrt((dx * dx + dy * dy) + dz * dz);
+}
+
+// flags: bit 0 a sample outside the grid (cannot happen), bit 1 next-level capacity exceeded
+__global__ __launch_bounds__(BLOCK) void k_vox_level(const Tri* __restrict__ in, long long n, double max_edge,
+                                                     VoxGrid g, uint8_t* __restrict__ grid, Tri* __restrict__ next,
+                                                     unsigned long long* __restrict__ next_count,
+                                                     unsigned long long next_cap, int* __restrict__ flags) {
+    long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= n) return;
+    const Tri tr = in[t];
+    const bool too_long = edge_len(tr, 0, 1) > max_edge || edge_len(tr, 1, 2) > max_edge ||
+                          edge_len(tr, 2, 0) > max_edge;
+    if (!too_long) {
+        for (int c = 0; c < 3; ++c) {
+            long long ix = (long long)rint(tr.v[3 * c + 0] / g.pitch) - g.lo[0];
+            long long iy = (long long)rint(tr.v[3 * c + 1] / g.pitch) - g.lo[1];
+            long long iz = (long long)rint(tr.v[3 * c + 2] / g.pitch) - g.lo[2];
+            if (ix < 0 || iy < 0 || iz < 0 || ix >= g.dims[0] || iy >= g.dims[1] || iz >= g.dims[2]) {
+                atomicOr(flags, 1);
+                continue;
+            }
+            grid[(ix * g.dims[1] + iy) * g.dims[2] + iz] = 1;
+        }
+        return;
+    }
+    unsigned long long base = atomicAdd(next_count, 4ull);
+    if (base + 4 > next_cap) {
+        atomicOr(flags, 2);
+        return;
+    }
+    double m[9];   // midpoints ab, bc, ca
+    for (int a = 0; a < 3; ++a) {
+        m[0 + a] = (tr.v[0 + a] + tr.v[3 + a]) / 2.0;
+        m[3 + a] = (tr.v[3 + a] + tr.v[6 + a]) / 2.0;
+        m[6 + a] = (tr.v[6 + a] + tr.v[0 + a]) / 2.0;
+    }
+    Tri ch[4];
+    for (int a = 0; a < 3; ++a) {
+        ch[0].v[a] = tr.v[a];     ch[0].v[3 + a] = m[a];       ch[0].v[6 + a] = m[6 + a];
+        ch[1].v[a] = m[a];        ch[1].v[3 + a] = tr.v[3 + a]; ch[1].v[6 + a] = m[3 + a];
+        ch[2].v[a] = m[6 + a];    ch[2].v[3 + a] = m[3 + a];   ch[2].v[6 + a] = tr.v[6 + a];
+        ch[3].v[a] = m[a];        ch[3].v[3 + a] = m[3 + a];   ch[3].v[6 + a] = m[6 + a];
+    }
+    for (int k = 0; k < 4; ++k) next[base + k] = ch[k];
+}
+
+// grid values: 0 empty, 1 occupied, 2 outside (empty and 6-connected to the border)
+__global__ __launch_bounds__(BLOCK) void k_vox_border(uint8_t* __restrict__ grid, VoxGrid g, long long nvox) {
+    long long f = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (f >= nvox) return;
+    long long iz = f % g.dims[2];
+    long long r = f / g.dims[2];
+    long long iy = r % g.dims[1];
+    long long ix = r / g.dims[1];
+    bool border = ix == 0 || iy == 0 || iz == 0 || ix == g.dims[0] - 1 || iy == g.dims[1] - 1 || iz == g.dims[2] - 1;
+    if (border && grid[f] == 0) grid[f] = 2;
+}
+
+// One thread per line along `axis`: spread "outside" forwards then backwards along the line.
+__global__ __launch_bounds__(BLOCK) void k_vox_sweep(uint8_t* __restrict__ grid, VoxGrid g, int axis, long long nlines,
+                                                     int* __restrict__ changed) {
+    long long l = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (l >= nlines) return;
+    const long long d0 = g.dims[0], d1 = g.dims[1], d2 = g.dims[2];
+    long long start, step, len;
+    if (axis == 2) {          // lines along z: l = ix * d1 + iy
+        start = l * d2; step = 1; len = d2;
+    } else if (axis == 1) {   // lines along y: l = ix * d2 + iz
+        start = (l / d2) * d1 * d2 + (l % d2); step = d2; len = d1;
+    } else {                  // lines along x: l = iy * d2 + iz
+        start = l; step = d1 * d2; len = d0;
+    }
+    int any = 0;
+    bool out = false;
+    for (long long s = 0; s < len; ++s) {
+        long long f = start + s * step;
+        uint8_t v = grid[f];
+        if (v == 0 && out) { grid[f] = 2; v = 2; any = 1; }
+        out = v == 2;
+    }
+    out = false;
+    for (long long s = len - 1; s >= 0; --s) {
+        long long f = start + s * step;
+        uint8_t v = grid[f];
+        if (v == 0 && out) { grid[f] = 2; v = 2; any = 1; }
+        out = v == 2;
+    }
+    if (any) atomicOr(changed, 1);
+}
+
+constexpr int VOX_PER_THREAD = 16;
+constexpr long long VOX_CHUNK = (long long)BLOCK * VOX_PER_THREAD;
+
+__device__ __forceinline__ int block_excl_scan_int(int v, int* total) {
+    __shared__ int wsum[WAVES_PER_BLOCK];
+    int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        int y = __shfl_up(x, o, WAVE);
+        if (lane >= o) x += y;
+    }
+    if (lane == WAVE - 1) wsum[w] = x;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int q = 0; q < WAVES_PER_BLOCK; ++q) {
+        if (q < w) off += wsum[q];
+        tot += wsum[q];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + x - v;
+}
+
+// Filled voxels (occupied or enclosed) per chunk of VOX_CHUNK voxels in flat x-major order
+__global__ __launch_bounds__(BLOCK) void k_vox_count(const uint8_t* __restrict__ grid, long long nvox,
+                                                     int* __restrict__ chunk_count) {
+    long long base = (long long)blockIdx.x * VOX_CHUNK + (long long)threadIdx.x * VOX_PER_THREAD;
+    int c = 0;
+    for (int k = 0; k < VOX_PER_THREAD; ++k) {
+        long long f = base + k;
+        if (f < nvox && grid[f] != 2) c++;
+    }
+    int tot;
+    (void)block_excl_scan_int(c, &tot);
+    if (threadIdx.x == 0) chunk_count[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_vox_emit(const uint8_t* __restrict__ grid, VoxGrid g, long long nvox,
+                                                    const long long* __restrict__ chunk_off, double ox, double oy,
+                                                    double oz, float* __restrict__ points) {
+    long long base = (long long)blockIdx.x * VOX_CHUNK + (long long)threadIdx.x * VOX_PER_THREAD;
+    int c = 0;
+    for (int k = 0; k < VOX_PER_THREAD; ++k) {
+        long long f = base + k;
+        if (f < nvox && grid[f] != 2) c++;
+    }
+    int tot;
+    long long o = chunk_off[blockIdx.x] + block_excl_scan_int(c, &tot);
+    for (int k = 0; k < VOX_PER_THREAD; ++k) {
+        long long f = base + k;
+        if (f >= nvox || grid[f] == 2) continue;
+        long long iz = f % g.dims[2];
+        long long r = f / g.dims[2];
+        long long iy = r % g.dims[1];
+        long long ix = r / g.dims[1];
+        points[3 * o + 0] = (float)((double)ix * g.pitch + ox);
+        points[3 * o + 1] = (float)((double)iy * g.pitch + oy);
+        points[3 * o + 2] = (float)((double)iz * g.pitch + oz);
+        o++;
+    }
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t bytes) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        return hipMalloc(&p, bytes ? bytes : 1) == hipSuccess ? IVP_OK : IVP_ERR_CAPACITY;
+    }
+};
+
+inline unsigned grid_blocks(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+}  // namespace
+
+extern "C" int ivp_voxelize(const double* vertices, int n_vertices, const int* faces, int n_faces, double pitch,
+                            int max_iter, float* points, long long capacity, long long* n_points, void* stream) {
+    if (!n_points) return IVP_ERR_ARG;
+    *n_points = 0;
+    if (n_vertices < 0 || n_faces < 0 || !(pitch > 0.0) || !isfinite(pitch) || max_iter < 0 || max_iter > 30)
+        return IVP_ERR_ARG;
+    if (n_faces == 0) return IVP_OK;
+    if (!vertices || !faces) return IVP_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    // level-0 triangles and the grid bounds (rounded corners), on the host
+    std::vector<Tri> tri((size_t)n_faces);
+    long long lo[3] = {LLONG_MAX, LLONG_MAX, LLONG_MAX}, hi[3] = {LLONG_MIN, LLONG_MIN, LLONG_MIN};
+    for (int f = 0; f < n_faces; ++f) {
+        for (int c = 0; c < 3; ++c) {
+            int vi = faces[3 * f + c];
+            if (vi < 0 || vi >= n_vertices) return IVP_ERR_ARG;
+            for (int a = 0; a < 3; ++a) {
+                double x = vertices[3 * vi + a];
+                if (!isfinite(x)) return IVP_ERR_ARG;
+                double q = rint(x / pitch);
+                if (!(fabs(q) < 1e15)) return IVP_ERR_ARG;
+                long long iq = (long long)q;
+                lo[a] = std::min(lo[a], iq);
+                hi[a] = std::max(hi[a], iq);
+                tri[f].v[3 * c + a] = x;
+            }
+        }
+    }
+    VoxGrid g;
+    g.pitch = pitch;
+    long long nvox = 1;
+    for (int a = 0; a < 3; ++a) {
+        g.lo[a] = lo[a];
+        g.dims[a] = hi[a] - lo[a] + 1;
+        if (g.dims[a] > (1ll << 20)) return IVP_ERR_CAPACITY;
+        nvox *= g.dims[a];
+        if (nvox > (1ll << 34)) return IVP_ERR_CAPACITY;
+    }
+    DevBuf d_grid, d_cur, d_next, d_cnt, d_flags;
+    IVP_TRY(d_grid.alloc((size_t)nvox));
+    IVP_TRY(d_cur.alloc(sizeof(Tri) * (size_t)n_faces));
+    IVP_TRY(d_cnt.alloc(sizeof(unsigned long long)));
+    IVP_TRY(d_flags.alloc(2 * sizeof(int)));
+    uint8_t* grid = (uint8_t*)d_grid.p;
+    int* flags = (int*)d_flags.p;
+    IVP_HIP(hipMemsetAsync(grid, 0, (size_t)nvox, st));
+    IVP_HIP(hipMemsetAsync(flags, 0, 2 * sizeof(int), st));
+    IVP_HIP(hipMemcpyAsync(d_cur.p, tri.data(), sizeof(Tri) * (size_t)n_faces, hipMemcpyHostToDevice, st));
+    const double max_edge = pitch / 2.0;
+    long long n = n_faces;
+    for (int level = 0; n > 0; ++level) {
+        // the last allowed level must finish every triangle
+        const unsigned long long cap = level < max_iter ? 4ull * (unsigned long long)n : 0ull;
+        IVP_TRY(d_next.alloc(sizeof(Tri) * (size_t)(cap ? cap : 1)));
+        IVP_HIP(hipMemsetAsync(d_cnt.p, 0, sizeof(unsigned long long), st));
+        hipLaunchKernelGGL(k_vox_level, dim3(grid_blocks(n)), dim3(BLOCK), 0, st, (const Tri*)d_cur.p, n, max_edge, g,
+                           grid, (Tri*)d_next.p, (unsigned long long*)d_cnt.p, cap, flags);
+        IVP_HIP(hipGetLastError());
+        unsigned long long nn = 0;
+        int fl = 0;
+        IVP_HIP(hipMemcpyAsync(&nn, d_cnt.p, sizeof(nn), hipMemcpyDeviceToHost, st));
+        IVP_HIP(hipMemcpyAsync(&fl, flags, sizeof(int), hipMemcpyDeviceToHost, st));
+        IVP_HIP(hipStreamSynchronize(st));
+        if (fl & 1) return IVP_ERR_INTERNAL;
+        if (nn > 0 && level >= max_iter) return IVP_ERR_ARG;   // trimesh: "max_iter exceeded"
+        if (fl & 2) return IVP_ERR_INTERNAL;
+        std::swap(d_cur.p, d_next.p);
+        n = (long long)nn;
+    }
+    // hole fill
+    hipLaunchKernelGGL(k_vox_border, dim3(grid_blocks(nvox)), dim3(BLOCK), 0, st, grid, g, nvox);
+    IVP_HIP(hipGetLastError());
+    int* changed = flags + 1;
+    for (int round = 0;; ++round) {
+        IVP_HIP(hipMemsetAsync(changed, 0, sizeof(int), st));
+        for (int axis = 0; axis < 3; ++axis) {
+            long long nl = nvox / g.dims[axis];
+            hipLaunchKernelGGL(k_vox_sweep, dim3(grid_blocks(nl)), dim3(BLOCK), 0, st, grid, g, axis, nl, changed);
+            IVP_HIP(hipGetLastError());
+        }
+        int ch = 0;
+        IVP_HIP(hipMemcpyAsync(&ch, changed, sizeof(int), hipMemcpyDeviceToHost, st));
+        IVP_HIP(hipStreamSynchronize(st));
+        if (!ch) break;
+        if (round > nvox) return IVP_ERR_INTERNAL;
+    }
+    // points
+    const long long nchunks = (nvox + VOX_CHUNK - 1) / VOX_CHUNK;
+    DevBuf d_ccount, d_coff;
+    IVP_TRY(d_ccount.alloc(sizeof(int) * (size_t)nchunks));
+    IVP_TRY(d_coff.alloc(sizeof(long long) * (size_t)nchunks));
+    hipLaunchKernelGGL(k_vox_count, dim3((unsigned)nchunks), dim3(BLOCK), 0, st, (const uint8_t*)grid, nvox,
+                       (int*)d_ccount.p);
+    IVP_HIP(hipGetLastError());
+    std::vector<int> cc((size_t)nchunks);
+    std::vector<long long> off((size_t)nchunks);
+    IVP_HIP(hipMemcpyAsync(cc.data(), d_ccount.p, sizeof(int) * (size_t)nchunks, hipMemcpyDeviceToHost, st));
+    IVP_HIP(hipStreamSynchronize(st));
+    long long total = 0;
+    for (long long k = 0; k < nchunks; ++k) {
+        off[k] = total;
+        total += cc[k];
+    }
+    *n_points = total;
+    if (total > capacity) return IVP_ERR_CAPACITY;
+    if (total == 0) return IVP_OK;
+    if (!points) return IVP_ERR_ARG;
+    IVP_HIP(hipMemcpyAsync(d_coff.p, off.data(), sizeof(long long) * (size_t)nchunks, hipMemcpyHostToDevice, st));
+    const double ox = (double)g.lo[0] * pitch, oy = (double)g.lo[1] * pitch, oz = (double)g.lo[2] * pitch;
+    hipLaunchKernelGGL(k_vox_emit, dim3((unsigned)nchunks), dim3(BLOCK), 0, st, (const uint8_t*)grid, g, nvox,
+                       (const long long*)d_coff.p, ox, oy, oz, points);
+    IVP_HIP(hipGetLastError());
+    IVP_HIP(hipStreamSynchronize(st));
+    return IVP_OK;
+}

@@ -270,6 +270,20 @@ int ivp_scene_history(ivp_ctx *ctx, int scene, int capacity, double *density_err
 #define IVP_OPT_SCENE_GRAPH 3
 #define IVP_OPT_SCENE_LANES 4
 
+/* ---- scene loading -------------------------------------------------------------------------- */
+/* SceneLoader.load_mesh (scene_loader.py:547-580): the voxel centres of the solid bounded by a
+ * placed triangle mesh, as the reference's trimesh call mesh.voxelized(pitch, max_iter).fill()
+ * .points computes them: triangles split at edge midpoints until no edge is longer than
+ * pitch / 2 (at most max_iter levels; more is IVP_ERR_ARG, trimesh's "max_iter exceeded"), every
+ * corner rounded to its voxel, round(v / pitch) half to even, and every empty voxel that is not
+ * 6-connected to the border of the occupied index box filled.  vertices (double[3 * n_vertices])
+ * and faces (int[3 * n_faces], 0-based) are HOST arrays; points is a DEVICE float[3 * capacity]
+ * that receives the centres in x-major voxel order (z fastest), (index - min) * pitch +
+ * min * pitch in float64 rounded to float32.  *n_points is the point count; when it exceeds
+ * capacity nothing is written and IVP_ERR_CAPACITY is returned.  Runs on `stream` (a
+ * hipStream_t) and synchronises it; needs no context. */
+int ivp_voxelize(const double *vertices, int n_vertices, const int *faces, int n_faces, double pitch,
+                 int max_iter, float *points, long long capacity, long long *n_points, void *stream);
 /* ---- instrumentation ---------------------------------------------------------------------- */
 /* Per-kernel-class event timing on the context's stream.  Only launches that did work are
  * counted (CG launches after convergence inside a launch chunk are excluded). */
