@@ -11,8 +11,7 @@
 //    to one thread, which walks it forwards and backwards; rounds of x, y and z sweeps repeat
 //    until one changes nothing.
 // 3. Points in x-major order of the grid (z fastest): per-chunk counts, an exclusive scan of the
-//    chunk counts, and a second pass that writes (index - min) * pitch + min * pitch (float64,
-//    rounded to float32).
+//    chunk counts, and a second pass that writes (index - min) * pitch + min * pitch in float64.
 //
 // The grid spans the rounded corners of the input triangles: every sample of the subdivision is
 // a convex combination of its triangle's corners, and rounding is monotonic.
@@ -174,7 +173,7 @@ __global__ __launch_bounds__(BLOCK) void k_vox_count(const uint8_t* __restrict__
 
 __global__ __launch_bounds__(BLOCK) void k_vox_emit(const uint8_t* __restrict__ grid, VoxGrid g, long long nvox,
                                                     const long long* __restrict__ chunk_off, double ox, double oy,
-                                                    double oz, float* __restrict__ points) {
+                                                    double oz, double* __restrict__ points) {
     long long base = (long long)blockIdx.x * VOX_CHUNK + (long long)threadIdx.x * VOX_PER_THREAD;
     int c = 0;
     for (int k = 0; k < VOX_PER_THREAD; ++k) {
@@ -190,9 +189,9 @@ __global__ __launch_bounds__(BLOCK) void k_vox_emit(const uint8_t* __restrict__ 
         long long r = f / g.dims[2];
         long long iy = r % g.dims[1];
         long long ix = r / g.dims[1];
-        points[3 * o + 0] = (float)((double)ix * g.pitch + ox);
-        points[3 * o + 1] = (float)((double)iy * g.pitch + oy);
-        points[3 * o + 2] = (float)((double)iz * g.pitch + oz);
+        points[3 * o + 0] = (double)ix * g.pitch + ox;
+        points[3 * o + 1] = (double)iy * g.pitch + oy;
+        points[3 * o + 2] = (double)iz * g.pitch + oz;
         o++;
     }
 }
@@ -211,7 +210,7 @@ inline unsigned grid_blocks(long long n) { return (unsigned)((n + BLOCK - 1) / B
 }  // namespace
 
 extern "C" int ivp_voxelize(const double* vertices, int n_vertices, const int* faces, int n_faces, double pitch,
-                            int max_iter, float* points, long long capacity, long long* n_points, void* stream) {
+                            int max_iter, double* points, long long capacity, long long* n_points, void* stream) {
     if (!n_points) return IVP_ERR_ARG;
     *n_points = 0;
     if (n_vertices < 0 || n_faces < 0 || !(pitch > 0.0) || !isfinite(pitch) || max_iter < 0 || max_iter > 30)

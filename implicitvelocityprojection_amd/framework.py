@@ -58,6 +58,7 @@ class Framework:
             raise NotImplementedError("pressureEnabled false is not supported")
         self.sync_arrays = True
         self.last_visc_iters = 0
+        self.scheduled_spawn_system = None        # set by the caller (main.py:73)
 
     def initialize(self):
         """framework.py:282-290, 323-330: neighbour rebuild, density and Aii at the current x."""
@@ -77,6 +78,8 @@ class Framework:
         ps.v[:n] = ps.v0[:n]
         ps.density.copy_(ps.density0)
         self.pressure.k.zero_()
+        if self.scheduled_spawn_system is not None:
+            self.scheduled_spawn_system.reset()
         # framework.py:265-270: the solvers' persistent multipliers (PRJ's warm-start lambdas) are
         # zeroed with k
         if ps.ctx is not None and ps.ctx.loaded:
@@ -97,10 +100,15 @@ class Framework:
         self.last_st_iters = ps.ctx.last_st_iters
         self.pressure.record(res)
         ps._cache_stale = True
-        if self.sync_arrays:
+        spawning = self.scheduled_spawn_system is not None
+        if self.sync_arrays or spawning:
             n = ps.particle_num
             ps.x_old[:n] = ps.x[:n]
             ps.sync_from_library(x=True, v=True, density=True, divergence=True)
             ps.ctx.read(N.FIELD_K, self.pressure.k[:n])
+        # bodies due at this step's start time join before the rebuild (framework.py:528-540): the
+        # library step has rebuilt for the old set, so a step that spawns rebuilds once more
+        if spawning and self.scheduled_spawn_system.step(self.time) > 0:
+            self.initialize()
         self.time += self.dt
         self.stats_step_ms = (time.perf_counter() - t0) * 1000.0

@@ -108,7 +108,7 @@ def place(mesh, obj):
 
 def voxelize(mesh, pitch, max_iter=15, device="cuda"):
     """Voxel centres of the solid bounded by `mesh` (libivp ivp_voxelize: trimesh's subdivide
-    voxeliser and hole fill, scene_loader.py:565-572), float32 (N, 3) on `device`, in the
+    voxeliser and hole fill, scene_loader.py:565-572), float64 (N, 3) on `device`, in the
     voxel grid's x-major order."""
     dev = torch.device(device)
     lib = N.load()
@@ -117,7 +117,7 @@ def voxelize(mesh, pitch, max_iter=15, device="cuda"):
     stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
     count = ctypes.c_longlong(0)
     cap = 0
-    out = torch.zeros(0, 3, dtype=torch.float32, device=dev)
+    out = torch.zeros(0, 3, dtype=torch.float64, device=dev)
     for _ in range(2):
         rc = lib.ivp_voxelize(verts.ctypes.data_as(ctypes.c_void_p), verts.shape[0],
                               faces.ctypes.data_as(ctypes.c_void_p), faces.shape[0], ctypes.c_double(pitch),
@@ -125,7 +125,7 @@ def voxelize(mesh, pitch, max_iter=15, device="cuda"):
                               ctypes.byref(count), stream)
         if rc == N.IVP_ERR_CAPACITY and count.value > cap:
             cap = count.value
-            out = torch.zeros(cap, 3, dtype=torch.float32, device=dev)
+            out = torch.zeros(cap, 3, dtype=torch.float64, device=dev)
             continue
         N.check(rc, "ivp_voxelize")
         return out[:count.value]

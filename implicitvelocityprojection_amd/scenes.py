@@ -1,5 +1,5 @@
-"""Scene construction: the procedural subset of the reference's ``SceneLoader``
-(scripts/io/scene_loader.py) plus the synthetic benchmark scenes.
+"""Scene construction: the reference's ``SceneLoader`` (scripts/io/scene_loader.py) block by
+block, plus the synthetic benchmark scenes.
 
 Supported scene JSON blocks (reference semantics):
   * ``FluidBlocks``      scene_loader.py:90-102, 319-336 and the lattice of
@@ -14,8 +14,19 @@ Extension keys (not in the reference), used by the synthetic scenes:
     coordinate gets ``numpy.random.default_rng(seed).uniform(-jitter*d, +jitter*d)`` added in
     float64 before the float32 store.
 
-Mesh-based blocks (``FluidBodies``, ``RigidBodies``, ``SolidBodies``, ``ScheduledSpawns``) need
-mesh voxelisation (trimesh in the reference) and raise ``NotImplementedError``.
+Mesh blocks (voxelised meshes, scene_loader.py:104-186, 359-400, 547-580):
+  * ``FluidBodies``      fluid particles at the voxel centres, optional ``angularVelocity``
+                         (v = v_lin + omega x (r - r_cm), r_cm the mean of the centres).
+  * ``RigidBodies``      rigid particles, dynamic only with ``"isDynamic": true`` (velocity 0
+                         otherwise).
+  * ``ScheduledSpawns``  expanded into timed entries: ``spawnNum`` waves ``interval`` apart from
+                         ``startTime``, an optional ``grid`` of copies centred on the
+                         translation (one object id per grid slot), ``waveOffset`` per wave,
+                         ``colorCycle``; material ``fluid`` or ``rigid``.
+The voxel centres come from a callable ``mesh_points(obj) -> float64 (N, 3)`` (the GPU
+voxeliser in ``SceneLoader``; the oracle's restatement in tests).  ``SolidBodies`` and
+deformable spawn materials belong to the elasticity solver, which is not part of this build,
+and raise ``NotImplementedError``.
 """
 
 from dataclasses import dataclass
@@ -101,14 +112,36 @@ def boundary_walls(domain_start, domain_size, num_layers, diam):
     return np.concatenate(out, axis=0).astype(np.float32)
 
 
-def load_particles(config: SimConfig) -> ParticleSet:
-    """Particles of a scene, in the reference's insertion order: fluid blocks, generated bodies,
-    boundary walls (scene_loader.py:319-426)."""
-    for key, getter in (("FluidBodies", config.get_fluid_bodies), ("RigidBodies", config.get_rigid_bodies),
-                        ("SolidBodies", config.get_solid_bodies),
-                        ("ScheduledSpawns", config.get_scheduled_spawns)):
-        if getter():
-            raise NotImplementedError(f"scene block {key!r} needs mesh voxelisation, which is not supported")
+DEFORMABLE_MATERIALS = ("solid", "deformable", "elastic")    # scheduled_spawn.py:11
+
+
+def _need_mesh_points(config, mesh_points):
+    if config.get_solid_bodies():
+        raise NotImplementedError("SolidBodies belong to the elasticity solver, which is not part of this build")
+    for sp in config.get_scheduled_spawns():
+        if str(sp.get("material", "fluid")).lower() in DEFORMABLE_MATERIALS:
+            raise NotImplementedError("deformable ScheduledSpawns belong to the elasticity solver, which is not "
+                                      "part of this build")
+    if mesh_points is None and (config.get_fluid_bodies() or config.get_rigid_bodies()
+                                or config.get_scheduled_spawns()):
+        raise NotImplementedError("mesh blocks need a voxeliser: load the scene through SceneLoader")
+
+
+def _spawn_object_ids(config):
+    """Object ids the ScheduledSpawns claim (scene_loader.py:158-169)."""
+    ids = []
+    for sp in config.get_scheduled_spawns():
+        n_slots = len(grid_offsets(sp))
+        base = int(sp["objectId"])
+        ids += [base + i for i in range(n_slots if n_slots > 1 else int(sp.get("spawnNum", 1)))]
+    return ids
+
+
+def load_particles(config: SimConfig, mesh_points=None) -> ParticleSet:
+    """Particles of a scene, in the reference's insertion order: fluid blocks, fluid bodies, rigid
+    bodies, generated bodies, boundary walls (scene_loader.py:319-426).  `mesh_points(obj)`
+    returns the float64 voxel centres of a mesh block."""
+    _need_mesh_points(config, mesh_points)
     diam = config.particle_diameter
     xs, vs, rho0, mat, dyn, oid = [], [], [], [], [], []
 
@@ -137,6 +170,20 @@ def load_particles(config: SimConfig) -> ParticleSet:
             pos = (pos.astype(np.float64) + rng.uniform(-amp, amp, size=pos.shape)).astype(np.float32)
         push(pos, fluid.get("velocity", [0.0, 0.0, 0.0]), float(fluid.get("density", 1000.0)),
              MATERIAL_FLUID, 1, int(fluid["objectId"]))
+    for fb in config.get_fluid_bodies():
+        object_ids.append(int(fb["objectId"]))
+        pts = np.asarray(mesh_points(fb), dtype=np.float64)
+        vel = np.array(fb.get("velocity", [0.0, 0.0, 0.0]), dtype=np.float32)
+        omega = np.array(fb.get("angularVelocity", [0.0, 0.0, 0.0]), dtype=np.float32)
+        if np.linalg.norm(omega) > 0.0:
+            vel = np.cross(omega, pts - pts.mean(axis=0)).astype(np.float32) + vel
+        push(pts, vel, float(fb.get("density", 1000.0)), MATERIAL_FLUID, 1, int(fb["objectId"]))
+    for rb in config.get_rigid_bodies():
+        object_ids.append(int(rb["objectId"]))
+        pts = np.asarray(mesh_points(rb), dtype=np.float64)
+        rdyn = int(bool(rb.get("isDynamic", False)))
+        vel = np.array(rb.get("velocity", [0.0, 0.0, 0.0]), dtype=np.float32) if rdyn else np.zeros(3, np.float32)
+        push(pts, vel, float(rb.get("density", 1000.0)), MATERIAL_RIGID, rdyn, int(rb["objectId"]))
     for gb in config.get_generated_bodies():
         object_ids.append(int(gb["objectId"]))
         if gb.get("type", "cube") != "cube":
@@ -147,6 +194,7 @@ def load_particles(config: SimConfig) -> ParticleSet:
     layers = int(config.get_cfg("boundaryLayers", 0))
     if layers > 0:
         pos = boundary_walls(config.domain_start, config.domain_size, layers, diam)
+        object_ids += _spawn_object_ids(config)
         num_objects = (max(object_ids) + 1) if object_ids else 1
         push(pos, [0.0, 0.0, 0.0], float(config.density0), MATERIAL_RIGID, 0, num_objects)
     if not xs:
@@ -155,6 +203,63 @@ def load_particles(config: SimConfig) -> ParticleSet:
                            np.zeros(0, np.int32), np.zeros(0, np.int32))
     return ParticleSet(np.concatenate(xs), np.concatenate(vs), np.concatenate(rho0),
                        np.concatenate(mat), np.concatenate(dyn), np.concatenate(oid))
+
+
+def grid_offsets(sp):
+    """Offsets of a ScheduledSpawn's grid copies, centred on its translation
+    (scene_loader.py:431-445): ``"grid": {"counts": [nx, ny, nz], "spacing": [dx, dy, dz]}``."""
+    grid = sp.get("grid")
+    if not grid:
+        return np.zeros((1, 3), dtype=np.float32)
+    counts = [max(1, int(c)) for c in grid.get("counts", [1, 1, 1])]
+    spacing = [float(s) for s in grid.get("spacing", [0.0, 0.0, 0.0])]
+    axes = [(np.arange(n) - 0.5 * (n - 1)) * s for n, s in zip(counts, spacing)]
+    gx, gy, gz = np.meshgrid(*axes, indexing="ij")
+    return np.stack([gx.ravel(), gy.ravel(), gz.ravel()], axis=1).astype(np.float32)
+
+
+@dataclass
+class SpawnEntry:
+    """One timed body of a ScheduledSpawns block (scheduled_spawn.py:14-32)."""
+    time: float
+    object_id: int
+    positions: np.ndarray      # (n, 3) float32
+    velocity: np.ndarray       # (3,) float32
+    density: float
+    material: int              # 0 rigid, 1 fluid
+    is_dynamic: int
+    color: np.ndarray
+
+
+def spawn_entries(config: SimConfig, mesh_points):
+    """ScheduledSpawns expanded as prepare_scene does (scene_loader.py:143-186): spawnNum waves,
+    grid copies, waveOffset shifts (added in float64 to the float64 voxel centres), colours
+    cycling per wave and slot."""
+    _need_mesh_points(config, mesh_points)
+    out = []
+    for sp in config.get_scheduled_spawns():
+        material = str(sp.get("material", "fluid")).lower()
+        vox = np.asarray(mesh_points(sp), dtype=np.float64)
+        num = int(sp.get("spawnNum", 1))
+        start = float(sp.get("startTime", sp.get("time", 0.0)))
+        interval = float(sp.get("interval", 1.0))
+        base = int(sp["objectId"])
+        slots = [vox + off for off in grid_offsets(sp)]
+        has_grid = len(slots) > 1
+        colors = sp.get("colorCycle") or [sp.get("color", [10, 100, 200])]
+        wave = np.array(sp.get("waveOffset", [0.0, 0.0, 0.0]), dtype=np.float32)
+        for k in range(num):
+            shift = k * wave
+            for g, pos in enumerate(slots):
+                pts = (pos + shift) if k else pos
+                out.append(SpawnEntry(time=start + k * interval, object_id=base + (g if has_grid else k),
+                                      positions=pts.astype(np.float32),
+                                      velocity=np.array(sp.get("velocity", [0.0, 0.0, 0.0]), dtype=np.float32),
+                                      density=float(sp.get("density", 1000.0)),
+                                      material=0 if material == "rigid" else 1,
+                                      is_dynamic=1 if bool(sp.get("isDynamic", True)) else 0,
+                                      color=np.array(colors[(k + g) % len(colors)], dtype=np.int32)))
+    return out
 
 
 # ----------------------------------------------------------------------------------------------

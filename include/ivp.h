@@ -277,13 +277,13 @@ int ivp_scene_history(ivp_ctx *ctx, int scene, int capacity, double *density_err
  * pitch / 2 (at most max_iter levels; more is IVP_ERR_ARG, trimesh's "max_iter exceeded"), every
  * corner rounded to its voxel, round(v / pitch) half to even, and every empty voxel that is not
  * 6-connected to the border of the occupied index box filled.  vertices (double[3 * n_vertices])
- * and faces (int[3 * n_faces], 0-based) are HOST arrays; points is a DEVICE float[3 * capacity]
+ * and faces (int[3 * n_faces], 0-based) are HOST arrays; points is a DEVICE double[3 * capacity]
  * that receives the centres in x-major voxel order (z fastest), (index - min) * pitch +
- * min * pitch in float64 rounded to float32.  *n_points is the point count; when it exceeds
+ * min * pitch in float64 (the reference rounds them to float32 when it adds the particles).  *n_points is the point count; when it exceeds
  * capacity nothing is written and IVP_ERR_CAPACITY is returned.  Runs on `stream` (a
  * hipStream_t) and synchronises it; needs no context. */
 int ivp_voxelize(const double *vertices, int n_vertices, const int *faces, int n_faces, double pitch,
-                 int max_iter, float *points, long long capacity, long long *n_points, void *stream);
+                 int max_iter, double *points, long long capacity, long long *n_points, void *stream);
 /* ---- instrumentation ---------------------------------------------------------------------- */
 /* Per-kernel-class event timing on the context's stream.  Only launches that did work are
  * counted (CG launches after convergence inside a launch chunk are excluded). */

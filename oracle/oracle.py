@@ -400,6 +400,9 @@ class OracleSim:
         )
         self.lam_prev = np.zeros(particles.n, np.float32)
         self.lam_df_prev = np.zeros(particles.n, np.float32)
+        self.time = 0.0
+        self.spawns = []
+        self.particle_max_num = particles.n
         self.last_ip = None
         self.last_prj = None
         self.last_visc_iters = 0
@@ -464,6 +467,44 @@ class OracleSim:
                                 f"(tol={self.prj_kw['tol_opt']:.2e}) after {res.outer_iters} iters")
         self.v, self.divergence = v, div
 
+    def add_spawns(self, entries, particle_max_num):
+        """Scheduled bodies (scenes.SpawnEntry list) and the particle capacity
+        (scheduled_spawn.py:68-117)."""
+        self.spawns = [[e, False] for e in sorted(entries, key=lambda e: e.time)]
+        self.particle_max_num = int(particle_max_num)
+
+    def _fire_spawns(self):
+        """ScheduledSpawnSystem.step(time): append every due body (cut to the capacity left),
+        masses by add_particle's convention (simulation_data.py:40-57)."""
+        for rec in self.spawns:
+            e, fired = rec
+            if fired or self.time < e.time:
+                continue
+            rec[1] = True
+            remain = self.particle_max_num - self.x.shape[0]
+            if remain <= 0:
+                continue
+            n = min(e.positions.shape[0], remain)
+            rho = np.full(n, e.density, np.float32)
+            m = (np.float32(0.8 * self.cfg.particle_diameter ** 3) * rho).astype(np.float32)
+            m_inv = np.zeros(n, np.float32)
+            if e.is_dynamic == 1:
+                m_inv[:] = np.float32(1.0) / m
+            else:
+                m = m * np.float32(10.0)
+            cat = lambda a, b: np.concatenate([a, b])
+            self.x = cat(self.x, e.positions[:n].astype(np.float32))
+            self.v = cat(self.v, np.tile(e.velocity[None, :].astype(np.float32), (n, 1)))
+            self.m, self.m_inv = cat(self.m, m), cat(self.m_inv, m_inv)
+            self.density0 = cat(self.density0, rho)
+            self.density = cat(self.density, rho)
+            self.is_dynamic = cat(self.is_dynamic, np.full(n, e.is_dynamic, np.int32))
+            self.material = cat(self.material, np.full(n, e.material, np.int32))
+            self.k = cat(self.k, np.zeros(n, np.float32))
+            self.divergence = cat(self.divergence, np.zeros(n, np.float32))
+            self.lam_prev = cat(self.lam_prev, np.zeros(n, np.float32))
+            self.lam_df_prev = cat(self.lam_df_prev, np.zeros(n, np.float32))
+
     def step(self):
         """Framework.forward (framework.py:423-544), velocity-based branch, fixed dt."""
         self.non_pressure()
@@ -471,4 +512,7 @@ class OracleSim:
         self.x, self.v = advect(self.x, self.v, self.is_dynamic, self.dt)
         self.x, self.v = enforce_boundary(self.x, self.v, self.is_dynamic,
                                           np.array(self.cfg.domain_size, np.float32), self.h)
+        if self.spawns:
+            self._fire_spawns()
         self.rebuild()
+        self.time += self.dt

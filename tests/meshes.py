@@ -87,3 +87,10 @@ def rotate(v, angle, axis, point):
     from implicitvelocityprojection_amd.mesh import rotation_matrix
     M = rotation_matrix(angle, axis, point)
     return (np.column_stack((v, np.ones(len(v)))) @ M.T)[:, :3]
+
+
+def to_obj(v, f):
+    """Wavefront OBJ text of a mesh (1-based indices)."""
+    lines = [f"v {float(x)!r} {float(y)!r} {float(z)!r}" for x, y, z in v]
+    lines += [f"f {a + 1} {b + 1} {c + 1}" for a, b, c in f]
+    return "\n".join(lines) + "\n"
