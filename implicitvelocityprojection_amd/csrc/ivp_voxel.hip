@@ -3,9 +3,11 @@
 //
 // 1. Subdivision, level-synchronous: one thread per triangle of the level.  A triangle whose
 //    three edges are at most pitch / 2 (sqrt((dx^2 + dy^2) + dz^2), float64) marks the voxels of
-//    its corners, round(v / pitch) half to even; otherwise it appends its four midpoint children
-//    ((a + b) / 2) to the next level through one atomic per triangle.  Marking is a set
-//    operation, so the order in which triangles land in a level does not matter.
+//    its corners, round(v / pitch) half to even; otherwise it forms its four midpoint children
+//    ((a + b) / 2), marks the corners of those that already meet the limit (the test the next
+//    level would apply, on the same coordinates) and appends only the others to the next level,
+//    through one atomic per triangle.  The deepest level is thus never written out.  Marking is a
+//    set operation, so the order in which triangles land in a level does not matter.
 // 2. Hole fill: every empty voxel 6-connected to the grid's border is outside; the rest is
 //    filled.  Outside status spreads along lines: each sweep kernel gives every line of one axis
 //    to one thread, which walks it forwards and backwards; rounds of x, y and z sweeps repeat
@@ -45,32 +47,42 @@ __device__ __forceinline__ double edge_len(const Tri& t, int p, int q) {
     return sqrt((dx * dx + dy * dy) + dz * dz);
 }
 
-// flags: bit 0 a sample outside the grid (cannot happen), bit 1 next-level capacity exceeded
+__device__ __forceinline__ bool is_long(const Tri& t, double max_edge) {
+    return edge_len(t, 0, 1) > max_edge || edge_len(t, 1, 2) > max_edge || edge_len(t, 2, 0) > max_edge;
+}
+
+__device__ __forceinline__ void mark_corners(const Tri& t, const VoxGrid& g, uint8_t* __restrict__ grid,
+                                             int* __restrict__ flags) {
+    for (int c = 0; c < 3; ++c) {
+        long long ix = (long long)rint(t.v[3 * c + 0] / g.pitch) - g.lo[0];
+        long long iy = (long long)rint(t.v[3 * c + 1] / g.pitch) - g.lo[1];
+        long long iz = (long long)rint(t.v[3 * c + 2] / g.pitch) - g.lo[2];
+        if (ix < 0 || iy < 0 || iz < 0 || ix >= g.dims[0] || iy >= g.dims[1] || iz >= g.dims[2]) {
+            atomicOr(flags, 1);
+            continue;
+        }
+        grid[(ix * g.dims[1] + iy) * g.dims[2] + iz] = 1;
+    }
+}
+
+// flags: bit 0 a sample outside the grid (cannot happen), bit 1 next-level capacity exceeded.
+// mode: LV_SPLIT stores the children that need more levels; LV_LAST_CHILDREN (the children are
+// on the last level allowed) and LV_NO_SPLIT (this level is the last allowed) only count the
+// triangles that would need one more level, which the caller reports as "max_iter exceeded".
+enum { LV_SPLIT = 0, LV_LAST_CHILDREN = 1, LV_NO_SPLIT = 2 };
 __global__ __launch_bounds__(BLOCK) void k_vox_level(const Tri* __restrict__ in, long long n, double max_edge,
                                                      VoxGrid g, uint8_t* __restrict__ grid, Tri* __restrict__ next,
                                                      unsigned long long* __restrict__ next_count,
-                                                     unsigned long long next_cap, int* __restrict__ flags) {
+                                                     unsigned long long next_cap, int mode, int* __restrict__ flags) {
     long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= n) return;
     const Tri tr = in[t];
-    const bool too_long = edge_len(tr, 0, 1) > max_edge || edge_len(tr, 1, 2) > max_edge ||
-                          edge_len(tr, 2, 0) > max_edge;
-    if (!too_long) {
-        for (int c = 0; c < 3; ++c) {
-            long long ix = (long long)rint(tr.v[3 * c + 0] / g.pitch) - g.lo[0];
-            long long iy = (long long)rint(tr.v[3 * c + 1] / g.pitch) - g.lo[1];
-            long long iz = (long long)rint(tr.v[3 * c + 2] / g.pitch) - g.lo[2];
-            if (ix < 0 || iy < 0 || iz < 0 || ix >= g.dims[0] || iy >= g.dims[1] || iz >= g.dims[2]) {
-                atomicOr(flags, 1);
-                continue;
-            }
-            grid[(ix * g.dims[1] + iy) * g.dims[2] + iz] = 1;
-        }
+    if (!is_long(tr, max_edge)) {
+        mark_corners(tr, g, grid, flags);
         return;
     }
-    unsigned long long base = atomicAdd(next_count, 4ull);
-    if (base + 4 > next_cap) {
-        atomicOr(flags, 2);
+    if (mode == LV_NO_SPLIT) {
+        atomicAdd(next_count, 1ull);
         return;
     }
     double m[9];   // midpoints ab, bc, ca
@@ -86,7 +98,22 @@ __global__ __launch_bounds__(BLOCK) void k_vox_level(const Tri* __restrict__ in,
         ch[2].v[a] = m[6 + a];    ch[2].v[3 + a] = m[3 + a];   ch[2].v[6 + a] = tr.v[6 + a];
         ch[3].v[a] = m[a];        ch[3].v[3 + a] = m[3 + a];   ch[3].v[6 + a] = m[6 + a];
     }
-    for (int k = 0; k < 4; ++k) next[base + k] = ch[k];
+    bool keep[4];
+    int nk = 0;
+    for (int k = 0; k < 4; ++k) {
+        keep[k] = is_long(ch[k], max_edge);
+        if (!keep[k]) mark_corners(ch[k], g, grid, flags);
+        nk += keep[k] ? 1 : 0;
+    }
+    if (nk == 0) return;
+    unsigned long long base = atomicAdd(next_count, (unsigned long long)nk);
+    if (mode != LV_SPLIT) return;
+    if (base + nk > next_cap) {
+        atomicOr(flags, 2);
+        return;
+    }
+    for (int k = 0; k < 4; ++k)
+        if (keep[k]) next[base++] = ch[k];
 }
 
 // grid values: 0 empty, 1 occupied, 2 outside (empty and 6-connected to the border)
@@ -260,12 +287,14 @@ extern "C" int ivp_voxelize(const double* vertices, int n_vertices, const int* f
     const double max_edge = pitch / 2.0;
     long long n = n_faces;
     for (int level = 0; n > 0; ++level) {
-        // the last allowed level must finish every triangle
-        const unsigned long long cap = level < max_iter ? 4ull * (unsigned long long)n : 0ull;
+        // levels 0 .. max_iter exist; triangles of level max_iter must meet the limit.  Children
+        // are tested one level ahead, so the loop stores levels below max_iter only.
+        const int mode = level == max_iter ? LV_NO_SPLIT : level + 1 == max_iter ? LV_LAST_CHILDREN : LV_SPLIT;
+        const unsigned long long cap = mode == LV_SPLIT ? 4ull * (unsigned long long)n : 0ull;
         IVP_TRY(d_next.alloc(sizeof(Tri) * (size_t)(cap ? cap : 1)));
         IVP_HIP(hipMemsetAsync(d_cnt.p, 0, sizeof(unsigned long long), st));
         hipLaunchKernelGGL(k_vox_level, dim3(grid_blocks(n)), dim3(BLOCK), 0, st, (const Tri*)d_cur.p, n, max_edge, g,
-                           grid, (Tri*)d_next.p, (unsigned long long*)d_cnt.p, cap, flags);
+                           grid, (Tri*)d_next.p, (unsigned long long*)d_cnt.p, cap, mode, flags);
         IVP_HIP(hipGetLastError());
         unsigned long long nn = 0;
         int fl = 0;
@@ -273,7 +302,7 @@ extern "C" int ivp_voxelize(const double* vertices, int n_vertices, const int* f
         IVP_HIP(hipMemcpyAsync(&fl, flags, sizeof(int), hipMemcpyDeviceToHost, st));
         IVP_HIP(hipStreamSynchronize(st));
         if (fl & 1) return IVP_ERR_INTERNAL;
-        if (nn > 0 && level >= max_iter) return IVP_ERR_ARG;   // trimesh: "max_iter exceeded"
+        if (nn > 0 && mode != LV_SPLIT) return IVP_ERR_ARG;   // trimesh: "max_iter exceeded"
         if (fl & 2) return IVP_ERR_INTERNAL;
         std::swap(d_cur.p, d_next.p);
         n = (long long)nn;

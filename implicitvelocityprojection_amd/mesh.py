@@ -116,8 +116,16 @@ def voxelize(mesh, pitch, max_iter=15, device="cuda"):
     faces = np.ascontiguousarray(mesh.faces, dtype=np.int32)
     stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
     count = ctypes.c_longlong(0)
+    # the points fit in the occupied index box of the rounded vertices: with a modest box, one
+    # call with that capacity; otherwise a counting call first
     cap = 0
-    out = torch.zeros(0, 3, dtype=torch.float64, device=dev)
+    if faces.shape[0] and faces.min() >= 0 and faces.max() < verts.shape[0]:
+        used = verts[np.unique(faces)]
+        box = np.rint(used.max(axis=0) / pitch) - np.rint(used.min(axis=0) / pitch) + 1
+        nvox = float(np.prod(box))
+        if nvox <= float(1 << 26):
+            cap = int(nvox)
+    out = torch.zeros(cap, 3, dtype=torch.float64, device=dev)
     for _ in range(2):
         rc = lib.ivp_voxelize(verts.ctypes.data_as(ctypes.c_void_p), verts.shape[0],
                               faces.ctypes.data_as(ctypes.c_void_p), faces.shape[0], ctypes.c_double(pitch),

@@ -60,3 +60,23 @@ def test_voxelize_errors_and_capacity():
                           ctypes.c_double(P), 15, ctypes.c_void_p(out.data_ptr()), 5, ctypes.byref(n), None)
     assert rc == N.IVP_ERR_CAPACITY and n.value == 18 ** 3
     assert torch.all(out == -7.0)
+
+
+def test_voxelize_level_limit_matches_oracle():
+    """The subdivision depth limit: a mesh that needs exactly L levels voxelises with
+    max_iter = L (bitwise as the oracle) and is refused with L - 1, as the oracle refuses it."""
+    from implicitvelocityprojection_amd import _native as N
+    from implicitvelocityprojection_amd.mesh import Mesh, voxelize
+    for v, f in (M.icosphere(0.7, 1), M.box((0.0, 0.0, 0.0), (0.02, 0.025, 0.01)),
+                 M.box((0.0, 0.0, 0.0), (0.01, 0.01, 0.01))):
+        _, _, L = OV.subdivide_samples(v[f], P / 2, 15)
+        want = OV.voxelize(v, f, P, max_iter=L)
+        got = voxelize(Mesh(v, f), P, max_iter=L).cpu().numpy()
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), L
+        if L > 0:
+            with pytest.raises(OV.MaxIterExceeded):
+                OV.voxelize(v, f, P, max_iter=L - 1)
+            with pytest.raises(N.IVPError):
+                voxelize(Mesh(v, f), P, max_iter=L - 1)
+        else:
+            assert got.shape[0] >= 1
