@@ -117,6 +117,9 @@ SIGNATURES = {
     "ivp_counts": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_longlong)]),
     "ivp_set_scenes": (_I, [_P, _I, ctypes.POINTER(Scene)]),
     "ivp_step_scenes": (_I, [_P, ctypes.POINTER(StepParams), ctypes.POINTER(SceneStats)]),
+    "ivp_max_speed2": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
+    "ivp_rigid_transform": (_I, [_P, _P, _P, _I, _I, ctypes.POINTER(_F), ctypes.POINTER(_F), ctypes.POINTER(_F),
+                                 _P]),
     "ivp_voxelize": (_I, [_P, _I, _P, _I, ctypes.c_double, _I, _P, ctypes.c_longlong,
                           ctypes.POINTER(ctypes.c_longlong), _P]),
     "ivp_scene_history": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I), ctypes.POINTER(_I),

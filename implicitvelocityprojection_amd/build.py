@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libivp.so")
-SOURCES = ["ivp_api.hip", "ivp_grid.hip", "ivp_solve.hip", "ivp_voxel.hip"]
+SOURCES = ["ivp_api.hip", "ivp_grid.hip", "ivp_solve.hip", "ivp_voxel.hip", "ivp_kinematic.hip"]
 HEADERS = ["ivp_device.h", "ivp_internal.h"]
 ARCH = os.environ.get("IVP_OFFLOAD_ARCH", "gfx950")
 

@@ -206,6 +206,12 @@ class Context:
                 "ivp_viscosity_weiler18")
         return cs.cg_iters
 
+    def max_speed2(self):
+        """max over dynamic particles of |v|^2 (float32 sum of squares) of the stored velocities."""
+        out = ctypes.c_double()
+        N.check(self.lib.ivp_max_speed2(self.h, ctypes.byref(out)), "ivp_max_speed2")
+        return out.value
+
     def apply_gravity(self, g, dt):
         ga = (ctypes.c_float * 3)(*[float(v) for v in g])
         N.check(self.lib.ivp_apply_gravity(self.h, ga, ctypes.c_float(dt)), "ivp_apply_gravity")

@@ -77,7 +77,7 @@ void free_all(ivp_ctx* c) {
     dfree(c->st_gt); dfree(c->st_div); dfree(c->st_b);
     dfree(c->prj_v0); dfree(c->prj_lam); dfree(c->prj_prev); dfree(c->prj_df_prev); dfree(c->prj_hist);
     c->prj_hist_cap = 0;
-    dfree(c->partials); dfree(c->state); dfree(c->err_flag);
+    dfree(c->partials); dfree(c->state); dfree(c->err_flag); dfree(c->speed_max);
     if (c->state_host) (void)hipHostFree(c->state_host);
     if (c->err_host) (void)hipHostFree(c->err_host);
     if (c->count_host) (void)hipHostFree(c->count_host);
@@ -114,7 +114,7 @@ int alloc_all(ivp_ctx* c) {
     A(w_Ap, 3 * cap); A(w_rhs, 3 * cap); A(w_fac, cap); A(v4, cap);
     c->max_blocks = blocks_for(c->cap) * MAX_PASS_LANES + 1;   // blocks of a pair pass at most
     A(partials, 4 * (size_t)c->max_blocks);
-    A(state, 1); A(err_flag, 1);
+    A(state, 1); A(err_flag, 1); A(speed_max, 1);
 #undef A
     IVP_HIP(hipHostMalloc((void**)&c->state_host, sizeof(SolverState), hipHostMallocDefault));
     IVP_HIP(hipHostMalloc((void**)&c->err_host, sizeof(int), hipHostMallocDefault));

@@ -234,6 +234,7 @@ struct ivp_ctx {
     ivp::SolverState* state = nullptr;      // device
     ivp::SolverState* state_host = nullptr; // pinned
     int* err_flag = nullptr;                // device: bit 0 neighbour overflow
+    unsigned* speed_max = nullptr;          // device: bits of max |v|^2 (ivp_max_speed2)
     int* err_host = nullptr;                // pinned
     int* count_host = nullptr;              // pinned scratch
 

@@ -9,6 +9,7 @@ every step, so velocity edits between steps take effect as in the reference; pos
 a rebuild (``initialize()``), as the reference's cached neighbour rows do.
 """
 
+import math
 import time
 
 from . import _native as N
@@ -27,9 +28,10 @@ class Framework:
         self.cfg = config
         self.dt = config.dt
         self.dt_default = config.dt
+        # CFL time step (framework.py:150-152, 427-440)
         self.use_cfl = bool(config.get_cfg("useCFL", False))
-        if self.use_cfl:
-            raise NotImplementedError("useCFL (adaptive time step) is not part of the IP hot path")
+        self.cfl_factor = float(config.get_cfg("cflFactor", 0.4))
+        self.cfl_min_dt = float(config.get_cfg("cflMinDt", 1e-4))
         self.time = 0.0
         self.initial_particle_num = simulation_data.particle_num
         self.stats_pressure_ms = 0.0
@@ -59,6 +61,7 @@ class Framework:
         self.sync_arrays = True
         self.last_visc_iters = 0
         self.scheduled_spawn_system = None        # set by the caller (main.py:73)
+        self.animation = None                     # set by the caller (main.py:76-80)
 
     def initialize(self):
         """framework.py:282-290, 323-330: neighbour rebuild, density and Aii at the current x."""
@@ -92,6 +95,14 @@ class Framework:
         """One step (framework.py:423-544, velocity-based branch)."""
         ps = self.ps
         ps.sync_to_library(positions=False, velocities=True)
+        if self.use_cfl:
+            # dt = cflFactor h / v_max, at most the scene's dt and at least cflMinDt
+            v_max_sq = ps.ctx.max_speed2()
+            if math.isfinite(v_max_sq) and v_max_sq > 1e-16:
+                self.dt = min(self.dt_default, self.cfl_factor * ps.support_radius / v_max_sq ** 0.5)
+            else:
+                self.dt = self.dt_default
+            self.dt = max(self.dt, self.cfl_min_dt)
         t0 = time.perf_counter()
         res, vit = ps.ctx.step(self.dt, self.g, self.cfg.viscosity_method, self.viscosity_coeff,
                                self.pressure.params(), self.surface_tension_method, self.surface_tension_coeff)
@@ -101,14 +112,26 @@ class Framework:
         self.pressure.record(res)
         ps._cache_stale = True
         spawning = self.scheduled_spawn_system is not None
-        if self.sync_arrays or spawning:
+        if self.sync_arrays or spawning or self.animation is not None:
             n = ps.particle_num
             ps.x_old[:n] = ps.x[:n]
             ps.sync_from_library(x=True, v=True, density=True, divergence=True)
             ps.ctx.read(N.FIELD_K, self.pressure.k[:n])
+        # Animated bodies take their pose at this step's start time (framework.py:442-444).  The
+        # reference moves them before the physics but solves with the neighbour cache, density and
+        # Aii of the previous build; animated bodies are static rigid here, so moving them after
+        # the library step, before the rebuild that ends the step, gives the same step.
+        moved = False
+        if self.animation is not None:
+            moved = self.animation.apply(self.time, self.dt)
+            self.animation.clear_kinematic()
         # bodies due at this step's start time join before the rebuild (framework.py:528-540): the
-        # library step has rebuilt for the old set, so a step that spawns rebuilds once more
+        # library step has rebuilt for the old set, so a step that spawns or moves bodies rebuilds
+        # once more, with the set reloaded
         if spawning and self.scheduled_spawn_system.step(self.time) > 0:
+            moved = True
+        if moved:
+            ps._dirty = True
             self.initialize()
         self.time += self.dt
         self.stats_step_ms = (time.perf_counter() - t0) * 1000.0

@@ -52,6 +52,7 @@ class SceneLoader:
         return {
             "particles": ps,
             "spawn_defs": spawns,
+            "object_map": self._object_map(ps, spawns),
             "particle_max_num": ps.n + spawn_capacity,
             "fluid_particle_num": int((ps.material == 1).sum()),
             "rigid_particle_num": int((ps.material == 0).sum()),
@@ -60,6 +61,30 @@ class SceneLoader:
             "num_rigid_bodies": len(self.cfg.get_rigid_bodies()),
             "object_ids": object_ids,
         }
+
+    def _object_map(self, ps, spawns):
+        """Scene object per id with its particles' rest positions (the "voxelizedPoints" /
+        "generatedPositions" the reference keeps, scene_loader.py:86-186), read by the animation
+        tracks' named pivots.  A spawn id maps to the last body spawned under it."""
+        cfg = self.cfg
+        out = {}
+        blocks = (cfg.get_fluid_blocks(), cfg.get_fluid_bodies(), cfg.get_rigid_bodies(), cfg.get_generated_bodies())
+        for group in blocks:
+            for obj in group:
+                oid = int(obj["objectId"])
+                entry = dict(obj)
+                entry["voxelizedPoints"] = ps.x[ps.object_id == oid]
+                out[oid] = entry
+        spawn_objs = {}
+        for sp in cfg.get_scheduled_spawns():
+            spawn_objs[int(sp["objectId"])] = sp
+        for e in spawns:
+            base = max(k for k in spawn_objs if k <= e.object_id)
+            entry = dict(spawn_objs[base])
+            entry["voxelizedPoints"] = e.positions
+            entry["objectId"] = e.object_id
+            out[e.object_id] = entry
+        return out
 
     def populate_scene(self, ps, scene_data: dict):
         ps.allocate(particle_max_num=scene_data["particle_max_num"], num_objects=scene_data["num_objects"],

@@ -284,6 +284,19 @@ int ivp_scene_history(ivp_ctx *ctx, int scene, int capacity, double *density_err
  * hipStream_t) and synchronises it; needs no context. */
 int ivp_voxelize(const double *vertices, int n_vertices, const int *faces, int n_faces, double pitch,
                  int max_iter, double *points, long long capacity, long long *n_points, void *stream);
+/* ---- time step and animation -------------------------------------------------------------- */
+/* compute_max_velocity_sq (framework.py:44-58): *out = max over dynamic particles of
+ * (vx vx + vy vy) + vz vz in float32 (0 without dynamic particles), from the stored velocities;
+ * Framework's CFL step dt = min(dt_default, cflFactor h / sqrt(out)) (framework.py:427-440)
+ * reads it.  Synchronises the stream. */
+int ivp_max_speed2(ivp_ctx *ctx, double *out);
+/* apply_transform_kernel (animation.py:7-34) on user-order DEVICE arrays of n particles: every
+ * particle with object_id == object gets x = (pivot + T) + R (x0 - pivot) in float32 (R row
+ * major, products summed left to right).  pivot, R, T are host arrays.  Needs no context; runs
+ * on `stream` without synchronising it.  The kinematic flags and velocities of the reference
+ * kernel are not touched (an animated static rigid body's velocity is zeroed by the advection). */
+int ivp_rigid_transform(const float *x0, float *x, const int *object_id, int n, int object,
+                        const float pivot[3], const float R[9], const float T[3], void *stream);
 /* ---- instrumentation ---------------------------------------------------------------------- */
 /* Per-kernel-class event timing on the context's stream.  Only launches that did work are
  * counted (CG launches after convergence inside a launch chunk are excluded). */

@@ -8,6 +8,7 @@ restated kernels.
 """
 
 import ctypes
+import math
 import os
 import subprocess
 
@@ -403,6 +404,15 @@ class OracleSim:
         self.time = 0.0
         self.spawns = []
         self.particle_max_num = particles.n
+        self.object_id = particles.object_id.copy()
+        self.x0 = particles.x.copy()
+        # CFL time step (framework.py:150-152, 427-440)
+        self.dt_default = self.dt
+        self.use_cfl = bool(config.get_cfg("useCFL", False))
+        self.cfl_factor = float(config.get_cfg("cflFactor", 0.4))
+        self.cfl_min_dt = float(config.get_cfg("cflMinDt", 1e-4))
+        # keyframe tracks: (object id, t -> (R (3, 3), T (3,), pivot (3,)) float32)
+        self.tracks = []
         self.last_ip = None
         self.last_prj = None
         self.last_visc_iters = 0
@@ -504,9 +514,42 @@ class OracleSim:
             self.divergence = cat(self.divergence, np.zeros(n, np.float32))
             self.lam_prev = cat(self.lam_prev, np.zeros(n, np.float32))
             self.lam_df_prev = cat(self.lam_df_prev, np.zeros(n, np.float32))
+            self.object_id = cat(self.object_id, np.full(n, e.object_id, np.int32))
+            self.x0 = cat(self.x0, e.positions[:n].astype(np.float32))
+
+    def cfl_dt(self):
+        """framework.py:427-440: dt = min(dt_default, cflFactor h / v_max), floored at cflMinDt;
+        v_max^2 the float32 max of (vx vx + vy vy) + vz vz over dynamic particles."""
+        dyn = self.is_dynamic == 1
+        v = self.v[dyn]
+        vsq = (v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2]
+        v_max_sq = float(vsq.max()) if vsq.size else 0.0
+        if math.isfinite(v_max_sq) and v_max_sq > 1e-16:
+            dt = min(self.dt_default, self.cfl_factor * self.cfg.support_radius / v_max_sq ** 0.5)
+        else:
+            dt = self.dt_default
+        return max(dt, self.cfl_min_dt)
+
+    def animate(self):
+        """apply_transform_kernel (animation.py:7-34) for every track at the current time:
+        x = (pivot + T) + R (x0 - pivot), float32, products summed left to right."""
+        for oid, pose in self.tracks:
+            sel = self.object_id == oid
+            if not sel.any():
+                continue
+            R, T, pivot = pose(self.time)
+            R, T, pivot = np.float32(R), np.float32(T), np.float32(pivot)
+            d = self.x0[sel] - pivot
+            r = np.stack([(R[i, 0] * d[:, 0] + R[i, 1] * d[:, 1]) + R[i, 2] * d[:, 2] for i in range(3)], axis=1)
+            self.x[sel] = (pivot + T) + r
 
     def step(self):
-        """Framework.forward (framework.py:423-544), velocity-based branch, fixed dt."""
+        """Framework.forward (framework.py:423-544), velocity-based branch: CFL time step, keyframe
+        poses, forces, pressure, advection, boundary, spawns, rebuild."""
+        if self.use_cfl:
+            self.dt = self.cfl_dt()
+        if self.tracks:
+            self.animate()
         self.non_pressure()
         self.pressure()
         self.x, self.v = advect(self.x, self.v, self.is_dynamic, self.dt)
