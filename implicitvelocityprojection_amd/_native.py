@@ -27,6 +27,7 @@ ROWS_IDS, ROWS_F32, ROWS_F16 = 0, 1, 2     # values of OPT_GRADIENT_CACHE
 OPT_SCENE_CG_CHUNK = 2
 OPT_SCENE_GRAPH = 3
 OPT_SCENE_LANES = 4
+OPT_BUILD_LANES = 5
 MIN_SCENE_DYNAMIC = 256
 
 # kernel classes of ivp_profile_read / ivp_profile_enable masks

@@ -463,6 +463,11 @@ int ivp_set_option(ivp_ctx* c, int option, int value) {
         c->scene_graph = value != 0;
         return IVP_OK;
     }
+    if (option == IVP_OPT_BUILD_LANES) {
+        if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16) return IVP_ERR_ARG;
+        c->build_lanes = value;
+        return IVP_OK;
+    }
     if (option == IVP_OPT_SCENE_LANES) {
         if (value != 1 && value != 2 && value != 4) return IVP_ERR_ARG;
         c->scene_lanes = value;

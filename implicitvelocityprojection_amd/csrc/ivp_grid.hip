@@ -325,9 +325,18 @@ struct BuildArgs {
 //     -- are padded with (id = row, gradW = 0).
 // FMT is the row format (include/ivp.h): ids only, ids + float32 gradW, or ids + float16 gradW
 // scaled by 1/inv_gs, packed as (gx, gy) and (gz, 0) half pairs.
+template <int FMT>
+constexpr unsigned rec_words() { return FMT == IVP_ROWS_F32 ? 256u : FMT == IVP_ROWS_F16 ? 192u : 64u; }
+
+// Phase B of the build for the wave's 64-row tile (one thread per row, lane = row's place in the
+// tile); cnt / cs / tot are the row's stored dynamic and static slots and its neighbour total.
+template <int FMT>
+__device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, int tile, bool valid, float4 pi,
+                                           int cnt, int cs, int tot);
+
 template <int FMT, bool SCENES>
 __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
-    constexpr unsigned REC = FMT == IVP_ROWS_F32 ? 256u : FMT == IVP_ROWS_F16 ? 192u : 64u;   // words per record
+    constexpr unsigned REC = rec_words<FMT>();
     int blk = xcd_block(blockIdx.x, gridDim.x);
     int i = blk * BLOCK + threadIdx.x;
     int lane = threadIdx.x & (WAVE - 1);
@@ -353,6 +362,129 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
         });
         if (tot > C) atomicOr(a.err_flag, 1);
     }
+    build_eval<FMT>(a, i, lane, tile, valid, pi, cnt, cs, tot);
+}
+
+// Candidate scan of one population with G lanes per row: lane g of the row's group tests the
+// candidates s0 + g, s0 + g + G, ... of every run, U batches in flight; the group's ballot
+// ranks the hits, so slots fill in the order of the one-lane scan (ascending candidate index).
+// `put(j, rank_in_batch, hits_before_batch)` stores a hit; returns the number of hits.
+template <int G, int U, class P>
+__device__ __forceinline__ int scan_group(const Grid& g, const int* __restrict__ start,
+                                          const float4* __restrict__ ppos, int self, float4 pi, float h, float h2_lo,
+                                          float h2_hi, int gl, P&& put) {
+    const int cx = cell_coord(g, pi.x, 0), cy = cell_coord(g, pi.y, 1), cz = cell_coord(g, pi.z, 2);
+    const int x0 = cx > 0 ? cx - 1 : 0;
+    const int x1 = cx < g.dims[0] - 1 ? cx + 1 : g.dims[0] - 1;
+    const int base = (int)(threadIdx.x & (WAVE - 1)) & ~(G - 1);
+    const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    const unsigned long long below = (1ull << gl) - 1ull;
+    int found = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        int zz = cz + dz;
+        if (zz < 0 || zz >= g.dims[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int yy = cy + dy;
+            if (yy < 0 || yy >= g.dims[1]) continue;
+            const int s0 = start[cell_key(g, x0, yy, zz)];
+            const int s1 = start[cell_key(g, x1, yy, zz) + 1];
+            for (int b = s0; b < s1; b += G * U) {
+                float4 p[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    int j = b + u * G + gl;
+                    p[u] = ppos[j < s1 ? j : s0];
+                }
+                __builtin_amdgcn_sched_barrier(0);   // every load of the batch in flight before the tests
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int j = b + u * G + gl;
+                    float rx = pi.x - p[u].x, ry = pi.y - p[u].y, rz = pi.z - p[u].z;
+                    float d2 = rx * rx + ry * ry + rz * rz;
+                    bool in = d2 < h2_lo;
+                    bool band = !in && d2 < h2_hi;
+                    if (__ballot(band)) {
+                        if (band) {
+                            float d = d2;
+                            asm volatile("" : "+v"(d));
+                            in = sqrtf(d) < h;
+                        }
+                    }
+                    in = in && j < s1 && j != self;
+                    const unsigned long long m = (__ballot(in) >> base) & gmask;
+                    if (in) put(j, __popcll(m & below), found);
+                    found += __popcll(m);
+                }
+            }
+        }
+    }
+    return found;
+}
+
+// Phase A alone with G lanes per row (threads = rows x G): ids into the rows' slots exactly as
+// k_build's scan stores them, the row's dynamic slot count into rowcnt and its neighbour total
+// into rowtot.  For particle counts that leave few waves per SIMD, where the one-lane scan is a
+// chain of dependent load rounds.
+template <int FMT, bool SCENES, int G>
+__global__ __launch_bounds__(BLOCK) void k_scan(BuildArgs a) {
+    constexpr unsigned REC = rec_words<FMT>();
+    constexpr int U = G == 2 ? 4 : G == 4 ? 2 : 1;   // batches in flight (measured best per G)
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int i = blk * (BLOCK / G) + (int)(threadIdx.x / G);
+    const int gl = (int)(threadIdx.x % G);
+    if (i >= a.nd) return;   // the G lanes of a row leave together
+    const int C = a.C, Cs = a.Cs;
+    uint32_t* tbase = a.nb + (size_t)(i >> 6) * (size_t)Cs * REC + (i & (WAVE - 1));
+    const float4 pi = a.pos[i];
+    const Grid g = SCENES ? a.grids[a.sid[i]] : a.g;
+    int tot = 0, cnt = 0, cs = 0;
+    // the k-th neighbour found (dynamic ones first) is stored when k < C
+    int nd_found = scan_group<G, U>(g, a.dstart, a.pos, i, pi, a.h, a.h2_lo, a.h2_hi, gl,
+                                    [&](int j, int rank, int before) {
+                                        if (before + rank < C) tbase[(size_t)(before + rank) * REC] = (uint32_t)j;
+                                    });
+    cnt = nd_found < C ? nd_found : C;
+    tot = nd_found;
+    int ns_found = scan_group<G, U>(g, a.sstart, a.pos + a.nd, -1, pi, a.h, a.h2_lo, a.h2_hi, gl,
+                                    [&](int j, int rank, int before) {
+                                        const int k = nd_found + before + rank;
+                                        if (k < C) tbase[(size_t)(Cs - 1 - (before + rank)) * REC] = (uint32_t)j;
+                                    });
+    tot += ns_found;
+    cs = (tot < C ? tot : C) - cnt;
+    (void)cs;
+    if (gl == 0) {
+        a.rowcnt[i] = cnt;
+        a.rowtot[i] = tot;
+        if (tot > C) atomicOr(a.err_flag, 1);
+    }
+}
+
+// Phase B after k_scan: one thread per row, the counts from k_scan
+template <int FMT>
+__global__ __launch_bounds__(BLOCK) void k_eval(BuildArgs a) {
+    int blk = xcd_block(blockIdx.x, gridDim.x);
+    int i = blk * BLOCK + threadIdx.x;
+    int lane = threadIdx.x & (WAVE - 1);
+    bool valid = i < a.nd;
+    int cnt = 0, tot = 0;
+    float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (valid) {
+        pi = a.pos[i];
+        cnt = a.rowcnt[i];
+        tot = a.rowtot[i];
+    }
+    int cs = (tot < a.C ? tot : a.C) - cnt;
+    build_eval<FMT>(a, i, lane, i >> 6, valid, pi, cnt, cs, tot);
+}
+
+template <int FMT>
+__device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, int tile, bool valid, float4 pi,
+                                           int cnt, int cs, int tot) {
+    constexpr unsigned REC = rec_words<FMT>();
+    const float h = a.h;
+    const int Cs = a.Cs;
+    uint32_t* tbase = a.nb + (size_t)tile * (size_t)Cs * REC + lane;
     int tl = wave_max(cnt);
     int ts = wave_max(cs);
     int tlp = (tl + ROW_PAD - 1) / ROW_PAD * ROW_PAD;   // <= Cs: tl <= C <= Cs, Cs a multiple of ROW_PAD
@@ -724,6 +856,28 @@ int grid_load(ivp_ctx* c, int n, const float* x, const float* v, const float* m,
     return IVP_OK;
 }
 
+// Lanes per row of the candidate scan: the option's value, or by the dynamic count.  Measured
+// (build and sort per call): 8k rows 132 us with one fused lane, 92 with 16 lanes; 102k rows
+// 145 -> 115 with 4 lanes; 1M rows 501 -> 404 with 2 lanes and 4 batches in flight.  Fewer rows
+// leave fewer waves per SIMD, and the one-lane scan is then a chain of dependent load rounds.
+int build_lanes_for(const ivp_ctx* c) {
+    if (c->build_lanes > 0) return c->build_lanes;
+    const int nd = c->nd;
+    return nd >= (1 << 18) ? 2 : nd >= (1 << 15) ? 4 : nd >= (1 << 12) ? 8 : 16;
+}
+
+template <int FMT, bool SCENES>
+int launch_scan(ivp_ctx* c, int G, const BuildArgs& ba) {
+    const dim3 grid(blocks_for(c->nd * G));
+    switch (G) {
+    case 2: return launch(c, KC_BUILD, -1, 0.0, k_scan<FMT, SCENES, 2>, grid, ba);
+    case 4: return launch(c, KC_BUILD, -1, 0.0, k_scan<FMT, SCENES, 4>, grid, ba);
+    case 8: return launch(c, KC_BUILD, -1, 0.0, k_scan<FMT, SCENES, 8>, grid, ba);
+    case 16: return launch(c, KC_BUILD, -1, 0.0, k_scan<FMT, SCENES, 16>, grid, ba);
+    default: return IVP_ERR_ARG;
+    }
+}
+
 int grid_build(ivp_ctx* c) {
     if (!c->loaded) return IVP_ERR_STATE;
     if (c->nd > 0) {
@@ -746,14 +900,32 @@ int grid_build(ivp_ctx* c) {
         double pairs = (double)c->state_host->total_pairs;
         double bytes = pairs * (4.0 * c->rec_words / 64 + 4.0) + 16.0 * c->n + 32.0 * c->nd;
         const dim3 nb(blocks_for(c->nd));
-        if (c->S > 0)   // independent scenes use ids-only rows (ivp_set_scenes fixes the format)
-            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_IDS, true>, nb, ba));
-        else if (c->row_fmt == IVP_ROWS_F32)
-            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_F32, false>, nb, ba));
-        else if (c->row_fmt == IVP_ROWS_F16)
-            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_F16, false>, nb, ba));
-        else
-            IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_IDS, false>, nb, ba));
+        const int G = build_lanes_for(c);
+        if (G == 1) {
+            if (c->S > 0)   // independent scenes use ids-only rows (ivp_set_scenes fixes the format)
+                IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_IDS, true>, nb, ba));
+            else if (c->row_fmt == IVP_ROWS_F32)
+                IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_F32, false>, nb, ba));
+            else if (c->row_fmt == IVP_ROWS_F16)
+                IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_F16, false>, nb, ba));
+            else
+                IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_build<IVP_ROWS_IDS, false>, nb, ba));
+        } else {
+            if (c->S > 0)
+                IVP_TRY((launch_scan<IVP_ROWS_IDS, true>(c, G, ba)));
+            else if (c->row_fmt == IVP_ROWS_F32)
+                IVP_TRY((launch_scan<IVP_ROWS_F32, false>(c, G, ba)));
+            else if (c->row_fmt == IVP_ROWS_F16)
+                IVP_TRY((launch_scan<IVP_ROWS_F16, false>(c, G, ba)));
+            else
+                IVP_TRY((launch_scan<IVP_ROWS_IDS, false>(c, G, ba)));
+            if (c->row_fmt == IVP_ROWS_F32 && c->S == 0)
+                IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_eval<IVP_ROWS_F32>, nb, ba));
+            else if (c->row_fmt == IVP_ROWS_F16 && c->S == 0)
+                IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_eval<IVP_ROWS_F16>, nb, ba));
+            else
+                IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_eval<IVP_ROWS_IDS>, nb, ba));
+        }
         IVP_TRY(finalize<FIN_BUILD>(c, nb.x));
     }
     c->built = true;

@@ -260,6 +260,7 @@ struct ivp_ctx {
     int cg_guess = 4;
     int w18_guess = 4;
     int st_guess = 4;
+    int build_lanes = 0;                  // lanes per row of the build's candidate scan (0: by count)
     int scene_chunk = 4;                  // CG iterations per replay
     int scene_lanes = 1;                  // lanes per row in the scene passes (1, 2 or 4)
     bool scene_graph = true;              // replay a captured hipGraph

@@ -269,6 +269,10 @@ int ivp_scene_history(ivp_ctx *ctx, int scene, int capacity, double *density_err
 #define IVP_OPT_SCENE_CG_CHUNK 2
 #define IVP_OPT_SCENE_GRAPH 3
 #define IVP_OPT_SCENE_LANES 4
+/* Lanes per row of the neighbour build's candidate scan: 0 (default) chooses by the dynamic
+ * count; 1 one lane per row, fused with the gradient/density/Aii pass; 2, 4, 8 or 16 lanes per row
+ * in a separate scan (slots filled in the same order).  The rows are identical either way. */
+#define IVP_OPT_BUILD_LANES 5
 
 /* ---- scene loading -------------------------------------------------------------------------- */
 /* SceneLoader.load_mesh (scene_loader.py:547-580): the voxel centres of the solid bounded by a
