@@ -894,57 +894,12 @@ __device__ __forceinline__ float4 w18_record(const float4* __restrict__ pos, con
     return make_float4(x.x, x.y, x.z, V);
 }
 
-// compute_precond_weiler18 (weiler18.py:21-85): inverse of I - factor_i L_ii
-template <class FM, bool ALLFLUID>
-__global__ __launch_bounds__(BLOCK) void k_w18_pre(Rows R, int fluid, const float4* __restrict__ v4,
-                                                   const float* __restrict__ fac, const int* __restrict__ mat,
-                                                   float* __restrict__ pinv) {
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int i = pass_row(blk), g = pass_lane();
-    auto loadA = [&](int j) { return w18_record<ALLFLUID>(R.pos, v4, mat, fluid, j); };
-    auto loadB = [&](int) { return Vec<0>{}; };
-    if (i >= R.nd) return;
-    float* P = pinv + 9 * (size_t)i;
-    if (!ALLFLUID && mat[i] != fluid) {
-        if (g == 0)
-            for (int q = 0; q < 9; ++q) P[q] = (q % 4) == 0 ? 1.0f : 0.0f;
-        return;
-    }
-    float4 xi = R.pos[i];
-    float eps2 = 0.01f * R.h * R.h;
-    float dd = 10.0f;
-    float L[9];
-    for (int q = 0; q < 9; ++q) L[q] = 0.0f;
-    row_pairs<FM, 0, PASS_LANES, PASS_UNROLL>(R, i, xi, loadA, loadB,
-                         [&](const float4& xj, const Vec<0>&, float gx, float gy, float gz) {
-                             float x[3] = {xi.x - xj.x, xi.y - xj.y, xi.z - xj.z};
-                             float r_sq = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
-                             float c = dd * xj.w * __builtin_amdgcn_rcpf(r_sq + eps2);
-                             float cg[3] = {c * gx, c * gy, c * gz};
-#pragma unroll
-                             for (int p = 0; p < 3; ++p)
-#pragma unroll
-                                 for (int q = 0; q < 3; ++q) L[3 * p + q] = fmaf(cg[p], x[q], L[3 * p + q]);
-                         }, g);
-#pragma unroll
-    for (int q = 0; q < 9; ++q) L[q] = group_sum<PASS_LANES>(L[q]);
-    if (g != 0) return;
-    float f = fac[i];
-    float M[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) M[q] = ((q % 4) == 0 ? 1.0f : 0.0f) - f * L[q];
-    float out[9];
-    inverse33(M, out);
-#pragma unroll
-    for (int q = 0; q < 9; ++q) P[q] = out[q];
-}
-
 struct W18Args {
     Rows R;
     int fluid;
     const float* fac;
     const int* mat;
-    const float* pinv;
+    float* pinv;
     float4* v4;   // operator input (v for the rhs, p for CG), w = m / rho
     float* out;   // rhs or Ap
     float* dv; float* r; float* z;
@@ -978,20 +933,82 @@ __device__ __forceinline__ float3 w18_Av_row(const W18Args& a, int i, float4 ui,
     return make_float3(ui.x - f * lx, ui.y - f * ly, ui.z - f * lz);
 }
 
-// rhs = v - A v (weiler18.py:390-393)
+// The solve's set-up in one pass over the pairs: compute_precond_weiler18 (weiler18.py:21-85,
+// the inverse of I - factor_i L_ii), the right-hand side v - A v (weiler18.py:390-393) and the
+// PCG prologue (weiler18.py:320-335: dv = 0, r = rhs, z = P r, p = z, r.z).  The preconditioner
+// block and the operator share each pair's gradient, (m_j / rho_j) and 1 / (r^2 + 0.01 h^2); every
+// accumulator sums the same terms in the same order as it would in a pass of its own.
 template <class FM, bool ALLFLUID>
-__global__ __launch_bounds__(BLOCK) void k_w18_rhs(W18Args a) {
+__global__ __launch_bounds__(BLOCK) void k_w18_setup(W18Args a, float4* __restrict__ p4) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = pass_row(blk), g = pass_lane();
     auto loadA = [&](int j) { return w18_record<ALLFLUID>(a.R.pos, a.v4, a.mat, a.fluid, j); };
     auto loadB = [&](int j) { return vec3_of(a.v4, j); };
-    if (i >= a.R.nd) return;
-    float4 vi = a.v4[i];
-    float3 Av = w18_Av_row<FM, ALLFLUID, PASS_LANES>(a, i, vi, loadA, loadB, g);
-    if (g != 0) return;
-    a.out[3 * i + 0] = vi.x + -1.0f * Av.x;
-    a.out[3 * i + 1] = vi.y + -1.0f * Av.y;
-    a.out[3 * i + 2] = vi.z + -1.0f * Av.z;
+    double rz = 0.0;
+    if (i < a.R.nd) {
+        const float4 vi = a.v4[i];
+        const bool fl = ALLFLUID || a.mat[i] == a.fluid;
+        float L[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) L[q] = 0.0f;
+        float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+        if (fl) {
+            const float4 xi = a.R.pos[i];
+            const float eps2 = 0.01f * a.R.h * a.R.h;
+            const float dd = 10.0f;
+            row_pairs<FM, 3, PASS_LANES, PASS_UNROLL>(a.R, i, xi, loadA, loadB,
+                [&](const float4& xj, const Vec<3>& uj, float gx, float gy, float gz) {
+                    float x[3] = {xi.x - xj.x, xi.y - xj.y, xi.z - xj.z};
+                    float v0 = vi.x - uj.v[0], v1 = vi.y - uj.v[1], v2 = vi.z - uj.v[2];
+                    float r_sq = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
+                    float rc = __builtin_amdgcn_rcpf(r_sq + eps2);
+                    float t = dd * xj.w;
+                    float c = t * rc;
+                    float cg[3] = {c * gx, c * gy, c * gz};
+#pragma unroll
+                    for (int p = 0; p < 3; ++p)
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) L[3 * p + q] = fmaf(cg[p], x[q], L[3 * p + q]);
+                    float vx = fmaf(v2, x[2], fmaf(v1, x[1], v0 * x[0]));
+                    float sc = t * (vx * rc);
+                    lx = fmaf(sc, gx, lx); ly = fmaf(sc, gy, ly); lz = fmaf(sc, gz, lz);
+                }, g);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) L[q] = group_sum<PASS_LANES>(L[q]);
+            lx = group_sum<PASS_LANES>(lx);
+            ly = group_sum<PASS_LANES>(ly);
+            lz = group_sum<PASS_LANES>(lz);
+        }
+        if (g == 0) {
+            float P[9];
+            float Av0 = vi.x, Av1 = vi.y, Av2 = vi.z;
+            if (fl) {
+                const float f = a.fac[i];
+                float M[9];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) M[q] = ((q % 4) == 0 ? 1.0f : 0.0f) - f * L[q];
+                inverse33(M, P);
+                Av0 = vi.x - f * lx; Av1 = vi.y - f * ly; Av2 = vi.z - f * lz;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) P[q] = (q % 4) == 0 ? 1.0f : 0.0f;
+            }
+            float* Pi = a.pinv + 9 * (size_t)i;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) Pi[q] = P[q];
+            const float r0 = vi.x + -1.0f * Av0, r1 = vi.y + -1.0f * Av1, r2 = vi.z + -1.0f * Av2;
+            const float z0 = P[0] * r0 + P[1] * r1 + P[2] * r2;
+            const float z1 = P[3] * r0 + P[4] * r1 + P[5] * r2;
+            const float z2 = P[6] * r0 + P[7] * r1 + P[8] * r2;
+            a.dv[3 * i + 0] = 0.0f; a.dv[3 * i + 1] = 0.0f; a.dv[3 * i + 2] = 0.0f;
+            a.r[3 * i + 0] = r0; a.r[3 * i + 1] = r1; a.r[3 * i + 2] = r2;
+            a.z[3 * i + 0] = z0; a.z[3 * i + 1] = z1; a.z[3 * i + 2] = z2;
+            p4[i] = make_float4(z0, z1, z2, vi.w);
+            rz = (double)(r0 * z0 + r1 * z1 + r2 * z2);
+        }
+    }
+    double v[1] = {rz};
+    block_partials<1>(v, a.partials, a.stride);
 }
 
 // PCG prologue (weiler18.py:320-335): dv = 0, r = rhs, z = P r, p = z, rz = r.z
@@ -1639,14 +1656,12 @@ int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
                    (const float*)c->density, (const float*)c->rho0, c->w_v4, c->w_fac));
     const double pairs = (double)c->state_host->total_pairs;
     const double pb = pair_bytes<FM>();
-    IVP_TRY(launch(c, KC_W18_PRE, -1, pb * pairs + 56.0 * nd, k_w18_pre<FM, ALLFLUID>, pass_grid(nd), R,
-                   c->cfg.material_fluid, (const float4*)c->w_v4, (const float*)c->w_fac, (const int*)c->mat,
-                   c->w_pinv));
     W18Args wa{R, c->cfg.material_fluid, c->w_fac, c->mat, c->w_pinv, c->w_v4, c->w_rhs,
                c->w_dv, c->w_r, c->w_z, c->partials, stride, c->state};
-    IVP_TRY(launch(c, KC_W18_AV, -1, pb * pairs + 48.0 * nd, k_w18_rhs<FM, ALLFLUID>, pass_grid(nd), wa));
-    IVP_TRY(launch(c, -1, -1, 0.0, k_w18_init, nb, wa, c->w_p4));
-    IVP_TRY(finalize<FIN_W18_INIT>(c, nb.x, 1e-5, 1000));
+    // one pair pass: preconditioner blocks, right-hand side and the PCG prologue
+    IVP_TRY(launch(c, KC_W18_PRE, -1, pb * pairs + (16.0 + 4.0 + 36.0 + 12.0 * 3 + 16.0) * nd,
+                   k_w18_setup<FM, ALLFLUID>, pass_grid(nd), wa, c->w_p4));
+    IVP_TRY(finalize<FIN_W18_INIT>(c, pass_grid(nd).x, 1e-5, 1000));
     IVP_TRY(read_state(c));
     int iters = 0, chunks = 0;
     if (c->state_host->active) {
