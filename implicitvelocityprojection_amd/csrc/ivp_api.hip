@@ -292,6 +292,13 @@ int prof_push(ivp_ctx* c, ProfEvent* ev) {
 }
 
 // Fold finished events.  Events of `epoch` with tag >= cg_iters_done are dropped.
+// Set the byte figure of the pending launches of one class in one solve (for byte models that
+// depend on counts the device reports only after the launches were enqueued).
+void prof_set_bytes(ivp_ctx* c, int epoch, int kclass, double bytes) {
+    for (auto& e : c->prof_pending)
+        if (e.epoch == epoch && e.kclass == kclass) e.bytes = bytes;
+}
+
 int prof_collect(ivp_ctx* c, int cg_iters_done, int epoch) {
     std::vector<ProfEvent> keep;
     for (auto& e : c->prof_pending) {

@@ -287,6 +287,7 @@ namespace ivp {
 int prof_take(ivp_ctx* c, int kclass, int tag, double bytes, ProfEvent* ev);
 int prof_push(ivp_ctx* c, ProfEvent* ev);
 int prof_collect(ivp_ctx* c, int cg_iters_done, int epoch);
+void prof_set_bytes(ivp_ctx* c, int epoch, int kclass, double bytes);
 
 // ivp_grid.hip
 int grid_sort_population(ivp_ctx* c, int offset, int count, int* start_table, bool dynamic);

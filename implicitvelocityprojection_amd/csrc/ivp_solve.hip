@@ -1495,7 +1495,6 @@ template <class L>
 int run_chunks(ivp_ctx* c, int max_iter, int guess, L&& launch_it, int* iters, int* chunks) {
     int issued = 0;
     int chunk = std::max(2, std::min(guess, 64));
-    double rr0 = c->state_host->rr0;
     for (;;) {
         int todo = std::min(chunk, max_iter - issued);
         if (todo <= 0) break;
@@ -1505,6 +1504,7 @@ int run_chunks(ivp_ctx* c, int max_iter, int guess, L&& launch_it, int* iters, i
         (*chunks)++;
         const SolverState& s = *c->state_host;
         if (s.active == 0) break;
+        const double rr0 = s.rr0;
         // geometric extrapolation of r.r to the tolerance
         double rate = s.iter > 0 && rr0 > 0.0 && s.rr_last > 0.0 ? pow(s.rr_last / rr0, 1.0 / s.iter) : 0.5;
         int est = 8;
@@ -1562,11 +1562,13 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
         CGInitArgs ia{nd, c->b, c->pos, c->sol, c->r, c->p4, c->y2, c->partials, stride};
         IVP_TRY(launch(c, -1, -1, 0.0, k_cg_init<false>, nb, ia));
         IVP_TRY(finalize<FIN_CG_INIT>(c, nb.x, prm->tol_pcg, prm->max_iteration_pcg));
-        IVP_TRY(read_state(c));
-        long long touch_pairs = c->state_host->touched_pairs;
-        int touch_rows = c->state_host->n_touched_rows;
+        // The first CG chunk is enqueued without waiting for the set-up: its kernels return at once
+        // when there is nothing to solve.  The J^T byte model (touched rows, reported by the set-up)
+        // is attached to the launches once the first chunk's state has been read.
+        long long touch_pairs = 0;
+        int touch_rows = 0;
         int iters = 0;
-        if (c->state_host->active) {
+        {
             c->prof_epoch++;
             int epoch = c->prof_epoch;
             // algorithmic bytes per launch: the pair stream once per traversed pair, per-row
@@ -1607,6 +1609,10 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
                               (const SolverState*)c->state, SceneRef{});
             };
             IVP_TRY(run_chunks(c, prm->max_iteration_pcg, c->cg_guess, one_iter, &iters, &st.cg_launch_chunks));
+            touch_pairs = c->state_host->touched_pairs;
+            touch_rows = c->state_host->n_touched_rows;
+            prof_set_bytes(c, epoch, KC_CG_JT,
+                           pbc * (double)touch_pairs + (double)touch_rows * (8 + 12) + (double)nd * (16 + 4 + 12));
             prof_collect(c, iters, epoch);
             c->cg_guess = std::max(2, iters);
             st.pairs_traversed += (long long)iters * (act_pairs + touch_pairs);
@@ -1662,9 +1668,9 @@ int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
     IVP_TRY(launch(c, KC_W18_PRE, -1, pb * pairs + (16.0 + 4.0 + 36.0 + 12.0 * 3 + 16.0) * nd,
                    k_w18_setup<FM, ALLFLUID>, pass_grid(nd), wa, c->w_p4));
     IVP_TRY(finalize<FIN_W18_INIT>(c, pass_grid(nd).x, 1e-5, 1000));
-    IVP_TRY(read_state(c));
+    // the first CG chunk follows without waiting (its kernels return at once when r.z is tiny)
     int iters = 0, chunks = 0;
-    if (c->state_host->active) {
+    {
         W18Args cg = wa;
         cg.v4 = c->w_p4;
         cg.out = c->w_Ap;
@@ -1680,6 +1686,8 @@ int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
         c->prof_epoch++;
         int epoch = c->prof_epoch;
         IVP_TRY(run_chunks(c, 1000, c->w18_guess, one_iter, &iters, &chunks));
+        // the pair count of the last build, now read back with the solver state
+        prof_set_bytes(c, epoch, KC_W18_AV, pb * (double)c->state_host->total_pairs + 48.0 * nd);
         prof_collect(c, iters, epoch);
         c->w18_guess = std::max(2, iters);
     }
@@ -1713,9 +1721,8 @@ int jeske23_body(ivp_ctx* c, float sigma, float dt, int* iters_out) {
     IVP_TRY(launch(c, -1, -1, 0.0, k_w18_init, nb, wa, c->w_p4));
     // Jeske23.PCG (jeske23.py:499-570): tol 1e-5 on r.r, at most 100 iterations
     IVP_TRY(finalize<FIN_W18_INIT>(c, nb.x, 1e-5, 100));
-    IVP_TRY(read_state(c));
     int iters = 0, chunks = 0;
-    if (c->state_host->active) {
+    {
         StArgs cga = sa;
         cga.u4 = c->w_p4;
         cga.out = c->w_Ap;
@@ -1734,6 +1741,7 @@ int jeske23_body(ivp_ctx* c, float sigma, float dt, int* iters_out) {
         c->prof_epoch++;
         int epoch = c->prof_epoch;
         IVP_TRY(run_chunks(c, 100, c->st_guess, one_iter, &iters, &chunks));
+        prof_set_bytes(c, epoch, KC_ST_AV, (pb + 4.0) * (double)c->state_host->total_pairs + 28.0 * nd);
         prof_collect(c, iters, epoch);
         c->st_guess = std::max(2, iters);
     }
