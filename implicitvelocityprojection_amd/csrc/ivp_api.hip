@@ -99,7 +99,7 @@ int alloc_all(ivp_ctx* c) {
     c->nd_alloc_tiles = (int)tiles;
     int rc = IVP_OK;
 #define A(p, n) if ((rc = dalloc(&c->p, (n))) != IVP_OK) return rc
-    A(pos, cap); A(pos_b, cap); A(vel, 3 * cap); A(vel_b, 3 * cap); A(minv, cap); A(minv_b, cap);
+    A(pos, cap + POS_PAD); A(pos_b, cap + POS_PAD); A(vel, 3 * cap); A(vel_b, 3 * cap); A(minv, cap); A(minv_b, cap);
     A(rho0, cap); A(rho0_b, cap); A(mat, cap); A(mat_b, cap); A(uid, cap); A(uid_b, cap); A(inv, cap);
     A(density, cap); A(kk, cap); A(Ss, 3 * cap); A(divg, cap); A(divg_b, cap); A(rowcnt, cap); A(rowtot, cap);
     A(tile_len, tiles);
@@ -123,6 +123,9 @@ int alloc_all(ivp_ctx* c) {
     IVP_HIP(hipMemsetAsync(c->ccount, 0, sizeof(int) * (size_t)c->ncell, c->stream));
     IVP_HIP(hipMemsetAsync(c->state, 0, sizeof(SolverState), c->stream));
     IVP_HIP(hipMemsetAsync(c->tile_len, 0, sizeof(int) * tiles, c->stream));
+    // the pads past cap are read by the candidate scan (and masked); keep them defined
+    IVP_HIP(hipMemsetAsync(c->pos, 0, sizeof(float4) * (cap + POS_PAD), c->stream));
+    IVP_HIP(hipMemsetAsync(c->pos_b, 0, sizeof(float4) * (cap + POS_PAD), c->stream));
     IVP_HIP(hipStreamSynchronize(c->stream));
     return IVP_OK;
 }

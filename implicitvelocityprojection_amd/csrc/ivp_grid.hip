@@ -365,10 +365,20 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
     build_eval<FMT>(a, i, lane, tile, valid, pi, cnt, cs, tot);
 }
 
+// Predicate codes of the wave-mask compare builtins (LLVM's CmpInst numbering)
+constexpr int FCMP_OLT = 4, ICMP_NE = 33, ICMP_ULT = 36;
+
 // Candidate scan of one population with G lanes per row: lane g of the row's group tests the
 // candidates s0 + g, s0 + g + G, ... of every run, U batches in flight; the group's ballot
 // ranks the hits, so slots fill in the order of the one-lane scan (ascending candidate index).
 // `put(j, rank_in_batch, hits_before_batch)` stores a hit; returns the number of hits.
+// A batch loads its G * U candidates unconditionally: entries past the run's end (at most
+// G * U - 1 <= POS_PAD - 1 past the last particle) lie inside the padded position buffer and are
+// masked by j < s1.  The predicate is decided as in scan_ids, on d2 formed with fused
+// multiply-adds: that d2 is within 5 * 2^-24 (relative) of the unfused d2 of the reference's
+// expression, far inside the 2^-20 margins of h2_lo and h2_hi, so outside the band both give
+// the same answer; inside the band the unfused d2 is formed again and the correctly rounded root
+// decides, exactly as in scan_ids.
 template <int G, int U, class P>
 __device__ __forceinline__ int scan_group(const Grid& g, const int* __restrict__ start,
                                           const float4* __restrict__ ppos, int self, float4 pi, float h, float h2_lo,
@@ -376,7 +386,8 @@ __device__ __forceinline__ int scan_group(const Grid& g, const int* __restrict__
     const int cx = cell_coord(g, pi.x, 0), cy = cell_coord(g, pi.y, 1), cz = cell_coord(g, pi.z, 2);
     const int x0 = cx > 0 ? cx - 1 : 0;
     const int x1 = cx < g.dims[0] - 1 ? cx + 1 : g.dims[0] - 1;
-    const int base = (int)(threadIdx.x & (WAVE - 1)) & ~(G - 1);
+    const int wl = (int)(threadIdx.x & (WAVE - 1));
+    const int base = wl & ~(G - 1);
     const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
     const unsigned long long below = (1ull << gl) - 1ull;
     int found = 0;
@@ -389,30 +400,33 @@ __device__ __forceinline__ int scan_group(const Grid& g, const int* __restrict__
             const int s0 = start[cell_key(g, x0, yy, zz)];
             const int s1 = start[cell_key(g, x1, yy, zz) + 1];
             for (int b = s0; b < s1; b += G * U) {
+                const float4* q = ppos + (b + gl);
                 float4 p[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    int j = b + u * G + gl;
-                    p[u] = ppos[j < s1 ? j : s0];
-                }
+                for (int u = 0; u < U; ++u) p[u] = q[u * G];
                 __builtin_amdgcn_sched_barrier(0);   // every load of the batch in flight before the tests
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int j = b + u * G + gl;
                     float rx = pi.x - p[u].x, ry = pi.y - p[u].y, rz = pi.z - p[u].z;
-                    float d2 = rx * rx + ry * ry + rz * rz;
-                    bool in = d2 < h2_lo;
-                    bool band = !in && d2 < h2_hi;
-                    if (__ballot(band)) {
-                        if (band) {
-                            float d = d2;
+                    float d2 = fmaf(rz, rz, fmaf(ry, ry, rx * rx));
+                    // wave masks straight from the compares: valid candidates (inside the run, not
+                    // the row itself), those inside h2_lo, and those in the band [h2_lo, h2_hi)
+                    const unsigned long long valid = __builtin_amdgcn_uicmp((unsigned)j, (unsigned)s1, ICMP_ULT) &
+                                                     __builtin_amdgcn_uicmp((unsigned)j, (unsigned)self, ICMP_NE);
+                    unsigned long long hit = __builtin_amdgcn_fcmpf(d2, h2_lo, FCMP_OLT) & valid;
+                    const unsigned long long band = (__builtin_amdgcn_fcmpf(d2, h2_hi, FCMP_OLT) & valid) & ~hit;
+                    if (band) {
+                        bool in = false;
+                        if ((band >> wl) & 1ull) {
+                            float d = rx * rx + ry * ry + rz * rz;
                             asm volatile("" : "+v"(d));
                             in = sqrtf(d) < h;
                         }
+                        hit |= __builtin_amdgcn_ballot_w64(in);
                     }
-                    in = in && j < s1 && j != self;
-                    const unsigned long long m = (__ballot(in) >> base) & gmask;
-                    if (in) put(j, __popcll(m & below), found);
+                    const unsigned long long m = (hit >> base) & gmask;
+                    if ((m >> gl) & 1ull) put(j, __popcll(m & below), found);
                     found += __popcll(m);
                 }
             }
@@ -434,21 +448,24 @@ __global__ __launch_bounds__(BLOCK) void k_scan(BuildArgs a) {
     const int gl = (int)(threadIdx.x % G);
     if (i >= a.nd) return;   // the G lanes of a row leave together
     const int C = a.C, Cs = a.Cs;
-    uint32_t* tbase = a.nb + (size_t)(i >> 6) * (size_t)Cs * REC + (i & (WAVE - 1));
+    // the wave's rows share one 64-row tile (64 / G rows from a multiple of 64 / G); slot k of the
+    // row is word k * REC + lane of the tile, a 32-bit index from the tile's (uniform) base
+    uint32_t* tnb = a.nb + (size_t)__builtin_amdgcn_readfirstlane(i >> 6) * (size_t)Cs * REC;
+    const uint32_t lane = (uint32_t)(i & (WAVE - 1));
     const float4 pi = a.pos[i];
     const Grid g = SCENES ? a.grids[a.sid[i]] : a.g;
     int tot = 0, cnt = 0, cs = 0;
     // the k-th neighbour found (dynamic ones first) is stored when k < C
     int nd_found = scan_group<G, U>(g, a.dstart, a.pos, i, pi, a.h, a.h2_lo, a.h2_hi, gl,
                                     [&](int j, int rank, int before) {
-                                        if (before + rank < C) tbase[(size_t)(before + rank) * REC] = (uint32_t)j;
+                                        if (before + rank < C) tnb[(uint32_t)(before + rank) * REC + lane] = (uint32_t)j;
                                     });
     cnt = nd_found < C ? nd_found : C;
     tot = nd_found;
     int ns_found = scan_group<G, U>(g, a.sstart, a.pos + a.nd, -1, pi, a.h, a.h2_lo, a.h2_hi, gl,
                                     [&](int j, int rank, int before) {
                                         const int k = nd_found + before + rank;
-                                        if (k < C) tbase[(size_t)(Cs - 1 - (before + rank)) * REC] = (uint32_t)j;
+                                        if (k < C) tnb[(uint32_t)(Cs - 1 - (before + rank)) * REC + lane] = (uint32_t)j;
                                     });
     tot += ns_found;
     cs = (tot < C ? tot : C) - cnt;

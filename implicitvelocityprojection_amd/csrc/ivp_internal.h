@@ -101,6 +101,8 @@ struct SceneParams {
 };
 
 constexpr int ROW_PAD = 8;        // tile lengths are padded to a multiple of this many slots
+constexpr int POS_PAD = 16;       // float4 entries past the last particle of the position buffers: a
+                                  // candidate batch of the scan (at most 16 entries) may read past its run
 
 enum KernelClass {
     KC_SORT = 0,
