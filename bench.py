@@ -422,11 +422,14 @@ def main():
     st_iters = 0
     outer = 0
     max_err = 0.0
+    cg_hist = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         res, vit = step()
-        cg_iters += sum(res.pcg_iters)
+        cg_step = sum(res.pcg_iters)
+        cg_iters += cg_step
+        cg_hist[cg_step] = cg_hist.get(cg_step, 0) + 1
         solve_ms += res.solve_ms
         visc_iters += vit
         st_iters += ctx.last_st_iters
@@ -441,6 +444,24 @@ def main():
     bytes_per_launch = pd["bytes"] / max(pd["launches"], 1)
     pairs_end = ctx.pair_count()
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+
+    # kernel time of one IP CG iteration (J, J^T with p.Ap, x/r update with r.r, p update), from
+    # dispatch events on a few steps after the timed region (launches past convergence excluded)
+    cg_us_per_iter = None
+    if args.pressure == "ip":
+        cg_classes = ("cg_J", "cg_JT", "cg_update", "cg_p")
+        ctx.profile_reset()
+        ctx.profile_enable(kclass_mask(*cg_classes))
+        it = 0
+        for _ in range(5):
+            res, _vit = step()
+            it += sum(res.pcg_iters)
+        torch.cuda.synchronize()
+        pc = ctx.profile_read()
+        ctx.profile_enable(0)
+        if it > 0:
+            cg_us_per_iter = {k: pc[k]["total_ms"] * 1e3 / it for k in cg_classes if k in pc}
+            cg_us_per_iter["total"] = sum(cg_us_per_iter.values())
 
     precision = None
     if args.precision_steps > 0 and st_coeff == 0.0 and args.pressure == "ip":
@@ -508,6 +529,8 @@ def main():
                    "maxOuterIters": prm.max_iteration_opt, "neighbour_pairs": pairs_end},
         "cg_iters_per_s": cg_iters / (solve_ms * 1e-3) if solve_ms > 0 else 0.0,
         "cg_iters": cg_iters,
+        "cg_iters_per_step_hist": {str(k): cg_hist[k] for k in sorted(cg_hist)},
+        "cg_kernel_us_per_iter": cg_us_per_iter,
         "ip_outer_iters": outer,
         "weiler18_cg_iters": visc_iters,
         "surface_tension_cg_iters": st_iters,
