@@ -510,15 +510,12 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
     float aii = 0.0f;
     float jx = 0.0f, jy = 0.0f, jz = 0.0f;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for (int k = 0; k < tl; ++k) {
-        if (!valid || k >= cnt) continue;
-        uint32_t* rec = tbase + (size_t)k * REC;
-        int j = (int)rec[0];
+    // one dynamic slot: gradW (stored next to the id in the cached formats), density, Aii, S
+    auto dyn_slot = [&](uint32_t* rec, int j, float4 pj, float minv_j) {
         if ((unsigned)j >= (unsigned)a.nd) {  // never taken: phase A stored a dynamic id here
             atomicOr(a.err_flag, 4);
-            continue;
+            return;
         }
-        float4 pj = a.pos[j];
         float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         float rn = len3(rx, ry, rz);
         float3 gw = spiky_grad(rx, ry, rz, h, a.ks);
@@ -532,8 +529,31 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
         }
         rho += pj.w * cubic_kernel(rn, h, a.kc);
         float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
-        aii += (Jx * Jx + Jy * Jy + Jz * Jz) * a.minv[j];
+        aii += (Jx * Jx + Jy * Jy + Jz * Jz) * minv_j;
         jx -= Jx; jy -= Jy; jz -= Jz;
+    };
+    // Two slots in flight: both ids, then both neighbours' (x, m) and 1/m, then the two slots in
+    // order (the accumulation order of a one-slot loop).  Slot k + 1 is read only inside the
+    // wave's tile length (<= Cs); an id past a row's count is read but never used, and the
+    // gathers take a clamped index.
+    for (int k = 0; k < tl; k += 2) {
+        uint32_t* r0 = tbase + (size_t)k * REC;
+        uint32_t* r1 = r0 + (k + 1 < tl ? REC : 0u);
+        const int j0 = (int)r0[0];
+        const int j1 = (int)r1[0];
+        // the empty asm statements keep each pair of loads together (the compiler would sink the
+        // second slot's loads into its conditional use)
+        asm volatile("" ::"v"(j0), "v"(j1));
+        const int c0 = (unsigned)j0 < (unsigned)a.nd ? j0 : 0;
+        const int c1 = (unsigned)j1 < (unsigned)a.nd ? j1 : 0;
+        const float4 p0 = a.pos[c0];
+        const float4 p1 = a.pos[c1];
+        const float m0 = a.minv[c0];
+        const float m1 = a.minv[c1];
+        asm volatile("" ::"v"(p0.x), "v"(p0.y), "v"(p0.z), "v"(p0.w), "v"(m0), "v"(p1.x), "v"(p1.y), "v"(p1.z),
+                     "v"(p1.w), "v"(m1));
+        if (valid && k < cnt) dyn_slot(r0, j0, p0, m0);
+        if (valid && k + 1 < cnt) dyn_slot(r1, j1, p1, m1);
     }
     for (int k = 0; k < ts; ++k) {
         if (!valid || k >= cs) continue;
