@@ -194,7 +194,7 @@ struct ivp_ctx {
 
     // IP solver vectors (dynamic rows)
     float4* d4 = nullptr;   // displacement d, w = m
-    float4* q4 = nullptr;   // (f k, m, dfdt k, 0)
+    float4* q4 = nullptr;   // (f k, dfdt k, m, 0)
     float* b = nullptr;  float* sol = nullptr;  float* r = nullptr;  float* Ap = nullptr;  // 3 * cap
     float4* p4 = nullptr;   // CG direction, w = m
     float2* y2 = nullptr;   // (dfdt k J p, m)

@@ -242,7 +242,7 @@ struct PackArgs {
     const float* kk;
     const float* rho0;
     const int* rowcnt;
-    float4* q4;      // (f k, m, dfdt k, 0) gathered by the rhs pass
+    float4* q4;      // (f k, dfdt k, m, 0) gathered by the rhs pass
     float* sfac;     // dfdt k
     float* jd;       // J d, kept for the divergence of the final d
     double* partials;
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(BLOCK) void k_ip_pack(PackArgs a) {
         float k = a.kk[i];
         float sk = dfdt * k;
         a.sfac[i] = sk;
-        a.q4[i] = make_float4(f * k, di.w, sk, 0.0f);
+        a.q4[i] = make_float4(f * k, sk, di.w, 0.0f);
         float denom = a.rho0[i];
         if (denom < 1e-6f) denom = 1e-6f;
         err = (double)fmaxf(f / denom, 0.0f);
@@ -317,21 +317,22 @@ __global__ __launch_bounds__(BLOCK) void k_ip_rhs(RhsArgs a) {
     const int i = pass_row<G>(blk), g = pass_lane<G>();
     Gate gt = gate_all(i, a.R.nd);
     if (SC && !gate_block<BLOCK / G>(a.sc, blk, i, a.R.nd, PH_RHS, gt)) return;
-    // neighbour record: A = (x, y, z, m), B = (f k, dfdt k); q4 = (f k, m, dfdt k, 0)
+    // neighbour record: A = (x, y, z, m), B = (f k, dfdt k); q4 = (f k, dfdt k, m, 0), so B is
+    // one 8-byte gather
     auto loadA = [&](int j) {
-        return FM::EVAL ? a.R.pos[j] : make_float4(0.0f, 0.0f, 0.0f, a.q4[j].y);
+        return FM::EVAL ? a.R.pos[j] : make_float4(0.0f, 0.0f, 0.0f, a.q4[j].z);
     };
     auto loadB = [&](int j) {
         float4 q = a.q4[j];
         Vec<2> B;
-        B.v[0] = q.x; B.v[1] = q.z;
+        B.v[0] = q.x; B.v[1] = q.y;
         return B;
     };
     double touched = 0.0, tpairs = 0.0;
     if (gt.on) {
         float4 qi = a.q4[i];
-        float fki = qi.x, mi = qi.y;
-        bool act = qi.z != 0.0f;
+        float fki = qi.x, mi = qi.z;
+        bool act = qi.y != 0.0f;
         float rx = 0.0f, ry = 0.0f, rz = 0.0f;
         row_pairs<FM, 2, G, PASS_UNROLL>(a.R, i, a.R.pos[i], loadA, loadB,
                              [&](const float4& Aj, const Vec<2>& Bj, float gx, float gy, float gz) {
