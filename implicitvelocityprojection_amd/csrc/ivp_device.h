@@ -82,6 +82,24 @@ __device__ __forceinline__ int cell_coord(const Grid& g, float x, int a) {
     return c;
 }
 
+// Neighbour records of a 64-row tile: the word of slot k of a row, relative to the tile's first
+// word, is row_word0(lane) + slot_word(k).  Ids-only rows (REC = 64 words per slot) keep slots
+// g and g + 4 of each 8-slot block of a row in adjacent words: block b holds, for row `lane`,
+// words 512 b + 8 lane + [slot 0, 4, 1, 5, 2, 6, 3, 7 of the block], so a lane that walks slots
+// g, g + 4, ... (four lanes per row) reads two ids with one 8-byte load, and the 16 rows x 4
+// lanes of a wave read 512 contiguous bytes.  Formats with per-pair gradient words (REC = 192,
+// 256) keep slot-major records: word k REC + lane, the gradient words 64 and 128 (and 192)
+// words further.
+template <unsigned REC>
+__device__ __forceinline__ uint32_t row_word0(int lane) {
+    return REC == 64u ? 8u * (uint32_t)lane : (uint32_t)lane;
+}
+template <unsigned REC>
+__device__ __forceinline__ uint32_t slot_word(int k) {
+    if (REC == 64u) return ((uint32_t)(k >> 3) << 9) | (uint32_t)(((k & 3) << 1) | ((k >> 2) & 1));
+    return (uint32_t)k * REC;
+}
+
 __device__ __forceinline__ int cell_key(const Grid& g, int cx, int cy, int cz) {
     return g.base + (cz * g.dims[1] + cy) * g.dims[0] + cx;
 }

@@ -344,19 +344,19 @@ __global__ __launch_bounds__(BLOCK) void k_build(BuildArgs a) {
     bool valid = i < a.nd;
     const float h = a.h;
     const int C = a.C, Cs = a.Cs;
-    uint32_t* tbase = a.nb + (size_t)tile * (size_t)Cs * REC + lane;
+    uint32_t* tbase = a.nb + (size_t)tile * (size_t)Cs * REC + row_word0<REC>(lane);
     int cnt = 0, cs = 0, tot = 0;
     float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (valid) {
         pi = a.pos[i];
         const Grid g = SCENES ? a.grids[a.sid[i]] : a.g;
         scan_ids(g, a.dstart, a.pos, i, pi, h, a.h2_lo, a.h2_hi, [&](int j) {
-            if (tot < C) tbase[(size_t)cnt * REC] = (uint32_t)j;
+            if (tot < C) tbase[slot_word<REC>(cnt)] = (uint32_t)j;
             if (tot < C) cnt++;
             tot++;
         });
         scan_ids(g, a.sstart, a.pos + a.nd, -1, pi, h, a.h2_lo, a.h2_hi, [&](int j) {
-            if (tot < C) tbase[(size_t)(Cs - 1 - cs) * REC] = (uint32_t)j;
+            if (tot < C) tbase[slot_word<REC>(Cs - 1 - cs)] = (uint32_t)j;
             if (tot < C) cs++;
             tot++;
         });
@@ -449,23 +449,24 @@ __global__ __launch_bounds__(BLOCK) void k_scan(BuildArgs a) {
     if (i >= a.nd) return;   // the G lanes of a row leave together
     const int C = a.C, Cs = a.Cs;
     // the wave's rows share one 64-row tile (64 / G rows from a multiple of 64 / G); slot k of the
-    // row is word k * REC + lane of the tile, a 32-bit index from the tile's (uniform) base
+    // row is word row_word0(lane) + slot_word(k) of the tile, a 32-bit index from the tile's
+    // (uniform) base
     uint32_t* tnb = a.nb + (size_t)__builtin_amdgcn_readfirstlane(i >> 6) * (size_t)Cs * REC;
-    const uint32_t lane = (uint32_t)(i & (WAVE - 1));
+    const uint32_t w0 = row_word0<REC>(i & (WAVE - 1));
     const float4 pi = a.pos[i];
     const Grid g = SCENES ? a.grids[a.sid[i]] : a.g;
     int tot = 0, cnt = 0, cs = 0;
     // the k-th neighbour found (dynamic ones first) is stored when k < C
     int nd_found = scan_group<G, U>(g, a.dstart, a.pos, i, pi, a.h, a.h2_lo, a.h2_hi, gl,
                                     [&](int j, int rank, int before) {
-                                        if (before + rank < C) tnb[(uint32_t)(before + rank) * REC + lane] = (uint32_t)j;
+                                        if (before + rank < C) tnb[slot_word<REC>(before + rank) + w0] = (uint32_t)j;
                                     });
     cnt = nd_found < C ? nd_found : C;
     tot = nd_found;
     int ns_found = scan_group<G, U>(g, a.sstart, a.pos + a.nd, -1, pi, a.h, a.h2_lo, a.h2_hi, gl,
                                     [&](int j, int rank, int before) {
                                         const int k = nd_found + before + rank;
-                                        if (k < C) tnb[(uint32_t)(Cs - 1 - (before + rank)) * REC + lane] = (uint32_t)j;
+                                        if (k < C) tnb[slot_word<REC>(Cs - 1 - (before + rank)) + w0] = (uint32_t)j;
                                     });
     tot += ns_found;
     cs = (tot < C ? tot : C) - cnt;
@@ -501,7 +502,7 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
     constexpr unsigned REC = rec_words<FMT>();
     const float h = a.h;
     const int Cs = a.Cs;
-    uint32_t* tbase = a.nb + (size_t)tile * (size_t)Cs * REC + lane;
+    uint32_t* tbase = a.nb + (size_t)tile * (size_t)Cs * REC + row_word0<REC>(lane);
     int tl = wave_max(cnt);
     int ts = wave_max(cs);
     int tlp = (tl + ROW_PAD - 1) / ROW_PAD * ROW_PAD;   // <= Cs: tl <= C <= Cs, Cs a multiple of ROW_PAD
@@ -533,14 +534,9 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
         jx -= Jx; jy -= Jy; jz -= Jz;
     };
     // Two slots in flight: both ids, then both neighbours' (x, m) and 1/m, then the two slots in
-    // order (the accumulation order of a one-slot loop).  Slot k + 1 is read only inside the
-    // wave's tile length (<= Cs); an id past a row's count is read but never used, and the
-    // gathers take a clamped index.
-    for (int k = 0; k < tl; k += 2) {
-        uint32_t* r0 = tbase + (size_t)k * REC;
-        uint32_t* r1 = r0 + (k + 1 < tl ? REC : 0u);
-        const int j0 = (int)r0[0];
-        const int j1 = (int)r1[0];
+    // order (the accumulation order of a one-slot loop).  An id past a row's count is read but
+    // never used, and the gathers take a clamped index.
+    auto dyn_pair = [&](uint32_t* r0, uint32_t* r1, int j0, int j1, int k) {
         // the empty asm statements keep each pair of loads together (the compiler would sink the
         // second slot's loads into its conditional use)
         asm volatile("" ::"v"(j0), "v"(j1));
@@ -554,10 +550,32 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
                      "v"(p1.w), "v"(m1));
         if (valid && k < cnt) dyn_slot(r0, j0, p0, m0);
         if (valid && k + 1 < cnt) dyn_slot(r1, j1, p1, m1);
+    };
+    if constexpr (REC == 64u) {
+        // ids-only rows: the row's 8 words of a block (slots 0, 4, 1, 5, 2, 6, 3, 7) with two
+        // 16-byte loads; whole blocks stay inside the tile (Cs is a multiple of 8)
+        for (int b = 0; b < tl; b += 8) {
+            uint32_t* blk = tbase + slot_word<REC>(b);
+            const uint4 w0 = *reinterpret_cast<const uint4*>(blk);
+            const uint4 w1 = *reinterpret_cast<const uint4*>(blk + 4);
+            const int ids[8] = {(int)w0.x, (int)w0.z, (int)w1.x, (int)w1.z,
+                                (int)w0.y, (int)w0.w, (int)w1.y, (int)w1.w};
+#pragma unroll
+            for (int t = 0; t < 8; t += 2) {
+                if (b + t >= tl) break;
+                dyn_pair(nullptr, nullptr, ids[t], ids[t + 1], b + t);
+            }
+        }
+    } else {
+        for (int k = 0; k < tl; k += 2) {
+            uint32_t* r0 = tbase + slot_word<REC>(k);
+            uint32_t* r1 = tbase + slot_word<REC>(k + 1 < tl ? k + 1 : k);
+            dyn_pair(r0, r1, (int)r0[0], (int)r1[0], k);
+        }
     }
     for (int k = 0; k < ts; ++k) {
         if (!valid || k >= cs) continue;
-        int j = (int)tbase[(size_t)(Cs - 1 - k) * REC];
+        int j = (int)tbase[slot_word<REC>(Cs - 1 - k)];
         if ((unsigned)j >= (unsigned)a.ns) {  // never taken: phase A stored a static id here
             atomicOr(a.err_flag, 4);
             continue;
@@ -575,7 +593,7 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
     // [cnt, tlp) follows the wave's longest row and can reach into them.
     for (int k = 0; k < tlp; ++k) {
         if (!valid || k < cnt) continue;
-        uint32_t* rec = tbase + (size_t)k * REC;
+        uint32_t* rec = tbase + slot_word<REC>(k);
         rec[0] = (uint32_t)i;
         if (FMT == IVP_ROWS_F32) {
             rec[64] = 0u;

@@ -177,16 +177,16 @@ struct Vec {
 // Walk the dynamic slots of row i in order, calling f(A_j, B_j, gx, gy, gz) with A_j = loadA(j) =
 // (x, y, z, a_j) and B_j = loadB(j).  The tile length is a multiple of ROW_PAD, and slots past a
 // row's count name the row itself, whose gradW is 0.
-// With G lanes per row, this lane (g) walks slots g, g + G, ... of the row.
+// With G lanes per row, this lane (g) walks slots g, g + G, ... of the row.  With ids-only rows and
+// four lanes per row, slots g + 8 b and g + 4 + 8 b are adjacent words (row_word0 / slot_word),
+// so the lane reads both ids with one 8-byte load; the slots are visited in the same order.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 template <class FM, int NB, int G = 1, int U = 4, class LA, class LB, class F>
 __device__ __forceinline__ void row_pairs(const Rows& R, int i, float4 xi, LA&& loadA, LB&& loadB, F&& f, int g = 0) {
     constexpr unsigned REC = FM::REC;
-    const uint32_t* e = R.nb + (size_t)(i >> 6) * (size_t)R.Cs * REC + (i & (WAVE - 1));
+    const uint32_t* e = R.nb + (size_t)(i >> 6) * (size_t)R.Cs * REC + row_word0<REC>(i & (WAVE - 1));
     const int tl = __builtin_amdgcn_readfirstlane(R.tile_len[i >> 6]);   // one tile per wave
-#pragma unroll(U)
-    for (int k = g; k < tl; k += G) {
-        const uint32_t* r = e + (size_t)k * REC;
-        int j = (int)__builtin_nontemporal_load(r);
+    auto visit = [&](int j, const uint32_t* r) {
         float4 Aj = loadA(j);
         Vec<NB> Bj = loadB(j);
         float gx, gy, gz;
@@ -203,6 +203,23 @@ __device__ __forceinline__ void row_pairs(const Rows& R, int i, float4 xi, LA&& 
             gx = R.gs * half_lo(w1); gy = R.gs * half_hi(w1); gz = R.gs * half_lo(w2);
         }
         f(Aj, Bj, gx, gy, gz);
+    };
+    if constexpr (REC == 64u && G == 4) {
+        // tl is a multiple of ROW_PAD = 8: whole blocks; block b's pair for this lane is the
+        // 8-byte word pair 2 g of the row's 8 words in block b (512 words per block)
+        const u32x2* e2 = reinterpret_cast<const u32x2*>(e + 2 * g);
+#pragma unroll((U + 1) / 2)
+        for (int b = 0; b < tl; b += 8) {
+            const u32x2 ids = __builtin_nontemporal_load(e2 + (b >> 3) * 256);
+            visit((int)ids.x, nullptr);
+            visit((int)ids.y, nullptr);
+        }
+    } else {
+#pragma unroll(U)
+        for (int k = g; k < tl; k += G) {
+            const uint32_t* r = e + slot_word<REC>(k);
+            visit((int)__builtin_nontemporal_load(r), r);
+        }
     }
 }
 
@@ -1197,7 +1214,7 @@ __global__ __launch_bounds__(BLOCK) void k_st_setup(StArgs a) {
         }
         return;
     }
-    const uint32_t* e = a.R.nb + (size_t)(i >> 6) * (size_t)a.R.Cs * REC + (i & (WAVE - 1));
+    const uint32_t* e = a.R.nb + (size_t)(i >> 6) * (size_t)a.R.Cs * REC + row_word0<REC>(i & (WAVE - 1));
     float* eg = a.gt + (size_t)(i >> 6) * (size_t)a.R.Cs * 64u + (i & (WAVE - 1));
     const int tl = __builtin_amdgcn_readfirstlane(a.R.tile_len[i >> 6]);
     const float4 xi = a.R.pos[i];
@@ -1205,7 +1222,7 @@ __global__ __launch_bounds__(BLOCK) void k_st_setup(StArgs a) {
     const float m_i = ui.w, rho_i = a.density[i], div_i = a.div[i];
     float bx = 0.0f, by = 0.0f, bz = 0.0f, diag = 0.0f;
     for (int k = g; k < tl; k += PASS_LANES) {
-        const int j = (int)__builtin_nontemporal_load(e + (size_t)k * REC);
+        const int j = (int)__builtin_nontemporal_load(e + slot_word<REC>(k));
         float gt = 0.0f;
         float4 xj = a.R.pos[j];
         float4 uj = a.u4[j];
@@ -1264,14 +1281,14 @@ __global__ __launch_bounds__(BLOCK) void k_st_Av(StArgs a) {
         const float4 ui = a.u4[i];
         float Mx = ui.x, My = ui.y, Mz = ui.z;
         if (ALLFLUID || a.mat[i] == a.fluid) {
-            const uint32_t* e = a.R.nb + (size_t)(i >> 6) * (size_t)a.R.Cs * REC + (i & (WAVE - 1));
+            const uint32_t* e = a.R.nb + (size_t)(i >> 6) * (size_t)a.R.Cs * REC + row_word0<REC>(i & (WAVE - 1));
             const float* eg = a.gt + (size_t)(i >> 6) * (size_t)a.R.Cs * 64u + (i & (WAVE - 1));
             const int tl = __builtin_amdgcn_readfirstlane(a.R.tile_len[i >> 6]);
             const float m_i = ui.w;
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
 #pragma unroll(PASS_UNROLL)
             for (int k = g; k < tl; k += PASS_LANES) {
-                const int j = (int)__builtin_nontemporal_load(e + (size_t)k * REC);
+                const int j = (int)__builtin_nontemporal_load(e + slot_word<REC>(k));
                 const float gt = __builtin_nontemporal_load(eg + (size_t)k * 64u);
                 float4 uj = a.u4[j];
                 float w = (0.5f * (m_i + uj.w)) * gt;
