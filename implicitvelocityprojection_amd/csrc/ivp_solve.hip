@@ -1215,7 +1215,8 @@ __global__ __launch_bounds__(BLOCK) void k_st_setup(StArgs a) {
         return;
     }
     const uint32_t* e = a.R.nb + (size_t)(i >> 6) * (size_t)a.R.Cs * REC + row_word0<REC>(i & (WAVE - 1));
-    float* eg = a.gt + (size_t)(i >> 6) * (size_t)a.R.Cs * 64u + (i & (WAVE - 1));
+    // the per-pair g-tilde array has one word per slot in the layout of ids-only rows
+    float* eg = a.gt + (size_t)(i >> 6) * (size_t)a.R.Cs * 64u + row_word0<64u>(i & (WAVE - 1));
     const int tl = __builtin_amdgcn_readfirstlane(a.R.tile_len[i >> 6]);
     const float4 xi = a.R.pos[i];
     const float4 ui = a.u4[i];
@@ -1241,7 +1242,7 @@ __global__ __launch_bounds__(BLOCK) void k_st_setup(StArgs a) {
             float term_density = wst / (2.0f * rho_bar * rho_bar) * (div_i + a.div[j]);
             gt = g_ij + a.dt * (term_kernel - term_density);
         }
-        eg[(size_t)k * 64u] = gt;
+        eg[slot_word<64u>(k)] = gt;
         if (use) {
             float m_bar = 0.5f * (m_i + uj.w);
             diag = diag + m_bar * gt;
@@ -1282,19 +1283,34 @@ __global__ __launch_bounds__(BLOCK) void k_st_Av(StArgs a) {
         float Mx = ui.x, My = ui.y, Mz = ui.z;
         if (ALLFLUID || a.mat[i] == a.fluid) {
             const uint32_t* e = a.R.nb + (size_t)(i >> 6) * (size_t)a.R.Cs * REC + row_word0<REC>(i & (WAVE - 1));
-            const float* eg = a.gt + (size_t)(i >> 6) * (size_t)a.R.Cs * 64u + (i & (WAVE - 1));
+            const float* eg = a.gt + (size_t)(i >> 6) * (size_t)a.R.Cs * 64u + row_word0<64u>(i & (WAVE - 1));
             const int tl = __builtin_amdgcn_readfirstlane(a.R.tile_len[i >> 6]);
             const float m_i = ui.w;
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-#pragma unroll(PASS_UNROLL)
-            for (int k = g; k < tl; k += PASS_LANES) {
-                const int j = (int)__builtin_nontemporal_load(e + slot_word<REC>(k));
-                const float gt = __builtin_nontemporal_load(eg + (size_t)k * 64u);
+            auto visit = [&](int j, float gt) {
                 float4 uj = a.u4[j];
                 float w = (0.5f * (m_i + uj.w)) * gt;
                 sx = sx + w * (ui.x - uj.x);
                 sy = sy + w * (ui.y - uj.y);
                 sz = sz + w * (ui.z - uj.z);
+            };
+            if constexpr (REC == 64u && PASS_LANES == 4) {
+                // slots g + 8 b and g + 4 + 8 b: one 8-byte load of ids and one of g-tilde
+                typedef float f32x2 __attribute__((ext_vector_type(2)));
+                const u32x2* e2 = reinterpret_cast<const u32x2*>(e + 2 * g);
+                const f32x2* g2 = reinterpret_cast<const f32x2*>(eg + 2 * g);
+#pragma unroll((PASS_UNROLL + 1) / 2)
+                for (int b = 0; b < tl; b += 8) {
+                    const u32x2 ids = __builtin_nontemporal_load(e2 + (b >> 3) * 256);
+                    const f32x2 gts = __builtin_nontemporal_load(g2 + (b >> 3) * 256);
+                    visit((int)ids.x, gts.x);
+                    visit((int)ids.y, gts.y);
+                }
+            } else {
+#pragma unroll(PASS_UNROLL)
+                for (int k = g; k < tl; k += PASS_LANES)
+                    visit((int)__builtin_nontemporal_load(e + slot_word<REC>(k)),
+                          __builtin_nontemporal_load(eg + slot_word<64u>(k)));
             }
             sx = group_sum<PASS_LANES>(sx);
             sy = group_sum<PASS_LANES>(sy);
