@@ -327,6 +327,24 @@ int prof_collect(ivp_ctx* c, int cg_iters_done, int epoch) {
 
 }  // namespace ivp
 
+// Status of the neighbour build just enqueued: its error flags (and the solver state, whose pair
+// count the byte figures of the following passes use), read with one synchronisation.
+static int build_status(ivp_ctx* c) {
+    IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    IVP_HIP(hipMemcpyAsync(c->state_host, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+    IVP_HIP(hipStreamSynchronize(c->stream));
+    c->build_err = *c->err_host;
+    if (c->build_err & 4) return IVP_ERR_INTERNAL;
+    return (c->build_err & 1) ? IVP_NEIGHBOR_OVERFLOW : IVP_OK;
+}
+
+// A step refuses to start on rows that the last build truncated (or whose build failed a
+// consistency guard): the velocities stay as they are.
+static int rows_usable(const ivp_ctx* c) {
+    if (c->build_err & 4) return IVP_ERR_INTERNAL;
+    return (c->build_err & 1) ? IVP_NEIGHBOR_OVERFLOW : IVP_OK;
+}
+
 static const char* kclass_name(int k) {
     static const char* names[KC_COUNT] = {"sort", "build", "ip_pack", "ip_rhs", "cg_J", "cg_JT", "cg_update", "cg_p",
                                           "visc", "w18_Av", "w18_update", "w18_p", "w18_pre", "cg_refresh", "st_setup", "st_Av", "prj_head", "prj_tail",
@@ -532,10 +550,7 @@ int ivp_neighbors_build(ivp_ctx* c) {
     if (!c) return IVP_ERR_ARG;
     int rc = grid_build(c);
     if (rc != IVP_OK) return rc;
-    IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    IVP_HIP(hipStreamSynchronize(c->stream));
-    if (*c->err_host & 4) return IVP_ERR_INTERNAL;
-    return (*c->err_host & 1) ? IVP_NEIGHBOR_OVERFLOW : IVP_OK;
+    return build_status(c);
 }
 
 int ivp_neighbors_export(ivp_ctx* c, int cache_size, int* count, int* ids, float* gradW) {
@@ -594,7 +609,9 @@ int ivp_step(ivp_ctx* c, const ivp_step_params* prm, ivp_ip_stats* stats, ivp_cg
     if (!c->built || c->S > 0) return IVP_ERR_STATE;
     if (prm->surface_tension_coeff != 0.0f && prm->surface_tension_method != 1) return IVP_ERR_ARG;
     if (prm->pressure_method != 0 && prm->pressure_method != 1) return IVP_ERR_ARG;
-    int rc = glue_gravity(c, prm->gravity, prm->dt);
+    int rc = rows_usable(c);
+    if (rc != IVP_OK) return rc;
+    rc = glue_gravity(c, prm->gravity, prm->dt);
     if (rc != IVP_OK) return rc;
     if (visc) { visc->cg_iters = 0; visc->residual_norm2 = 0.0; }
     if (st) { st->cg_iters = 0; st->residual_norm2 = 0.0; }
@@ -617,7 +634,10 @@ int ivp_step(ivp_ctx* c, const ivp_step_params* prm, ivp_ip_stats* stats, ivp_cg
     if (rc != IVP_OK) return rc;
     rc = glue_advect_boundary(c, prm->dt);
     if (rc != IVP_OK) return rc;
-    return grid_build(c);
+    // the rebuild that ends the step: a row over the capacity is reported by this step
+    rc = grid_build(c);
+    if (rc != IVP_OK) return rc;
+    return build_status(c);
 }
 
 int ivp_set_scenes(ivp_ctx* c, int n_scenes, const ivp_scene* scenes) {
@@ -631,7 +651,9 @@ int ivp_step_scenes(ivp_ctx* c, const ivp_step_params* prm, ivp_scene_stats* sta
     // viscosity and surface tension are not part of the scene step, which solves with IP
     if (prm->viscosity_coeff != 0.0f || prm->surface_tension_coeff != 0.0f) return IVP_ERR_ARG;
     if (prm->pressure_method != 1) return IVP_ERR_ARG;
-    int rc = glue_gravity(c, prm->gravity, prm->dt);
+    int rc = rows_usable(c);
+    if (rc != IVP_OK) return rc;
+    rc = glue_gravity(c, prm->gravity, prm->dt);
     if (rc != IVP_OK) return rc;
     rc = ip_solve_scenes(c, prm->dt, &prm->ip);
     if (rc != IVP_OK) return rc;
@@ -647,6 +669,8 @@ int ivp_step_scenes(ivp_ctx* c, const ivp_step_params* prm, ivp_scene_stats* sta
     rc = glue_advect_boundary(c, prm->dt);
     if (rc != IVP_OK) return rc;
     rc = grid_build(c);
+    if (rc != IVP_OK) return rc;
+    rc = build_status(c);
     if (rc != IVP_OK) return rc;
     return status;
 }

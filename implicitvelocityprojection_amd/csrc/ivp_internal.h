@@ -8,15 +8,20 @@
 //
 // Neighbour rows exist for dynamic particles only and hold their DYNAMIC neighbours; the
 // static neighbours of a row enter the operators only through S_i = sum_static m_j gradW_ij,
-// density and Aii, all accumulated during the build.  Rows are stored in wave tiles:
-// tile t = rows [64t, 64t+64), slot k, component c (0 = neighbour id, 1..3 = gradW xyz when
-// gradients are cached):
-//     nb[((t * Cs + k) * R + c) * 64 + lane],   R = 4 (cached gradW) or 1 (ids only)
-// (Cs = cache size rounded up to ROW_PAD) so every (tile, slot) is one contiguous record (1 KiB
-// or 256 B) read by one wave.  The tile length is the tile's longest row rounded up to ROW_PAD;
-// slots past a row's count hold (id = the row itself, gradW = 0): r_ii = 0 gives gradW = 0
-// either way, which adds exactly 0 to every operator, so the hot loops run a wave-uniform trip
-// count.
+// density and Aii, all accumulated during the build.  Rows are stored in wave tiles: tile t =
+// rows [64t, 64t+64), Cs slots each (Cs = cache size rounded up to ROW_PAD), lane = row - 64t.
+// The word of (tile t, slot k, row lane) is t * Cs * REC + row_word0<REC>(lane) + slot_word<REC>(k)
+// (ivp_device.h), REC = words per slot of a tile:
+//   * ids only (REC = 64): 8-slot blocks of 512 words; block b holds, for row `lane`, the words
+//     512 b + 8 lane + [slot 0, 4, 1, 5, 2, 6, 3, 7 of the block], so a pass lane walking slots
+//     g, g + 4, ... (four lanes per row) reads two ids with one 8-byte load and a wave reads 512
+//     contiguous bytes;
+//   * cached gradW (REC = 256 float32: id, gx, gy, gz; REC = 192 float16: id, (gx, gy), (gz, 0)):
+//     slot-major records, word k REC + c 64 + lane for component c, one contiguous record per
+//     (tile, slot) read by one wave.
+// The tile length is the tile's longest row rounded up to ROW_PAD; slots past a row's count hold
+// (id = the row itself, gradW = 0): r_ii = 0 gives gradW = 0 either way, which adds exactly 0 to
+// every operator, so the hot loops run a wave-uniform trip count.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -101,6 +106,8 @@ struct SceneParams {
 };
 
 constexpr int ROW_PAD = 8;        // tile lengths are padded to a multiple of this many slots
+// slot_word's 8-slot blocks and the paired (g, g + 4) id loads of the pair passes assume both
+static_assert(ROW_PAD % 8 == 0, "ids-only rows are stored in 8-slot blocks");
 constexpr int POS_PAD = 16;       // float4 entries past the last particle of the position buffers: a
                                   // candidate batch of the scan (at most 16 entries) may read past its run
 
@@ -216,7 +223,7 @@ struct ivp_ctx {
     float4* v4 = nullptr;     // (v, rho)
 
     // Jeske23 surface tension (allocated at first use)
-    float* st_gt = nullptr;   // g~ per stored pair, tile layout of the ids: [(t*Cs + k)*64 + lane]
+    float* st_gt = nullptr;   // g~ per stored pair, at the word of its id in the ids-only layout
     float* st_div = nullptr;  // velocity divergence per row
     float* st_b = nullptr;    // extra right-hand side, 3 * cap
 
@@ -235,7 +242,9 @@ struct ivp_ctx {
     int max_blocks = 0;
     ivp::SolverState* state = nullptr;      // device
     ivp::SolverState* state_host = nullptr; // pinned
-    int* err_flag = nullptr;                // device: bit 0 neighbour overflow
+    int* err_flag = nullptr;                // device: bit 0 neighbour overflow, bit 2 a kernel's
+                                            // consistency guard (set by the build)
+    int build_err = 0;                      // err_flag of the last build, as read on the host
     unsigned* speed_max = nullptr;          // device: bits of max |v|^2 (ivp_max_speed2)
     int* err_host = nullptr;                // pinned
     int* count_host = nullptr;              // pinned scratch

@@ -1,6 +1,7 @@
 """Edge cases of the HIP path: empty and all-static particle sets, an isolated particle (free
-fall in closed form), neighbour-row overflow, a fluid count that leaves the last 64-row tile
-ragged (one step against the oracle), and the argument checks of independent scenes."""
+fall in closed form), neighbour-row overflow (at a build, and at the rebuild that ends a step,
+single-scene and batched), a fluid count that leaves the last 64-row tile ragged (one step
+against the oracle), and the argument checks of independent scenes."""
 
 import numpy as np
 import pytest
@@ -88,6 +89,72 @@ def test_neighbour_overflow_is_reported():
     ctx = _ctx(ps.n, cache=8, domain=cfg.domain_size)
     _load(ctx, ps.x, ps.v, ps.is_dynamic)
     assert ctx.neighbors_build() == 3          # IVP_NEIGHBOR_OVERFLOW
+    ctx.close()
+
+
+def _converging(ps, dt, f=0.7):
+    """Velocities that carry the fluid block to f times its size about its centre in one step."""
+    dyn = ps.is_dynamic == 1
+    v = np.zeros_like(ps.x)
+    c = ps.x[dyn].mean(axis=0)
+    v[dyn] = (c - ps.x[dyn]) * np.float32((1.0 - f) / dt)
+    return v.astype(np.float32)
+
+
+# no pressure correction (fixedOuterIters with 0 iterations), so the converging block compresses
+def _no_correction():
+    from implicitvelocityprojection_amd.context import IPParams
+    return IPParams(max_iteration_opt=0, use_fixed_iters=True)
+
+
+def test_overflow_reported_by_the_step_that_overflowed():
+    """The rebuild that ends a step overflows (rows of 37 neighbours at load, 92 after the step,
+    capacity 48): that step returns IVP_NEIGHBOR_OVERFLOW, and the next step refuses to start,
+    leaving the velocities and positions untouched."""
+    cfg = SimConfig.from_dict(synthetic_scene(0, scale=(8, 8, 8), box=(0.32, 0.32, 0.32)))
+    ps = load_particles(cfg)
+    ctx = _ctx(ps.n, cache=48, domain=cfg.domain_size)
+    _load(ctx, ps.x, _converging(ps, 1e-3), ps.is_dynamic)
+    assert ctx.neighbors_build() == 0
+    res, _ = ctx.step(1e-3, G, 0, 0.0, _no_correction())
+    assert res.status == 3                      # IVP_NEIGHBOR_OVERFLOW
+    v0 = ctx.read(1, torch.zeros((ps.n, 3), device="cuda")).cpu().numpy()
+    x0 = ctx.read(0, torch.zeros((ps.n, 3), device="cuda")).cpu().numpy()
+    res, _ = ctx.step(1e-3, G, 0, 0.0, _no_correction())
+    assert res.status == 3
+    v1 = ctx.read(1, torch.zeros((ps.n, 3), device="cuda")).cpu().numpy()
+    x1 = ctx.read(0, torch.zeros((ps.n, 3), device="cuda")).cpu().numpy()
+    assert np.array_equal(v0.view(np.uint32), v1.view(np.uint32))
+    assert np.array_equal(x0.view(np.uint32), x1.view(np.uint32))
+    ctx.close()
+
+
+def test_scene_step_overflow_is_reported():
+    """The same in a context of two independent scenes: the scene step whose rebuild overflows
+    returns IVP_NEIGHBOR_OVERFLOW, and the next one refuses to start."""
+    from implicitvelocityprojection_amd.context import Context
+    cfgs = [SimConfig.from_dict(s) for s in synthetic_batch(n_scenes=2, seed=4, count_range=(2048, 3000))]
+    pss = [load_particles(c) for c in cfgs]
+    n = sum(p.n for p in pss)
+    ctx = Context(float(np.float32(cfgs[0].support_radius)), cfgs[0].domain_start, cfgs[0].domain_size, n,
+                  cache_size=48)
+    ctx.set_scenes([p.n for p in pss], [c.domain_start for c in cfgs], [c.domain_size for c in cfgs])
+    dev = torch.device("cuda")
+    t = lambda a, dt=torch.float32: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)
+    cat = lambda f: np.concatenate([f(c, p) for c, p in zip(cfgs, pss)])
+    m = cat(lambda c, p: particle_masses(c, p)[0])
+    m_inv = cat(lambda c, p: particle_masses(c, p)[1])
+    ctx.set_particles(t(cat(lambda c, p: p.x)), t(cat(lambda c, p: _converging(p, c.dt))), t(m), t(m_inv),
+                      t(cat(lambda c, p: p.density0)), t(cat(lambda c, p: p.is_dynamic), torch.int32),
+                      t(cat(lambda c, p: p.material), torch.int32))
+    assert ctx.neighbors_build() == 0
+    status, _ = ctx.step_scenes(cfgs[0].dt, G, _no_correction())
+    assert status == 3
+    v0 = ctx.read(1, torch.zeros((n, 3), device="cuda")).cpu().numpy()
+    status, _ = ctx.step_scenes(cfgs[0].dt, G, _no_correction())
+    assert status == 3
+    v1 = ctx.read(1, torch.zeros((n, 3), device="cuda")).cpu().numpy()
+    assert np.array_equal(v0.view(np.uint32), v1.view(np.uint32))
     ctx.close()
 
 
