@@ -5,7 +5,8 @@
 * 1M-particle dam break at full size: neighbour counts, density and k bit-exact against the
   oracle's build of the initial state (sets and orders are checked on the small scenes), then
   size-independent properties of three steps: every final density error at or below
-  densityErrorTol, static particles untouched, all values finite.
+  densityErrorTol, static particles untouched, all values finite; and one whole step from the
+  state after 20 steps, stage by stage against the oracle (both row formats).
 """
 
 import numpy as np
@@ -89,4 +90,71 @@ def test_dambreak_1m_full_size():
     assert np.all(np.isfinite(x)) and np.all(np.isfinite(v))
     assert np.array_equal(x[~dyn], x0[~dyn])
     assert np.all(v[~dyn] == 0.0)
+    ctx.close()
+
+
+@pytest.mark.parametrize("cache", [0, 1], ids=["ids_only", "gradW_f32"])
+def test_dambreak_1m_step_matches_oracle(cache):
+    """One whole step of the 1M dam break from the state the GPU reached after 20 steps, stage by
+    stage, each stage of the oracle fed the GPU's input to that stage:
+      gravity + Weiler18 viscosity: velocities within 1e-5, CG iterations within +-1;
+      IP solve: velocities within 1e-5, outer iterations equal, CG iterations within +-1, final
+        density error at or below densityErrorTol, divergence within 1e-4;
+      advection + domain clamp: positions and velocities bitwise;
+      rebuild: density and k bitwise.
+    Stages are fed the GPU's input because the IP's active set (dfdt = [t >= 1e-6], ip.py:165)
+    is a threshold: a Weiler18 output that differs by 6e-8 (relative) moves a few particles of the
+    1M across it, and the oracle's own IP, given the two inputs, then differs by ~1e-3 at those
+    particles -- the float32 algorithm's sensitivity, not a difference of the GPU path."""
+    from implicitvelocityprojection_amd.context import IPParams
+    cfg, ps = scene(2)
+    sim = O.OracleSim(cfg, ps)
+    sim.rebuild()
+    ctx = gpu_context(sim, torch, cache)
+    assert ctx.neighbors_build() == 0
+    kw = ip_kwargs(cfg)
+    for _ in range(20):
+        res, _ = ctx.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, IPParams(**kw))
+        assert res.status == 0
+    n = sim.x.shape[0]
+    rd = lambda f, shp: _np(ctx.read(f, torch.zeros(shp, device="cuda")))
+    sim.x, sim.v = rd(0, (n, 3)), rd(1, (n, 3))
+    sim.rebuild()
+    assert np.array_equal(rd(2, (n,)).view(np.uint32), sim.density.view(np.uint32))
+    dyn = sim.is_dynamic == 1
+    # gravity + Weiler18 (framework.py:400-421, weiler18.py:371-401)
+    ctx.apply_gravity(cfg.gravity, cfg.dt)
+    v1 = O.gravity(sim.v, sim.is_dynamic, sim.g, sim.dt)
+    assert np.array_equal(rd(1, (n, 3)).view(np.uint32), v1.view(np.uint32))
+    it_g = ctx.viscosity_weiler18(cfg.viscosity_coeff, cfg.dt)
+    v2o, it_o = O.weiler18(sim.x, v1, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h,
+                           sim.C, sim.count, sim.ids, sim.gradW, cfg.viscosity_coeff, cfg.dt)
+    v2g = rd(1, (n, 3))
+    assert abs(it_g - it_o) <= 1, (it_g, it_o)
+    assert rel_inf(v2g[dyn], v2o[dyn]) <= V_TOL, rel_inf(v2g[dyn], v2o[dyn])
+    # IP (ip.py:331-404) from the GPU's viscosity output
+    res = ctx.ip_solve(cfg.dt, IPParams(**kw))
+    v3o, div_o, res_o = O.ip_solve(v2g, sim.m, sim.density, sim.density0, sim.is_dynamic, sim.C, sim.count,
+                                   sim.ids, sim.gradW, sim.k, sim.dt, **kw)
+    v3g = rd(1, (n, 3))
+    assert res.status == 0 and res_o.status == O.ORC_OK
+    assert res.outer_iters == res_o.outer_iters, (res.outer_iters, res_o.outer_iters)
+    assert len(res.pcg_iters) == len(res_o.pcg_iters)
+    for a, b in zip(res.pcg_iters, res_o.pcg_iters):
+        assert abs(a - b) <= 1, (res.pcg_iters, res_o.pcg_iters)
+    assert res.final_density_error <= kw["tol_opt"]
+    assert rel_inf(v3g[dyn], v3o[dyn]) <= V_TOL, rel_inf(v3g[dyn], v3o[dyn])
+    assert rel_inf(rd(4, (n,)), div_o) <= 1e-4
+    # advection and the domain clamp (framework.py:117-136, boundary.py:27-91), then the rebuild
+    ctx.advect_boundary(cfg.dt)
+    pad = float(np.float32(cfg.support_radius))
+    x4o, v4o = O.advect(sim.x, v3g, sim.is_dynamic, sim.dt)
+    x4o, v4o = O.enforce_boundary(x4o, v4o, sim.is_dynamic, cfg.domain_size, pad)
+    assert np.array_equal(rd(0, (n, 3)).view(np.uint32), x4o.view(np.uint32))
+    assert np.array_equal(rd(1, (n, 3)).view(np.uint32), v4o.view(np.uint32))
+    assert ctx.neighbors_build() == 0
+    sim.x, sim.v = x4o, v4o
+    sim.rebuild()
+    assert np.array_equal(rd(2, (n,)).view(np.uint32), sim.density.view(np.uint32))
+    assert np.array_equal(rd(3, (n,)).view(np.uint32), sim.k.view(np.uint32))
     ctx.close()

@@ -93,6 +93,42 @@ def test_ip_solve_matches_oracle(step, cache):
 
 
 @CACHE
+@pytest.mark.parametrize("step", [0, 5, 20])
+@pytest.mark.parametrize("tol_pcg", [1e-12, 1e-14])
+def test_ip_solve_deep_cg_matches_oracle(step, tol_pcg, cache):
+    """The float32 CG run to a tight absolute r.r bound (ip.py:319), so every solve takes many
+    iterations (the oracle: 5-6 at step 0's second update, 11-18 at steps 5 and 20): the
+    multi-iteration beta recursion, the chunked device-side stop and the iteration count are
+    compared with the oracle, not only the two iterations the default 1e-6 needs."""
+    sim = oracle_at(0, step)
+    v_in = O.gravity(sim.v, sim.is_dynamic, sim.g, sim.dt)
+    kw = ip_kwargs(sim.cfg)
+    kw.update(tol_pcg=tol_pcg, max_iteration_pcg=200)
+    v_o, div_o, res_o = O.ip_solve(v_in, sim.m, sim.density, sim.density0, sim.is_dynamic, sim.C, sim.count,
+                                   sim.ids, sim.gradW, sim.k, sim.dt, **kw)
+    assert res_o.status == O.ORC_OK
+    assert max(res_o.pcg_iters) >= 5, res_o.pcg_iters
+    ctx = gpu_context(sim, torch, cache)
+    assert ctx.neighbors_build() == 0
+    n = sim.x.shape[0]
+    ctx.write(1, torch.from_numpy(v_in).cuda())
+    from implicitvelocityprojection_amd.context import IPParams
+    res_g = ctx.ip_solve(sim.dt, IPParams(**kw))
+    v_g = _np(ctx.read(1, torch.zeros((n, 3), device="cuda")))
+    div_g = _np(ctx.read(4, torch.zeros(n, device="cuda")))
+    dyn = sim.is_dynamic == 1
+    assert res_g.status == 0
+    assert res_g.outer_iters == res_o.outer_iters
+    assert len(res_g.pcg_iters) == len(res_o.pcg_iters)
+    for a, b in zip(res_g.pcg_iters, res_o.pcg_iters):
+        assert abs(a - b) <= 1, (res_g.pcg_iters, res_o.pcg_iters)
+    assert res_g.final_density_error <= kw["tol_opt"]
+    assert rel_inf(v_g[dyn], v_o[dyn]) <= V_TOL, rel_inf(v_g[dyn], v_o[dyn])
+    assert rel_inf(div_g, div_o) <= 1e-4
+    ctx.close()
+
+
+@CACHE
 def test_full_step_matches_oracle(cache):
     sim = oracle_at(0, 8)
     ctx = gpu_context(sim, torch, cache)
