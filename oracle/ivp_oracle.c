@@ -31,6 +31,12 @@
  *     against brute force.
  *   - tile_sum reduction tree of dot_vec3_tiled: sequential inside each 128-wide tile.
  *   - wp.inverse(mat33) singular case: returns the zero matrix.
+ *
+ * Built a second time with -fopenmp (oracle/_build/libivp_oracle_omp.so) for the all-cores CPU
+ * baseline only: the row loops then run in parallel, the scatter of compute_Ax_part2 uses
+ * atomics (as the reference's wp.atomic_add / atomic_sub do), and the float sums reduce in
+ * per-thread partials.  The in-place (serial-order) explicit viscosity stays serial.  Only the
+ * serial build is the checker; the parallel one changes summation order.
  */
 #include <math.h>
 #include <stdint.h>
@@ -38,6 +44,35 @@
 #include <string.h>
 
 #include "ivp_oracle.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#define ORC_FOR _Pragma("omp parallel for schedule(static)")
+#define ORC_FOR_ROWS _Pragma("omp parallel for schedule(dynamic, 256)")
+#define ORC_FOR_ROWS_MAX_MX _Pragma("omp parallel for schedule(dynamic, 256) reduction(max : mx)")
+#define ORC_FOR_SUM_TOTAL _Pragma("omp parallel for schedule(static) reduction(+ : total)")
+#define ORC_FOR_SUM_TOTAL_COUNT _Pragma("omp parallel for schedule(static) reduction(+ : total, count)")
+#define ORC_ATOMIC _Pragma("omp atomic")
+#define ORC_FOR_ROWS_IF_JACOBI _Pragma("omp parallel for schedule(dynamic, 256) if (jacobi)")
+#else
+#define ORC_FOR
+#define ORC_FOR_ROWS
+#define ORC_FOR_ROWS_MAX_MX
+#define ORC_FOR_SUM_TOTAL
+#define ORC_FOR_SUM_TOTAL_COUNT
+#define ORC_ATOMIC
+#define ORC_FOR_ROWS_IF_JACOBI
+#endif
+
+/* threads the row loops use: 1 in the serial build */
+int orc_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 /* ------------------------------------------------------------------------------------------ */
 /* SPH kernels: scripts/sim/sph_kernel.py                                                      */
@@ -118,6 +153,7 @@ static int cell_key(const orc_grid *g, int cx, int cy, int cz)
 
 void orc_cell_keys(const orc_grid *g, int n, const float *x, int *key)
 {
+    ORC_FOR
     for (int i = 0; i < n; ++i) {
         int cx = cell_coord(g, x[3 * i + 0], 0);
         int cy = cell_coord(g, x[3 * i + 1], 1);
@@ -162,6 +198,7 @@ int orc_build_neighbors(const orc_grid *g, int n, const float *x, const int *is_
     sort_population(n, key, is_dynamic, 0, ncell, sta, sstart);
 
     int mx = 0;
+    ORC_FOR_ROWS_MAX_MX
     for (int i = 0; i < n; ++i) {
         const float *xi = x + 3 * i;
         int cx = cell_coord(g, xi[0], 0), cy = cell_coord(g, xi[1], 1), cz = cell_coord(g, xi[2], 2);
@@ -236,6 +273,7 @@ static int capped(const int *count, int i, int C)
 void orc_density(int n, const float *x, const float *m, const int *is_dynamic, float h, int C,
                  const int *count, const int *ids, float *density)
 {
+    ORC_FOR_ROWS
     for (int i = 0; i < n; ++i) {
         if (is_dynamic[i] == 0) continue;
         const float *xi = x + 3 * i;
@@ -257,6 +295,7 @@ void orc_density(int n, const float *x, const float *m, const int *is_dynamic, f
 void orc_aii(int n, const float *m, const float *m_inv, const int *is_dynamic, int C,
              const int *count, const int *ids, const float *gradW, float *k_out)
 {
+    ORC_FOR_ROWS
     for (int i = 0; i < n; ++i) {
         if (is_dynamic[i] == 0) { k_out[i] = 0.0f; continue; }
         float eps = 1e-3f;
@@ -287,6 +326,7 @@ static void constraint(int n, const float *density, const float *density0, const
 void orc_J(int n, const float *xv, const float *m, const int *is_dynamic, int C, const int *count,
            const int *ids, const float *gradW, float *ret)
 {
+    ORC_FOR_ROWS
     for (int i = 0; i < n; ++i) {
         if (is_dynamic[i] == 0) { ret[i] = 0.0f; continue; }
         const float *vi = xv + 3 * i;
@@ -311,6 +351,7 @@ void orc_J(int n, const float *xv, const float *m, const int *is_dynamic, int C,
 void orc_JT(int n, const float *y, const float *m, const int *is_dynamic, int C, const int *count,
             const int *ids, const float *gradW, float *ret)
 {
+    ORC_FOR_ROWS
     for (int i = 0; i < n; ++i) {
         float *o = ret + 3 * i;
         if (is_dynamic[i] == 0) { o[0] = o[1] = o[2] = 0.0f; continue; }
@@ -336,6 +377,7 @@ void orc_JT(int n, const float *y, const float *m, const int *is_dynamic, int C,
 static double dot_tiled(int n, const float *a, const float *b, const int *is_dynamic)
 {
     float total = 0.0f;
+    ORC_FOR_SUM_TOTAL
     for (int t0 = 0; t0 < n; t0 += 128) {
         float s = 0.0f;
         int t1 = t0 + 128 < n ? t0 + 128 : n;
@@ -350,6 +392,7 @@ static double dot_tiled(int n, const float *a, const float *b, const int *is_dyn
 static double dot_atomic(int n, const float *a, const float *b, const int *is_dynamic)
 {
     float total = 0.0f;
+    ORC_FOR_SUM_TOTAL
     for (int i = 0; i < n; ++i)
         if (is_dynamic[i] != 0) total += dot3(a + 3 * i, b + 3 * i);
     return (double)total;
@@ -360,12 +403,14 @@ static double avg_density_error(int n, float *t, const float *num, const float *
                                 const int *is_dynamic)
 {
     float total = 0.0f, count = 0.0f;
+    ORC_FOR
     for (int i = 0; i < n; ++i) {
         if (is_dynamic[i] == 0) continue;
         float denom = density0[i];
         if (denom < 1e-6f) denom = 1e-6f;
         t[i] = num[i] / denom;
     }
+    ORC_FOR_SUM_TOTAL_COUNT
     for (int i = 0; i < n; ++i) {
         if (is_dynamic[i] == 0) continue;
         total += fmaxf(t[i], 0.0f);
@@ -390,6 +435,7 @@ typedef struct {
 /* compute_Jx_pcg, ip.py:19-60 */
 static void ip_Jx(const ip_op *o, const float *xv)
 {
+    ORC_FOR_ROWS
     for (int i = 0; i < o->n; ++i) {
         o->Jx[i] = 0.0f;
         if (o->is_dynamic[i] == 0) continue;
@@ -418,11 +464,13 @@ static void ip_Jx(const ip_op *o, const float *xv)
 static void ip_Ax(const ip_op *o, const float *p, float *Ax)
 {
     ip_Jx(o, p);
+    ORC_FOR
     for (int i = 0; i < o->n; ++i) {
         float *a = Ax + 3 * i;
         if (o->is_dynamic[i] == 0) { a[0] = a[1] = a[2] = 0.0f; continue; }
         a[0] = o->m[i] * p[3 * i + 0]; a[1] = o->m[i] * p[3 * i + 1]; a[2] = o->m[i] * p[3 * i + 2];
     }
+    ORC_FOR_ROWS
     for (int i = 0; i < o->n; ++i) {
         if (o->is_dynamic[i] == 0) continue;
         if (o->dfdt[i] <= 0.0f) continue;
@@ -435,9 +483,19 @@ static void ip_Ax(const ip_op *o, const float *p, float *Ax)
             float val[3] = {o->m[j] * g[0], o->m[j] * g[1], o->m[j] * g[2]};
             float s = o->omega * Jxi;
             float c[3] = {s * val[0], s * val[1], s * val[2]};
-            Ax[3 * i + 0] += c[0]; Ax[3 * i + 1] += c[1]; Ax[3 * i + 2] += c[2];
+            ORC_ATOMIC
+            Ax[3 * i + 0] += c[0];
+            ORC_ATOMIC
+            Ax[3 * i + 1] += c[1];
+            ORC_ATOMIC
+            Ax[3 * i + 2] += c[2];
             if (o->is_dynamic[j] == 1) {
-                Ax[3 * j + 0] -= c[0]; Ax[3 * j + 1] -= c[1]; Ax[3 * j + 2] -= c[2];
+                ORC_ATOMIC
+                Ax[3 * j + 0] -= c[0];
+                ORC_ATOMIC
+                Ax[3 * j + 1] -= c[1];
+                ORC_ATOMIC
+                Ax[3 * j + 2] -= c[2];
             }
         }
     }
@@ -469,6 +527,7 @@ static int ip_pcg(const ip_op *o, float *x, const float *b, float *r, float *p, 
             if (pAp < 0.0) { *pAp_bad = pAp; return -1 - it; }
             double alpha = rr_old / (pAp > 1e-30 ? pAp : 1e-30);
             float af = (float)alpha;
+            ORC_FOR
             for (int i = 0; i < n; ++i) {
                 if (o->is_dynamic[i] == 0) continue;
                 for (int c = 0; c < 3; ++c) {
@@ -481,6 +540,7 @@ static int ip_pcg(const ip_op *o, float *x, const float *b, float *r, float *p, 
             if ((rr_new < tol_pcg && it >= 2) || it >= max_it) break;
             double beta = rr_new / (rr_old > 1e-30 ? rr_old : 1e-30);
             float bf = (float)beta;
+            ORC_FOR
             for (int i = 0; i < n; ++i) {
                 if (o->is_dynamic[i] == 0) continue;
                 for (int c = 0; c < 3; ++c) p[3 * i + c] = r[3 * i + c] + bf * p[3 * i + c];
@@ -518,6 +578,7 @@ int orc_ip_solve(int n, float *v, const float *m, const float *density, const fl
     st->n_density_errors = 0;
     st->n_pcg = 0;
     float dtf = dt;
+    ORC_FOR
     for (int i = 0; i < n; ++i) {
         if (is_dynamic[i] == 0) continue;
         for (int a = 0; a < 3; ++a) d[3 * i + a] = dtf * v[3 * i + a];
@@ -527,6 +588,7 @@ int orc_ip_solve(int n, float *v, const float *m, const float *density, const fl
     for (;;) {
         /* ip_pack_fused, ip.py:124-171 */
         orc_J(n, d, m, is_dynamic, C, count, ids, gradW, Jx);
+        ORC_FOR
         for (int i = 0; i < n; ++i) {
             if (is_dynamic[i] == 0) { t[i] = 0.0f; f[i] = 0.0f; dfdt[i] = 0.0f; continue; }
             float ti = Jx[i] + c[i];
@@ -540,7 +602,9 @@ int orc_ip_solve(int n, float *v, const float *m, const float *density, const fl
         int converged = err < prm->tol_opt && !prm->use_fixed_iters;
         if ((converged && opt_iter >= 1) || opt_iter >= prm->max_iteration_opt) break;
         /* ip_rhs_fused, ip.py:174-215 */
+        ORC_FOR
         for (int i = 0; i < n; ++i) fk[i] = f[i] * k[i];
+        ORC_FOR
         for (int i = 0; i < n; ++i) {
             float *o3 = rhs + 3 * i;
             if (is_dynamic[i] == 0) { o3[0] = o3[1] = o3[2] = 0.0f; continue; }
@@ -562,6 +626,7 @@ int orc_ip_solve(int n, float *v, const float *m, const float *density, const fl
             }
             o3[0] = prm->omega * res[0]; o3[1] = prm->omega * res[1]; o3[2] = prm->omega * res[2];
         }
+        ORC_FOR
         for (int i = 0; i < n; ++i)
             if (is_dynamic[i] != 0) sol[3 * i] = sol[3 * i + 1] = sol[3 * i + 2] = 0.0f;
         double bad = 0.0;
@@ -574,6 +639,7 @@ int orc_ip_solve(int n, float *v, const float *m, const float *density, const fl
         }
         if (st->n_pcg < st->capacity) st->pcg_iters[st->n_pcg] = it;
         st->n_pcg++;
+        ORC_FOR
         for (int i = 0; i < n; ++i) {
             if (is_dynamic[i] == 0) continue;
             for (int a = 0; a < 3; ++a) d[3 * i + a] = d[3 * i + a] + -1.0f * sol[3 * i + a];
@@ -586,18 +652,21 @@ int orc_ip_solve(int n, float *v, const float *m, const float *density, const fl
     if (status == ORC_OK) {
         /* v = d / dt: scale_vec3 with 1.0/dt computed in double and passed as float32 */
         float inv_dt = (float)(1.0 / (double)dt);
+        ORC_FOR
         for (int i = 0; i < n; ++i) {
             if (is_dynamic[i] == 0) continue;
             for (int a = 0; a < 3; ++a) v[3 * i + a] = inv_dt * d[3 * i + a];
         }
         /* final_density_error, pressure_base.py:438-447 */
         orc_J(n, d, m, is_dynamic, C, count, ids, gradW, Jx);
+        ORC_FOR
         for (int i = 0; i < n; ++i)
             if (is_dynamic[i] != 0) t[i] = Jx[i] + 1.0f * c[i];
         st->final_density_error = avg_density_error(n, t, t, density0, is_dynamic);
         /* compute_divergence, pressure_base.py:449-457 */
         orc_J(n, v, m, is_dynamic, C, count, ids, gradW, Jx);
         if (divergence)
+            ORC_FOR
             for (int i = 0; i < n; ++i) divergence[i] = is_dynamic[i] == 1 ? fmaxf(Jx[i], 0.0f) : 0.0f;
     }
 done:
@@ -801,6 +870,9 @@ void orc_viscosity(int n, const float *x, float *v, const float *m, const float 
         memcpy(copy, v, sizeof(float) * 3 * (size_t)n);
         vin = copy;
     }
+    int jacobi = mode == ORC_VISC_JACOBI;
+    (void)jacobi;
+    ORC_FOR_ROWS_IF_JACOBI
     for (int i = 0; i < n; ++i) {
         if (is_dynamic[i] == 0) continue;
         if (material[i] != material_fluid) continue;
@@ -868,6 +940,7 @@ void orc_inverse33(const float a[9], float out[9])
 /* compute_precond_weiler18, weiler18.py:21-85 */
 static void w18_precond(const w18_op *o, float *Pinv)
 {
+    ORC_FOR_ROWS
     for (int i = 0; i < o->n; ++i) {
         float *P = Pinv + 9 * (size_t)i;
         if (o->is_dynamic[i] == 0 || o->material[i] != o->fluid) {
@@ -903,6 +976,7 @@ static void w18_precond(const w18_op *o, float *Pinv)
 /* compute_Av_weiler18, weiler18.py:103-172 */
 static void w18_Av(const w18_op *o, const float *vin, float *Av)
 {
+    ORC_FOR_ROWS
     for (int i = 0; i < o->n; ++i) {
         float *out = Av + 3 * (size_t)i;
         if (o->is_dynamic[i] == 0 || o->material[i] != o->fluid) {
@@ -946,6 +1020,7 @@ void orc_w18_apply_A(int n, const float *x, const float *m, const float *density
 static void add_vec3(int n, float *ret, const float *v0, float scale, const float *v1,
                      const int *is_dynamic)
 {
+    ORC_FOR
     for (int i = 0; i < n; ++i) {
         if (is_dynamic[i] == 0) continue;
         for (int a = 0; a < 3; ++a) ret[3 * i + a] = v0[3 * i + a] + scale * v1[3 * i + a];
@@ -973,6 +1048,7 @@ int orc_weiler18(int n, const float *x, float *v, const float *m, const float *d
     add_vec3(n, rhs, v, -1.0f, Ap, is_dynamic);
     /* PCG, weiler18.py:298-365 */
     memcpy(r, rhs, sizeof(float) * 3 * (size_t)n);
+    ORC_FOR
     for (int i = 0; i < n; ++i) {
         if (is_dynamic[i] == 0) continue;
         const float *P = Pinv + 9 * (size_t)i;
@@ -993,6 +1069,7 @@ int orc_weiler18(int n, const float *x, float *v, const float *m, const float *d
             double rn = dot_atomic(n, r, r, is_dynamic);
             it++;
             if ((rn < 1e-5 && it >= 2) || it >= 1000) break;
+            ORC_FOR
             for (int i = 0; i < n; ++i) {
                 if (is_dynamic[i] == 0) continue;
                 const float *P = Pinv + 9 * (size_t)i;
@@ -1270,6 +1347,7 @@ int orc_jeske23(int n, const float *x, float *v, const float *m, const float *de
 void orc_gravity(int n, float *v, const int *is_dynamic, const float g[3], float dt)
 {
     float gd[3] = {g[0] * dt, g[1] * dt, g[2] * dt};
+    ORC_FOR
     for (int i = 0; i < n; ++i) {
         if (is_dynamic[i] == 0) continue;
         for (int a = 0; a < 3; ++a) v[3 * i + a] = v[3 * i + a] + gd[a];
@@ -1279,6 +1357,7 @@ void orc_gravity(int n, float *v, const int *is_dynamic, const float g[3], float
 /* advect_position, framework.py:117-136 */
 void orc_advect(int n, float *x, float *x_old, float *v, const int *is_dynamic, float dt)
 {
+    ORC_FOR
     for (int i = 0; i < n; ++i) {
         if (x_old) for (int a = 0; a < 3; ++a) x_old[3 * i + a] = x[3 * i + a];
         if (is_dynamic[i] == 0) v[3 * i] = v[3 * i + 1] = v[3 * i + 2] = 0.0f;
@@ -1290,6 +1369,7 @@ void orc_advect(int n, float *x, float *x_old, float *v, const int *is_dynamic, 
 void orc_enforce_boundary(int n, float *x, float *v, const int *is_dynamic,
                           const float domain_size[3], float padding)
 {
+    ORC_FOR
     for (int i = 0; i < n; ++i) {
         if (is_dynamic[i] == 0) continue;
         float *p = x + 3 * i;

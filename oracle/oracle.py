@@ -16,7 +16,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libivp_oracle.so")
+_OMP_PATH = os.path.join(_HERE, "_build", "libivp_oracle_omp.so")
 _lib = None
+_serial = None
 
 ORC_OK, ORC_NOT_CONVERGED, ORC_NOT_PD = 0, 1, 2
 VISC_INPLACE, VISC_JACOBI = 0, 1
@@ -55,19 +57,34 @@ class OrcPRJStats(ctypes.Structure):
 
 def build(force=False):
     """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
-    if force or not os.path.exists(_LIB_PATH):
+    if force or not os.path.exists(_LIB_PATH) or not os.path.exists(_OMP_PATH):
         subprocess.run(["make", "-s", "-B" if force else "-s"], cwd=_HERE, check=True)
 
 
+def _open(path):
+    if not os.path.exists(path):
+        build()
+    h = ctypes.CDLL(path)
+    h.orc_cubic_kernel.restype = ctypes.c_float
+    h.orc_cubic_kernel.argtypes = [ctypes.c_float, ctypes.c_float]
+    return h
+
+
 def lib():
-    global _lib
+    global _lib, _serial
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
-            build()
-        _lib = ctypes.CDLL(_LIB_PATH)
-        _lib.orc_cubic_kernel.restype = ctypes.c_float
-        _lib.orc_cubic_kernel.argtypes = [ctypes.c_float, ctypes.c_float]
+        _serial = _lib = _open(_LIB_PATH)
     return _lib
+
+
+def use_threads(parallel):
+    """Select the serial checker (False, the default) or the OpenMP build of the same source
+    (True: row loops on every core, atomics in the scatter, per-thread partial sums), which
+    only the all-cores CPU baseline uses.  Returns the thread count of the selected build."""
+    global _lib, _serial
+    lib()
+    _lib = _open(_OMP_PATH) if parallel else _serial
+    return int(_lib.orc_threads())
 
 
 def _p(a):
