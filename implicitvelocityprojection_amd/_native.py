@@ -123,6 +123,8 @@ SIGNATURES = {
                                  _P]),
     "ivp_voxelize": (_I, [_P, _I, _P, _I, ctypes.c_double, _I, _P, ctypes.c_longlong,
                           ctypes.POINTER(ctypes.c_longlong), _P]),
+    "ivp_voxelize_noalloc": (_I, [_P, _I, _P, _I, ctypes.c_double, _I, _P, ctypes.c_longlong,
+                                  ctypes.POINTER(ctypes.c_longlong), _P]),
     "ivp_scene_history": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I), ctypes.POINTER(_I),
                                ctypes.POINTER(_I)]),
 }

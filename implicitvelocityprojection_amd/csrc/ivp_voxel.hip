@@ -21,6 +21,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "ivp_internal.h"
@@ -223,6 +224,42 @@ __global__ __launch_bounds__(BLOCK) void k_vox_emit(const uint8_t* __restrict__ 
     }
 }
 
+// ---- allocation-free variant: kernels over a fixed scratch array ----------------------------
+
+// Level-0 triangles from device copies of the vertex and face arrays (faces checked on the host)
+__global__ __launch_bounds__(BLOCK) void k_vox_tris(const double* __restrict__ verts, const int* __restrict__ faces,
+                                                    long long nf, Tri* __restrict__ out) {
+    long long f = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (f >= nf) return;
+    Tri t;
+    for (int c = 0; c < 3; ++c) {
+        const long long vi = faces[3 * f + c];
+        for (int a = 0; a < 3; ++a) t.v[3 * c + a] = verts[3 * vi + a];
+    }
+    out[f] = t;
+}
+
+// Zero `words` 16-byte words (clears the voxel grid and the counters with vector stores)
+__global__ __launch_bounds__(BLOCK) void k_vox_zero(uint4* __restrict__ p, long long words) {
+    long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < words) p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Exclusive scan of the per-chunk counts in one block: off[k] = sum of count[0..k), *total = sum
+__global__ __launch_bounds__(BLOCK) void k_vox_chunk_scan(const int* __restrict__ count, long long n,
+                                                          long long* __restrict__ off, long long* __restrict__ total) {
+    long long run = 0;
+    for (long long base = 0; base < n; base += BLOCK) {
+        const long long k = base + threadIdx.x;
+        const int c = k < n ? count[k] : 0;
+        int tile;
+        const int ex = block_excl_scan_int(c, &tile);
+        if (k < n) off[k] = run + ex;
+        run += tile;
+    }
+    if (threadIdx.x == 0) *total = run;
+}
+
 struct DevBuf {
     void* p = nullptr;
     ~DevBuf() { if (p) (void)hipFree(p); }
@@ -349,6 +386,182 @@ extern "C" int ivp_voxelize(const double* vertices, int n_vertices, const int* f
     const double ox = (double)g.lo[0] * pitch, oy = (double)g.lo[1] * pitch, oz = (double)g.lo[2] * pitch;
     hipLaunchKernelGGL(k_vox_emit, dim3((unsigned)nchunks), dim3(BLOCK), 0, st, (const uint8_t*)grid, g, nvox,
                        (const long long*)d_coff.p, ox, oy, oz, points);
+    IVP_HIP(hipGetLastError());
+    IVP_HIP(hipStreamSynchronize(st));
+    return IVP_OK;
+}
+
+// ---- ivp_voxelize_noalloc ----------------------------------------------------------------------
+// The same voxelisation with every device buffer carved from one file-scope array, `scratch`, of
+// VOX_SCRATCH_BYTES: no hipMalloc / hipFree, for callers that cannot allocate.  The array's
+// address on each device comes from hipGetSymbolAddress at the first call on that device (the
+// host-side name of a __device__ array is not a device address); kernels, copies and fills only
+// ever see that address.  Layout (256-byte aligned regions): counters, voxel grid, chunk counts
+// and offsets, then the rest split into two halves that hold one subdivision level each (the
+// level being read and the level being written; the vertex and face copies sit in the second
+// half until the level-0 triangles are formed from them).
+constexpr size_t VOX_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[VOX_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int VOX_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[VOX_MAX_DEVICES] = {};
+
+int scratch_base(unsigned char** out) {
+    int dev = 0;
+    IVP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= VOX_MAX_DEVICES) return IVP_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        IVP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return IVP_ERR_HIP;
+        g_scratch_ptr[dev] = (unsigned char*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return IVP_OK;
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+int zero_bytes(void* p, size_t bytes, hipStream_t st) {
+    // regions are 256-byte aligned and sized in whole 16-byte words (align256)
+    const long long words = (long long)(align256(bytes) / 16);
+    if (words == 0) return IVP_OK;
+    hipLaunchKernelGGL(k_vox_zero, dim3(grid_blocks(words)), dim3(BLOCK), 0, st, (uint4*)p, words);
+    IVP_HIP(hipGetLastError());
+    return IVP_OK;
+}
+
+}  // namespace
+
+extern "C" int ivp_voxelize_noalloc(const double* vertices, int n_vertices, const int* faces, int n_faces,
+                                    double pitch, int max_iter, double* points, long long capacity,
+                                    long long* n_points, void* stream) {
+    if (!n_points) return IVP_ERR_ARG;
+    *n_points = 0;
+    if (n_vertices < 0 || n_faces < 0 || !(pitch > 0.0) || !isfinite(pitch) || max_iter < 0 || max_iter > 30)
+        return IVP_ERR_ARG;
+    if (n_faces == 0) return IVP_OK;
+    if (!vertices || !faces) return IVP_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    // grid bounds: the rounded corners of the input triangles (host loop, no copies)
+    long long lo[3] = {LLONG_MAX, LLONG_MAX, LLONG_MAX}, hi[3] = {LLONG_MIN, LLONG_MIN, LLONG_MIN};
+    for (int f = 0; f < n_faces; ++f) {
+        for (int c = 0; c < 3; ++c) {
+            int vi = faces[3 * f + c];
+            if (vi < 0 || vi >= n_vertices) return IVP_ERR_ARG;
+            for (int a = 0; a < 3; ++a) {
+                double x = vertices[3 * vi + a];
+                if (!isfinite(x)) return IVP_ERR_ARG;
+                double q = rint(x / pitch);
+                if (!(fabs(q) < 1e15)) return IVP_ERR_ARG;
+                long long iq = (long long)q;
+                lo[a] = std::min(lo[a], iq);
+                hi[a] = std::max(hi[a], iq);
+            }
+        }
+    }
+    VoxGrid g;
+    g.pitch = pitch;
+    long long nvox = 1;
+    for (int a = 0; a < 3; ++a) {
+        g.lo[a] = lo[a];
+        g.dims[a] = hi[a] - lo[a] + 1;
+        if (g.dims[a] > (1ll << 20)) return IVP_ERR_CAPACITY;
+        nvox *= g.dims[a];
+        if (nvox > (long long)VOX_SCRATCH_BYTES) return IVP_ERR_CAPACITY;
+    }
+    const long long nchunks = (nvox + VOX_CHUNK - 1) / VOX_CHUNK;
+    // carve the scratch array
+    const size_t hdr = 256;   // [0, 8) next-level count, [8, 16) flags, [16, 24) point total
+    const size_t b_grid = align256((size_t)nvox);
+    const size_t b_cnt = align256(sizeof(int) * (size_t)nchunks);
+    const size_t b_off = align256(sizeof(long long) * (size_t)nchunks);
+    const size_t fixed = hdr + b_grid + b_cnt + b_off;
+    if (fixed >= VOX_SCRATCH_BYTES) return IVP_ERR_CAPACITY;
+    const size_t half = ((VOX_SCRATCH_BYTES - fixed) / 2) & ~size_t(255);
+    const unsigned long long tri_cap = half / sizeof(Tri);
+    const size_t b_verts = align256(sizeof(double) * 3 * (size_t)n_vertices);
+    const size_t b_faces = align256(sizeof(int) * 3 * (size_t)n_faces);
+    if ((unsigned long long)n_faces > tri_cap || b_verts + b_faces > half) return IVP_ERR_CAPACITY;
+    unsigned char* base = nullptr;
+    IVP_TRY(scratch_base(&base));
+    unsigned long long* d_count = (unsigned long long*)base;
+    int* flags = (int*)(base + 8);
+    long long* d_total = (long long*)(base + 16);
+    uint8_t* grid = base + hdr;
+    int* d_ccount = (int*)(base + hdr + b_grid);
+    long long* d_coff = (long long*)(base + hdr + b_grid + b_cnt);
+    unsigned char* half_a = base + fixed;
+    unsigned char* half_b = half_a + half;
+    // every call starts from zeroed counters and an empty grid (the array keeps the last call's
+    // contents)
+    IVP_TRY(zero_bytes(base, hdr + b_grid, st));
+    double* d_verts = (double*)half_b;
+    int* d_faces = (int*)(half_b + b_verts);
+    IVP_HIP(hipMemcpyAsync(d_verts, vertices, sizeof(double) * 3 * (size_t)n_vertices, hipMemcpyHostToDevice, st));
+    IVP_HIP(hipMemcpyAsync(d_faces, faces, sizeof(int) * 3 * (size_t)n_faces, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_vox_tris, dim3(grid_blocks(n_faces)), dim3(BLOCK), 0, st, (const double*)d_verts,
+                       (const int*)d_faces, (long long)n_faces, (Tri*)half_a);
+    IVP_HIP(hipGetLastError());
+    const double max_edge = pitch / 2.0;
+    Tri* cur = (Tri*)half_a;
+    Tri* next = (Tri*)half_b;
+    long long n = n_faces;
+    for (int level = 0; n > 0; ++level) {
+        const int mode = level == max_iter ? LV_NO_SPLIT : level + 1 == max_iter ? LV_LAST_CHILDREN : LV_SPLIT;
+        const unsigned long long cap = mode == LV_SPLIT ? tri_cap : 0ull;
+        IVP_TRY(zero_bytes(d_count, sizeof(unsigned long long), st));
+        hipLaunchKernelGGL(k_vox_level, dim3(grid_blocks(n)), dim3(BLOCK), 0, st, (const Tri*)cur, n, max_edge, g,
+                           grid, next, d_count, cap, mode, flags);
+        IVP_HIP(hipGetLastError());
+        unsigned long long nn = 0;
+        int fl = 0;
+        IVP_HIP(hipMemcpyAsync(&nn, d_count, sizeof(nn), hipMemcpyDeviceToHost, st));
+        IVP_HIP(hipMemcpyAsync(&fl, flags, sizeof(int), hipMemcpyDeviceToHost, st));
+        IVP_HIP(hipStreamSynchronize(st));
+        if (fl & 1) return IVP_ERR_INTERNAL;
+        if (nn > 0 && mode != LV_SPLIT) return IVP_ERR_ARG;   // trimesh: "max_iter exceeded"
+        if (fl & 2) return IVP_ERR_CAPACITY;                   // a level larger than half the scratch
+        std::swap(cur, next);
+        n = (long long)nn;
+    }
+    // hole fill
+    hipLaunchKernelGGL(k_vox_border, dim3(grid_blocks(nvox)), dim3(BLOCK), 0, st, grid, g, nvox);
+    IVP_HIP(hipGetLastError());
+    int* changed = flags + 1;
+    for (int round = 0;; ++round) {
+        IVP_TRY(zero_bytes(base, hdr, st));   // the counters (changed among them)
+        for (int axis = 0; axis < 3; ++axis) {
+            long long nl = nvox / g.dims[axis];
+            hipLaunchKernelGGL(k_vox_sweep, dim3(grid_blocks(nl)), dim3(BLOCK), 0, st, grid, g, axis, nl, changed);
+            IVP_HIP(hipGetLastError());
+        }
+        int ch = 0;
+        IVP_HIP(hipMemcpyAsync(&ch, changed, sizeof(int), hipMemcpyDeviceToHost, st));
+        IVP_HIP(hipStreamSynchronize(st));
+        if (!ch) break;
+        if (round > nvox) return IVP_ERR_INTERNAL;
+    }
+    // points: chunk counts, their scan on the device, then the emit pass
+    hipLaunchKernelGGL(k_vox_count, dim3((unsigned)nchunks), dim3(BLOCK), 0, st, (const uint8_t*)grid, nvox,
+                       d_ccount);
+    IVP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_vox_chunk_scan, dim3(1), dim3(BLOCK), 0, st, (const int*)d_ccount, nchunks, d_coff, d_total);
+    IVP_HIP(hipGetLastError());
+    long long total = 0;
+    IVP_HIP(hipMemcpyAsync(&total, d_total, sizeof(total), hipMemcpyDeviceToHost, st));
+    IVP_HIP(hipStreamSynchronize(st));
+    *n_points = total;
+    if (total > capacity) return IVP_ERR_CAPACITY;
+    if (total == 0) return IVP_OK;
+    if (!points) return IVP_ERR_ARG;
+    const double ox = (double)g.lo[0] * pitch, oy = (double)g.lo[1] * pitch, oz = (double)g.lo[2] * pitch;
+    hipLaunchKernelGGL(k_vox_emit, dim3((unsigned)nchunks), dim3(BLOCK), 0, st, (const uint8_t*)grid, g, nvox,
+                       (const long long*)d_coff, ox, oy, oz, points);
     IVP_HIP(hipGetLastError());
     IVP_HIP(hipStreamSynchronize(st));
     return IVP_OK;

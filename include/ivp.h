@@ -288,6 +288,16 @@ int ivp_scene_history(ivp_ctx *ctx, int scene, int capacity, double *density_err
  * hipStream_t) and synchronises it; needs no context. */
 int ivp_voxelize(const double *vertices, int n_vertices, const int *faces, int n_faces, double pitch,
                  int max_iter, double *points, long long capacity, long long *n_points, void *stream);
+/* ivp_voxelize without any device allocation, for callers that cannot allocate device memory
+ * (serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * arguments, result and errors as ivp_voxelize; runs on the current device.  Every device buffer
+ * (voxel grid, subdivision levels, counters, chunk counts) lives in one 64 MiB array that the
+ * library holds on each device it runs on until the process ends: meshes whose voxel grid, or
+ * one subdivision level of triangles still too long (72 B each), do not fit in it return
+ * IVP_ERR_CAPACITY.  All calls on one device share that array, so calls that overlap in time on
+ * one device (other threads, other streams) race: serialise them. */
+int ivp_voxelize_noalloc(const double *vertices, int n_vertices, const int *faces, int n_faces, double pitch,
+                         int max_iter, double *points, long long capacity, long long *n_points, void *stream);
 /* ---- time step and animation -------------------------------------------------------------- */
 /* compute_max_velocity_sq (framework.py:44-58): *out = max over dynamic particles of
  * (vx vx + vy vy) + vz vz in float32 (0 without dynamic particles), from the stored velocities;
