@@ -28,6 +28,8 @@ OPT_SCENE_CG_CHUNK = 2
 OPT_SCENE_GRAPH = 3
 OPT_SCENE_LANES = 4
 OPT_BUILD_LANES = 5
+OPT_W18_WARM_START = 6
+OPT_W18_PRECONDITIONER = 7
 MIN_SCENE_DYNAMIC = 256
 
 # kernel classes of ivp_profile_read / ivp_profile_enable masks
@@ -51,7 +53,8 @@ class IPStats(ctypes.Structure):
                 ("n_pcg", ctypes.c_int), ("outer_iters", ctypes.c_int), ("final_density_error", ctypes.c_double),
                 ("pAp_bad", ctypes.c_double), ("pcg_iter_bad", ctypes.c_int), ("pairs_traversed", ctypes.c_longlong),
                 ("rows_traversed", ctypes.c_longlong), ("cg_launch_chunks", ctypes.c_int),
-                ("solve_ms", ctypes.c_double), ("df_outer_iters", ctypes.c_int), ("df_ms", ctypes.c_double)]
+                ("solve_ms", ctypes.c_double), ("df_outer_iters", ctypes.c_int), ("df_ms", ctypes.c_double),
+                ("iter_costs", ctypes.POINTER(ctypes.c_double)), ("n_iter_costs", ctypes.c_int)]
 
 
 class PRJParams(ctypes.Structure):

@@ -75,6 +75,8 @@ class IPResult:
         self.solve_ms = st.solve_ms
         self.df_outer_iters = st.df_outer_iters
         self.df_ms = st.df_ms
+        costs = getattr(st, "_costs", None)
+        self.iter_costs = [] if costs is None else [float(c) for c in costs[:min(st.n_iter_costs, len(costs))]]
 
     @property
     def last_err(self):
@@ -169,10 +171,13 @@ class Context:
     def _stats(self, cap):
         errs = np.zeros(cap, np.float64)
         its = np.zeros(cap, np.int32)
+        costs = np.zeros(cap, np.float64)
         st = N.IPStats()
         st.capacity = cap
         st.density_errors = errs.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         st.pcg_iters = its.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        st.iter_costs = costs.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        st._costs = costs          # keeps the array alive with the struct
         return st, errs, its
 
     def ip_solve(self, dt, params: IPParams):
@@ -321,6 +326,24 @@ class Context:
     @scene_graph.setter
     def scene_graph(self, on):
         N.check(self.lib.ivp_set_option(self.h, N.OPT_SCENE_GRAPH, int(bool(on))), "ivp_set_option")
+
+    @property
+    def w18_warm_start(self):
+        """Weiler18.warm_start: start the viscosity PCG from the previous solve's correction."""
+        return bool(self._opt(N.OPT_W18_WARM_START))
+
+    @w18_warm_start.setter
+    def w18_warm_start(self, on):
+        N.check(self.lib.ivp_set_option(self.h, N.OPT_W18_WARM_START, int(bool(on))), "ivp_set_option")
+
+    @property
+    def w18_preconditioner(self):
+        """Weiler18.use_preconditioner: block-Jacobi PCG (True) or plain CG (False)."""
+        return bool(self._opt(N.OPT_W18_PRECONDITIONER))
+
+    @w18_preconditioner.setter
+    def w18_preconditioner(self, on):
+        N.check(self.lib.ivp_set_option(self.h, N.OPT_W18_PRECONDITIONER, int(bool(on))), "ivp_set_option")
 
     # -- instrumentation ----------------------------------------------------------------------
     def profile_enable(self, mask=-1):

@@ -73,7 +73,7 @@ void free_all(ivp_ctx* c) {
     dfree(c->d4); dfree(c->q4); dfree(c->b); dfree(c->sol); dfree(c->r); dfree(c->Ap); dfree(c->p4); dfree(c->y2);
     dfree(c->cc); dfree(c->sfac); dfree(c->jd); dfree(c->xs4); dfree(c->nbact);
     dfree(c->w_pinv); dfree(c->w_p4); dfree(c->w_v4); dfree(c->w_dv); dfree(c->w_r); dfree(c->w_z);
-    dfree(c->w_Ap); dfree(c->w_rhs); dfree(c->w_fac); dfree(c->v4);
+    dfree(c->w_Ap); dfree(c->w_rhs); dfree(c->w_fac); dfree(c->v4); dfree(c->w_dv_prev);
     dfree(c->st_gt); dfree(c->st_div); dfree(c->st_b);
     dfree(c->prj_v0); dfree(c->prj_lam); dfree(c->prj_prev); dfree(c->prj_df_prev); dfree(c->prj_hist);
     c->prj_hist_cap = 0;
@@ -91,6 +91,8 @@ void free_all(ivp_ctx* c) {
     c->prof_pending.clear();
     for (auto e : c->prof_free) (void)hipEventDestroy(e);
     c->prof_free.clear();
+    for (auto e : c->iter_ev) (void)hipEventDestroy(e);
+    c->iter_ev.clear();
 }
 
 int alloc_all(ivp_ctx* c) {
@@ -158,6 +160,16 @@ int st_buffers(ivp_ctx* c) {
     if (!c->st_gt) {
         size_t tiles = (size_t)blocks_for(std::max(c->nd, 1)) * WAVES_PER_BLOCK;
         IVP_TRY(dalloc(&c->st_gt, tiles * (size_t)c->Cs * 64u));
+    }
+    return IVP_OK;
+}
+
+// The Weiler18 correction kept for warm starts, in user order; zero until a solve writes it (the
+// reference allocates dv with wp.zeros, weiler18.py:193).
+int w18_buffers(ivp_ctx* c) {
+    if (!c->w_dv_prev) {
+        IVP_TRY(dalloc(&c->w_dv_prev, 3 * (size_t)c->cap));
+        IVP_HIP(hipMemsetAsync(c->w_dv_prev, 0, 3 * (size_t)c->cap * sizeof(float), c->stream));
     }
     return IVP_OK;
 }
@@ -300,6 +312,27 @@ int prof_push(ivp_ctx* c, ProfEvent* ev) {
 void prof_set_bytes(ivp_ctx* c, int epoch, int kclass, double bytes) {
     for (auto& e : c->prof_pending)
         if (e.epoch == epoch && e.kclass == kclass) e.bytes = bytes;
+}
+
+int iter_mark(ivp_ctx* c, int k) {
+    while ((int)c->iter_ev.size() <= k) {
+        hipEvent_t e = nullptr;
+        // read after the stream is synchronised: no system-scope fence needed
+        IVP_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+        c->iter_ev.push_back(e);
+    }
+    IVP_HIP(hipEventRecord(c->iter_ev[(size_t)k], c->stream));
+    return IVP_OK;
+}
+
+void iter_costs(ivp_ctx* c, int n, ivp_ip_stats* st) {
+    st->n_iter_costs = n;
+    if (!st->iter_costs) return;
+    for (int k = 0; k < n && k < st->capacity && k + 1 < (int)c->iter_ev.size(); ++k) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, c->iter_ev[(size_t)k], c->iter_ev[(size_t)k + 1]) != hipSuccess) ms = 0.0f;
+        st->iter_costs[k] = (double)ms;
+    }
 }
 
 int prof_collect(ivp_ctx* c, int cg_iters_done, int epoch) {
@@ -475,6 +508,8 @@ int ivp_reset(ivp_ctx* c) {
     const size_t cap = (size_t)c->cap;
     if (c->prj_prev) IVP_HIP(hipMemsetAsync(c->prj_prev, 0, cap * sizeof(float), c->stream));
     if (c->prj_df_prev) IVP_HIP(hipMemsetAsync(c->prj_df_prev, 0, cap * sizeof(float), c->stream));
+    // Weiler18.reset (weiler18.py:213-220), called by Framework.reset: the kept correction
+    if (c->w_dv_prev) IVP_HIP(hipMemsetAsync(c->w_dv_prev, 0, 3 * cap * sizeof(float), c->stream));
     IVP_HIP(hipStreamSynchronize(c->stream));
     c->built = false;
     return IVP_OK;
@@ -489,6 +524,14 @@ int ivp_set_option(ivp_ctx* c, int option, int value) {
     }
     if (option == IVP_OPT_SCENE_GRAPH) {
         c->scene_graph = value != 0;
+        return IVP_OK;
+    }
+    if (option == IVP_OPT_W18_WARM_START) {
+        c->w18_warm = value != 0;
+        return IVP_OK;
+    }
+    if (option == IVP_OPT_W18_PRECONDITIONER) {
+        c->w18_precond = value != 0;
         return IVP_OK;
     }
     if (option == IVP_OPT_BUILD_LANES) {
@@ -531,6 +574,14 @@ int ivp_get_option(ivp_ctx* c, int option, int* value) {
     }
     if (option == IVP_OPT_SCENE_LANES) {
         *value = c->scene_lanes;
+        return IVP_OK;
+    }
+    if (option == IVP_OPT_W18_WARM_START) {
+        *value = c->w18_warm ? 1 : 0;
+        return IVP_OK;
+    }
+    if (option == IVP_OPT_W18_PRECONDITIONER) {
+        *value = c->w18_precond ? 1 : 0;
         return IVP_OK;
     }
     return IVP_ERR_ARG;

@@ -218,6 +218,10 @@ struct ivp_ctx {
     float* w_dv = nullptr; float* w_r = nullptr; float* w_z = nullptr;
     float* w_Ap = nullptr; float* w_rhs = nullptr;
     float* w_fac = nullptr;   // dt * mu_i / rho_i
+    float* w_dv_prev = nullptr;  // the last solve's correction in USER order (allocated at first
+                                 // use, zeroed by ivp_reset): the warm start's initial guess
+    bool w18_warm = false;       // Weiler18.warm_start
+    bool w18_precond = true;     // Weiler18.use_preconditioner
 
     // explicit viscosity
     float4* v4 = nullptr;     // (v, rho)
@@ -280,6 +284,9 @@ struct ivp_ctx {
     int sgraph_next = 0;
     long long scene_replays = 0;          // replays of the last batched solve
 
+    // stream marks at outer-iteration / pass boundaries (stats_iter_costs)
+    std::vector<hipEvent_t> iter_ev;
+
     // profiling
     bool prof = false;
     unsigned prof_mask = 0xffffffffu;
@@ -299,6 +306,10 @@ int prof_take(ivp_ctx* c, int kclass, int tag, double bytes, ProfEvent* ev);
 int prof_push(ivp_ctx* c, ProfEvent* ev);
 int prof_collect(ivp_ctx* c, int cg_iters_done, int epoch);
 void prof_set_bytes(ivp_ctx* c, int epoch, int kclass, double bytes);
+// stats_iter_costs: mark k on the stream at the start of outer-loop evaluation k (mark n after the
+// last); iter_costs fills the n intervals once the stream has been synchronised
+int iter_mark(ivp_ctx* c, int k);
+void iter_costs(ivp_ctx* c, int n, ivp_ip_stats* st);
 
 // ivp_grid.hip
 int grid_sort_population(ivp_ctx* c, int offset, int count, int* start_table, bool dynamic);
@@ -318,6 +329,7 @@ int visc_explicit(ivp_ctx* c, float coeff, float dt);
 int visc_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* st);
 int st_jeske23(ivp_ctx* c, float sigma, float dt, ivp_cg_stats* st);
 int st_buffers(ivp_ctx* c);   // ivp_api.hip: allocate the Jeske23 buffers on first use
+int w18_buffers(ivp_ctx* c);  // ivp_api.hip: the Weiler18 warm-start buffer, zeroed, on first use
 enum PrjPart { PRJ_BOTH = 0, PRJ_CD = 1, PRJ_DF = 2 };   // solve (+ solve_df if enabled), solve, solve_df
 int prj_solve(ivp_ctx* c, float dt, const ivp_prj_params* prm, ivp_ip_stats* st, int part);
 int prj_buffers(ivp_ctx* c, int passes);   // ivp_api.hip: PRJ buffers, history for `passes`

@@ -924,6 +924,9 @@ struct W18Args {
     double* partials;
     int stride;
     const SolverState* st;
+    int precond;            // use_preconditioner: block-Jacobi blocks, else identity (plain CG)
+    const int* uid;         // user index of each internal row
+    const float* dv_prev;   // the last solve's correction in user order (warm starts)
 };
 
 // (A u)_i = u_i - factor_i * 10 sum_fluid (m_j/rho_j) (u_ij . x_ij) / (r^2 + 0.01 h^2) gradW_ij
@@ -956,7 +959,11 @@ __device__ __forceinline__ float3 w18_Av_row(const W18Args& a, int i, float4 ui,
 // PCG prologue (weiler18.py:320-335: dv = 0, r = rhs, z = P r, p = z, r.z).  The preconditioner
 // block and the operator share each pair's gradient, (m_j / rho_j) and 1 / (r^2 + 0.01 h^2); every
 // accumulator sums the same terms in the same order as it would in a pass of its own.
-template <class FM, bool ALLFLUID>
+// With WARM (warm_start, weiler18.py:311-316) the pass stores the blocks and the right-hand side
+// (a.out) and loads the initial correction dv = dv_prev into dv and p4; r = rhs - A dv follows
+// from the operator pass on p4 and k_w18_warm_init.  Without the preconditioner the blocks are
+// the identity, so z = P r is r exactly (apply_preconditioner's copy, weiler18.py:279-292).
+template <class FM, bool ALLFLUID, bool WARM = false>
 __global__ __launch_bounds__(BLOCK) void k_w18_setup(W18Args a, float4* __restrict__ p4) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = pass_row(blk), g = pass_lane();
@@ -1002,10 +1009,15 @@ __global__ __launch_bounds__(BLOCK) void k_w18_setup(W18Args a, float4* __restri
             float Av0 = vi.x, Av1 = vi.y, Av2 = vi.z;
             if (fl) {
                 const float f = a.fac[i];
-                float M[9];
+                if (a.precond) {
+                    float M[9];
 #pragma unroll
-                for (int q = 0; q < 9; ++q) M[q] = ((q % 4) == 0 ? 1.0f : 0.0f) - f * L[q];
-                inverse33(M, P);
+                    for (int q = 0; q < 9; ++q) M[q] = ((q % 4) == 0 ? 1.0f : 0.0f) - f * L[q];
+                    inverse33(M, P);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) P[q] = (q % 4) == 0 ? 1.0f : 0.0f;
+                }
                 Av0 = vi.x - f * lx; Av1 = vi.y - f * ly; Av2 = vi.z - f * lz;
             } else {
 #pragma unroll
@@ -1015,6 +1027,15 @@ __global__ __launch_bounds__(BLOCK) void k_w18_setup(W18Args a, float4* __restri
 #pragma unroll
             for (int q = 0; q < 9; ++q) Pi[q] = P[q];
             const float r0 = vi.x + -1.0f * Av0, r1 = vi.y + -1.0f * Av1, r2 = vi.z + -1.0f * Av2;
+            if constexpr (WARM) {
+                // rhs = v - A v; dv = the last correction of this particle
+                a.out[3 * i + 0] = r0; a.out[3 * i + 1] = r1; a.out[3 * i + 2] = r2;
+                const float* d = a.dv_prev + 3 * (size_t)a.uid[i];
+                const float d0 = d[0], d1 = d[1], d2 = d[2];
+                a.dv[3 * i + 0] = d0; a.dv[3 * i + 1] = d1; a.dv[3 * i + 2] = d2;
+                p4[i] = make_float4(d0, d1, d2, vi.w);
+                return;
+            }
             const float z0 = P[0] * r0 + P[1] * r1 + P[2] * r2;
             const float z1 = P[3] * r0 + P[4] * r1 + P[5] * r2;
             const float z2 = P[6] * r0 + P[7] * r1 + P[8] * r2;
@@ -1025,8 +1046,43 @@ __global__ __launch_bounds__(BLOCK) void k_w18_setup(W18Args a, float4* __restri
             rz = (double)(r0 * z0 + r1 * z1 + r2 * z2);
         }
     }
+    if constexpr (!WARM) {
+        double v[1] = {rz};
+        block_partials<1>(v, a.partials, a.stride);
+    }
+}
+
+// Warm-start prologue (weiler18.py:311-335): r = rhs - A dv (a.out = A dv), z = P r, p = z,
+// r.z partials; dv keeps the loaded correction
+__global__ __launch_bounds__(BLOCK) void k_w18_warm_init(W18Args a, const float* __restrict__ rhs,
+                                                         float4* __restrict__ p4) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    double rz = 0.0;
+    if (i < a.R.nd) {
+        const float* P = a.pinv + 9 * (size_t)i;
+        float r0 = rhs[3 * i + 0] + -1.0f * a.out[3 * i + 0];
+        float r1 = rhs[3 * i + 1] + -1.0f * a.out[3 * i + 1];
+        float r2 = rhs[3 * i + 2] + -1.0f * a.out[3 * i + 2];
+        float z0 = P[0] * r0 + P[1] * r1 + P[2] * r2;
+        float z1 = P[3] * r0 + P[4] * r1 + P[5] * r2;
+        float z2 = P[6] * r0 + P[7] * r1 + P[8] * r2;
+        a.r[3 * i + 0] = r0; a.r[3 * i + 1] = r1; a.r[3 * i + 2] = r2;
+        a.z[3 * i + 0] = z0; a.z[3 * i + 1] = z1; a.z[3 * i + 2] = z2;
+        p4[i] = make_float4(z0, z1, z2, a.v4[i].w);
+        rz = (double)(r0 * z0 + r1 * z1 + r2 * z2);
+    }
     double v[1] = {rz};
     block_partials<1>(v, a.partials, a.stride);
+}
+
+// The correction of this solve in user order, the next warm start's initial guess (the
+// reference's self.dv, indexed by particle)
+__global__ __launch_bounds__(BLOCK) void k_w18_save(int nd, const float* __restrict__ dv, const int* __restrict__ uid,
+                                                    float* __restrict__ dv_prev) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nd) return;
+    float* d = dv_prev + 3 * (size_t)uid[i];
+    d[0] = dv[3 * i + 0]; d[1] = dv[3 * i + 1]; d[2] = dv[3 * i + 2];
 }
 
 // PCG prologue (weiler18.py:320-335): dv = 0, r = rhs, z = P r, p = z, rz = r.z
@@ -1049,10 +1105,11 @@ __global__ __launch_bounds__(BLOCK) void k_w18_init(W18Args a, float4* __restric
     block_partials<1>(v, a.partials, a.stride);
 }
 
-// Ap = A p, p.Ap (weiler18.py:337-338)
-template <class FM, bool ALLFLUID>
+// Ap = A p, p.Ap (weiler18.py:337-338).  Without CHECK (the warm-start residual) it runs
+// regardless of the solver state.
+template <class FM, bool ALLFLUID, bool CHECK = true>
 __global__ __launch_bounds__(BLOCK) void k_w18_Av(W18Args a) {
-    if (a.st->active == 0) return;
+    if (CHECK && a.st->active == 0) return;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = pass_row(blk), g = pass_lane();
     auto loadA = [&](int j) { return w18_record<ALLFLUID>(a.R.pos, a.v4, a.mat, a.fluid, j); };
@@ -1573,6 +1630,7 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
                    (const float*)c->vel, (const float4*)c->pos, c->cc, d4));
     int opt_iter = 0;
     double err = 0.0;
+    IVP_TRY(iter_mark(c, 0));
     for (;;) {
         PackArgs pa{R, d4, c->Ss, c->cc, c->kk, c->rho0, c->rowcnt, c->q4, c->sfac, c->jd, c->partials, stride};
         double total_pairs = (double)c->state_host->total_pairs;
@@ -1586,7 +1644,10 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
         if (st.n_density_errors < st.capacity && st.density_errors) st.density_errors[st.n_density_errors] = err;
         st.n_density_errors++;
         bool converged = err < prm->tol_opt && !prm->use_fixed_iters;
-        if ((converged && opt_iter >= 1) || opt_iter >= prm->max_iteration_opt) break;
+        if ((converged && opt_iter >= 1) || opt_iter >= prm->max_iteration_opt) {
+            IVP_TRY(iter_mark(c, opt_iter + 1));
+            break;
+        }
         long long act_pairs = c->state_host->active_pairs;
         int act_rows = c->state_host->n_active_rows;
 
@@ -1661,6 +1722,7 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
         st.n_pcg++;
         IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_dupdate<false>, nb, nd, d4, (const float*)c->sol, SceneRef{}));
         opt_iter++;
+        IVP_TRY(iter_mark(c, opt_iter));
     }
     st.outer_iters = opt_iter;
     if (!prm->use_fixed_iters && err >= prm->tol_opt && opt_iter >= prm->max_iteration_opt) return IVP_NOT_CONVERGED;
@@ -1697,11 +1759,24 @@ int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
     const double pairs = (double)c->state_host->total_pairs;
     const double pb = pair_bytes<FM>();
     W18Args wa{R, c->cfg.material_fluid, c->w_fac, c->mat, c->w_pinv, c->w_v4, c->w_rhs,
-               c->w_dv, c->w_r, c->w_z, c->partials, stride, c->state};
-    // one pair pass: preconditioner blocks, right-hand side and the PCG prologue
-    IVP_TRY(launch(c, KC_W18_PRE, -1, pb * pairs + (16.0 + 4.0 + 36.0 + 12.0 * 3 + 16.0) * nd,
-                   k_w18_setup<FM, ALLFLUID>, pass_grid(nd), wa, c->w_p4));
-    IVP_TRY(finalize<FIN_W18_INIT>(c, pass_grid(nd).x, 1e-5, 1000));
+               c->w_dv, c->w_r, c->w_z, c->partials, stride, c->state, c->w18_precond ? 1 : 0, c->uid,
+               c->w_dv_prev};
+    if (!c->w18_warm) {
+        // one pair pass: preconditioner blocks, right-hand side and the PCG prologue
+        IVP_TRY(launch(c, KC_W18_PRE, -1, pb * pairs + (16.0 + 4.0 + 36.0 + 12.0 * 3 + 16.0) * nd,
+                       k_w18_setup<FM, ALLFLUID>, pass_grid(nd), wa, c->w_p4));
+    } else {
+        // warm start: blocks, rhs and dv = the last correction; then r = rhs - A dv
+        IVP_TRY(launch(c, KC_W18_PRE, -1, pb * pairs + (16.0 + 4.0 + 36.0 + 12.0 * 3 + 16.0) * nd,
+                       k_w18_setup<FM, ALLFLUID, true>, pass_grid(nd), wa, c->w_p4));
+        W18Args res = wa;
+        res.v4 = c->w_p4;
+        res.out = c->w_Ap;
+        IVP_TRY(launch(c, KC_W18_PRE, -1, pb * pairs + 48.0 * nd, k_w18_Av<FM, ALLFLUID, false>, pass_grid(nd),
+                       res));
+        IVP_TRY(launch(c, KC_W18_PRE, -1, 0.0, k_w18_warm_init, nb, res, (const float*)c->w_rhs, c->w_p4));
+    }
+    IVP_TRY(finalize<FIN_W18_INIT>(c, c->w18_warm ? nb.x : pass_grid(nd).x, 1e-5, 1000));
     // the first CG chunk follows without waiting (its kernels return at once when r.z is tiny)
     int iters = 0, chunks = 0;
     {
@@ -1726,6 +1801,8 @@ int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
         c->w18_guess = std::max(2, iters);
     }
     *iters_out = iters;
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_w18_save, nb, nd, (const float*)c->w_dv, (const int*)c->uid,
+                   c->w_dv_prev));
     return launch(c, KC_GLUE, -1, 0.0, k_w18_apply, dim3(blocks_for(3 * nd)), 3 * nd, (const float*)c->w_dv, c->vel);
 }
 
@@ -1806,6 +1883,7 @@ int prj_loop(ivp_ctx* c, const ivp_prj_params* prm, double tol, int* passes, dou
     const int total = prm->max_iteration_opt + 1;
     int issued = 0;
     int chunk = std::max(2, std::min(c->prj_guess, 128));
+    if (!DF) IVP_TRY(iter_mark(c, 0));
     for (;;) {
         const int todo = std::min(chunk, total - issued);
         if (todo <= 0) break;
@@ -1815,6 +1893,7 @@ int prj_loop(ivp_ctx* c, const ivp_prj_params* prm, double tol, int* passes, dou
             IVP_TRY(launch(c, KC_PRJ_TAIL, tag, bT, k_prj_tail<FM, DF>, np, pa));
             hipLaunchKernelGGL(k_fin_prj, dim3(1), dim3(FIN_BLOCK), 0, c->stream, fa);
             IVP_HIP(hipGetLastError());
+            if (!DF) IVP_TRY(iter_mark(c, tag + 1));
         }
         issued += todo;
         IVP_TRY(read_state(c));
@@ -1867,12 +1946,14 @@ int prj_solve_body(ivp_ctx* c, float dt, const ivp_prj_params* prm, ivp_ip_stats
     c->prj_guess = std::max(2, passes + 1);
     if (!prm->use_fixed_iters && err >= prm->tol_opt && passes >= prm->max_iteration_opt) {
         IVP_HIP(hipStreamSynchronize(c->stream));
+        iter_costs(c, n_err, &st);
         return IVP_NOT_CONVERGED;
     }
     // v = dt v / dt and the divergence max(J v, 0) from J (dt v) of the last pass
     IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_finish, nb, nd, inv_dt, (const float4*)c->d4, (const float*)c->jd,
                    c->vel, c->divg));
     IVP_HIP(hipStreamSynchronize(c->stream));
+    iter_costs(c, n_err, &st);
     st.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return IVP_OK;
 }
@@ -1957,6 +2038,7 @@ int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats
     st.cg_launch_chunks = 0;
     st.df_outer_iters = 0;
     st.df_ms = 0.0;
+    st.n_iter_costs = 0;
     auto t0 = std::chrono::steady_clock::now();
     int rc;
     switch (c->row_fmt) {
@@ -1964,7 +2046,10 @@ int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats
     case IVP_ROWS_F16: rc = ip_solve_body<FmtF16Eval, FmtF16>(c, dt, prm, stats); break;
     default: rc = ip_solve_body<FmtIds, FmtIds>(c, dt, prm, stats); break;
     }
-    if (rc >= 0) IVP_HIP(hipStreamSynchronize(c->stream));
+    if (rc >= 0) {
+        IVP_HIP(hipStreamSynchronize(c->stream));
+        if (c->nd > 0) iter_costs(c, st.n_density_errors, stats);
+    }
     stats->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
 }
@@ -1990,6 +2075,7 @@ int prj_solve(ivp_ctx* c, float dt, const ivp_prj_params* prm, ivp_ip_stats* sta
     st.solve_ms = 0.0;
     st.df_outer_iters = 0;
     st.df_ms = 0.0;
+    st.n_iter_costs = 0;
     IVP_TRY(prj_buffers(c, prm->max_iteration_opt + 1));
     // a row that overflowed the neighbour capacity at the last build (the IP solve checks the
     // same flag in its first constraint pass)
@@ -2023,6 +2109,7 @@ int visc_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* out) {
     if (!c->built) return IVP_ERR_STATE;
     if (out) { out->cg_iters = 0; out->residual_norm2 = 0.0; }
     if (c->nd == 0) return IVP_OK;
+    IVP_TRY(w18_buffers(c));
     int iters = 0, rc;
     switch (c->row_fmt) {
     case IVP_ROWS_F32:

@@ -103,6 +103,8 @@ class Framework:
             else:
                 self.dt = self.dt_default
             self.dt = max(self.dt, self.cfl_min_dt)
+        if isinstance(self.viscosity, Weiler18):
+            self.viscosity.apply_options(ps.ctx)
         t0 = time.perf_counter()
         res, vit = ps.ctx.step(self.dt, self.g, self.cfg.viscosity_method, self.viscosity_coeff,
                                self.pressure.params(), self.surface_tension_method, self.surface_tension_coeff)

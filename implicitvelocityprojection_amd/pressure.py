@@ -81,6 +81,7 @@ class IP(PressureBase):
         self.last_result = res
         self.reset_stats()
         self.stats_density_errors = list(res.density_errors)
+        self.stats_iter_costs = list(res.iter_costs)
         self.stats_pcg_iters = list(res.pcg_iters)
         self.stats_outer_iters = res.outer_iters
         self.stats_total_cost_ms = res.solve_ms
@@ -130,6 +131,7 @@ class PRJ(PressureBase):
         self.last_result = res
         self.reset_stats()
         self.stats_density_errors = list(res.density_errors)
+        self.stats_iter_costs = list(res.iter_costs)
         self.outer_iter = res.outer_iters
         self.stats_outer_iters = res.outer_iters
         self.stats_total_cost_ms = res.solve_ms

@@ -73,6 +73,11 @@ typedef struct {
     double solve_ms;             /* host wall time of the solve, synchronisations included */
     int df_outer_iters;          /* PRJ: stats_df_outer_iters of the divergence-free pass   */
     double df_ms;                /* PRJ: stats_df_total_cost_ms                             */
+    double *iter_costs;          /* stats_iter_costs (ip.py:369, 392; prj.py:189): ms per
+                                    outer-loop evaluation (IP) or pass (PRJ), one per density
+                                    error, timed by events on the stream; caller-owned, at most
+                                    `capacity` entries, NULL to skip                          */
+    int n_iter_costs;
 } ivp_ip_stats;
 /* ivp_ip_stats holds PressureBase's statistics (pressure_base.py:332-339), shared by both
  * solvers: PRJ leaves pcg_iters empty (n_pcg = 0) and fills the df_ fields. */
@@ -198,7 +203,8 @@ int ivp_prj_solve_df(ivp_ctx *ctx, float dt, const ivp_prj_params *prm, ivp_ip_s
 /* Viscosity.solve (viscosity.py:134-164) with out-of-place (Jacobi) reads. */
 int ivp_viscosity_explicit(ivp_ctx *ctx, float coeff, float dt);
 
-/* Weiler18.solve (weiler18.py:371-401), block-Jacobi PCG, cold start. */
+/* Weiler18.solve (weiler18.py:371-401): block-Jacobi PCG from dv = 0, or as the options
+ * IVP_OPT_W18_WARM_START / IVP_OPT_W18_PRECONDITIONER select. */
 int ivp_viscosity_weiler18(ivp_ctx *ctx, float coeff, float dt, ivp_cg_stats *stats);
 
 /* Jeske23.solve (jeske23.py:590-655): implicit surface tension, diagonal-preconditioned CG on
@@ -273,6 +279,14 @@ int ivp_scene_history(ivp_ctx *ctx, int scene, int capacity, double *density_err
  * count; 1 one lane per row, fused with the gradient/density/Aii pass; 2, 4, 8 or 16 lanes per row
  * in a separate scan (slots filled in the same order).  The rows are identical either way. */
 #define IVP_OPT_BUILD_LANES 5
+/* Weiler18's mutable solver attributes, read at every Weiler18 solve (the reference's GUI toggles
+ * them live, gui.py:429-432): IVP_OPT_W18_WARM_START 1 starts the PCG from the previous solve's
+ * correction dv, r_0 = b - A dv (warm_start, weiler18.py:311-316, 385-388; default 0: dv = 0);
+ * the correction is kept per particle through re-sorts and zeroed by ivp_reset (Weiler18.reset).
+ * IVP_OPT_W18_PRECONDITIONER 0 runs plain CG, z = r (use_preconditioner, weiler18.py:279-292,
+ * 382-383; default 1: the 3x3 block-Jacobi preconditioner). */
+#define IVP_OPT_W18_WARM_START 6
+#define IVP_OPT_W18_PRECONDITIONER 7
 
 /* ---- scene loading -------------------------------------------------------------------------- */
 /* SceneLoader.load_mesh (scene_loader.py:547-580): the voxel centres of the solid bounded by a

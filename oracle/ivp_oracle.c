@@ -1027,27 +1027,15 @@ static void add_vec3(int n, float *ret, const float *v0, float scale, const floa
     }
 }
 
-/* Weiler18.solve + Weiler18.PCG, weiler18.py:298-401 (cold start, preconditioner on).
- * Returns the CG iteration count. */
-int orc_weiler18(int n, const float *x, float *v, const float *m, const float *density,
-                 const float *density0, const int *material, const int *is_dynamic, float h, int C,
-                 const int *count, const int *ids, const float *gradW, float coeff, float dt,
-                 int material_fluid)
+/* z = M^-1 r (apply_precond_weiler18), or z = r (wp.copy) without the preconditioner:
+ * weiler18.py:279-292 */
+static void w18_apply_precond(int n, const float *Pinv, const float *r, float *z, const int *is_dynamic,
+                              int precond)
 {
-    size_t n3 = 3 * (size_t)(n > 0 ? n : 1);
-    w18_op o = {n, C, x, m, density, density0, material, is_dynamic, count, ids, gradW, h, coeff, dt, material_fluid};
-    float *Pinv = (float *)calloc(9 * (size_t)(n > 0 ? n : 1), sizeof(float));
-    float *dv = (float *)calloc(n3, sizeof(float));
-    float *r = (float *)calloc(n3, sizeof(float));
-    float *z = (float *)calloc(n3, sizeof(float));
-    float *p = (float *)calloc(n3, sizeof(float));
-    float *Ap = (float *)calloc(n3, sizeof(float));
-    float *rhs = (float *)calloc(n3, sizeof(float));
-    w18_precond(&o, Pinv);
-    w18_Av(&o, v, Ap);
-    add_vec3(n, rhs, v, -1.0f, Ap, is_dynamic);
-    /* PCG, weiler18.py:298-365 */
-    memcpy(r, rhs, sizeof(float) * 3 * (size_t)n);
+    if (!precond) {
+        memcpy(z, r, sizeof(float) * 3 * (size_t)n);
+        return;
+    }
     ORC_FOR
     for (int i = 0; i < n; ++i) {
         if (is_dynamic[i] == 0) continue;
@@ -1055,6 +1043,37 @@ int orc_weiler18(int n, const float *x, float *v, const float *m, const float *d
         const float *ri = r + 3 * i;
         for (int a = 0; a < 3; ++a) z[3 * i + a] = P[3 * a] * ri[0] + P[3 * a + 1] * ri[1] + P[3 * a + 2] * ri[2];
     }
+}
+
+/* Weiler18.solve + Weiler18.PCG, weiler18.py:298-401.  warm: start from dv (the previous solve's
+ * correction, weiler18.py:311-316, r_0 = b - A dv), else from dv = 0 (weiler18.py:387-388,
+ * r_0 = b); precond: block-Jacobi preconditioner (use_preconditioner) or plain CG.  dv is in/out
+ * (3n floats, the solver's self.dv).  Returns the CG iteration count. */
+int orc_weiler18_ex(int n, const float *x, float *v, const float *m, const float *density,
+                    const float *density0, const int *material, const int *is_dynamic, float h, int C,
+                    const int *count, const int *ids, const float *gradW, float coeff, float dt,
+                    int material_fluid, int warm, int precond, float *dv)
+{
+    size_t n3 = 3 * (size_t)(n > 0 ? n : 1);
+    w18_op o = {n, C, x, m, density, density0, material, is_dynamic, count, ids, gradW, h, coeff, dt, material_fluid};
+    float *Pinv = (float *)calloc(9 * (size_t)(n > 0 ? n : 1), sizeof(float));
+    float *r = (float *)calloc(n3, sizeof(float));
+    float *z = (float *)calloc(n3, sizeof(float));
+    float *p = (float *)calloc(n3, sizeof(float));
+    float *Ap = (float *)calloc(n3, sizeof(float));
+    float *rhs = (float *)calloc(n3, sizeof(float));
+    if (precond) w18_precond(&o, Pinv);
+    if (!warm) memset(dv, 0, sizeof(float) * 3 * (size_t)n);
+    w18_Av(&o, v, Ap);
+    add_vec3(n, rhs, v, -1.0f, Ap, is_dynamic);
+    /* PCG, weiler18.py:298-365 */
+    if (warm) {
+        w18_Av(&o, dv, Ap);
+        add_vec3(n, r, rhs, -1.0f, Ap, is_dynamic);
+    } else {
+        memcpy(r, rhs, sizeof(float) * 3 * (size_t)n);
+    }
+    w18_apply_precond(n, Pinv, r, z, is_dynamic, precond);
     memcpy(p, z, sizeof(float) * 3 * (size_t)n);
     double rz_old = dot_atomic(n, r, z, is_dynamic);
     int it = 0;
@@ -1069,13 +1088,7 @@ int orc_weiler18(int n, const float *x, float *v, const float *m, const float *d
             double rn = dot_atomic(n, r, r, is_dynamic);
             it++;
             if ((rn < 1e-5 && it >= 2) || it >= 1000) break;
-            ORC_FOR
-            for (int i = 0; i < n; ++i) {
-                if (is_dynamic[i] == 0) continue;
-                const float *P = Pinv + 9 * (size_t)i;
-                const float *ri = r + 3 * i;
-                for (int a = 0; a < 3; ++a) z[3 * i + a] = P[3 * a] * ri[0] + P[3 * a + 1] * ri[1] + P[3 * a + 2] * ri[2];
-            }
+            w18_apply_precond(n, Pinv, r, z, is_dynamic, precond);
             double rz_new = dot_atomic(n, r, z, is_dynamic);
             double beta = rz_new / (rz_old > 1e-12 ? rz_old : 1e-12);
             add_vec3(n, p, z, (float)beta, p, is_dynamic);
@@ -1083,7 +1096,20 @@ int orc_weiler18(int n, const float *x, float *v, const float *m, const float *d
         }
     }
     add_vec3(n, v, v, 1.0f, dv, is_dynamic);
-    free(Pinv); free(dv); free(r); free(z); free(p); free(Ap); free(rhs);
+    free(Pinv); free(r); free(z); free(p); free(Ap); free(rhs);
+    return it;
+}
+
+/* the reference's defaults: cold start, block-Jacobi preconditioner */
+int orc_weiler18(int n, const float *x, float *v, const float *m, const float *density,
+                 const float *density0, const int *material, const int *is_dynamic, float h, int C,
+                 const int *count, const int *ids, const float *gradW, float coeff, float dt,
+                 int material_fluid)
+{
+    float *dv = (float *)calloc(3 * (size_t)(n > 0 ? n : 1), sizeof(float));
+    int it = orc_weiler18_ex(n, x, v, m, density, density0, material, is_dynamic, h, C, count, ids, gradW, coeff,
+                             dt, material_fluid, 0, 1, dv);
+    free(dv);
     return it;
 }
 

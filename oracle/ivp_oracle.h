@@ -102,6 +102,10 @@ int orc_weiler18(int n, const float *x, float *v, const float *m, const float *d
                  const float *density0, const int *material, const int *is_dynamic, float h, int C,
                  const int *count, const int *ids, const float *gradW, float coeff, float dt,
                  int material_fluid);
+int orc_weiler18_ex(int n, const float *x, float *v, const float *m, const float *density,
+                    const float *density0, const int *material, const int *is_dynamic, float h, int C,
+                    const int *count, const int *ids, const float *gradW, float coeff, float dt,
+                    int material_fluid, int warm, int precond, float *dv);
 float orc_wst(float r_ij, float d, float h_st);
 void orc_grad_wst(const float x_ij[3], float r_ij, float d, float h_st, float out[3]);
 void orc_jeske23_setup(int n, const float *x, const float *v, const float *m, const float *density,

@@ -296,12 +296,20 @@ def w18_apply(x, m, dens, density0, material, is_dynamic, h, C, count, ids, grad
 
 
 def weiler18(x, v, m, dens, density0, material, is_dynamic, h, C, count, ids, gradW, coeff, dt,
-             material_fluid=1):
+             material_fluid=1, warm_start=False, use_preconditioner=True, dv=None):
+    """Weiler18.solve (weiler18.py:371-401).  dv is the solver's correction buffer (self.dv): the
+    initial guess with warm_start (zeros when None), replaced by the new correction; returns
+    (v, CG iterations) and, when dv is given, fills it in place."""
     v = _f32(v).copy()
-    it = lib().orc_weiler18(ctypes.c_int(v.shape[0]), _p(_f32(x)), _p(v), _p(_f32(m)), _p(_f32(dens)),
-                            _p(_f32(density0)), _p(_i32(material)), _p(_i32(is_dynamic)), ctypes.c_float(h),
-                            ctypes.c_int(C), _p(_i32(count)), _p(_i32(ids)), _p(_f32(gradW)),
-                            ctypes.c_float(coeff), ctypes.c_float(dt), ctypes.c_int(material_fluid))
+    n = v.shape[0]
+    buf = np.zeros((n, 3), np.float32) if dv is None else dv
+    assert buf.dtype == np.float32 and buf.flags.c_contiguous and buf.shape == (n, 3)
+    it = lib().orc_weiler18_ex(ctypes.c_int(n), _p(_f32(x)), _p(v), _p(_f32(m)), _p(_f32(dens)),
+                               _p(_f32(density0)), _p(_i32(material)), _p(_i32(is_dynamic)), ctypes.c_float(h),
+                               ctypes.c_int(C), _p(_i32(count)), _p(_i32(ids)), _p(_f32(gradW)),
+                               ctypes.c_float(coeff), ctypes.c_float(dt), ctypes.c_int(material_fluid),
+                               ctypes.c_int(int(bool(warm_start))), ctypes.c_int(int(bool(use_preconditioner))),
+                               _p(buf))
     return v, int(it)
 
 

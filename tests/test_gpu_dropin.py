@@ -49,6 +49,10 @@ def test_framework_forward_matches_oracle_and_exposes_reference_attributes():
         assert fw.pressure.stats_outer_iters == sim.last_ip.outer_iters
         assert fw.pressure.stats_final_density_error <= fw.pressure.tol_opt
         assert len(fw.pressure.stats_density_errors) == len(sim.last_ip.density_errors)
+        # stats_iter_costs (ip.py:369, 392): one cost per outer-loop evaluation, within the solve
+        costs = fw.pressure.stats_iter_costs
+        assert len(costs) == len(fw.pressure.stats_density_errors)
+        assert all(c > 0.0 for c in costs) and sum(costs) <= fw.pressure.stats_total_cost_ms
     # reference-layout cache on demand
     cnt = ps.neighbor_count[:n].cpu().numpy()
     np.testing.assert_array_equal(cnt, sim.count)

@@ -187,6 +187,75 @@ def test_weiler18_matches_oracle(cache):
 
 
 @CACHE_ALL
+def test_weiler18_plain_cg_matches_oracle(cache):
+    """use_preconditioner = False (weiler18.py:279-292, 382-383): plain CG."""
+    sim = oracle_at(0, 10)
+    coeff = 0.05
+    v_o, it_o = O.weiler18(sim.x, sim.v, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h,
+                           sim.C, sim.count, sim.ids, sim.gradW, coeff, sim.dt, use_preconditioner=False)
+    ctx = gpu_context(sim, torch, cache)
+    assert ctx.neighbors_build() == 0
+    ctx.w18_preconditioner = False
+    it_g = ctx.viscosity_weiler18(coeff, sim.dt)
+    n = sim.x.shape[0]
+    v_g = _np(ctx.read(1, torch.zeros((n, 3), device="cuda")))
+    dyn = sim.is_dynamic == 1
+    assert abs(it_g - it_o) <= 1, (it_g, it_o)
+    assert rel_inf(v_g[dyn], v_o[dyn]) <= V_TOL
+    ctx.close()
+
+
+@CACHE_ALL
+def test_weiler18_warm_start_across_a_resort_matches_oracle(cache):
+    """warm_start (weiler18.py:311-316, 385-388): a cold solve, advection and a rebuild that
+    re-sorts the particles, then a warm solve from the first solve's correction, against the
+    oracle given the same correction per particle; after ivp_reset the kept correction is zero,
+    so a warm solve equals a cold one bitwise."""
+    from implicitvelocityprojection_amd.context import IPParams
+    sim = oracle_at(0, 10)
+    coeff = 0.05
+    n = sim.x.shape[0]
+    dyn = sim.is_dynamic == 1
+    ctx = gpu_context(sim, torch, cache)
+    assert ctx.neighbors_build() == 0
+    dv = np.zeros((n, 3), np.float32)
+    v1, it1 = O.weiler18(sim.x, sim.v, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h,
+                         sim.C, sim.count, sim.ids, sim.gradW, coeff, sim.dt, dv=dv)
+    assert abs(ctx.viscosity_weiler18(coeff, sim.dt) - it1) <= 1
+    # move on: IP solve, advect, clamp, rebuild (the store re-sorts)
+    res = ctx.ip_solve(sim.dt, IPParams(**ip_kwargs(sim.cfg)))
+    assert res.status == 0
+    ctx.advect_boundary(sim.dt)
+    assert ctx.neighbors_build() == 0
+    sim.x = _np(ctx.read(0, torch.zeros((n, 3), device="cuda")))
+    sim.v = _np(ctx.read(1, torch.zeros((n, 3), device="cuda")))
+    sim.rebuild()
+    ctx.w18_warm_start = True
+    it_g = ctx.viscosity_weiler18(coeff, sim.dt)
+    v_g = _np(ctx.read(1, torch.zeros((n, 3), device="cuda")))
+    v_o, it_o = O.weiler18(sim.x, sim.v, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h,
+                           sim.C, sim.count, sim.ids, sim.gradW, coeff, sim.dt, warm_start=True, dv=dv)
+    v_c, it_c = O.weiler18(sim.x, sim.v, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h,
+                           sim.C, sim.count, sim.ids, sim.gradW, coeff, sim.dt)
+    assert abs(it_g - it_o) <= 1, (it_g, it_o, it_c)
+    assert rel_inf(v_g[dyn], v_o[dyn]) <= V_TOL
+    # reset: the kept correction is zero, a warm solve is the cold solve
+    ctx.reset()
+    ctx.write(0, torch.from_numpy(sim.x).cuda())
+    ctx.write(1, torch.from_numpy(sim.v).cuda())
+    assert ctx.neighbors_build() == 0
+    it_w = ctx.viscosity_weiler18(coeff, sim.dt)
+    v_w = _np(ctx.read(1, torch.zeros((n, 3), device="cuda")))
+    ctx.write(1, torch.from_numpy(sim.v).cuda())
+    ctx.w18_warm_start = False
+    it_k = ctx.viscosity_weiler18(coeff, sim.dt)
+    v_k = _np(ctx.read(1, torch.zeros((n, 3), device="cuda")))
+    assert it_w == it_k and np.array_equal(v_w.view(np.uint32), v_k.view(np.uint32))
+    assert rel_inf(v_k[dyn], v_c[dyn]) <= V_TOL
+    ctx.close()
+
+
+@CACHE_ALL
 def test_deterministic_rerun(cache):
     sim = oracle_at(0, 6)
     from implicitvelocityprojection_amd.context import IPParams

@@ -56,6 +56,8 @@ def test_prj_solve_matches_oracle(step, tol, cache):
     _errors_close(res_g.density_errors, res_o.density_errors)
     assert res_g.final_density_error < tol
     assert res_g.pcg_iters == []
+    # stats_iter_costs (prj.py:189): one cost per pass
+    assert len(res_g.iter_costs) == len(res_g.density_errors) and all(c > 0.0 for c in res_g.iter_costs)
     assert rel_inf(v_g[dyn], v_o[dyn]) <= V_TOL, rel_inf(v_g[dyn], v_o[dyn])
     assert np.array_equal(v_g[~dyn], v_in[~dyn])
     assert rel_inf(div_g, div_o) <= 1e-4

@@ -639,6 +639,76 @@ def test_weiler18_operator_preconditioner_and_solve():
     np.testing.assert_allclose(v2[dyn].reshape(-1), expect, rtol=0, atol=5e-3 * np.abs(expect).max())
 
 
+def _cg64_iters(A, b, M=None, x0=None, tol=1e-5, cap=1000):
+    """Textbook (P)CG in float64 with Weiler18.PCG's stop rules (weiler18.py:320-365)."""
+    x = np.zeros_like(b) if x0 is None else x0.copy()
+    r = b - A @ x
+    z = r if M is None else M @ r
+    p = z.copy()
+    rz = r @ z
+    it = 0
+    if rz <= 1e-12:
+        return x, 0
+    while True:
+        Ap = A @ p
+        pAp = p @ Ap
+        if pAp <= 0.0:
+            break
+        a = rz / max(pAp, 1e-12)
+        x += a * p
+        r -= a * Ap
+        it += 1
+        if (r @ r < tol and it >= 2) or it >= cap:
+            break
+        z = r if M is None else M @ r
+        rz_new = r @ z
+        p = z + (rz_new / max(rz, 1e-12)) * p
+        rz = rz_new
+    return x, it
+
+
+def test_weiler18_plain_cg_warm_start_and_iteration_counts():
+    """use_preconditioner=False (plain CG, z = r) and warm_start (r_0 = b - A dv_prev,
+    weiler18.py:298-319, 382-388): solutions equal the dense solve, iteration counts equal the
+    float64 textbook (P)CG's within 1, a warm start from the exact correction stops at the
+    minimum of two iterations, a warm start from zero is the cold solve bitwise, and the new
+    velocity is v + dv with dv the returned correction."""
+    sim = tiny_sim(n=(5, 5, 5))
+    coeff, dt = 0.5, 2e-3
+    A, dyn = _dense_w18(sim, coeff, dt)
+    _, P = O.w18_apply(sim.x, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h, sim.C,
+                       sim.count, sim.ids, sim.gradW, coeff, dt)
+    n = sim.x.shape[0]
+    rng = np.random.default_rng(9)
+    v = rng.normal(scale=1.0, size=(n, 3)).astype(np.float32)
+    v[sim.is_dynamic == 0] = 0.0
+    args = (sim.x, v, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h, sim.C, sim.count,
+            sim.ids, sim.gradW, coeff, dt)
+    vd = v[dyn].reshape(-1).astype(np.float64)
+    b = vd - A @ vd
+    Mblk = np.zeros_like(A)
+    for bi, i in enumerate(dyn):
+        Mblk[3 * bi:3 * bi + 3, 3 * bi:3 * bi + 3] = P[i]
+    expect = np.linalg.solve(A, vd)
+    for precond in (True, False):
+        dv = np.zeros((n, 3), np.float32)
+        v2, it = O.weiler18(*args, use_preconditioner=precond, dv=dv)
+        _, it64 = _cg64_iters(A, b, Mblk if precond else None)
+        assert it >= 2 and abs(it - it64) <= 1, (precond, it, it64)
+        np.testing.assert_allclose(v2[dyn].reshape(-1), expect, rtol=0, atol=5e-3 * np.abs(expect).max())
+        assert np.array_equal(v2.view(np.uint32), (v + np.float32(1.0) * dv).view(np.uint32))
+        # warm start from zero: the cold solve, bitwise
+        z = np.zeros((n, 3), np.float32)
+        v3, it3 = O.weiler18(*args, use_preconditioner=precond, warm_start=True, dv=z)
+        assert it3 == it and np.array_equal(v3.view(np.uint32), v2.view(np.uint32))
+        # warm start from the exact correction: two iterations (the minimum), same solution
+        exact = np.zeros((n, 3), np.float32)
+        exact[dyn] = (expect - vd).reshape(-1, 3).astype(np.float32)
+        v4, it4 = O.weiler18(*args, use_preconditioner=precond, warm_start=True, dv=exact)
+        assert it4 <= 2 < it, (it4, it)
+        np.testing.assert_allclose(v4[dyn].reshape(-1), expect, rtol=0, atol=5e-3 * np.abs(expect).max())
+
+
 # ---------------------------------------------------------------------------------------------
 # Jeske23 implicit surface tension (jeske23.py:23-655)
 # ---------------------------------------------------------------------------------------------
