@@ -20,9 +20,14 @@ Prints one JSON line.  `value` is dynamic-particle-steps per second over the K t
 per launch (the byte model the library attaches to each launch: every stored neighbour-pair
 record once -- 4 B of id, plus 12 B of gradW when gradients are cached -- and every per-particle
 array once) over its average launch time, measured with HIP events that the kernel dispatch
-itself stamps (hipExtLaunchKernelGGL) on the library's stream.  `cpu_baseline` is the CPU oracle (a serial float32 C restatement of the
-reference kernels) timed on a few steps of the same workload, starting from the GPU state at the
-start of the timed region.
+itself stamps (hipExtLaunchKernelGGL) on the library's stream.  `ip_matvec_roofline` gives the
+same figures for the IP CG matvec's two pair passes (cg_J, cg_JT) on a few steps after the timed
+region.  `cpu_baseline` is the CPU oracle (a serial float32 C restatement of the reference
+kernels) timed on 10 steps of the same workload, starting from the GPU state at the start of the
+timed region, on one core; `all_cores` times the same steps with the oracle's OpenMP build on
+every core the process may use, and the CPU model and core counts are reported with both.
+`precision` compares one step of each GPU form, and of the float32 oracle, with the float64
+restatement of that step.
 """
 
 import argparse
@@ -54,6 +59,18 @@ def copy_calibration(torch, mib):
 
 
 WORKLOADS = {"column_8k": 0, "column_102k": 1, "dambreak_1m": 2}
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def measured_traffic(kernel, workload="dambreak_1m"):
@@ -95,6 +112,7 @@ def precision_study(ctxs, cfg, ps, m, m_inv, prm, steps, torch, dev, n, timed_st
                 raise RuntimeError("neighbour overflow")
 
     rows = {nm: [] for nm in names}
+    rows_oracle = []
     t0 = time.perf_counter()
     for _ in range(steps):
         xd, vd = rd(first, 0), rd(first, 1)
@@ -104,6 +122,19 @@ def precision_study(ctxs, cfg, ps, m, m_inv, prm, steps, torch, dev, n, timed_st
         _, x64, v64, st = fp64.step(x, v, m, m_inv, ps.density0, ps.is_dynamic, ps.material, h, count, ids, C,
                                     cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff,
                                     cfg.domain_size, ip_kw)
+        # the float32 oracle's own step from the same state: the float32 algorithm's distance to
+        # float64, next to the GPU forms'
+        sim = O.OracleSim(cfg, ps)
+        sim.x, sim.v = x.astype(np.float32), v.astype(np.float32)
+        sim.rebuild()
+        sim.step()
+        vo, xo = sim.v.astype(np.float64), sim.x.astype(np.float64)
+        rows_oracle.append({"velocity_rel_l2": float(np.linalg.norm(vo[dyn] - v64[dyn]) / np.linalg.norm(v64[dyn])),
+                            "velocity_rel_max": float(np.abs(vo[dyn] - v64[dyn]).max() / np.abs(v64[dyn]).max()),
+                            "position_max_abs": float(np.abs(xo[dyn] - x64[dyn]).max()),
+                            "density_error_oracle": sim.last_ip.final_density_error,
+                            "density_error_fp64": st["final_density_error"],
+                            "outer_iters_oracle": sim.last_ip.outer_iters, "outer_iters_fp64": st["outer_iters"]})
         for nm in names:
             c = ctxs[nm]
             res, _ = c.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, prm)
@@ -138,9 +169,10 @@ def precision_study(ctxs, cfg, ps, m, m_inv, prm, steps, torch, dev, n, timed_st
                       "max_final_density_error": errs}
     gw = {0: "evaluated in float32", 1: "float32 cache", 2: "float16 cache in the CG operator, float32 "
                                                              "residual replacement every 8 CG iterations"}
+    forms = {nm: {"gradW": gw[ctxs[nm].gradient_cache], "steps": rows[nm]} for nm in names}
+    forms["oracle_f32"] = {"gradW": "float32 cache (CPU oracle, the reference's arithmetic)", "steps": rows_oracle}
     return {"reference": "float64 restatement of the step (CPU), from the GPU state", "dtype_gpu": "f32",
-            "forms": {nm: {"gradW": gw[ctxs[nm].gradient_cache], "steps": rows[nm]} for nm in names},
-            "timing": timing, "seconds": seconds}
+            "forms": forms, "timing": timing, "seconds": seconds}
 
 
 def solver_study(src, cfg, ps, m, m_inv, prm_ip, steps, torch, dev, n, gradient_cache):
@@ -323,7 +355,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--gradient-cache", type=int, default=0, choices=[0, 1, 2],
                     help="neighbour-row format: 0 ids only, gradW from positions (4 B per pair); 1 float32 gradW "
@@ -448,6 +480,7 @@ def main():
     # kernel time of one IP CG iteration (J, J^T with p.Ap, x/r update with r.r, p update), from
     # dispatch events on a few steps after the timed region (launches past convergence excluded)
     cg_us_per_iter = None
+    matvec_roofline = {}
     if args.pressure == "ip":
         cg_classes = ("cg_J", "cg_JT", "cg_update", "cg_p")
         ctx.profile_reset()
@@ -462,6 +495,18 @@ def main():
         if it > 0:
             cg_us_per_iter = {k: pc[k]["total_ms"] * 1e3 / it for k in cg_classes if k in pc}
             cg_us_per_iter["total"] = sum(cg_us_per_iter.values())
+            # the IP matvec's two pair passes against the HBM roofline, on their own algorithmic
+            # bytes (the byte model attached to each launch)
+            for k in ("cg_J", "cg_JT"):
+                q = pc[k]
+                if q["launches"]:
+                    us = q["total_ms"] * 1e3 / q["launches"]
+                    b = q["bytes"] / q["launches"]
+                    ach = b / (us * 1e-6) / 1e9
+                    matvec_roofline[k] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                          "frac": ach / HBM_PEAK_GBS, "avg_launch_us": us,
+                                          "launches": q["launches"], "algorithmic_bytes_per_launch": b,
+                                          "traffic": measured_traffic(k, args.workload)}
 
     precision = None
     if args.precision_steps > 0 and st_coeff == 0.0 and args.pressure == "ip":
@@ -492,19 +537,30 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
-        sim = O.OracleSim(cfg, ps)
-        if args.pressure == "prj":
-            sim.pressure_method = 0
-        sim.x = x0.cpu().numpy().astype(np.float32)
-        sim.v = v0.cpu().numpy().astype(np.float32)
-        sim.rebuild()
-        tc = time.perf_counter()
-        for _ in range(args.cpu_steps):
-            sim.step()
-        tcpu = time.perf_counter() - tc
+
+        def cpu_run(parallel):
+            threads = O.use_threads(parallel)
+            sim = O.OracleSim(cfg, ps)
+            if args.pressure == "prj":
+                sim.pressure_method = 0
+            sim.x = x0.cpu().numpy().astype(np.float32)
+            sim.v = v0.cpu().numpy().astype(np.float32)
+            sim.rebuild()
+            tc = time.perf_counter()
+            for _ in range(args.cpu_steps):
+                sim.step()
+            return threads, time.perf_counter() - tc
+
+        _, tcpu = cpu_run(False)
+        threads, tomp = cpu_run(True)
+        O.use_threads(False)
         cpu = {"value": nd * args.cpu_steps / tcpu, "unit": "particle-steps/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_steps} steps of {args.workload} from the GPU state at the start of the timed "
-                         f"region, serial float32 C oracle ({tcpu:.1f} s)"}
+                         f"region, serial float32 C oracle ({tcpu:.1f} s)",
+               "cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+               "all_cores": {"value": nd * args.cpu_steps / tomp, "cores": threads, "seconds": tomp,
+                             "sample": f"the same {args.cpu_steps} steps, the oracle's OpenMP build (row loops on "
+                                       f"{threads} threads, atomics in the J^T scatter)"}}
 
     out = {
         "metric": METRIC,
@@ -531,6 +587,7 @@ def main():
         "cg_iters": cg_iters,
         "cg_iters_per_step_hist": {str(k): cg_hist[k] for k in sorted(cg_hist)},
         "cg_kernel_us_per_iter": cg_us_per_iter,
+        "ip_matvec_roofline": matvec_roofline or None,
         "ip_outer_iters": outer,
         "weiler18_cg_iters": visc_iters,
         "surface_tension_cg_iters": st_iters,
