@@ -63,53 +63,41 @@ def cube_lattice(lower_corner, cube_size, diam):
     return pos.reshape(-1, n).transpose().copy()
 
 
+def _shell_layer(lo, hi, diam):
+    """One layer of a box's particle shell between corners lo and hi: the faces y = lo[1],
+    y = hi[1], x = lo[0], x = hi[0], z = lo[2], z = hi[2] in that order, each a lattice at spacing
+    diam over the two other axes (the lower-numbered axis outer), edge and corner particles
+    repeated on every face that holds them -- the order and the np.arange bounds that the
+    reference's generators use, so the particle arrays are identical (scene_loader.py:447-545)."""
+    axes = [np.arange(lo[a], hi[a] + diam * 0.5, diam) for a in range(3)]
+    faces = []
+    for normal in (1, 0, 2):
+        u, w = (a for a in range(3) if a != normal)
+        gu, gw = np.meshgrid(axes[u], axes[w], indexing="ij")
+        for level in (lo[normal], hi[normal]):
+            face = np.empty((gu.size, 3), np.float64)
+            face[:, u], face[:, w], face[:, normal] = gu.ravel(), gw.ravel(), level
+            faces.append(face)
+    return faces
+
+
 def cube_shell(translation, size, layers, diam):
-    """``SceneLoader._generate_body`` for ``type == "cube"`` (scene_loader.py:447-492)."""
-    translation = np.array(translation, dtype=np.float32)
-    size = np.array(size, dtype=np.float32)
-    half = size / 2.0
-    start = translation - half
-    end = translation + half
-    out = []
-    for layer in range(int(layers)):
-        offset = layer * diam
-        s = start - offset
-        e = end + offset
-        xs = np.arange(s[0], e[0] + diam * 0.5, diam)
-        ys = np.arange(s[1], e[1] + diam * 0.5, diam)
-        zs = np.arange(s[2], e[2] + diam * 0.5, diam)
-        gx, gz = np.meshgrid(xs, zs, indexing="ij")
-        out.append(np.stack([gx.ravel(), np.full(gx.size, s[1]), gz.ravel()], axis=1))
-        out.append(np.stack([gx.ravel(), np.full(gx.size, e[1]), gz.ravel()], axis=1))
-        gy, gz2 = np.meshgrid(ys, zs, indexing="ij")
-        out.append(np.stack([np.full(gy.size, s[0]), gy.ravel(), gz2.ravel()], axis=1))
-        out.append(np.stack([np.full(gy.size, e[0]), gy.ravel(), gz2.ravel()], axis=1))
-        gx2, gy2 = np.meshgrid(xs, ys, indexing="ij")
-        out.append(np.stack([gx2.ravel(), gy2.ravel(), np.full(gx2.size, s[2])], axis=1))
-        out.append(np.stack([gx2.ravel(), gy2.ravel(), np.full(gx2.size, e[2])], axis=1))
-    return np.concatenate(out, axis=0).astype(np.float32)
+    """``SceneLoader._generate_body`` for ``type == "cube"``: `layers` shells around the box of
+    the given centre and size, each one diam further out."""
+    centre = np.array(translation, dtype=np.float32)
+    half = np.array(size, dtype=np.float32) / 2.0
+    lo, hi = centre - half, centre + half
+    faces = [f for k in range(int(layers)) for f in _shell_layer(lo - k * diam, hi + k * diam, diam)]
+    return np.concatenate(faces, axis=0).astype(np.float32)
 
 
 def boundary_walls(domain_start, domain_size, num_layers, diam):
-    """``SceneLoader._generate_boundary_walls`` (scene_loader.py:494-545)."""
-    start = np.array(domain_start, dtype=np.float32)
-    end = start + np.array(domain_size, dtype=np.float32)
-    out = []
-    for layer in range(int(num_layers)):
-        offset = layer * diam
-        xs = np.arange(start[0] - offset, end[0] + offset + diam * 0.5, diam)
-        ys = np.arange(start[1] - offset, end[1] + offset + diam * 0.5, diam)
-        zs = np.arange(start[2] - offset, end[2] + offset + diam * 0.5, diam)
-        gx, gz = np.meshgrid(xs, zs, indexing="ij")
-        out.append(np.stack([gx.ravel(), np.full(gx.size, start[1] - offset), gz.ravel()], axis=1))
-        out.append(np.stack([gx.ravel(), np.full(gx.size, end[1] + offset), gz.ravel()], axis=1))
-        gy, gz2 = np.meshgrid(ys, zs, indexing="ij")
-        out.append(np.stack([np.full(gy.size, start[0] - offset), gy.ravel(), gz2.ravel()], axis=1))
-        out.append(np.stack([np.full(gy.size, end[0] + offset), gy.ravel(), gz2.ravel()], axis=1))
-        gx2, gy2 = np.meshgrid(xs, ys, indexing="ij")
-        out.append(np.stack([gx2.ravel(), gy2.ravel(), np.full(gx2.size, start[2] - offset)], axis=1))
-        out.append(np.stack([gx2.ravel(), gy2.ravel(), np.full(gx2.size, end[2] + offset)], axis=1))
-    return np.concatenate(out, axis=0).astype(np.float32)
+    """``SceneLoader._generate_boundary_walls``: `num_layers` shells around the domain, each one
+    diam further out."""
+    lo = np.array(domain_start, dtype=np.float32)
+    hi = lo + np.array(domain_size, dtype=np.float32)
+    faces = [f for k in range(int(num_layers)) for f in _shell_layer(lo - k * diam, hi + k * diam, diam)]
+    return np.concatenate(faces, axis=0).astype(np.float32)
 
 
 DEFORMABLE_MATERIALS = ("solid", "deformable", "elastic")    # scheduled_spawn.py:11

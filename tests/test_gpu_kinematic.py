@@ -18,14 +18,14 @@ pytestmark = pytest.mark.gpu
 
 def test_rigid_transform_and_max_speed_match_numpy():
     from implicitvelocityprojection_amd import _native as N
-    from implicitvelocityprojection_amd.animation import axis_angle_matrix
+    from implicitvelocityprojection_amd.animation import rotation_matrix
     from implicitvelocityprojection_amd.context import Context
     lib = N.load()
     rng = np.random.default_rng(1)
     n = 5000
     x0 = rng.uniform(-1, 1, size=(n, 3)).astype(np.float32)
     oid = rng.integers(0, 3, size=n).astype(np.int32)
-    R = axis_angle_matrix(np.array([0.3, 0.8, -0.52]) / np.linalg.norm([0.3, 0.8, -0.52]), 0.77)
+    R = rotation_matrix(np.array([0.3, 0.8, -0.52]) / np.linalg.norm([0.3, 0.8, -0.52]), 0.77)
     pivot = np.array([0.1, -0.2, 0.3], np.float32)
     T = np.array([0.5, 0.25, -1.0], np.float32)
     dx0, dx = torch.from_numpy(x0).cuda(), torch.from_numpy(x0 * 3).cuda()
@@ -76,7 +76,7 @@ def _scene(tmp_path):
 
 
 def test_framework_cfl_and_animated_paddle_match_oracle(tmp_path):
-    from implicitvelocityprojection_amd.animation import AnimationEngine, track_transform
+    from implicitvelocityprojection_amd.animation import AnimationEngine
     from implicitvelocityprojection_amd.framework import Framework
     from implicitvelocityprojection_amd.neighbour_search import NeighbourSearch
     from implicitvelocityprojection_amd.scene_loader import SceneLoader
@@ -94,7 +94,7 @@ def test_framework_cfl_and_animated_paddle_match_oracle(tmp_path):
     fw.animation = anim
     fw.initialize()
     sim = O.OracleSim(cfg, data["particles"])
-    sim.tracks = [(tr.object_id, lambda t, tr=tr: (*track_transform(tr, t), tr.pivot)) for tr in anim.tracks]
+    sim.tracks = [(tr.object_id, lambda t, tr=tr: (*tr.pose(t), tr.pivot)) for tr in anim.tracks]
     sim.rebuild()
     n = ps.particle_num
     paddle = data["particles"].object_id == 1
