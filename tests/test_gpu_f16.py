@@ -16,7 +16,9 @@ is the exact float32 path, bit-for-bit the same as the ids-only format.  Bars:
   1e-4 of the float64 solve: replacement brings the float16 operator's solution to the float32
   one (measured 2.5e-5), where a solve without it would stop at the 7e-4 operator error;
 * a CG tolerance below float32's attainable residual ends in finite values, through the
-  stall test, not in a blow-up of the search direction.
+  stall test, not in a blow-up of the search direction;
+* at full size (the 1M dam break), one step stage by stage against the float32 oracle at these
+  bars, then ten steps that all converge.
 """
 
 import numpy as np
@@ -124,3 +126,62 @@ def test_f16_unattainable_tolerance_stays_finite():
     assert all(it < 200 for it in res.pcg_iters), res.pcg_iters     # the stall test ended each solve
     assert res.final_density_error <= kw["tol_opt"]
     assert rel_inf(v_g[dyn], v64[dyn]) <= V_TOL16
+
+
+def test_f16_form_dambreak_1m():
+    """The precision form on the 1M dam break (float16 gradW in the IP CG operator, residual
+    replacement every 8 iterations): 20 steps from the initial state, then one step stage by
+    stage against the float32 oracle, each stage fed the GPU's input -- Weiler18 viscosity at the
+    exact-format bar (1e-5: it evaluates gradW from the positions), the IP solve at the float16
+    bars (velocities 5e-4, the correction 5e-3, same outer iterations, CG within +-1), advection
+    and clamp bitwise -- then ten more steps with every final density error at or below
+    densityErrorTol, the static particles untouched and every value finite."""
+    from implicitvelocityprojection_amd.context import IPParams
+    from tests.helpers import scene
+    cfg, ps = scene(2)
+    sim = O.OracleSim(cfg, ps)
+    sim.rebuild()
+    ctx = gpu_context(sim, torch, F16)
+    assert ctx.neighbors_build() == 0
+    kw = ip_kwargs(cfg)
+    n = sim.x.shape[0]
+    rd = lambda f, shp: _np(ctx.read(f, torch.zeros(shp, device="cuda")))
+    x0 = rd(0, (n, 3))
+    for _ in range(20):
+        res, _ = ctx.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, IPParams(**kw))
+        assert res.status == 0 and res.final_density_error <= kw["tol_opt"]
+    sim.x, sim.v = rd(0, (n, 3)), rd(1, (n, 3))
+    sim.rebuild()
+    dyn = sim.is_dynamic == 1
+    ctx.apply_gravity(cfg.gravity, cfg.dt)
+    v1 = O.gravity(sim.v, sim.is_dynamic, sim.g, sim.dt)
+    it_g = ctx.viscosity_weiler18(cfg.viscosity_coeff, cfg.dt)
+    v2o, it_o = O.weiler18(sim.x, v1, sim.m, sim.density, sim.density0, sim.material, sim.is_dynamic, sim.h,
+                           sim.C, sim.count, sim.ids, sim.gradW, cfg.viscosity_coeff, cfg.dt)
+    v2g = rd(1, (n, 3))
+    assert abs(it_g - it_o) <= 1
+    assert rel_inf(v2g[dyn], v2o[dyn]) <= 1e-5
+    res = ctx.ip_solve(cfg.dt, IPParams(**kw))
+    v3o, _, res_o = O.ip_solve(v2g, sim.m, sim.density, sim.density0, sim.is_dynamic, sim.C, sim.count, sim.ids,
+                               sim.gradW, sim.k, sim.dt, **kw)
+    v3g = rd(1, (n, 3))
+    assert res.status == 0 and res.outer_iters == res_o.outer_iters
+    for a, b in zip(res.pcg_iters, res_o.pcg_iters):
+        assert abs(a - b) <= 1, (res.pcg_iters, res_o.pcg_iters)
+    assert res.final_density_error <= kw["tol_opt"]
+    assert rel_inf(v3g[dyn], v3o[dyn]) <= V_TOL16, rel_inf(v3g[dyn], v3o[dyn])
+    assert rel_inf(v3g[dyn] - v2g[dyn], v3o[dyn] - v2g[dyn]) <= DV_TOL16
+    ctx.advect_boundary(cfg.dt)
+    x4o, v4o = O.advect(sim.x, v3g, sim.is_dynamic, sim.dt)
+    x4o, v4o = O.enforce_boundary(x4o, v4o, sim.is_dynamic, cfg.domain_size, sim.h)
+    assert np.array_equal(rd(0, (n, 3)).view(np.uint32), x4o.view(np.uint32))
+    assert np.array_equal(rd(1, (n, 3)).view(np.uint32), v4o.view(np.uint32))
+    assert ctx.neighbors_build() == 0
+    for _ in range(10):
+        res, _ = ctx.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, IPParams(**kw))
+        assert res.status == 0 and res.final_density_error <= kw["tol_opt"]
+    x, v = rd(0, (n, 3)), rd(1, (n, 3))
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(v))
+    assert np.array_equal(x[~dyn].view(np.uint32), x0[~dyn].view(np.uint32))
+    assert np.all(v[~dyn] == 0.0)
+    ctx.close()

@@ -202,3 +202,45 @@ def test_scene_reload_recaptures_graphs():
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     batch.close()
     ref.close()
+
+
+def test_scene_batch_256_full_size():
+    """The batched workload at its full size: 256 independent columns of 2 048-32 768 fluid
+    particles (seed 4, 4.58M fluid particles), autotuned chunk and lanes.  Three steps, every
+    scene converged (status 0, final density error at or below densityErrorTol), static particles
+    untouched, all values finite; four sampled scenes (the smallest, the largest, two more) equal
+    their one-scene contexts at the single-scene bars after the same three steps."""
+    from implicitvelocityprojection_amd.context import IPParams
+    from implicitvelocityprojection_amd.scene_batch import SceneBatch
+    scenes = synthetic_batch(n_scenes=256, seed=4)
+    batch = SceneBatch(scenes)
+    batch.autotune(chunks=(2, 4), lanes=(1, 4), steps=1)
+    x0 = batch.read(0, (batch.n, 3))
+    for _ in range(STEPS):
+        status, stats = batch.step()
+        assert status == 0
+        assert len(stats) == 256
+        assert all(st[0] == 0 for st in stats)
+        assert max(st[3] for st in stats) <= 1e-3
+    x_b = batch.read(0, (batch.n, 3))
+    v_b = batch.read(1, (batch.n, 3))
+    assert np.all(np.isfinite(x_b)) and np.all(np.isfinite(v_b))
+    stat = batch.is_dynamic == 0
+    assert np.array_equal(x_b[stat].view(np.uint32), x0[stat].view(np.uint32))
+    assert np.all(v_b[stat] == 0.0)
+    sizes = [int(np.count_nonzero(load_particles(SimConfig.from_dict(sc)).is_dynamic == 1)) for sc in scenes]
+    picks = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), 17, 200})
+    for s in picks:
+        cfg, ps, ctx = _single(scenes[s])
+        kw = ip_kwargs(cfg)
+        for _ in range(STEPS):
+            res, _ = ctx.step(cfg.dt, cfg.gravity, 0, 0.0, IPParams(**kw))
+            assert res.status == 0
+        x = _np(ctx.read(0, torch.zeros((ps.n, 3), device="cuda")))
+        v = _np(ctx.read(1, torch.zeros((ps.n, 3), device="cuda")))
+        dyn = ps.is_dynamic == 1
+        sl = batch.scene_slice(s)
+        assert rel_inf(x_b[sl][dyn], x[dyn]) <= V_TOL, s
+        assert rel_inf(v_b[sl][dyn], v[dyn]) <= V_TOL, s
+        ctx.close()
+    batch.close()
