@@ -165,3 +165,10 @@ def check(status, where=""):
     if status < 0:
         raise IVPError(status, where)
     return status
+
+
+def check_ok(status, where=""):
+    """Raise IVPError on any status but IVP_OK (entry points without a solver status to report)."""
+    if status != IVP_OK:
+        raise IVPError(status, where)
+    return status

@@ -202,13 +202,15 @@ class Context:
         return IPResult(status, st, errs, its)
 
     def viscosity_explicit(self, coeff, dt):
-        N.check(self.lib.ivp_viscosity_explicit(self.h, ctypes.c_float(coeff), ctypes.c_float(dt)),
-                "ivp_viscosity_explicit")
+        N.check_ok(self.lib.ivp_viscosity_explicit(self.h, ctypes.c_float(coeff), ctypes.c_float(dt)),
+                   "ivp_viscosity_explicit")
 
     def viscosity_weiler18(self, coeff, dt):
+        """Weiler18.solve; returns the CG iteration count (raises IVPError on any other status,
+        e.g. rows that the last rebuild truncated)."""
         cs = N.CGStats()
-        N.check(self.lib.ivp_viscosity_weiler18(self.h, ctypes.c_float(coeff), ctypes.c_float(dt), ctypes.byref(cs)),
-                "ivp_viscosity_weiler18")
+        N.check_ok(self.lib.ivp_viscosity_weiler18(self.h, ctypes.c_float(coeff), ctypes.c_float(dt),
+                                                   ctypes.byref(cs)), "ivp_viscosity_weiler18")
         return cs.cg_iters
 
     def max_speed2(self):
@@ -253,8 +255,8 @@ class Context:
     def surface_tension_jeske23(self, sigma, dt):
         """Jeske23.solve; returns the CG iteration count."""
         cs = N.CGStats()
-        N.check(self.lib.ivp_surface_tension_jeske23(self.h, ctypes.c_float(sigma), ctypes.c_float(dt),
-                                                     ctypes.byref(cs)), "ivp_surface_tension_jeske23")
+        N.check_ok(self.lib.ivp_surface_tension_jeske23(self.h, ctypes.c_float(sigma), ctypes.c_float(dt),
+                                                        ctypes.byref(cs)), "ivp_surface_tension_jeske23")
         return cs.cg_iters
 
     # -- independent scenes (one context, many scenes) ----------------------------------------

@@ -81,6 +81,7 @@ void free_all(ivp_ctx* c) {
     if (c->state_host) (void)hipHostFree(c->state_host);
     if (c->err_host) (void)hipHostFree(c->err_host);
     if (c->count_host) (void)hipHostFree(c->count_host);
+
     c->state_host = nullptr;
     c->err_host = nullptr;
     c->count_host = nullptr;
@@ -121,6 +122,7 @@ int alloc_all(ivp_ctx* c) {
     IVP_HIP(hipHostMalloc((void**)&c->state_host, sizeof(SolverState), hipHostMallocDefault));
     IVP_HIP(hipHostMalloc((void**)&c->err_host, sizeof(int), hipHostMallocDefault));
     IVP_HIP(hipHostMalloc((void**)&c->count_host, 4 * sizeof(int), hipHostMallocDefault));
+
     memset(c->state_host, 0, sizeof(SolverState));
     IVP_HIP(hipMemsetAsync(c->ccount, 0, sizeof(int) * (size_t)c->ncell, c->stream));
     IVP_HIP(hipMemsetAsync(c->state, 0, sizeof(SolverState), c->stream));
@@ -360,22 +362,18 @@ int prof_collect(ivp_ctx* c, int cg_iters_done, int epoch) {
 
 }  // namespace ivp
 
-// Status of the neighbour build just enqueued: its error flags (and the solver state, whose pair
-// count the byte figures of the following passes use), read with one synchronisation.
+static int build_code(int err) {
+    if (err & 4) return IVP_ERR_INTERNAL;
+    return (err & 1) ? IVP_NEIGHBOR_OVERFLOW : IVP_OK;
+}
+
+// Status of the neighbour build last enqueued, read with one synchronisation: its error flags
+// (and the solver state, whose pair count the byte figures of the following passes use).
 static int build_status(ivp_ctx* c) {
     IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     IVP_HIP(hipMemcpyAsync(c->state_host, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
     IVP_HIP(hipStreamSynchronize(c->stream));
-    c->build_err = *c->err_host;
-    if (c->build_err & 4) return IVP_ERR_INTERNAL;
-    return (c->build_err & 1) ? IVP_NEIGHBOR_OVERFLOW : IVP_OK;
-}
-
-// A step refuses to start on rows that the last build truncated (or whose build failed a
-// consistency guard): the velocities stay as they are.
-static int rows_usable(const ivp_ctx* c) {
-    if (c->build_err & 4) return IVP_ERR_INTERNAL;
-    return (c->build_err & 1) ? IVP_NEIGHBOR_OVERFLOW : IVP_OK;
+    return build_code(*c->err_host);
 }
 
 static const char* kclass_name(int k) {
@@ -618,12 +616,16 @@ int ivp_ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* s
 int ivp_viscosity_explicit(ivp_ctx* c, float coeff, float dt) {
     if (!c) return IVP_ERR_ARG;
     if (c->S > 0) return IVP_ERR_STATE;
+    // rows that the last rebuild truncated are refused before v changes
+    if (c->built) IVP_TRY(build_status(c));
     return visc_explicit(c, coeff, dt);
 }
 
 int ivp_viscosity_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* stats) {
     if (!c) return IVP_ERR_ARG;
     if (c->S > 0) return IVP_ERR_STATE;
+    // rows that the last rebuild truncated are refused before v changes
+    if (c->built) IVP_TRY(build_status(c));
     return visc_weiler18(c, coeff, dt, stats);
 }
 
@@ -642,6 +644,8 @@ int ivp_prj_solve_df(ivp_ctx* c, float dt, const ivp_prj_params* prm, ivp_ip_sta
 int ivp_surface_tension_jeske23(ivp_ctx* c, float sigma, float dt, ivp_cg_stats* stats) {
     if (!c) return IVP_ERR_ARG;
     if (c->S > 0) return IVP_ERR_STATE;
+    // rows that the last rebuild truncated are refused before v changes
+    if (c->built) IVP_TRY(build_status(c));
     return st_jeske23(c, sigma, dt, stats);
 }
 
@@ -660,10 +664,15 @@ int ivp_step(ivp_ctx* c, const ivp_step_params* prm, ivp_ip_stats* stats, ivp_cg
     if (!c->built || c->S > 0) return IVP_ERR_STATE;
     if (prm->surface_tension_coeff != 0.0f && prm->surface_tension_method != 1) return IVP_ERR_ARG;
     if (prm->pressure_method != 0 && prm->pressure_method != 1) return IVP_ERR_ARG;
-    int rc = rows_usable(c);
+    // gravity writes the step's velocities into the second buffer; a stage that finds the rows
+    // truncated by the last rebuild (the pressure solve's first synchronisation reads its flags)
+    // swaps the first one back, so such a step leaves v and x as they were
+    int rc = glue_gravity(c, prm->gravity, prm->dt);
     if (rc != IVP_OK) return rc;
-    rc = glue_gravity(c, prm->gravity, prm->dt);
-    if (rc != IVP_OK) return rc;
+    auto truncated = [&](int r) {
+        if (r == IVP_NEIGHBOR_OVERFLOW || r == IVP_ERR_INTERNAL) std::swap(c->vel, c->vel_b);
+        return r;
+    };
     if (visc) { visc->cg_iters = 0; visc->residual_norm2 = 0.0; }
     if (st) { st->cg_iters = 0; st->residual_norm2 = 0.0; }
     // compute_non_pressure_forces order (framework.py:400-421): gravity, surface tension, viscosity
@@ -682,13 +691,12 @@ int ivp_step(ivp_ctx* c, const ivp_step_params* prm, ivp_ip_stats* stats, ivp_cg
     }
     // pressure solve (framework.py:481-487; PRJ's divergence-free pass runs inside prj_solve)
     rc = prm->pressure_method == 0 ? prj_solve(c, prm->dt, &prm->prj, stats, PRJ_BOTH) : ip_solve(c, prm->dt, &prm->ip, stats);
-    if (rc != IVP_OK) return rc;
+    if (rc != IVP_OK) return truncated(rc);
     rc = glue_advect_boundary(c, prm->dt);
     if (rc != IVP_OK) return rc;
-    // the rebuild that ends the step: a row over the capacity is reported by this step
-    rc = grid_build(c);
-    if (rc != IVP_OK) return rc;
-    return build_status(c);
+    // the rebuild that ends the step, not waited for: the step's results come from the rows built
+    // before it, and a row of the new build over the capacity stops the next step (above)
+    return grid_build(c);
 }
 
 int ivp_set_scenes(ivp_ctx* c, int n_scenes, const ivp_scene* scenes) {
@@ -702,11 +710,10 @@ int ivp_step_scenes(ivp_ctx* c, const ivp_step_params* prm, ivp_scene_stats* sta
     // viscosity and surface tension are not part of the scene step, which solves with IP
     if (prm->viscosity_coeff != 0.0f || prm->surface_tension_coeff != 0.0f) return IVP_ERR_ARG;
     if (prm->pressure_method != 1) return IVP_ERR_ARG;
-    int rc = rows_usable(c);
-    if (rc != IVP_OK) return rc;
-    rc = glue_gravity(c, prm->gravity, prm->dt);
+    int rc = glue_gravity(c, prm->gravity, prm->dt);
     if (rc != IVP_OK) return rc;
     rc = ip_solve_scenes(c, prm->dt, &prm->ip);
+    if (rc == IVP_NEIGHBOR_OVERFLOW || rc == IVP_ERR_INTERNAL) std::swap(c->vel, c->vel_b);   // v as it was
     if (rc != IVP_OK) return rc;
     int status = IVP_OK;
     for (int s = 0; s < c->S; ++s) {
@@ -720,8 +727,6 @@ int ivp_step_scenes(ivp_ctx* c, const ivp_step_params* prm, ivp_scene_stats* sta
     rc = glue_advect_boundary(c, prm->dt);
     if (rc != IVP_OK) return rc;
     rc = grid_build(c);
-    if (rc != IVP_OK) return rc;
-    rc = build_status(c);
     if (rc != IVP_OK) return rc;
     return status;
 }

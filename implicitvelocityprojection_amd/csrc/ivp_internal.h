@@ -248,7 +248,6 @@ struct ivp_ctx {
     ivp::SolverState* state_host = nullptr; // pinned
     int* err_flag = nullptr;                // device: bit 0 neighbour overflow, bit 2 a kernel's
                                             // consistency guard (set by the build)
-    int build_err = 0;                      // err_flag of the last build, as read on the host
     unsigned* speed_max = nullptr;          // device: bits of max |v|^2 (ivp_max_speed2)
     int* err_host = nullptr;                // pinned
     int* count_host = nullptr;              // pinned scratch

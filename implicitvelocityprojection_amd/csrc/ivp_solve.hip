@@ -1406,12 +1406,13 @@ __global__ __launch_bounds__(BLOCK) void k_st_prep(int nd, const float* __restri
 // ---- glue ----------------------------------------------------------------------------------
 
 // apply_gravity (framework.py:26-41)
-__global__ __launch_bounds__(BLOCK) void k_gravity(int nd, float gx, float gy, float gz, float* __restrict__ vel) {
+__global__ __launch_bounds__(BLOCK) void k_gravity(int nd, float gx, float gy, float gz, const float* __restrict__ vin,
+                                                   float* __restrict__ vout) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= nd) return;
-    vel[3 * i + 0] = vel[3 * i + 0] + gx;
-    vel[3 * i + 1] = vel[3 * i + 1] + gy;
-    vel[3 * i + 2] = vel[3 * i + 2] + gz;
+    vout[3 * i + 0] = vin[3 * i + 0] + gx;
+    vout[3 * i + 1] = vin[3 * i + 1] + gy;
+    vout[3 * i + 2] = vin[3 * i + 2] + gz;
 }
 
 // advect_position (framework.py:117-136) + enforce_boundary (boundary.py:27-91); with SC each
@@ -2165,7 +2166,13 @@ int ip_solve_scenes(ivp_ctx* c, float dt, const ivp_ip_params* prm) {
             IVP_TRY(enqueue_scene_replay(c, c->stream, prm->omega, K));
         c->scene_replays++;
         IVP_HIP(hipMemcpyAsync(c->sactive_host, c->sactive, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        if (c->scene_replays == 1)   // the flags of the rebuild that produced these rows
+            IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         IVP_HIP(hipStreamSynchronize(c->stream));
+        if (c->scene_replays == 1) {
+            if (*c->err_host & 4) return IVP_ERR_INTERNAL;
+            if (*c->err_host & 1) return IVP_NEIGHBOR_OVERFLOW;
+        }
         if (*c->sactive_host == 0) break;
         if (c->scene_replays >= cap) return IVP_ERR_INTERNAL;
     }
@@ -2177,11 +2184,15 @@ int ip_solve_scenes(ivp_ctx* c, float dt, const ivp_ip_params* prm) {
     return IVP_OK;
 }
 
+// v + g dt into the second velocity buffer, which becomes the first; the previous velocities stay
+// in the second until the next re-sort, so a step that must not start can restore them
 int glue_gravity(ivp_ctx* c, const float g[3], float dt) {
     if (!c->loaded) return IVP_ERR_STATE;
     if (c->nd == 0) return IVP_OK;
-    return launch(c, KC_GLUE, -1, 0.0, k_gravity, dim3(blocks_for(c->nd)), c->nd, g[0] * dt, g[1] * dt, g[2] * dt,
-                  c->vel);
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_gravity, dim3(blocks_for(c->nd)), c->nd, g[0] * dt, g[1] * dt, g[2] * dt,
+                   (const float*)c->vel, c->vel_b));
+    std::swap(c->vel, c->vel_b);
+    return IVP_OK;
 }
 
 int glue_advect_boundary(ivp_ctx* c, float dt) {

@@ -223,7 +223,8 @@ int ivp_advect_boundary(ivp_ctx *ctx, float dt);
  * viscosity, the pressure solve of prm->pressure_method (IP, or PRJ with its optional
  * divergence-free pass), advect, boundary, neighbour rebuild, density, Aii.  stats receives the
  * pressure solver's statistics; visc and st (either may be NULL) the viscosity and
- * surface-tension CG statistics. */
+ * surface-tension CG statistics.  The closing rebuild is not waited for; if a row of it exceeds
+ * cache_size, the next step (or solve) returns IVP_NEIGHBOR_OVERFLOW with v and x unchanged. */
 int ivp_step(ivp_ctx *ctx, const ivp_step_params *prm, ivp_ip_stats *stats, ivp_cg_stats *visc,
              ivp_cg_stats *st);
 

@@ -107,31 +107,39 @@ def _no_correction():
     return IPParams(max_iteration_opt=0, use_fixed_iters=True)
 
 
-def test_overflow_reported_by_the_step_that_overflowed():
+def test_overflow_of_a_step_rebuild_stops_the_next_step():
     """The rebuild that ends a step overflows (rows of 37 neighbours at load, 92 after the step,
-    capacity 48): that step returns IVP_NEIGHBOR_OVERFLOW, and the next step refuses to start,
-    leaving the velocities and positions untouched."""
+    capacity 48).  That step's own results come from the rows built before it and stand; every
+    later call that would read the truncated rows (a step, a solve) returns IVP_NEIGHBOR_OVERFLOW
+    before touching the velocities or positions."""
+    from implicitvelocityprojection_amd import _native as N
     cfg = SimConfig.from_dict(synthetic_scene(0, scale=(8, 8, 8), box=(0.32, 0.32, 0.32)))
     ps = load_particles(cfg)
     ctx = _ctx(ps.n, cache=48, domain=cfg.domain_size)
     _load(ctx, ps.x, _converging(ps, 1e-3), ps.is_dynamic)
     assert ctx.neighbors_build() == 0
     res, _ = ctx.step(1e-3, G, 0, 0.0, _no_correction())
-    assert res.status == 3                      # IVP_NEIGHBOR_OVERFLOW
+    assert res.status == 0
     v0 = ctx.read(1, torch.zeros((ps.n, 3), device="cuda")).cpu().numpy()
     x0 = ctx.read(0, torch.zeros((ps.n, 3), device="cuda")).cpu().numpy()
-    res, _ = ctx.step(1e-3, G, 0, 0.0, _no_correction())
-    assert res.status == 3
+    for _ in range(2):
+        res, _ = ctx.step(1e-3, G, 0, 0.0, _no_correction())
+        assert res.status == N.IVP_NEIGHBOR_OVERFLOW
+    assert ctx.ip_solve(1e-3, _no_correction()).status == N.IVP_NEIGHBOR_OVERFLOW
+    with pytest.raises(N.IVPError):
+        ctx.viscosity_weiler18(0.01, 1e-3)
     v1 = ctx.read(1, torch.zeros((ps.n, 3), device="cuda")).cpu().numpy()
     x1 = ctx.read(0, torch.zeros((ps.n, 3), device="cuda")).cpu().numpy()
     assert np.array_equal(v0.view(np.uint32), v1.view(np.uint32))
     assert np.array_equal(x0.view(np.uint32), x1.view(np.uint32))
+    # the same rows built directly report at once
+    assert ctx.neighbors_build() == N.IVP_NEIGHBOR_OVERFLOW
     ctx.close()
 
 
 def test_scene_step_overflow_is_reported():
-    """The same in a context of two independent scenes: the scene step whose rebuild overflows
-    returns IVP_NEIGHBOR_OVERFLOW, and the next one refuses to start."""
+    """The same in a context of two independent scenes: after the scene step whose rebuild
+    overflowed, the next scene step refuses to start and leaves the velocities untouched."""
     from implicitvelocityprojection_amd.context import Context
     cfgs = [SimConfig.from_dict(s) for s in synthetic_batch(n_scenes=2, seed=4, count_range=(2048, 3000))]
     pss = [load_particles(c) for c in cfgs]
@@ -149,7 +157,7 @@ def test_scene_step_overflow_is_reported():
                       t(cat(lambda c, p: p.material), torch.int32))
     assert ctx.neighbors_build() == 0
     status, _ = ctx.step_scenes(cfgs[0].dt, G, _no_correction())
-    assert status == 3
+    assert status == 0
     v0 = ctx.read(1, torch.zeros((n, 3), device="cuda")).cpu().numpy()
     status, _ = ctx.step_scenes(cfgs[0].dt, G, _no_correction())
     assert status == 3
