@@ -141,12 +141,14 @@ int ensure_rows(ivp_ctx* c) {
     size_t slots = tiles * (size_t)c->Cs * 64u;
     if (c->nb) return IVP_OK;
     IVP_TRY(dalloc(&c->nb, slots * (size_t)(c->rec_words / 64)));
+    if (c->row_fmt == IVP_ROWS_F16) IVP_TRY(dalloc(&c->g16, 2 * slots));
     return IVP_OK;
 }
 
 void free_rows(ivp_ctx* c) {
     scene_graphs_clear(c);
     dfree(c->nb);
+    dfree(c->g16);
     dfree(c->st_gt);
 }
 
@@ -549,7 +551,7 @@ int ivp_set_option(ivp_ctx* c, int option, int value) {
         IVP_HIP(hipStreamSynchronize(c->stream));
         free_rows(c);
         c->row_fmt = value;
-        c->rec_words = value == IVP_ROWS_F32 ? 256 : value == IVP_ROWS_F16 ? 192 : 64;
+        c->rec_words = value == IVP_ROWS_F32 ? 256 : 64;
         c->built = false;
         return c->loaded ? ensure_rows(c) : IVP_OK;
     }

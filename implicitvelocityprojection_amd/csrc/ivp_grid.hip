@@ -303,6 +303,7 @@ struct BuildArgs {
     const int* dstart;
     const int* sstart;
     uint32_t* nb;
+    uint32_t* g16;   // float16 format: gradW / scale as two half-pair words at twice each id's word
     int* tile_len;
     int* rowcnt;
     int* rowtot;
@@ -326,7 +327,7 @@ struct BuildArgs {
 // FMT is the row format (include/ivp.h): ids only, ids + float32 gradW, or ids + float16 gradW
 // scaled by 1/inv_gs, packed as (gx, gy) and (gz, 0) half pairs.
 template <int FMT>
-constexpr unsigned rec_words() { return FMT == IVP_ROWS_F32 ? 256u : FMT == IVP_ROWS_F16 ? 192u : 64u; }
+constexpr unsigned rec_words() { return FMT == IVP_ROWS_F32 ? 256u : 64u; }
 
 // Phase B of the build for the wave's 64-row tile (one thread per row, lane = row's place in the
 // tile); cnt / cs / tot are the row's stored dynamic and static slots and its neighbour total.
@@ -511,8 +512,9 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
     float aii = 0.0f;
     float jx = 0.0f, jy = 0.0f, jz = 0.0f;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    // one dynamic slot: gradW (stored next to the id in the cached formats), density, Aii, S
-    auto dyn_slot = [&](uint32_t* rec, int j, float4 pj, float minv_j) {
+    // one dynamic slot: gradW (stored next to the id in the float32 format; returned as the two
+    // float16 side-array words h0, h1 in the float16 format), density, Aii, S
+    auto dyn_slot = [&](uint32_t* rec, int j, float4 pj, float minv_j, uint32_t& h0, uint32_t& h1) {
         if ((unsigned)j >= (unsigned)a.nd) {  // never taken: phase A stored a dynamic id here
             atomicOr(a.err_flag, 4);
             return;
@@ -525,8 +527,8 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
             rec[128] = __float_as_uint(gw.y);
             rec[192] = __float_as_uint(gw.z);
         } else if (FMT == IVP_ROWS_F16) {
-            rec[64] = pack_half2(gw.x * a.inv_gs, gw.y * a.inv_gs);
-            rec[128] = pack_half2(gw.z * a.inv_gs, 0.0f);
+            h0 = pack_half2(gw.x * a.inv_gs, gw.y * a.inv_gs);
+            h1 = pack_half2(gw.z * a.inv_gs, 0.0f);
         }
         rho += pj.w * cubic_kernel(rn, h, a.kc);
         float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
@@ -536,7 +538,7 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
     // Two slots in flight: both ids, then both neighbours' (x, m) and 1/m, then the two slots in
     // order (the accumulation order of a one-slot loop).  An id past a row's count is read but
     // never used, and the gathers take a clamped index.
-    auto dyn_pair = [&](uint32_t* r0, uint32_t* r1, int j0, int j1, int k) {
+    auto dyn_pair = [&](uint32_t* r0, uint32_t* r1, int j0, int j1, int k, uint32_t (&hq)[4]) {
         // the empty asm statements keep each pair of loads together (the compiler would sink the
         // second slot's loads into its conditional use)
         asm volatile("" ::"v"(j0), "v"(j1));
@@ -548,29 +550,46 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
         const float m1 = a.minv[c1];
         asm volatile("" ::"v"(p0.x), "v"(p0.y), "v"(p0.z), "v"(p0.w), "v"(m0), "v"(p1.x), "v"(p1.y), "v"(p1.z),
                      "v"(p1.w), "v"(m1));
-        if (valid && k < cnt) dyn_slot(r0, j0, p0, m0);
-        if (valid && k + 1 < cnt) dyn_slot(r1, j1, p1, m1);
+        if (valid && k < cnt) dyn_slot(r0, j0, p0, m0, hq[0], hq[1]);
+        if (valid && k + 1 < cnt) dyn_slot(r1, j1, p1, m1, hq[2], hq[3]);
     };
     if constexpr (REC == 64u) {
         // ids-only rows: the row's 8 words of a block (slots 0, 4, 1, 5, 2, 6, 3, 7) with two
-        // 16-byte loads; whole blocks stay inside the tile (Cs is a multiple of 8)
+        // 16-byte loads; whole blocks stay inside the tile (Cs is a multiple of 8).  The float16
+        // format collects the block's gradient words (slot s at word position
+        // ((s & 3) << 1) | (s >> 2), zero past the row's count) and writes them as one 64-byte run
+        // at twice the block's word index.
         for (int b = 0; b < tl; b += 8) {
             uint32_t* blk = tbase + slot_word<REC>(b);
             const uint4 w0 = *reinterpret_cast<const uint4*>(blk);
             const uint4 w1 = *reinterpret_cast<const uint4*>(blk + 4);
             const int ids[8] = {(int)w0.x, (int)w0.z, (int)w1.x, (int)w1.z,
                                 (int)w0.y, (int)w0.w, (int)w1.y, (int)w1.w};
+            uint32_t gq[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) gq[q] = 0u;
 #pragma unroll
             for (int t = 0; t < 8; t += 2) {
                 if (b + t >= tl) break;
-                dyn_pair(nullptr, nullptr, ids[t], ids[t + 1], b + t);
+                uint32_t hq[4] = {0u, 0u, 0u, 0u};
+                dyn_pair(nullptr, nullptr, ids[t], ids[t + 1], b + t, hq);
+                const int p0 = ((t & 3) << 1) | (t >> 2), p1 = (((t + 1) & 3) << 1) | ((t + 1) >> 2);
+                gq[2 * p0] = hq[0]; gq[2 * p0 + 1] = hq[1];
+                gq[2 * p1] = hq[2]; gq[2 * p1 + 1] = hq[3];
+            }
+            if (FMT == IVP_ROWS_F16 && valid) {
+                typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                u32x4* dst = reinterpret_cast<u32x4*>(a.g16 + 2 * (size_t)(blk - a.nb));
+#pragma unroll
+                for (int q = 0; q < 4; ++q) dst[q] = u32x4{gq[4 * q], gq[4 * q + 1], gq[4 * q + 2], gq[4 * q + 3]};
             }
         }
     } else {
         for (int k = 0; k < tl; k += 2) {
             uint32_t* r0 = tbase + slot_word<REC>(k);
             uint32_t* r1 = tbase + slot_word<REC>(k + 1 < tl ? k + 1 : k);
-            dyn_pair(r0, r1, (int)r0[0], (int)r1[0], k);
+            uint32_t hq[4];
+            dyn_pair(r0, r1, (int)r0[0], (int)r1[0], k, hq);
         }
     }
     for (int k = 0; k < ts; ++k) {
@@ -599,10 +618,7 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
             rec[64] = 0u;
             rec[128] = 0u;
             rec[192] = 0u;
-        } else if (FMT == IVP_ROWS_F16) {
-            rec[64] = 0u;
-            rec[128] = 0u;
-        }
+        }   // float16 gradient words past the count were written as zeros with their block
     }
     if (valid) {
         aii += (jx * jx + jy * jy + jz * jz) * a.minv[i];
@@ -948,12 +964,13 @@ int grid_build(ivp_ctx* c) {
         float h2_lo = std::nextafter((float)(hh * (1.0 - 1.0 / 1048576.0)), 0.0f);
         float h2_hi = std::nextafter((float)(hh * (1.0 + 1.0 / 1048576.0)), 1.0f);
         BuildArgs ba{c->nd, c->ns, c->C, c->Cs, c->h, h2_lo, h2_hi, c->kcubic, c->kspiky, 1.0f / c->gscale, c->grid,
-                     c->grids, c->sid, c->pos, c->minv, c->dstart, c->sstart, c->nb, c->tile_len,
+                     c->grids, c->sid, c->pos, c->minv, c->dstart, c->sstart, c->nb, c->g16, c->tile_len,
                      c->rowcnt, c->rowtot, c->density, c->kk, c->Ss, c->err_flag, c->partials, c->max_blocks};
         // bytes (pair count of the previous build): the pair records written, the ids read back
         // once, x and m of every particle once, the per-row outputs (rho, k, S, counts)
         double pairs = (double)c->state_host->total_pairs;
-        double bytes = pairs * (4.0 * c->rec_words / 64 + 4.0) + 16.0 * c->n + 32.0 * c->nd;
+        double bytes = pairs * (4.0 * c->rec_words / 64 + 4.0 + (c->row_fmt == IVP_ROWS_F16 ? 8.0 : 0.0)) +
+                       16.0 * c->n + 32.0 * c->nd;
         const dim3 nb(blocks_for(c->nd));
         const int G = build_lanes_for(c);
         if (G == 1) {

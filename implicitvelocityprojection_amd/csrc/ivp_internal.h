@@ -187,6 +187,8 @@ struct ivp_ctx {
 
     // neighbour rows
     uint32_t* nb = nullptr;
+    uint32_t* g16 = nullptr;   // float16 format: gradW / gscale, two half-pair words per slot at
+                               // twice the word index of the slot's id
     int* tile_len = nullptr;
 
     // cell tables and sort scratch

@@ -40,8 +40,9 @@ using namespace ivp;
 namespace {
 
 // Row format seen by one pass: REC words per (tile, slot) record and the source of gradW --
-// evaluated from positions (GS_EVAL), or read from the record as float32 (GS_F32) or as two
-// packed float16 words scaled by Rows::gs (GS_F16).
+// evaluated from positions (GS_EVAL), read from the record as float32 (GS_F32), or read as two
+// packed float16 words scaled by Rows::gs from the side array Rows::g16 (GS_F16: the ids-only
+// layout, REC = 64, the two words of a slot at twice the word index of its id).
 enum { GS_EVAL = 0, GS_F32 = 1, GS_F16 = 2 };
 template <unsigned REC_, int GS_>
 struct Fmt {
@@ -51,8 +52,8 @@ struct Fmt {
 };
 using FmtIds = Fmt<64u, GS_EVAL>;
 using FmtF32 = Fmt<256u, GS_F32>;
-using FmtF16 = Fmt<192u, GS_F16>;
-using FmtF16Eval = Fmt<192u, GS_EVAL>;   // FP16 rows, gradW evaluated (the exact operator)
+using FmtF16 = Fmt<64u, GS_F16>;
+using FmtF16Eval = FmtIds;   // the float16 format's passes outside the IP CG: ids, gradW evaluated
 
 __device__ __forceinline__ float half_lo(uint32_t w) {
     return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
@@ -63,6 +64,7 @@ __device__ __forceinline__ float half_hi(uint32_t w) {
 
 struct Rows {
     const uint32_t* nb;
+    const uint32_t* g16;   // float16 gradW side array (float16 format): words 2w, 2w + 1 for id word w
     const int* tile_len;   // per tile: slots to walk, a multiple of ROW_PAD, padded with the row itself
     int Cs;                // slots per row in the tile layout
     int nd;
@@ -181,12 +183,16 @@ struct Vec {
 // four lanes per row, slots g + 8 b and g + 4 + 8 b are adjacent words (row_word0 / slot_word),
 // so the lane reads both ids with one 8-byte load; the slots are visited in the same order.
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 template <class FM, int NB, int G = 1, int U = 4, class LA, class LB, class F>
 __device__ __forceinline__ void row_pairs(const Rows& R, int i, float4 xi, LA&& loadA, LB&& loadB, F&& f, int g = 0) {
     constexpr unsigned REC = FM::REC;
-    const uint32_t* e = R.nb + (size_t)(i >> 6) * (size_t)R.Cs * REC + row_word0<REC>(i & (WAVE - 1));
+    static_assert(FM::GS != GS_F16 || REC == 64u, "float16 gradients live beside the ids-only layout");
+    const size_t w_row = (size_t)(i >> 6) * (size_t)R.Cs * REC + row_word0<REC>(i & (WAVE - 1));
+    const uint32_t* e = R.nb + w_row;
     const int tl = __builtin_amdgcn_readfirstlane(R.tile_len[i >> 6]);   // one tile per wave
-    auto visit = [&](int j, const uint32_t* r) {
+    // gw: the slot's gradient words (float32 x, y, z at 64-word strides, or two float16 words)
+    auto visit = [&](int j, const uint32_t* gw) {
         float4 Aj = loadA(j);
         Vec<NB> Bj = loadB(j);
         float gx, gy, gz;
@@ -195,30 +201,40 @@ __device__ __forceinline__ void row_pairs(const Rows& R, int i, float4 xi, LA&& 
             float sc = grad_scale(R.h, R.ks, rx, ry, rz);
             gx = sc * rx; gy = sc * ry; gz = sc * rz;
         } else if (FM::GS == GS_F32) {
-            gx = __uint_as_float(__builtin_nontemporal_load(r + 64));
-            gy = __uint_as_float(__builtin_nontemporal_load(r + 128));
-            gz = __uint_as_float(__builtin_nontemporal_load(r + 192));
+            gx = __uint_as_float(__builtin_nontemporal_load(gw + 64));
+            gy = __uint_as_float(__builtin_nontemporal_load(gw + 128));
+            gz = __uint_as_float(__builtin_nontemporal_load(gw + 192));
         } else {
-            uint32_t w1 = __builtin_nontemporal_load(r + 64), w2 = __builtin_nontemporal_load(r + 128);
+            const uint32_t w1 = gw[0], w2 = gw[1];
             gx = R.gs * half_lo(w1); gy = R.gs * half_hi(w1); gz = R.gs * half_lo(w2);
         }
         f(Aj, Bj, gx, gy, gz);
     };
     if constexpr (REC == 64u && G == 4) {
         // tl is a multiple of ROW_PAD = 8: whole blocks; block b's pair for this lane is the
-        // 8-byte word pair 2 g of the row's 8 words in block b (512 words per block)
+        // 8-byte word pair 2 g of the row's 8 words in block b (512 words per block), and its
+        // float16 gradients the 16-byte quad at twice that word index
         const u32x2* e2 = reinterpret_cast<const u32x2*>(e + 2 * g);
 #pragma unroll((U + 1) / 2)
         for (int b = 0; b < tl; b += 8) {
             const u32x2 ids = __builtin_nontemporal_load(e2 + (b >> 3) * 256);
-            visit((int)ids.x, nullptr);
-            visit((int)ids.y, nullptr);
+            if constexpr (FM::GS == GS_F16) {
+                const u32x4* g4 = reinterpret_cast<const u32x4*>(R.g16 + 2 * (w_row + 2 * g));
+                const u32x4 q = __builtin_nontemporal_load(g4 + (b >> 3) * 256);
+                const uint32_t w0[2] = {q.x, q.y}, w1[2] = {q.z, q.w};
+                visit((int)ids.x, w0);
+                visit((int)ids.y, w1);
+            } else {
+                visit((int)ids.x, nullptr);
+                visit((int)ids.y, nullptr);
+            }
         }
     } else {
 #pragma unroll(U)
         for (int k = g; k < tl; k += G) {
             const uint32_t* r = e + slot_word<REC>(k);
-            visit((int)__builtin_nontemporal_load(r), r);
+            const uint32_t* gw = FM::GS == GS_F16 ? R.g16 + 2 * (size_t)(r - R.nb) : r;
+            visit((int)__builtin_nontemporal_load(r), gw);
         }
     }
 }
@@ -1448,7 +1464,7 @@ __global__ __launch_bounds__(BLOCK) void k_advect_boundary(int nd, float dt, flo
     pos[i] = make_float4(x[0], x[1], x[2], p.w);
 }
 
-Rows rows_of(ivp_ctx* c) { return Rows{c->nb, c->tile_len, c->Cs, c->nd, c->pos, c->h, c->kspiky, c->gscale}; }
+Rows rows_of(ivp_ctx* c) { return Rows{c->nb, c->g16, c->tile_len, c->Cs, c->nd, c->pos, c->h, c->kspiky, c->gscale}; }
 
 // ---- independent scenes -------------------------------------------------------------------
 
