@@ -1092,13 +1092,17 @@ __global__ __launch_bounds__(BLOCK) void k_w18_warm_init(W18Args a, const float*
 }
 
 // The correction of this solve in user order, the next warm start's initial guess (the
-// reference's self.dv, indexed by particle)
-__global__ __launch_bounds__(BLOCK) void k_w18_save(int nd, const float* __restrict__ dv, const int* __restrict__ uid,
-                                                    float* __restrict__ dv_prev) {
-    int i = blockIdx.x * BLOCK + threadIdx.x;
+// reference's self.dv, indexed by particle): one thread per user index, contiguous writes,
+// the internal row found through inv (static particles keep their zero)
+__global__ __launch_bounds__(BLOCK) void k_w18_save(int n, int nd, const float* __restrict__ dv,
+                                                    const int* __restrict__ inv, float* __restrict__ dv_prev) {
+    int u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= n) return;
+    const int i = inv[u];
     if (i >= nd) return;
-    float* d = dv_prev + 3 * (size_t)uid[i];
-    d[0] = dv[3 * i + 0]; d[1] = dv[3 * i + 1]; d[2] = dv[3 * i + 2];
+    dv_prev[3 * u + 0] = dv[3 * i + 0];
+    dv_prev[3 * u + 1] = dv[3 * i + 1];
+    dv_prev[3 * u + 2] = dv[3 * i + 2];
 }
 
 // PCG prologue (weiler18.py:320-335): dv = 0, r = rhs, z = P r, p = z, rz = r.z
@@ -1818,8 +1822,8 @@ int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
         c->w18_guess = std::max(2, iters);
     }
     *iters_out = iters;
-    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_w18_save, nb, nd, (const float*)c->w_dv, (const int*)c->uid,
-                   c->w_dv_prev));
+    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_w18_save, dim3(blocks_for(c->n)), c->n, nd, (const float*)c->w_dv,
+                   (const int*)c->inv, c->w_dv_prev));
     return launch(c, KC_GLUE, -1, 0.0, k_w18_apply, dim3(blocks_for(3 * nd)), 3 * nd, (const float*)c->w_dv, c->vel);
 }
 
