@@ -152,10 +152,16 @@ def precision_study(ctxs, cfg, ps, m, m_inv, prm, steps, torch, dev, n, timed_st
     xd, vd = rd(first, 0), rd(first, 1)
     for nm in names:
         c = ctxs[nm]
-        c.write(0, xd)
-        c.write(1, vd)
-        if c.neighbors_build() != 0:
-            raise RuntimeError("neighbour overflow")
+        # three untimed steps first (the GPU idled through the CPU restatement above), then the
+        # timed steps from the common state
+        for warm in (True, False):
+            c.write(0, xd)
+            c.write(1, vd)
+            if c.neighbors_build() != 0:
+                raise RuntimeError("neighbour overflow")
+            if warm:
+                for _ in range(3):
+                    c.step(cfg.dt, cfg.gravity, cfg.viscosity_method, cfg.viscosity_coeff, prm)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         errs = 0.0
@@ -361,6 +367,9 @@ def main():
                     help="neighbour-row format: 0 ids only, gradW from positions (4 B per pair); 1 float32 gradW "
                          "cached (16 B per pair); 2 float16 gradW cached for the IP CG operator (12 B per pair) "
                          "with a float32 true-residual refresh every 8 CG iterations")
+    ap.add_argument("--tile-windows", type=int, default=1, choices=[0, 1],
+                    help="1 (default): the pair passes stage each 64-row tile's distinct neighbours in LDS; "
+                         "0: every pair gathers its neighbour from memory (same results)")
     ap.add_argument("--precision-steps", type=int, default=1,
                     help="steps (from the state after the timed region) whose GPU result, in float32 and in the "
                          "float16 precision form, is compared with the float64 restatement of the step; then 20 "
@@ -411,6 +420,7 @@ def main():
     t = lambda a, dt=torch.float32: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)
     ctx = Context(h, cfg.domain_start, cfg.domain_size, ps.n, cache_size=int(cfg.get_cfg("neighborCacheSize", 60)))
     ctx.gradient_cache = args.gradient_cache
+    ctx.tile_windows = bool(args.tile_windows)
     ctx.set_particles(t(ps.x), t(ps.v), t(m), t(m_inv), t(ps.density0), t(ps.is_dynamic, torch.int32),
                       t(ps.material, torch.int32))
     if ctx.neighbors_build() != 0:
@@ -582,7 +592,8 @@ def main():
                    "surface_tension": f"Jeske23 {st_coeff:g}" if st_coeff != 0.0 else "off",
                    "pressure": {"ip": "IP", "prj": "PRJ"}[args.pressure],
                    "densityErrorTol": prm.tol_opt,
-                   "maxOuterIters": prm.max_iteration_opt, "neighbour_pairs": pairs_end},
+                   "maxOuterIters": prm.max_iteration_opt, "neighbour_pairs": pairs_end,
+                   "tile_windows": dict(zip(("tiles", "windowed", "staged_ids"), ctx.window_stats()))},
         "cg_iters_per_s": cg_iters / (solve_ms * 1e-3) if solve_ms > 0 else 0.0,
         "cg_iters": cg_iters,
         "cg_iters_per_step_hist": {str(k): cg_hist[k] for k in sorted(cg_hist)},

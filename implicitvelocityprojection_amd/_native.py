@@ -30,11 +30,13 @@ OPT_SCENE_LANES = 4
 OPT_BUILD_LANES = 5
 OPT_W18_WARM_START = 6
 OPT_W18_PRECONDITIONER = 7
+OPT_TILE_WINDOWS = 8
 MIN_SCENE_DYNAMIC = 256
 
 # kernel classes of ivp_profile_read / ivp_profile_enable masks
 KCLASSES = ["sort", "build", "ip_pack", "ip_rhs", "cg_J", "cg_JT", "cg_update", "cg_p", "visc", "w18_Av",
-            "w18_update", "w18_p", "w18_pre", "cg_refresh", "st_setup", "st_Av", "prj_head", "prj_tail", "glue"]
+            "w18_update", "w18_p", "w18_pre", "cg_refresh", "st_setup", "st_Av", "prj_head", "prj_tail", "glue",
+            "window", "scan"]
 
 
 class Config(ctypes.Structure):
@@ -119,6 +121,7 @@ SIGNATURES = {
     "ivp_profile_reset": (_I, [_P]),
     "ivp_reset": (_I, [_P]),
     "ivp_counts": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_longlong)]),
+    "ivp_tile_windows": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_longlong)]),
     "ivp_set_scenes": (_I, [_P, _I, ctypes.POINTER(Scene)]),
     "ivp_step_scenes": (_I, [_P, ctypes.POINTER(StepParams), ctypes.POINTER(SceneStats)]),
     "ivp_max_speed2": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),

@@ -127,6 +127,13 @@ class Context:
         N.check(self.lib.ivp_counts(self.h, None, None, ctypes.byref(p)))
         return p.value
 
+    def window_stats(self):
+        """(tiles, tiles with a window, neighbour ids staged per pass) of the last build."""
+        t, w, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+        N.check(self.lib.ivp_tile_windows(self.h, ctypes.byref(t), ctypes.byref(w), ctypes.byref(k)),
+                "ivp_tile_windows")
+        return t.value, w.value, k.value
+
     @property
     def gradient_cache(self):
         """Neighbour-row format: ROWS_IDS (0, gradW evaluated per pass), ROWS_F32 (1, float32
@@ -346,6 +353,17 @@ class Context:
     @w18_preconditioner.setter
     def w18_preconditioner(self, on):
         N.check(self.lib.ivp_set_option(self.h, N.OPT_W18_PRECONDITIONER, int(bool(on))), "ivp_set_option")
+
+    @property
+    def tile_windows(self):
+        """Pair passes load each 64-row tile's distinct neighbours into on-chip memory once
+        (True, default) instead of gathering every pair's neighbour; a change applies from the
+        next neighbour build on.  Results are bitwise the same."""
+        return bool(self._opt(N.OPT_TILE_WINDOWS))
+
+    @tile_windows.setter
+    def tile_windows(self, on):
+        N.check(self.lib.ivp_set_option(self.h, N.OPT_TILE_WINDOWS, int(bool(on))), "ivp_set_option")
 
     # -- instrumentation ----------------------------------------------------------------------
     def profile_enable(self, mask=-1):

@@ -142,6 +142,13 @@ int ensure_rows(ivp_ctx* c) {
     if (c->nb) return IVP_OK;
     IVP_TRY(dalloc(&c->nb, slots * (size_t)(c->rec_words / 64)));
     if (c->row_fmt == IVP_ROWS_F16) IVP_TRY(dalloc(&c->g16, 2 * slots));
+    if (c->rec_words == 64) {
+        // tile windows: 16 bits per slot, WIN_MAX ids and a count per tile
+        IVP_TRY(dalloc(&c->loc, slots / 2));
+        IVP_TRY(dalloc(&c->win_ids, tiles * (size_t)WIN_MAX));
+        IVP_TRY(dalloc(&c->win_n, tiles));
+    }
+    c->win_built = false;
     return IVP_OK;
 }
 
@@ -149,6 +156,10 @@ void free_rows(ivp_ctx* c) {
     scene_graphs_clear(c);
     dfree(c->nb);
     dfree(c->g16);
+    dfree(c->loc);
+    dfree(c->win_ids);
+    dfree(c->win_n);
+    c->win_built = false;
     dfree(c->st_gt);
 }
 
@@ -381,7 +392,7 @@ static int build_status(ivp_ctx* c) {
 static const char* kclass_name(int k) {
     static const char* names[KC_COUNT] = {"sort", "build", "ip_pack", "ip_rhs", "cg_J", "cg_JT", "cg_update", "cg_p",
                                           "visc", "w18_Av", "w18_update", "w18_p", "w18_pre", "cg_refresh", "st_setup", "st_Av", "prj_head", "prj_tail",
-                                          "glue"};
+                                          "glue", "window", "scan"};
     return (k >= 0 && k < KC_COUNT) ? names[k] : "?";
 }
 
@@ -534,6 +545,12 @@ int ivp_set_option(ivp_ctx* c, int option, int value) {
         c->w18_precond = value != 0;
         return IVP_OK;
     }
+    if (option == IVP_OPT_TILE_WINDOWS) {
+        if (value < 0) return IVP_ERR_ARG;
+        c->windows = value != 0;
+        c->win_cap = value > 1 ? std::min(value, WIN_MAX) : WIN_MAX;
+        return IVP_OK;
+    }
     if (option == IVP_OPT_BUILD_LANES) {
         if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16) return IVP_ERR_ARG;
         c->build_lanes = value;
@@ -582,6 +599,10 @@ int ivp_get_option(ivp_ctx* c, int option, int* value) {
     }
     if (option == IVP_OPT_W18_PRECONDITIONER) {
         *value = c->w18_precond ? 1 : 0;
+        return IVP_OK;
+    }
+    if (option == IVP_OPT_TILE_WINDOWS) {
+        *value = !c->windows ? 0 : c->win_cap < WIN_MAX ? c->win_cap : 1;
         return IVP_OK;
     }
     return IVP_ERR_ARG;
@@ -799,6 +820,27 @@ int ivp_counts(ivp_ctx* c, int* n_total, int* n_dynamic, long long* dyn_pairs) {
             *dyn_pairs = c->state_host->total_pairs;
         }
     }
+    return IVP_OK;
+}
+
+int ivp_tile_windows(ivp_ctx* c, int* tiles, int* windowed, long long* window_ids) {
+    if (!c) return IVP_ERR_ARG;
+    int t = 0, w = 0;
+    long long ids = 0;
+    if (c->win_built && c->nd > 0) {
+        t = (c->nd + WAVE - 1) / WAVE;
+        std::vector<int> n((size_t)t);
+        IVP_HIP(hipMemcpyAsync(n.data(), c->win_n, sizeof(int) * (size_t)t, hipMemcpyDeviceToHost, c->stream));
+        IVP_HIP(hipStreamSynchronize(c->stream));
+        for (int k : n)
+            if (k >= 0) {
+                w++;
+                ids += k;
+            }
+    }
+    if (tiles) *tiles = t;
+    if (windowed) *windowed = w;
+    if (window_ids) *window_ids = ids;
     return IVP_OK;
 }
 

@@ -15,6 +15,12 @@ constexpr int WAVE = 64;
 constexpr int BLOCK = 256;            // 4 waves; every row kernel maps wave w of a block to tile
 constexpr int WAVES_PER_BLOCK = BLOCK / WAVE;
 
+// Tile windows: the pair passes stage the distinct neighbours of a 64-row tile in LDS, at most
+// WIN_MAX of them, found by the build in a bitmap of WIN_SPAN_WORDS words over the tile's id
+// range; a tile outside either bound gathers its neighbours from memory.
+constexpr int WIN_MAX = 608;
+constexpr int WIN_SPAN_WORDS = 3072;
+
 struct Grid {
     float origin[3];
     float inv_w;
@@ -117,6 +123,15 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks) {
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        int u = __shfl_xor(v, o, WAVE);
+        v = u < v ? u : v;
+    }
     return v;
 }
 

@@ -630,9 +630,128 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
         a.rowcnt[i] = cnt;
         a.rowtot[i] = tot;
     }
-    double v[1] = {(double)cnt};
-    block_partials<1>(v, a.partials, a.stride);
+    // row 1 starts at zero: the window pass adds its tiles' window sizes there
+    double v[2] = {(double)cnt, 0.0};
+    block_partials<2>(v, a.partials, a.stride);
     if (lane == 0) a.tile_len[tile] = tlp;
+}
+
+struct WinArgs {
+    int nd, Cs;
+    int cap;   // largest window kept, <= WIN_MAX
+    const uint32_t* nb;
+    const int* tile_len;
+    uint32_t* loc;
+    int* win_ids;
+    int* win_n;
+    double* partials;   // the build's partials (row 1: window sizes)
+    int stride;
+};
+
+// Tile windows of ids-only rows, one block per 64-row tile.  The window is the ascending list of
+// the distinct ids in the walked slots [0, tile_len) of the tile's valid rows (padding slots name
+// the row itself); each such slot gets the index of its id in that list, 16 bits at the slot's id
+// word (the 8-byte word pair of slots g, g + 4 becomes one 32-bit word).  The ids are marked in
+// an LDS bitmap over [lo, hi] of the tile's ids, and an id's index is the number of marks below
+// it; every slot then writes its id at its index into the list.  A tile whose range exceeds
+// WIN_SPAN_WORDS words, or whose window exceeds the cap (at most WIN_MAX ids), gets win_n = -1
+// and no indices.  Each thread keeps its PPT word pairs of the tile in registers from one
+// coalesced load.  Measured on the 1M dam break: 83 us per build at the start, 107 us after 40
+// steps (a bit loop writing the list, thread-contiguous pairs or per-word merged marks were slower).
+template <int PPT>
+__global__ __launch_bounds__(BLOCK) void k_window(WinArgs a) {
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    __shared__ uint32_t bm[WIN_SPAN_WORDS];
+    __shared__ uint16_t pre[WIN_SPAN_WORDS];
+    __shared__ int red[2 * WAVES_PER_BLOCK];
+    const int t = xcd_block(blockIdx.x, gridDim.x);
+    const int nrow = min(WAVE, a.nd - t * WAVE);
+    const int npair = a.tile_len[t] * (WAVE / 2);
+    const size_t base = (size_t)t * (size_t)a.Cs * WAVE;
+    const u32x2* ids2 = reinterpret_cast<const u32x2*>(a.nb + base);
+    // word pair q of a tile: block q >> 8, row (q >> 2) & 63; thread pairs q = thread + BLOCK k
+    u32x2 v[PPT];
+    bool on[PPT];
+    int lo = 0x7fffffff, hi = -1;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int q = (int)threadIdx.x + BLOCK * k;
+        on[k] = q < npair && ((q >> 2) & (WAVE - 1)) < nrow;
+        v[k] = on[k] ? ids2[q] : u32x2{0u, 0u};
+    }
+#pragma unroll
+    for (int k = 0; k < PPT; ++k)
+        if (on[k]) {
+            lo = min(lo, (int)min(v[k].x, v[k].y));
+            hi = max(hi, (int)max(v[k].x, v[k].y));
+        }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        red[w] = lo;
+        red[WAVES_PER_BLOCK + w] = hi;
+    }
+    __syncthreads();
+    lo = min(min(red[0], red[1]), min(red[2], red[3]));
+    hi = max(max(red[4], red[5]), max(red[6], red[7]));
+    if (hi < 0) {   // no walked slot
+        if (threadIdx.x == 0) a.win_n[t] = 0;
+        return;
+    }
+    const int words = ((hi - lo) >> 5) + 1;
+    if (words > WIN_SPAN_WORDS) {
+        if (threadIdx.x == 0) a.win_n[t] = -1;
+        return;
+    }
+    for (int k = threadIdx.x; k < words; k += BLOCK) bm[k] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PPT; ++k)
+        if (on[k]) {
+            const uint32_t p0 = v[k].x - (uint32_t)lo, p1 = v[k].y - (uint32_t)lo;
+            atomicOr(&bm[p0 >> 5], 1u << (p0 & 31));
+            atomicOr(&bm[p1 >> 5], 1u << (p1 & 31));
+        }
+    __syncthreads();
+    // thread k counts the marks of words [w0, w1); exclusive scan over the block
+    const int per = (words + BLOCK - 1) / BLOCK;
+    const int w0 = min((int)threadIdx.x * per, words), w1 = min(w0 + per, words);
+    int cnt = 0;
+    for (int k = w0; k < w1; ++k) cnt += __popc(bm[k]);
+    int total;
+    int off = block_excl_scan(cnt, &total);
+    if (total > a.cap) {
+        if (threadIdx.x == 0) a.win_n[t] = -1;
+        return;
+    }
+    for (int k = w0; k < w1; ++k) {
+        pre[k] = (uint16_t)off;
+        off += __popc(bm[k]);
+    }
+    __syncthreads();
+    auto index = [&](uint32_t j) {
+        const uint32_t p = j - (uint32_t)lo;
+        return (uint32_t)pre[p >> 5] + (uint32_t)__popc(bm[p >> 5] & ((1u << (p & 31)) - 1u));
+    };
+    // every window id is named by some walked slot: each slot writes its id at its index (slots
+    // of one id write the same word)
+    int* wid = a.win_ids + (size_t)t * WIN_MAX;
+    uint32_t* l32 = a.loc + base / 2;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k)
+        if (on[k]) {
+            const uint32_t i0 = index(v[k].x), i1 = index(v[k].y);
+            wid[i0] = (int)v[k].x;
+            wid[i1] = (int)v[k].y;
+            l32[threadIdx.x + BLOCK * k] = i0 | (i1 << 16);
+        }
+    if (threadIdx.x == 0) {
+        a.win_n[t] = total;
+        // the build's partial row 1 (zeroed by the build's blocks of four tiles; the sums are
+        // integers, exact in any order)
+        atomicAdd(&a.partials[a.stride + t / WAVES_PER_BLOCK], (double)total);
+    }
 }
 
 struct ExportArgs {
@@ -941,10 +1060,10 @@ template <int FMT, bool SCENES>
 int launch_scan(ivp_ctx* c, int G, const BuildArgs& ba) {
     const dim3 grid(blocks_for(c->nd * G));
     switch (G) {
-    case 2: return launch(c, KC_BUILD, -1, 0.0, k_scan<FMT, SCENES, 2>, grid, ba);
-    case 4: return launch(c, KC_BUILD, -1, 0.0, k_scan<FMT, SCENES, 4>, grid, ba);
-    case 8: return launch(c, KC_BUILD, -1, 0.0, k_scan<FMT, SCENES, 8>, grid, ba);
-    case 16: return launch(c, KC_BUILD, -1, 0.0, k_scan<FMT, SCENES, 16>, grid, ba);
+    case 2: return launch(c, KC_SCAN, -1, 0.0, k_scan<FMT, SCENES, 2>, grid, ba);
+    case 4: return launch(c, KC_SCAN, -1, 0.0, k_scan<FMT, SCENES, 4>, grid, ba);
+    case 8: return launch(c, KC_SCAN, -1, 0.0, k_scan<FMT, SCENES, 8>, grid, ba);
+    case 16: return launch(c, KC_SCAN, -1, 0.0, k_scan<FMT, SCENES, 16>, grid, ba);
     default: return IVP_ERR_ARG;
     }
 }
@@ -998,8 +1117,24 @@ int grid_build(ivp_ctx* c) {
             else
                 IVP_TRY(launch(c, KC_BUILD, -1, bytes, k_eval<IVP_ROWS_IDS>, nb, ba));
         }
-        IVP_TRY(finalize<FIN_BUILD>(c, nb.x));
     }
+    c->win_built = false;
+    // a tile walks at most Cs slots: Cs * 32 word pairs, PPT per thread
+    const int ppt = (c->Cs * (WAVE / 2) + BLOCK - 1) / BLOCK;
+    if (c->nd > 0 && c->windows && c->rec_words == 64 && c->S == 0 && ppt <= 32) {
+        WinArgs wa{c->nd, c->Cs, c->win_cap, c->nb, c->tile_len, c->loc, c->win_ids, c->win_n, c->partials,
+                   c->max_blocks};
+        const dim3 grid((c->nd + WAVE - 1) / WAVE);
+        if (ppt <= 8)
+            IVP_TRY(launch(c, KC_WINDOW, -1, 0.0, k_window<8>, grid, wa));
+        else if (ppt <= 16)
+            IVP_TRY(launch(c, KC_WINDOW, -1, 0.0, k_window<16>, grid, wa));
+        else
+            IVP_TRY(launch(c, KC_WINDOW, -1, 0.0, k_window<32>, grid, wa));
+        c->win_built = true;
+    }
+    // pair count (row 0) and window sizes (row 1) of the build's blocks
+    if (c->nd > 0) IVP_TRY(finalize<FIN_BUILD>(c, (int)blocks_for(c->nd)));
     c->built = true;
     return IVP_OK;
 }

@@ -65,6 +65,7 @@ struct SolverState {
     long long touched_pairs;
     long long active_pairs;
     long long total_pairs;  // stored (dynamic-dynamic) neighbour pairs of the last build
+    long long win_entries;  // neighbour ids in the tile windows of the last build
 };
 
 // Independent scenes in one context.  Each scene has its own cell grid (keys of scene s follow
@@ -131,6 +132,8 @@ enum KernelClass {
     KC_PRJ_HEAD,
     KC_PRJ_TAIL,
     KC_GLUE,
+    KC_WINDOW,
+    KC_SCAN,
     KC_COUNT
 };
 
@@ -190,6 +193,16 @@ struct ivp_ctx {
     uint32_t* g16 = nullptr;   // float16 format: gradW / gscale, two half-pair words per slot at
                                // twice the word index of the slot's id
     int* tile_len = nullptr;
+    // tile windows (ids-only layout, one scene): per 64-row tile its distinct neighbour ids in
+    // ascending order (win_ids, WIN_MAX per tile) and their count (win_n, -1: the tile has no
+    // window); per walked slot of a valid row the index of its id in the window, 16 bits at the
+    // slot's id word (loc).  win_built: the last build wrote them.
+    uint32_t* loc = nullptr;
+    int* win_ids = nullptr;
+    int* win_n = nullptr;
+    bool windows = true;
+    int win_cap = ivp::WIN_MAX;   // largest window kept (a smaller cap leaves more tiles without)
+    bool win_built = false;
 
     // cell tables and sort scratch
     int* dstart = nullptr;  // ncell + 1

@@ -81,6 +81,7 @@ template <int OP>
 __device__ __forceinline__ void fin_apply(SolverState* st, const double (&t)[3], double tol, int max_iter) {
     if (OP == FIN_BUILD) {
         st->total_pairs = (long long)t[0];
+        st->win_entries = (long long)t[1];
     } else if (OP == FIN_PACK) {
         st->err_sum = t[0];
         st->n_active_rows = (int)t[1];
@@ -175,7 +176,7 @@ __device__ __forceinline__ void fin_apply(SolverState* st, const double (&t)[3],
 
 template <int OP>
 constexpr int fin_nv() {
-    return OP == FIN_PACK ? 3 : (OP == FIN_RHS || OP == FIN_W18_UPDATE || OP == FIN_CG_REPLACE) ? 2 : 1;
+    return OP == FIN_PACK ? 3 : (OP == FIN_BUILD || OP == FIN_RHS || OP == FIN_W18_UPDATE || OP == FIN_CG_REPLACE) ? 2 : 1;
 }
 
 // One block: fold the partials of the previous kernel in a fixed order and advance the solver

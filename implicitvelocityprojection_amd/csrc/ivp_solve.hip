@@ -72,6 +72,9 @@ struct Rows {
     float h;
     float ks;            // 45 / (pi h^6)
     float gs;            // scale of float16 gradW records
+    const uint32_t* loc; // tile windows: the window index of each walked slot, 16 bits per slot
+    const int* win_ids;  // WIN_MAX ids per tile
+    const int* win_n;    // ids per tile window (-1: none); nullptr when the rows have no windows
 };
 
 // gradW_ij = s * r_ij in the evaluated form; returns s
@@ -184,7 +187,9 @@ struct Vec {
 // so the lane reads both ids with one 8-byte load; the slots are visited in the same order.
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-template <class FM, int NB, int G = 1, int U = 4, class LA, class LB, class F>
+// With LOC (tile windows, four lanes per row) the lane reads its two slots' window indices as
+// one 32-bit word at half the word index of their ids, and loadA / loadB take window indices.
+template <class FM, int NB, int G = 1, int U = 4, bool LOC = false, class LA, class LB, class F>
 __device__ __forceinline__ void row_pairs(const Rows& R, int i, float4 xi, LA&& loadA, LB&& loadB, F&& f, int g = 0) {
     constexpr unsigned REC = FM::REC;
     static_assert(FM::GS != GS_F16 || REC == 64u, "float16 gradients live beside the ids-only layout");
@@ -210,7 +215,24 @@ __device__ __forceinline__ void row_pairs(const Rows& R, int i, float4 xi, LA&& 
         }
         f(Aj, Bj, gx, gy, gz);
     };
-    if constexpr (REC == 64u && G == 4) {
+    static_assert(!LOC || (REC == 64u && G == 4), "tile windows index the paired ids-only layout");
+    if constexpr (LOC) {
+        const uint32_t* l = R.loc + ((w_row + 2 * g) >> 1);
+#pragma unroll((U + 1) / 2)
+        for (int b = 0; b < tl; b += 8) {
+            const uint32_t q = __builtin_nontemporal_load(l + (b >> 3) * 256);
+            if constexpr (FM::GS == GS_F16) {
+                const u32x4* g4 = reinterpret_cast<const u32x4*>(R.g16 + 2 * (w_row + 2 * g));
+                const u32x4 h = __builtin_nontemporal_load(g4 + (b >> 3) * 256);
+                const uint32_t w0[2] = {h.x, h.y}, w1[2] = {h.z, h.w};
+                visit((int)(q & 0xffffu), w0);
+                visit((int)(q >> 16), w1);
+            } else {
+                visit((int)(q & 0xffffu), nullptr);
+                visit((int)(q >> 16), nullptr);
+            }
+        }
+    } else if constexpr (REC == 64u && G == 4) {
         // tl is a multiple of ROW_PAD = 8: whole blocks; block b's pair for this lane is the
         // 8-byte word pair 2 g of the row's 8 words in block b (512 words per block), and its
         // float16 gradients the 16-byte quad at twice that word index
@@ -237,6 +259,64 @@ __device__ __forceinline__ void row_pairs(const Rows& R, int i, float4 xi, LA&& 
             visit((int)__builtin_nontemporal_load(r), gw);
         }
     }
+}
+
+// Tile windows: the block of a four-lane pass over one scene is one 64-row tile.  stage_window
+// loads the tile's window into LDS, A[k] = la(win[k]) and B[k] = lb(win[k]) (the pass's B record
+// in the first NB components of a float4), and returns true; it returns false, staging nothing,
+// when the rows have no windows or this tile has none.  Every thread of the block calls it (it
+// synchronises the block); the result is block-uniform.  row_pairs_w then walks a row through
+// the window, or through memory when there is none: the same values in the same order.
+template <class FM, int G>
+constexpr bool has_windows() { return FM::REC == 64u && G == 4; }
+
+template <class FM, int G>
+struct WinLds {
+    static constexpr int N = has_windows<FM, G>() ? WIN_MAX : 1;
+    float4 A[N];
+    float4 B[N];
+};
+
+template <class FM, int G, class LA, class LB>
+__device__ __forceinline__ bool stage_window(const Rows& R, int tile, WinLds<FM, G>& w, LA&& la, LB&& lb) {
+    if constexpr (!has_windows<FM, G>()) {
+        return false;
+    } else {
+        static_assert(BLOCK / G == WAVE, "a four-lane pass block is one tile");
+        if (R.win_n == nullptr) return false;
+        const int nw = __builtin_amdgcn_readfirstlane(R.win_n[tile]);
+        if (nw < 0) return false;
+        const int* wid = R.win_ids + (size_t)tile * WIN_MAX;
+        for (int k = threadIdx.x; k < nw; k += BLOCK) {
+            const int j = wid[k];
+            w.A[k] = la(j);
+            w.B[k] = lb(j);
+        }
+        __syncthreads();
+        return true;
+    }
+}
+
+template <int NB>
+__device__ __forceinline__ Vec<NB> vec_of(const float4& b) {
+    Vec<NB> B;
+    const float c[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int q = 0; q < (NB > 0 ? NB : 1); ++q) B.v[q] = c[q];
+    return B;
+}
+
+template <class FM, int NB, int G, class LA, class LB, class F>
+__device__ __forceinline__ void row_pairs_w(bool win, const WinLds<FM, G>& w, const Rows& R, int i, float4 xi,
+                                            LA&& loadA, LB&& loadB, F&& f, int g) {
+    if constexpr (has_windows<FM, G>()) {
+        if (win) {
+            row_pairs<FM, NB, G, PASS_UNROLL, true>(R, i, xi, [&](int k) { return w.A[k]; },
+                                                    [&](int k) { return vec_of<NB>(w.B[k]); }, f, g);
+            return;
+        }
+    }
+    row_pairs<FM, NB, G, PASS_UNROLL>(R, i, xi, loadA, loadB, f, g);
 }
 
 // ---- IP ------------------------------------------------------------------------------------
@@ -295,11 +375,13 @@ __global__ __launch_bounds__(BLOCK) void k_ip_pack(PackArgs a) {
     // neighbour record: A = (x, y, z, m), B = d
     auto loadA = [&](int j) { return j_record<FM::EVAL>(a.R.pos, a.d4, j); };
     auto loadB = [&](int j) { return vec3_of(a.d4, j); };
+    __shared__ WinLds<FM, G> w;
+    const bool win = stage_window(a.R, blk, w, loadA, [&](int j) { return a.d4[j]; });
     double err = 0.0, nact = 0.0, npairs = 0.0;
     float acc = 0.0f;
     if (gt.on) {
         float4 di = a.d4[i];
-        row_pairs<FM, 3, G, PASS_UNROLL>(a.R, i, a.R.pos[i], loadA, loadB,
+        row_pairs_w<FM, 3, G>(win, w, a.R, i, a.R.pos[i], loadA, loadB,
                              [&](const float4& Aj, const Vec<3>& Bj, float gx, float gy, float gz) {
                                  float dx = di.x - Bj.v[0], dy = di.y - Bj.v[1], dz = di.z - Bj.v[2];
                                  acc = fmaf(Aj.w, fmaf(dz, gz, fmaf(dy, gy, dx * gx)), acc);
@@ -361,13 +443,15 @@ __global__ __launch_bounds__(BLOCK) void k_ip_rhs(RhsArgs a) {
         B.v[0] = q.x; B.v[1] = q.y;
         return B;
     };
+    __shared__ WinLds<FM, G> w;
+    const bool win = stage_window(a.R, blk, w, loadA, [&](int j) { return a.q4[j]; });
     double touched = 0.0, tpairs = 0.0;
     if (gt.on) {
         float4 qi = a.q4[i];
         float fki = qi.x, mi = qi.z;
         bool act = qi.y != 0.0f;
         float rx = 0.0f, ry = 0.0f, rz = 0.0f;
-        row_pairs<FM, 2, G, PASS_UNROLL>(a.R, i, a.R.pos[i], loadA, loadB,
+        row_pairs_w<FM, 2, G>(win, w, a.R, i, a.R.pos[i], loadA, loadB,
                              [&](const float4& Aj, const Vec<2>& Bj, float gx, float gy, float gz) {
                                  float sc = fmaf(Aj.w, fki, mi * Bj.v[0]);
                                  rx = fmaf(sc, gx, rx); ry = fmaf(sc, gy, ry); rz = fmaf(sc, gz, rz);
@@ -448,15 +532,21 @@ __global__ __launch_bounds__(BLOCK) void k_cg_J(CGJArgs a) {
     const int i = pass_row<G>(blk), g = pass_lane<G>();
     Gate gt = gate_all(i, a.R.nd);
     if (SC && !gate_block<BLOCK / G>(a.sc, blk, i, a.R.nd, PH_CG, gt)) return;
-    if (!gt.on) return;
-    const float sk = a.sfac[i];
-    if (sk == 0.0f) return;
+    const float sk = gt.on ? a.sfac[i] : 0.0f;
     // neighbour record: A = (x, y, z, m), B = p
     auto loadA = [&](int j) { return j_record<FM::EVAL>(a.R.pos, a.p4, j); };
     auto loadB = [&](int j) { return vec3_of(a.p4, j); };
+    __shared__ WinLds<FM, G> w;
+    bool win = false;
+    if constexpr (has_windows<FM, G>()) {
+        // a tile without active rows neither stages nor walks
+        if (!__syncthreads_or(sk != 0.0f)) return;
+        win = stage_window(a.R, blk, w, loadA, [&](int j) { return a.p4[j]; });
+    }
+    if (sk == 0.0f) return;
     float4 pi = a.p4[i];
     float acc = 0.0f;
-    row_pairs<FM, 3, G, PASS_UNROLL>(a.R, i, a.R.pos[i], loadA, loadB,
+    row_pairs_w<FM, 3, G>(win, w, a.R, i, a.R.pos[i], loadA, loadB,
                          [&](const float4& Aj, const Vec<3>& Bj, float gx, float gy, float gz) {
                              float dx = pi.x - Bj.v[0], dy = pi.y - Bj.v[1], dz = pi.z - Bj.v[2];
                              acc = fmaf(Aj.w, fmaf(dz, gz, fmaf(dy, gy, dx * gx)), acc);
@@ -500,6 +590,16 @@ __global__ __launch_bounds__(BLOCK) void k_cg_JT(CGJTArgs a) {
         B.v[0] = a.y2[j].x;
         return B;
     };
+    __shared__ WinLds<FM, G> w;
+    bool win = false;
+    if constexpr (has_windows<FM, G>()) {
+        // a tile without touched rows stages nothing (its rows only write m p)
+        if (__syncthreads_or(touched))
+            win = stage_window(a.R, blk, w, loadA, [&](int j) {
+                const float2 y = a.y2[j];
+                return make_float4(y.x, y.y, 0.0f, 0.0f);
+            });
+    }
     double pap = 0.0;
     if (gt.on) {
         float4 pi = a.p4[i];
@@ -507,7 +607,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_JT(CGJTArgs a) {
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         if (touched) {
             float yi = a.y2[i].x;
-            row_pairs<FM, 1, G, PASS_UNROLL>(a.R, i, a.R.pos[i], loadA, loadB,
+            row_pairs_w<FM, 1, G>(win, w, a.R, i, a.R.pos[i], loadA, loadB,
                                  [&](const float4& Aj, const Vec<1>& Bj, float gx, float gy, float gz) {
                                      float sc = fmaf(Aj.w, yi, mi * Bj.v[0]);
                                      ax = fmaf(sc, gx, ax); ay = fmaf(sc, gy, ay); az = fmaf(sc, gz, az);
@@ -698,7 +798,6 @@ __global__ __launch_bounds__(BLOCK) void k_prj_head(PrjArgs a) {
     constexpr int G = PASS_LANES;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = pass_row<G>(blk), g = pass_lane<G>();
-    if (i >= a.R.nd) return;   // the G lanes of a row leave together
     // neighbour record: A = (x, y, z, m), B = lambda; lam = (lambda, m)
     auto loadA = [&](int j) {
         return FM::EVAL ? a.R.pos[j] : make_float4(0.0f, 0.0f, 0.0f, a.lam[j].y);
@@ -708,9 +807,15 @@ __global__ __launch_bounds__(BLOCK) void k_prj_head(PrjArgs a) {
         B.v[0] = a.lam[j].x;
         return B;
     };
+    __shared__ WinLds<FM, G> w;
+    const bool win = stage_window(a.R, blk, w, loadA, [&](int j) {
+        const float2 l = a.lam[j];
+        return make_float4(l.x, l.y, 0.0f, 0.0f);
+    });
+    if (i >= a.R.nd) return;   // the G lanes of a row leave together
     const float2 li = a.lam[i];
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    row_pairs<FM, 1, G, PASS_UNROLL>(a.R, i, a.R.pos[i], loadA, loadB,
+    row_pairs_w<FM, 1, G>(win, w, a.R, i, a.R.pos[i], loadA, loadB,
                                      [&](const float4& Aj, const Vec<1>& Bj, float gx, float gy, float gz) {
                                          float s = fmaf(Aj.w, li.x, li.y * Bj.v[0]);
                                          ax = fmaf(s, gx, ax); ay = fmaf(s, gy, ay); az = fmaf(s, gz, az);
@@ -739,11 +844,13 @@ __global__ __launch_bounds__(BLOCK) void k_prj_tail(PrjArgs a) {
     // neighbour record: A = (x, y, z, m), B = dt v
     auto loadA = [&](int j) { return j_record<FM::EVAL>(a.R.pos, a.d4, j); };
     auto loadB = [&](int j) { return vec3_of(a.d4, j); };
+    __shared__ WinLds<FM, G> w;
+    const bool win = stage_window(a.R, blk, w, loadA, [&](int j) { return a.d4[j]; });
     double err = 0.0;
     if (on) {
         const float4 di = a.d4[i];
         float acc = 0.0f;
-        row_pairs<FM, 3, G, PASS_UNROLL>(a.R, i, a.R.pos[i], loadA, loadB,
+        row_pairs_w<FM, 3, G>(win, w, a.R, i, a.R.pos[i], loadA, loadB,
                                          [&](const float4& Aj, const Vec<3>& Bj, float gx, float gy, float gz) {
                                              float dx = di.x - Bj.v[0], dy = di.y - Bj.v[1], dz = di.z - Bj.v[2];
                                              acc = fmaf(Aj.w, fmaf(dz, gz, fmaf(dy, gy, dx * gx)), acc);
@@ -852,6 +959,8 @@ __global__ __launch_bounds__(BLOCK) void k_visc(Rows R, float coeff, float dt, i
         B.v[0] = u.x; B.v[1] = u.y; B.v[2] = u.z; B.v[3] = u.w;
         return B;
     };
+    __shared__ WinLds<FM, PASS_LANES> w;
+    const bool win = stage_window(R, blk, w, loadA, [&](int j) { return v4[j]; });
     if (i >= R.nd) return;
     if (!ALLFLUID && mat[i] != fluid) return;
     float4 xi = R.pos[i];
@@ -860,7 +969,7 @@ __global__ __launch_bounds__(BLOCK) void k_visc(Rows R, float coeff, float dt, i
     float eps2 = (0.01f * R.h) * (0.01f * R.h);
     float k_visc = 2.0f * (3.0f + 2.0f);
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    row_pairs<FM, 4, PASS_LANES, PASS_UNROLL>(R, i, xi, loadA, loadB,
+    row_pairs_w<FM, 4, PASS_LANES>(win, w, R, i, xi, loadA, loadB,
                          [&](const float4& xj, const Vec<4>& vj, float gx, float gy, float gz) {
                              float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
                              float rn = len3(rx, ry, rz);
@@ -947,14 +1056,15 @@ struct W18Args {
 
 // (A u)_i = u_i - factor_i * 10 sum_fluid (m_j/rho_j) (u_ij . x_ij) / (r^2 + 0.01 h^2) gradW_ij
 // (compute_Av_weiler18, weiler18.py:103-172).  The caller stages the block's window.
-template <class FM, bool ALLFLUID, int G = 1, class LA, class LB>
-__device__ __forceinline__ float3 w18_Av_row(const W18Args& a, int i, float4 ui, LA&& loadA, LB&& loadB, int g = 0) {
+template <class FM, bool ALLFLUID, int G, class LA, class LB>
+__device__ __forceinline__ float3 w18_Av_row(const W18Args& a, int i, float4 ui, bool win, const WinLds<FM, G>& w,
+                                             LA&& loadA, LB&& loadB, int g = 0) {
     if (!ALLFLUID && a.mat[i] != a.fluid) return make_float3(ui.x, ui.y, ui.z);
     float4 xi = a.R.pos[i];
     float lx = 0.0f, ly = 0.0f, lz = 0.0f;
     float eps2 = 0.01f * a.R.h * a.R.h;
     float dd = 10.0f;
-    row_pairs<FM, 3, G, PASS_UNROLL>(a.R, i, xi, loadA, loadB,
+    row_pairs_w<FM, 3, G>(win, w, a.R, i, xi, loadA, loadB,
                          [&](const float4& xj, const Vec<3>& uj, float gx, float gy, float gz) {
                              float x0 = xi.x - xj.x, x1 = xi.y - xj.y, x2 = xi.z - xj.z;
                              float v0 = ui.x - uj.v[0], v1 = ui.y - uj.v[1], v2 = ui.z - uj.v[2];
@@ -985,6 +1095,8 @@ __global__ __launch_bounds__(BLOCK) void k_w18_setup(W18Args a, float4* __restri
     const int i = pass_row(blk), g = pass_lane();
     auto loadA = [&](int j) { return w18_record<ALLFLUID>(a.R.pos, a.v4, a.mat, a.fluid, j); };
     auto loadB = [&](int j) { return vec3_of(a.v4, j); };
+    __shared__ WinLds<FM, PASS_LANES> w;
+    const bool win = stage_window(a.R, blk, w, loadA, [&](int j) { return a.v4[j]; });
     double rz = 0.0;
     if (i < a.R.nd) {
         const float4 vi = a.v4[i];
@@ -997,7 +1109,7 @@ __global__ __launch_bounds__(BLOCK) void k_w18_setup(W18Args a, float4* __restri
             const float4 xi = a.R.pos[i];
             const float eps2 = 0.01f * a.R.h * a.R.h;
             const float dd = 10.0f;
-            row_pairs<FM, 3, PASS_LANES, PASS_UNROLL>(a.R, i, xi, loadA, loadB,
+            row_pairs_w<FM, 3, PASS_LANES>(win, w, a.R, i, xi, loadA, loadB,
                 [&](const float4& xj, const Vec<3>& uj, float gx, float gy, float gz) {
                     float x[3] = {xi.x - xj.x, xi.y - xj.y, xi.z - xj.z};
                     float v0 = vi.x - uj.v[0], v1 = vi.y - uj.v[1], v2 = vi.z - uj.v[2];
@@ -1134,10 +1246,12 @@ __global__ __launch_bounds__(BLOCK) void k_w18_Av(W18Args a) {
     const int i = pass_row(blk), g = pass_lane();
     auto loadA = [&](int j) { return w18_record<ALLFLUID>(a.R.pos, a.v4, a.mat, a.fluid, j); };
     auto loadB = [&](int j) { return vec3_of(a.v4, j); };
+    __shared__ WinLds<FM, PASS_LANES> w;
+    const bool win = stage_window(a.R, blk, w, loadA, [&](int j) { return a.v4[j]; });
     double pap = 0.0;
     if (i < a.R.nd) {
         float4 pi = a.v4[i];
-        float3 Ap = w18_Av_row<FM, ALLFLUID, PASS_LANES>(a, i, pi, loadA, loadB, g);
+        float3 Ap = w18_Av_row<FM, ALLFLUID, PASS_LANES>(a, i, pi, win, w, loadA, loadB, g);
         if (g == 0) {
             a.out[3 * i + 0] = Ap.x; a.out[3 * i + 1] = Ap.y; a.out[3 * i + 2] = Ap.z;
             pap = (double)(pi.x * Ap.x + pi.y * Ap.y + pi.z * Ap.z);
@@ -1251,20 +1365,22 @@ template <class FM, bool ALLFLUID>
 __global__ __launch_bounds__(BLOCK) void k_st_div(StArgs a) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = pass_row(blk), g = pass_lane();
-    if (i >= a.R.nd) return;
-    if (!ALLFLUID && a.mat[i] != a.fluid) {
-        if (g == 0) a.div[i] = 0.0f;
-        return;
-    }
     auto loadA = [&](int j) {
         float4 A = a.R.pos[j];
         if (!ALLFLUID && a.mat[j] != a.fluid) A.w = 0.0f;
         return A;
     };
     auto loadB = [&](int j) { return vec3_of(a.u4, j); };
+    __shared__ WinLds<FM, PASS_LANES> w;
+    const bool win = stage_window(a.R, blk, w, loadA, [&](int j) { return a.u4[j]; });
+    if (i >= a.R.nd) return;
+    if (!ALLFLUID && a.mat[i] != a.fluid) {
+        if (g == 0) a.div[i] = 0.0f;
+        return;
+    }
     float4 ui = a.u4[i];
     float acc = 0.0f;
-    row_pairs<FM, 3, PASS_LANES, PASS_UNROLL>(a.R, i, a.R.pos[i], loadA, loadB,
+    row_pairs_w<FM, 3, PASS_LANES>(win, w, a.R, i, a.R.pos[i], loadA, loadB,
                          [&](const float4& Aj, const Vec<3>& Bj, float gx, float gy, float gz) {
                              float vx = ui.x - Bj.v[0], vy = ui.y - Bj.v[1], vz = ui.z - Bj.v[2];
                              acc = acc + Aj.w * (vx * gx + vy * gy + vz * gz);
@@ -1354,6 +1470,10 @@ __global__ __launch_bounds__(BLOCK) void k_st_Av(StArgs a) {
     constexpr unsigned REC = FM::REC;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = pass_row(blk), g = pass_lane();
+    // only (u_j, m_j) is read per neighbour: the window's B records
+    __shared__ WinLds<FM, PASS_LANES> w;
+    const bool win = stage_window(a.R, blk, w, [](int) { return make_float4(0.0f, 0.0f, 0.0f, 0.0f); },
+                                  [&](int j) { return a.u4[j]; });
     double pap = 0.0;
     if (i < a.R.nd) {
         const float4 ui = a.u4[i];
@@ -1364,14 +1484,26 @@ __global__ __launch_bounds__(BLOCK) void k_st_Av(StArgs a) {
             const int tl = __builtin_amdgcn_readfirstlane(a.R.tile_len[i >> 6]);
             const float m_i = ui.w;
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-            auto visit = [&](int j, float gt) {
-                float4 uj = a.u4[j];
+            auto visit_u = [&](const float4& uj, float gt) {
                 float w = (0.5f * (m_i + uj.w)) * gt;
                 sx = sx + w * (ui.x - uj.x);
                 sy = sy + w * (ui.y - uj.y);
                 sz = sz + w * (ui.z - uj.z);
             };
-            if constexpr (REC == 64u && PASS_LANES == 4) {
+            auto visit = [&](int j, float gt) { visit_u(a.u4[j], gt); };
+            if (has_windows<FM, PASS_LANES>() && win) {
+                // window indices of slots g + 8 b and g + 4 + 8 b: one 32-bit word
+                typedef float f32x2 __attribute__((ext_vector_type(2)));
+                const uint32_t* l = a.R.loc + ((size_t)(i >> 6) * (size_t)a.R.Cs * 32u + 4u * (i & (WAVE - 1)) + g);
+                const f32x2* g2 = reinterpret_cast<const f32x2*>(eg + 2 * g);
+#pragma unroll((PASS_UNROLL + 1) / 2)
+                for (int b = 0; b < tl; b += 8) {
+                    const uint32_t q = __builtin_nontemporal_load(l + (b >> 3) * 256);
+                    const f32x2 gts = __builtin_nontemporal_load(g2 + (b >> 3) * 256);
+                    visit_u(w.B[q & 0xffffu], gts.x);
+                    visit_u(w.B[q >> 16], gts.y);
+                }
+            } else if constexpr (REC == 64u && PASS_LANES == 4) {
                 // slots g + 8 b and g + 4 + 8 b: one 8-byte load of ids and one of g-tilde
                 typedef float f32x2 __attribute__((ext_vector_type(2)));
                 const u32x2* e2 = reinterpret_cast<const u32x2*>(e + 2 * g);
@@ -1468,7 +1600,11 @@ __global__ __launch_bounds__(BLOCK) void k_advect_boundary(int nd, float dt, flo
     pos[i] = make_float4(x[0], x[1], x[2], p.w);
 }
 
-Rows rows_of(ivp_ctx* c) { return Rows{c->nb, c->g16, c->tile_len, c->Cs, c->nd, c->pos, c->h, c->kspiky, c->gscale}; }
+Rows rows_of(ivp_ctx* c) {
+    const bool win = c->win_built && c->windows;
+    return Rows{c->nb, c->g16, c->tile_len, c->Cs, c->nd, c->pos, c->h, c->kspiky, c->gscale,
+                c->loc, c->win_ids, win ? c->win_n : nullptr};
+}
 
 // ---- independent scenes -------------------------------------------------------------------
 
@@ -1633,6 +1769,15 @@ int run_chunks(ivp_ctx* c, int max_iter, int guess, L&& launch_it, int* iters, i
 template <class FM>
 constexpr double pair_bytes() { return FM::GS == GS_EVAL ? 4.0 : FM::GS == GS_F32 ? 16.0 : 12.0; }
 
+// Bytes of a four-lane pass's pair stream over `pairs` pairs: with tile windows a 2-byte window
+// index replaces each 4-byte id, and the window lists (4 bytes per entry) are read once.
+template <class FM>
+double stream_bytes(const ivp_ctx* c, double pairs) {
+    if (has_windows<FM, PASS_LANES>() && c->win_built && c->windows)
+        return (pair_bytes<FM>() - 2.0) * pairs + 4.0 * (double)c->state_host->win_entries;
+    return pair_bytes<FM>() * pairs;
+}
+
 // CG iterations between true-residual replacements when the CG operator reads float16 gradW
 constexpr int REFRESH_EVERY = 8;
 
@@ -1655,7 +1800,7 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
     for (;;) {
         PackArgs pa{R, d4, c->Ss, c->cc, c->kk, c->rho0, c->rowcnt, c->q4, c->sfac, c->jd, c->partials, stride};
         double total_pairs = (double)c->state_host->total_pairs;
-        IVP_TRY(launch(c, KC_IP_PACK, -1, pb * total_pairs + 68.0 * nd, k_ip_pack<FP, false>, np, pa));
+        IVP_TRY(launch(c, KC_IP_PACK, -1, stream_bytes<FP>(c, total_pairs) + 68.0 * nd, k_ip_pack<FP, false>, np, pa));
         IVP_TRY(finalize<FIN_PACK>(c, np.x));
         IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         IVP_TRY(read_state(c));
@@ -1673,7 +1818,7 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
         int act_rows = c->state_host->n_active_rows;
 
         RhsArgs ra{R, c->q4, c->Ss, prm->omega, c->b, c->nbact, c->rowcnt, c->partials, stride};
-        IVP_TRY(launch(c, KC_IP_RHS, -1, pb * total_pairs + 48.0 * nd, k_ip_rhs<FP, false>, np, ra));
+        IVP_TRY(launch(c, KC_IP_RHS, -1, stream_bytes<FP>(c, total_pairs) + 48.0 * nd, k_ip_rhs<FP, false>, np, ra));
         IVP_TRY(finalize<FIN_RHS>(c, np.x));
         CGInitArgs ia{nd, c->b, c->pos, c->sol, c->r, c->p4, c->y2, c->partials, stride};
         IVP_TRY(launch(c, -1, -1, 0.0, k_cg_init<false>, nb, ia));
@@ -1689,8 +1834,8 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
             int epoch = c->prof_epoch;
             // algorithmic bytes per launch: the pair stream once per traversed pair, per-row
             // arrays once (gathered neighbour values counted once per particle)
-            double bJ = pbc * (double)act_pairs + (double)act_rows * (16 + 12 + 8) + (double)nd * 4;
-            double bJT = pbc * (double)touch_pairs + (double)touch_rows * (8 + 12) + (double)nd * (16 + 4 + 12);
+            double bJ = stream_bytes<FC>(c, (double)act_pairs) + (double)act_rows * (16 + 12 + 8) + (double)nd * 4;
+            double bJT = stream_bytes<FC>(c, (double)touch_pairs) + (double)touch_rows * (8 + 12) + (double)nd * (16 + 4 + 12);
             double bU = (double)nd * (12 + 12 + 16 + 12 + 12 + 12);
             double bP = (double)nd * (12 + 16 + 16);
             CGJArgs ja{R, c->p4, c->Ss, c->sfac, c->y2, c->state};
@@ -1728,7 +1873,8 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
             touch_pairs = c->state_host->touched_pairs;
             touch_rows = c->state_host->n_touched_rows;
             prof_set_bytes(c, epoch, KC_CG_JT,
-                           pbc * (double)touch_pairs + (double)touch_rows * (8 + 12) + (double)nd * (16 + 4 + 12));
+                           stream_bytes<FC>(c, (double)touch_pairs) + (double)touch_rows * (8 + 12) +
+                               (double)nd * (16 + 4 + 12));
             prof_collect(c, iters, epoch);
             c->cg_guess = std::max(2, iters);
             st.pairs_traversed += (long long)iters * (act_pairs + touch_pairs);
@@ -1760,7 +1906,7 @@ int visc_explicit_body(ivp_ctx* c, float coeff, float dt) {
     const int nd = c->nd;
     const dim3 nb(blocks_for(nd));
     IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_v4, nb, nd, (const float*)c->vel, (const float*)c->density, c->v4));
-    double bytes = pair_bytes<FM>() * (double)c->state_host->total_pairs + 48.0 * nd;
+    double bytes = stream_bytes<FM>(c, (double)c->state_host->total_pairs) + 48.0 * nd;
     Rows R = rows_of(c);
     if (c->all_dyn_fluid)
         return launch(c, KC_VISC, -1, bytes, k_visc<FM, true>, pass_grid(nd), R, coeff, dt, c->cfg.material_fluid,
@@ -1784,16 +1930,16 @@ int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
                c->w_dv_prev};
     if (!c->w18_warm) {
         // one pair pass: preconditioner blocks, right-hand side and the PCG prologue
-        IVP_TRY(launch(c, KC_W18_PRE, -1, pb * pairs + (16.0 + 4.0 + 36.0 + 12.0 * 3 + 16.0) * nd,
+        IVP_TRY(launch(c, KC_W18_PRE, -1, stream_bytes<FM>(c, pairs) + (16.0 + 4.0 + 36.0 + 12.0 * 3 + 16.0) * nd,
                        k_w18_setup<FM, ALLFLUID>, pass_grid(nd), wa, c->w_p4));
     } else {
         // warm start: blocks, rhs and dv = the last correction; then r = rhs - A dv
-        IVP_TRY(launch(c, KC_W18_PRE, -1, pb * pairs + (16.0 + 4.0 + 36.0 + 12.0 * 3 + 16.0) * nd,
+        IVP_TRY(launch(c, KC_W18_PRE, -1, stream_bytes<FM>(c, pairs) + (16.0 + 4.0 + 36.0 + 12.0 * 3 + 16.0) * nd,
                        k_w18_setup<FM, ALLFLUID, true>, pass_grid(nd), wa, c->w_p4));
         W18Args res = wa;
         res.v4 = c->w_p4;
         res.out = c->w_Ap;
-        IVP_TRY(launch(c, KC_W18_PRE, -1, pb * pairs + 48.0 * nd, k_w18_Av<FM, ALLFLUID, false>, pass_grid(nd),
+        IVP_TRY(launch(c, KC_W18_PRE, -1, stream_bytes<FM>(c, pairs) + 48.0 * nd, k_w18_Av<FM, ALLFLUID, false>, pass_grid(nd),
                        res));
         IVP_TRY(launch(c, KC_W18_PRE, -1, 0.0, k_w18_warm_init, nb, res, (const float*)c->w_rhs, c->w_p4));
     }
@@ -1804,7 +1950,7 @@ int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
         W18Args cg = wa;
         cg.v4 = c->w_p4;
         cg.out = c->w_Ap;
-        const double bAv = pb * (double)c->state_host->total_pairs + 48.0 * nd;
+        const double bAv = stream_bytes<FM>(c, (double)c->state_host->total_pairs) + 48.0 * nd;
         auto one_iter = [&](int tag) -> int {
             IVP_TRY(launch(c, KC_W18_AV, tag, bAv, k_w18_Av<FM, ALLFLUID>, pass_grid(nd), cg));
             IVP_TRY(finalize<FIN_W18_PAP>(c, pass_grid(nd).x));
@@ -1817,7 +1963,7 @@ int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
         int epoch = c->prof_epoch;
         IVP_TRY(run_chunks(c, 1000, c->w18_guess, one_iter, &iters, &chunks));
         // the pair count of the last build, now read back with the solver state
-        prof_set_bytes(c, epoch, KC_W18_AV, pb * (double)c->state_host->total_pairs + 48.0 * nd);
+        prof_set_bytes(c, epoch, KC_W18_AV, stream_bytes<FM>(c, (double)c->state_host->total_pairs) + 48.0 * nd);
         prof_collect(c, iters, epoch);
         c->w18_guess = std::max(2, iters);
     }
@@ -1845,9 +1991,9 @@ int jeske23_body(ivp_ctx* c, float sigma, float dt, int* iters_out) {
     const double pairs = (double)c->state_host->total_pairs;
     const double pb = pair_bytes<FM>();
     IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_st_prep, nb, nd, (const float*)c->vel, (const float4*)c->pos, c->w_v4));
-    IVP_TRY(launch(c, KC_ST_SETUP, -1, pb * pairs + 36.0 * nd, k_st_div<FM, ALLFLUID>, np, sa));
+    IVP_TRY(launch(c, KC_ST_SETUP, -1, stream_bytes<FM>(c, pairs) + 36.0 * nd, k_st_div<FM, ALLFLUID>, np, sa));
     IVP_TRY(launch(c, KC_ST_SETUP, -1, (pb + 4.0) * pairs + 72.0 * nd, k_st_setup<FM, ALLFLUID>, np, sa));
-    IVP_TRY(launch(c, KC_ST_AV, -1, (pb + 4.0) * pairs + 40.0 * nd, k_st_Av<FM, ALLFLUID, true>, np, sa));
+    IVP_TRY(launch(c, KC_ST_AV, -1, stream_bytes<FM>(c, pairs) + 4.0 * pairs + 40.0 * nd, k_st_Av<FM, ALLFLUID, true>, np, sa));
     W18Args wa{R, c->cfg.material_fluid, c->w_fac, c->mat, c->w_pinv, c->w_v4, c->w_rhs,
                c->w_dv, c->w_r, c->w_z, c->partials, stride, c->state};
     IVP_TRY(launch(c, -1, -1, 0.0, k_w18_init, nb, wa, c->w_p4));
@@ -1861,7 +2007,7 @@ int jeske23_body(ivp_ctx* c, float sigma, float dt, int* iters_out) {
         W18Args cg = wa;
         cg.v4 = c->w_p4;
         cg.out = c->w_Ap;
-        const double bAv = (pb + 4.0) * pairs + 28.0 * nd;
+        const double bAv = stream_bytes<FM>(c, pairs) + 4.0 * pairs + 28.0 * nd;
         auto one_iter = [&](int tag) -> int {
             IVP_TRY(launch(c, KC_ST_AV, tag, bAv, k_st_Av<FM, ALLFLUID, false>, np, cga));
             IVP_TRY(finalize<FIN_W18_PAP>(c, np.x));
@@ -1873,7 +2019,7 @@ int jeske23_body(ivp_ctx* c, float sigma, float dt, int* iters_out) {
         c->prof_epoch++;
         int epoch = c->prof_epoch;
         IVP_TRY(run_chunks(c, 100, c->st_guess, one_iter, &iters, &chunks));
-        prof_set_bytes(c, epoch, KC_ST_AV, (pb + 4.0) * (double)c->state_host->total_pairs + 28.0 * nd);
+        prof_set_bytes(c, epoch, KC_ST_AV, stream_bytes<FM>(c, (double)c->state_host->total_pairs) + 4.0 * (double)c->state_host->total_pairs + 28.0 * nd);
         prof_collect(c, iters, epoch);
         c->st_guess = std::max(2, iters);
     }
@@ -1897,8 +2043,8 @@ int prj_loop(ivp_ctx* c, const ivp_prj_params* prm, double tol, int* passes, dou
     PrjFinArgs fa{c->partials, c->max_blocks, (int)np.x, c->state, tol, prm->max_iteration_opt,
                   prm->use_fixed_iters ? 1 : 0, nd, DF ? nullptr : c->prj_hist, DF ? 0 : c->prj_hist_cap};
     // algorithmic bytes: the pair stream once, the gathered per-particle data once per row
-    const double bH = pb * pairs + (double)nd * (8 + 12 + 12 + 16);
-    const double bT = pb * pairs + (double)nd * (16 + 12 + 4 + 4 + 4 + 8 + 4 + 8 + (DF ? 4 : 0));
+    const double bH = stream_bytes<FM>(c, pairs) + (double)nd * (8 + 12 + 12 + 16);
+    const double bT = stream_bytes<FM>(c, pairs) + (double)nd * (16 + 12 + 4 + 4 + 4 + 8 + 4 + 8 + (DF ? 4 : 0));
     c->prof_epoch++;
     const int epoch = c->prof_epoch;
     const int total = prm->max_iteration_opt + 1;

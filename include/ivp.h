@@ -288,6 +288,12 @@ int ivp_scene_history(ivp_ctx *ctx, int scene, int capacity, double *density_err
  * 382-383; default 1: the 3x3 block-Jacobi preconditioner). */
 #define IVP_OPT_W18_WARM_START 6
 #define IVP_OPT_W18_PRECONDITIONER 7
+/* Tile windows (1, default; ids-only and float16 rows, one scene): each neighbour build also
+ * lists, per 64-row tile, the distinct neighbours of its rows, and the pair passes load them into
+ * on-chip memory once per tile instead of gathering each pair's neighbour from memory.  Results
+ * are bitwise the same either way; 0 turns them off from the next build on, and a value v > 1
+ * keeps windows of at most v neighbours only (tiles with more gather from memory). */
+#define IVP_OPT_TILE_WINDOWS 8
 
 /* ---- scene loading -------------------------------------------------------------------------- */
 /* SceneLoader.load_mesh (scene_loader.py:547-580): the voxel centres of the solid bounded by a
@@ -345,6 +351,11 @@ int ivp_profile_reset(ivp_ctx *ctx);
 /* Particle counts of the loaded set (n_total, n_dynamic) and the number of dynamic-dynamic
  * neighbour pairs stored by the last build (synchronises the stream when requested). */
 int ivp_counts(ivp_ctx *ctx, int *n_total, int *n_dynamic, long long *dyn_pairs);
+
+/* Tile windows of the last build (IVP_OPT_TILE_WINDOWS): 64-row tiles, tiles with a window, and
+ * the neighbour ids staged per pass over all windows (synchronises the stream).  All zero when
+ * the last build made no windows. */
+int ivp_tile_windows(ivp_ctx *ctx, int *tiles, int *windowed, long long *window_ids);
 
 #ifdef __cplusplus
 }
