@@ -187,6 +187,10 @@ struct Vec {
 // so the lane reads both ids with one 8-byte load; the slots are visited in the same order.
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+#ifndef IVP_LOC_PF
+#define IVP_LOC_PF 8
+#endif
+constexpr int LOC_PF = IVP_LOC_PF;   // window-index words requested at once per lane
 // With LOC (tile windows, four lanes per row) the lane reads its two slots' window indices as
 // one 32-bit word at half the word index of their ids, and loadA / loadB take window indices.
 template <class FM, int NB, int G = 1, int U = 4, bool LOC = false, class LA, class LB, class F>
@@ -216,7 +220,23 @@ __device__ __forceinline__ void row_pairs(const Rows& R, int i, float4 xi, LA&& 
         f(Aj, Bj, gx, gy, gz);
     };
     static_assert(!LOC || (REC == 64u && G == 4), "tile windows index the paired ids-only layout");
-    if constexpr (LOC) {
+    if constexpr (LOC && FM::GS != GS_F16) {
+        // all index words of up to LOC_PF blocks are requested before the first is used (the
+        // words come from HBM; the neighbours from LDS)
+        const uint32_t* l = R.loc + ((w_row + 2 * g) >> 1);
+        for (int b0 = 0; b0 < tl; b0 += 8 * LOC_PF) {
+            uint32_t q[LOC_PF];
+#pragma unroll
+            for (int k = 0; k < LOC_PF; ++k)
+                q[k] = b0 + 8 * k < tl ? __builtin_nontemporal_load(l + ((b0 >> 3) + k) * 256) : 0u;
+#pragma unroll
+            for (int k = 0; k < LOC_PF; ++k)
+                if (b0 + 8 * k < tl) {
+                    visit((int)(q[k] & 0xffffu), nullptr);
+                    visit((int)(q[k] >> 16), nullptr);
+                }
+        }
+    } else if constexpr (LOC) {
         const uint32_t* l = R.loc + ((w_row + 2 * g) >> 1);
 #pragma unroll((U + 1) / 2)
         for (int b = 0; b < tl; b += 8) {
