@@ -153,8 +153,8 @@ int ivp_reset(ivp_ctx *ctx);
  *     gradW from the positions.
  *   IVP_ROWS_F32 (1): float32 gradW cached next to every id (the reference's neighbor_gradW
  *     cache, 16 B per pair and pass) and read by every operator pass.
- *   IVP_ROWS_F16 (2): float16 gradW cached next to every id (12 B per pair and pass) with one
- *     power-of-two scale per context; the IP CG's J and J^T passes read it, and every 8th CG
+ *   IVP_ROWS_F16 (2): the ids-only rows plus float16 gradW per pair in a side array (8 B per
+ *     pair) with one power-of-two scale per context; the IP CG's J and J^T passes read it, and every 8th CG
  *     iteration replaces the recursive residual by r = b - A x evaluated in float32 from the
  *     positions.  The outer-loop passes (constraint, right-hand side) and viscosity evaluate
  *     gradW from the positions as in IVP_ROWS_IDS.
