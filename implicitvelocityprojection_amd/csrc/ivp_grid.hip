@@ -1121,6 +1121,9 @@ int grid_build(ivp_ctx* c) {
     c->win_built = false;
     // a tile walks at most Cs slots: Cs * 32 word pairs, PPT per thread
     const int ppt = (c->Cs * (WAVE / 2) + BLOCK - 1) / BLOCK;
+    // (not for independent scenes: their passes run late in a batched solve on few active blocks,
+    // and the window pass over every scene's rows cost more than it saved -- 4.28 -> 4.48 ms per
+    // step on 256 columns)
     if (c->nd > 0 && c->windows && c->rec_words == 64 && c->S == 0 && ppt <= 32) {
         WinArgs wa{c->nd, c->Cs, c->win_cap, c->nb, c->tile_len, c->loc, c->win_ids, c->win_n, c->partials,
                    c->max_blocks};

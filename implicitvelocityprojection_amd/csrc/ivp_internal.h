@@ -98,6 +98,7 @@ struct SceneGraph {
     float omega = 0.0f;
     int nd = 0;
     int lanes = 0;
+    bool win = false;            // the captured passes read the tile windows
 };
 
 // IP parameters of the current batched solve (device copy, read by the finalize kernels)

@@ -1717,9 +1717,10 @@ int enqueue_scene_replay(ivp_ctx* c, hipStream_t st, float omega, int K) {
 // The instantiated graph of one replay for (K, omega) and the current store buffers (the sort
 // alternates two buffers, so at most two captures are live).
 int scene_graph(ivp_ctx* c, float omega, int K, hipGraphExec_t* out) {
+    const bool win = rows_of(c).win_n != nullptr;
     for (auto& g : c->sgraphs) {
         if (g.exec && g.pos == (const void*)c->pos && g.chunk == K && g.omega == omega && g.nd == c->nd &&
-            g.lanes == c->scene_lanes) {
+            g.lanes == c->scene_lanes && g.win == win) {
             *out = g.exec;
             return IVP_OK;
         }
@@ -1747,6 +1748,7 @@ int scene_graph(ivp_ctx* c, float omega, int K, hipGraphExec_t* out) {
     slot.omega = omega;
     slot.nd = c->nd;
     slot.lanes = c->scene_lanes;
+    slot.win = win;
     *out = exec;
     return IVP_OK;
 }
