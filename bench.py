@@ -18,16 +18,17 @@ neighbour rebuild, density and Aii -- all of it on the GPU, inputs resident in H
 Prints one JSON line.  `value` is dynamic-particle-steps per second over the K timed steps.
 `roofline` reports the kernel with the largest share of the timed region: its algorithmic bytes
 per launch (the byte model the library attaches to each launch: every stored neighbour-pair
-record once -- 4 B of id, plus 12 B of gradW when gradients are cached -- and every per-particle
-array once) over its average launch time, measured with HIP events that the kernel dispatch
+record once -- a 2 B tile-window index with the windows' lists once, or a 4 B id without them,
+plus 12 B of gradW when gradients are cached -- and every per-particle array once) over its
+average launch time, measured with HIP events that the kernel dispatch
 itself stamps (hipExtLaunchKernelGGL) on the library's stream.  `ip_matvec_roofline` gives the
 same figures for the IP CG matvec's two pair passes (cg_J, cg_JT) on a few steps after the timed
 region.  `cpu_baseline` is the CPU oracle (a serial float32 C restatement of the reference
 kernels) timed on 10 steps of the same workload, starting from the GPU state at the start of the
 timed region, on one core; `all_cores` times the same steps with the oracle's OpenMP build on
 every core the process may use, and the CPU model and core counts are reported with both.
-`precision` compares one step of each GPU form, and of the float32 oracle, with the float64
-restatement of that step.
+`precision` compares `--precision-steps` consecutive steps (5 by default) of each GPU form, and
+of the float32 oracle, with the float64 restatement of each step.
 """
 
 import argparse
@@ -370,10 +371,11 @@ def main():
     ap.add_argument("--tile-windows", type=int, default=1, choices=[0, 1],
                     help="1 (default): the pair passes stage each 64-row tile's distinct neighbours in LDS; "
                          "0: every pair gathers its neighbour from memory (same results)")
-    ap.add_argument("--precision-steps", type=int, default=1,
-                    help="steps (from the state after the timed region) whose GPU result, in float32 and in the "
-                         "float16 precision form, is compared with the float64 restatement of the step; then 20 "
-                         "steps of each form are timed from one state; 0 skips both")
+    ap.add_argument("--precision-steps", type=int, default=5,
+                    help="consecutive steps (from the state after the timed region) whose GPU result, in float32 and "
+                         "in the float16 precision form, is compared with the float64 restatement of each step "
+                         "(about 10 s of CPU per step at 1M); then 20 steps of each form are timed from one state; "
+                         "0 skips both")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="dambreak_1m")
     ap.add_argument("--surface-tension", type=float, default=0.0, metavar="SIGMA",
                     help="add Jeske23 implicit surface tension (surfaceTensionMethod 1) with this coefficient "

@@ -5,17 +5,33 @@ float64 -- kernels, density, Aii, the IP outer loop with its CG, explicit or Wei
 viscosity, advection and the domain clamp -- so that the float32 GPU path's per-step error
 can be stated against a higher-precision evaluation of the same algorithm.  It takes the
 neighbour SETS from the float32 oracle's build of the same positions (sets are exact), and
-vectorises every row pass over the (row, neighbour) pair list with numpy.  Summation order is
-numpy's, not the reference's: this is a precision yardstick, not a bit-level restatement.
+vectorises every row pass over the (row, neighbour) pair list with numpy, on a thread pool over
+chunks of whole rows.  Summation order is numpy's (each row's pairs in order, whatever the
+thread count), not the reference's: this is a precision yardstick, not a bit-level restatement.
 
 Followed files (reference root): sph_kernel.py:8-61; pressure_base.py:12-203, 394-447;
 ip.py:19-404; viscosity.py:9-108; weiler18.py:21-401; framework.py:26-41, 117-136;
 boundary.py:27-91.  Only tests/ and bench.py load this module.
 """
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 PI_LIT = 3.14159265359
+
+# Row sums over the pair list run on threads, each over whole rows (numpy releases the GIL in
+# the gathers, the arithmetic and bincount); every row sums its pairs in the same order as one
+# bincount over all pairs would, so the results do not depend on the thread count.
+_POOL = None
+
+
+def _pool():
+    global _POOL
+    if _POOL is None:
+        _POOL = ThreadPoolExecutor(max(1, min(16, os.cpu_count() or 1)))
+    return _POOL
 
 
 def cubic_kernel(r, h):
@@ -50,6 +66,35 @@ class Pairs:
         self.i = np.repeat(rows, cnt[rows])
         self.j = ids2[rows][mask].astype(np.int64)
         self.n = n
+        # pair ranges of whole rows (the pairs are in row order), about 4 per thread
+        P = self.i.shape[0]
+        K = max(1, min(64, P // 65536))
+        cuts = [0]
+        for k in range(1, K):
+            c = int(np.searchsorted(self.i, self.i[min(P - 1, k * P // K)], side="left"))
+            if c > cuts[-1]:
+                cuts.append(c)
+        cuts.append(P)
+        self.chunks = [(a, b) for a, b in zip(cuts[:-1], cuts[1:]) if b > a]
+
+    def row_sums(self, weights, width=0):
+        """sum over each row's pairs of weights(sl) (per-pair values of the pair slice sl, shape
+        (len,) or (len, width)); rows without pairs get 0."""
+        out = np.zeros((self.n, width) if width else self.n)
+
+        def task(ch):
+            a, b = ch
+            r0, r1 = int(self.i[a]), int(self.i[b - 1]) + 1
+            w = weights(slice(a, b))
+            loc = self.i[a:b] - r0
+            if width:
+                for c in range(width):
+                    out[r0:r1, c] = np.bincount(loc, weights=w[:, c], minlength=r1 - r0)
+            else:
+                out[r0:r1] = np.bincount(loc, weights=w, minlength=r1 - r0)
+
+        list(_pool().map(task, self.chunks))
+        return out
 
 
 def _sum_rows(n, i, w):
@@ -71,33 +116,62 @@ class Step64:
         self.h = float(h)
         self.P = pairs
         i, j = pairs.i, pairs.j
-        self.xij = self.x[i] - self.x[j]
-        self.rn = np.linalg.norm(self.xij, axis=1)
-        self.g = spiky_grad(self.xij, self.h)
-        self.dynj = self.dyn[j].astype(np.float64)
+        P = i.shape[0]
+        # per-pair arrays, filled chunk by chunk on the pool (disjoint slices)
+        self.xij = np.empty((P, 3))
+        self.rn = np.empty(P)
+        self.g = np.empty((P, 3))
+        self.dynj = np.empty(P)
+        self.mj = np.empty(P)
+        self.mi = np.empty(P)
+
+        def fill(ch):
+            sl = slice(*ch)
+            self.xij[sl] = self.x[i[sl]] - self.x[j[sl]]
+            self.rn[sl] = np.linalg.norm(self.xij[sl], axis=1)
+            self.g[sl] = spiky_grad(self.xij[sl], self.h)
+            self.dynj[sl] = self.dyn[j[sl]]
+            self.mj[sl] = self.m[j[sl]]
+            self.mi[sl] = self.m[i[sl]]
+
+        list(_pool().map(fill, pairs.chunks))
         n = pairs.n
+        h, mj, dynj, g, rn = self.h, self.mj, self.dynj, self.g, self.rn
         # compute_density_kernel, pressure_base.py:12-46
-        rho = self.m * cubic_kernel(np.zeros(n), self.h) + _sum_rows(n, i, self.m[j] * cubic_kernel(self.rn, self.h))
+        rho = self.m * cubic_kernel(np.zeros(n), h) + pairs.row_sums(lambda sl: mj[sl] * cubic_kernel(rn[sl], h))
         self.density = np.where(self.dyn, rho, self.rho0)
         # compute_Aii_kernel, pressure_base.py:49-87
-        Jij = self.m[j][:, None] * self.g
-        a1 = _sum_rows(n, i, (Jij * Jij).sum(1) * self.m_inv[j] * self.dynj)
-        Jii = -_sum_rows(n, i, Jij)
+        minvj = lambda sl: self.m_inv[j[sl]]
+
+        def a1w(sl):
+            Jij = mj[sl][:, None] * g[sl]
+            return (Jij * Jij).sum(1) * minvj(sl) * dynj[sl]
+
+        a1 = pairs.row_sums(a1w)
+        Jii = -pairs.row_sums(lambda sl: mj[sl][:, None] * g[sl], 3)
         aii = a1 + (Jii * Jii).sum(1) * self.m_inv
         self.k = np.where(self.dyn, 1.0 / (aii + 1e-3), 0.0)
 
     # -- IP (ip.py) -------------------------------------------------------------------------
     def J(self, d):
         """compute_J_x_kernel form: sum_j m_j (d_i - [dyn_j] d_j) . gradW_ij"""
-        i, j = self.P.i, self.P.j
-        dl = d[i] - self.dynj[:, None] * d[j]
-        return np.where(self.dyn, _sum_rows(self.P.n, i, self.m[j] * (dl * self.g).sum(1)), 0.0)
+        i, j, mj, dynj, g = self.P.i, self.P.j, self.mj, self.dynj, self.g
+
+        def w(sl):
+            dl = d[i[sl]] - dynj[sl, None] * d[j[sl]]
+            return mj[sl] * (dl * g[sl]).sum(1)
+
+        return np.where(self.dyn, self.P.row_sums(w), 0.0)
 
     def JT(self, y):
         """sum_j (m_j y_i + [dyn_j] m_i y_j) gradW_ij"""
-        i, j = self.P.i, self.P.j
-        s = self.m[j] * y[i] + self.dynj * self.m[i] * y[j]
-        out = _sum_rows(self.P.n, i, s[:, None] * self.g)
+        i, j, mj, mi, dynj, g = self.P.i, self.P.j, self.mj, self.mi, self.dynj, self.g
+
+        def w(sl):
+            s = mj[sl] * y[i[sl]] + dynj[sl] * mi[sl] * y[j[sl]]
+            return s[:, None] * g[sl]
+
+        out = self.P.row_sums(w, 3)
         out[~self.dyn] = 0.0
         return out
 
@@ -168,20 +242,30 @@ class Step64:
         i, j = self.P.i, self.P.j
         n = self.P.n
         rows = self.dyn & self.fluid
-        ok = self.fluid[j].astype(np.float64)
         eps2 = 0.01 * self.h * self.h
         fac = dt * coeff * self.rho0 / self.density
-        cij = 10.0 * (self.m[j] / self.density[j]) / (self.rn ** 2 + eps2) * ok
+        cij = np.empty(i.shape[0])
+
+        def fill(ch):
+            sl = slice(*ch)
+            ok = self.fluid[j[sl]].astype(np.float64)
+            cij[sl] = 10.0 * (self.mj[sl] / self.density[j[sl]]) / (self.rn[sl] ** 2 + eps2) * ok
+
+        list(_pool().map(fill, self.P.chunks))
+
+        xij, g = self.xij, self.g
 
         def A(u):
-            uij = u[i] - u[j]
-            w = cij * (uij * self.xij).sum(1)
-            lap = _sum_rows(n, i, w[:, None] * self.g)
+            def wts(sl):
+                uij = u[i[sl]] - u[j[sl]]
+                w = cij[sl] * (uij * xij[sl]).sum(1)
+                return w[:, None] * g[sl]
+
+            lap = self.P.row_sums(wts, 3)
             return np.where(rows[:, None], u - fac[:, None] * lap, u)
 
-        L = np.zeros((n, 3, 3))
-        outer = (cij[:, None, None] * self.g[:, :, None] * self.xij[:, None, :]).reshape(-1, 9)
-        L = np.stack([np.bincount(i, weights=outer[:, q], minlength=n) for q in range(9)], axis=1).reshape(n, 3, 3)
+        L = self.P.row_sums(lambda sl: (cij[sl, None, None] * g[sl, :, None] * xij[sl, None, :]).reshape(-1, 9),
+                            9).reshape(n, 3, 3)
         M = np.eye(3)[None] - fac[:, None, None] * L
         Pinv = np.linalg.inv(np.where(rows[:, None, None], M, np.eye(3)[None]))
 
