@@ -1412,11 +1412,43 @@ __global__ __launch_bounds__(BLOCK) void k_st_div(StArgs a) {
 // compute_g_tilde_jeske23 + compute_rhs_jeske23 + compute_precond_jeske23 (jeske23.py:115-350)
 // in one pass over each row's slots: g~_ij (stored per slot), b_i and the diagonal
 // preconditioner.  Padding slots (id = the row) and non-fluid neighbours get g~ = 0.
+// The set-up's window records: A = (x, y, z, rho_j), B = (v_j, m_j), C = div_j, D = j, with bit
+// 31 set for a non-fluid neighbour.
+template <class FM>
+struct StWinLds {
+    static constexpr int N = has_windows<FM, PASS_LANES>() ? WIN_MAX : 1;
+    float4 A[N];
+    float4 B[N];
+    float C[N];
+    int D[N];
+};
+
 template <class FM, bool ALLFLUID>
 __global__ __launch_bounds__(BLOCK) void k_st_setup(StArgs a) {
     constexpr unsigned REC = FM::REC;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = pass_row(blk), g = pass_lane();
+    // the tile window (every thread of the block stages before any row leaves)
+    __shared__ StWinLds<FM> w;
+    bool win = false;
+    if constexpr (has_windows<FM, PASS_LANES>()) {
+        if (a.R.win_n != nullptr) {
+            const int nw = __builtin_amdgcn_readfirstlane(a.R.win_n[blk]);
+            if (nw >= 0) {
+                const int* wid = a.R.win_ids + (size_t)blk * WIN_MAX;
+                for (int k = threadIdx.x; k < nw; k += BLOCK) {
+                    const int j = wid[k];
+                    const float4 xj = a.R.pos[j];
+                    w.A[k] = make_float4(xj.x, xj.y, xj.z, a.density[j]);
+                    w.B[k] = a.u4[j];
+                    w.C[k] = a.div[j];
+                    w.D[k] = j | ((ALLFLUID || a.mat[j] == a.fluid) ? 0 : (int)0x80000000u);
+                }
+                __syncthreads();
+                win = true;
+            }
+        }
+    }
     if (i >= a.R.nd) return;
     float* P = a.pinv + 9 * (size_t)i;
     const bool fluid_i = ALLFLUID || a.mat[i] == a.fluid;
@@ -1435,27 +1467,24 @@ __global__ __launch_bounds__(BLOCK) void k_st_setup(StArgs a) {
     const float4 ui = a.u4[i];
     const float m_i = ui.w, rho_i = a.density[i], div_i = a.div[i];
     float bx = 0.0f, by = 0.0f, bz = 0.0f, diag = 0.0f;
-    for (int k = g; k < tl; k += PASS_LANES) {
-        const int j = (int)__builtin_nontemporal_load(e + slot_word<REC>(k));
+    // one slot: g~ of the pair (stored by the caller) and the sums; rho_j and div_j are read only
+    // for a used pair within h_st
+    auto pair = [&](float xjx, float xjy, float xjz, const float4& uj, bool use, auto&& rho_j, auto&& div_j) {
         float gt = 0.0f;
-        float4 xj = a.R.pos[j];
-        float4 uj = a.u4[j];
-        bool use = j != i && (ALLFLUID || a.mat[j] == a.fluid);
-        float x0 = xi.x - xj.x, x1 = xi.y - xj.y, x2 = xi.z - xj.z;
+        float x0 = xi.x - xjx, x1 = xi.y - xjy, x2 = xi.z - xjz;
         float r = len3(x0, x1, x2);
         if (use && r < a.hst) {
             float v0 = ui.x - uj.x, v1 = ui.y - uj.y, v2 = ui.z - uj.z;
-            float rho_bar = 0.5f * (rho_i + a.density[j]);
+            float rho_bar = 0.5f * (rho_i + rho_j());
             float wst = st_wst(r, a.d, a.hst, a.tn);
             float s = st_grad_scale(r, a.d, a.hst, a.tnh);
             float g0 = s * (x0 / r), g1 = s * (x1 / r), g2 = s * (x2 / r);
             if (s == 0.0f) { g0 = 0.0f; g1 = 0.0f; g2 = 0.0f; }
             float g_ij = wst / rho_bar;
             float term_kernel = (g0 * v0 + g1 * v1 + g2 * v2) / rho_bar;
-            float term_density = wst / (2.0f * rho_bar * rho_bar) * (div_i + a.div[j]);
+            float term_density = wst / (2.0f * rho_bar * rho_bar) * (div_i + div_j());
             gt = g_ij + a.dt * (term_kernel - term_density);
         }
-        eg[slot_word<64u>(k)] = gt;
         if (use) {
             float m_bar = 0.5f * (m_i + uj.w);
             diag = diag + m_bar * gt;
@@ -1464,6 +1493,35 @@ __global__ __launch_bounds__(BLOCK) void k_st_setup(StArgs a) {
                 by = by + m_bar * x1 * gt;
                 bz = bz + m_bar * x2 * gt;
             }
+        }
+        return gt;
+    };
+    if (has_windows<FM, PASS_LANES>() && win) {
+        // slots g + 8 b and g + 4 + 8 b (the lane's order of the loop below): one window-index
+        // word and one 8-byte g~ store per block
+        typedef float f32x2 __attribute__((ext_vector_type(2)));
+        const uint32_t* l = a.R.loc + ((size_t)(i >> 6) * (size_t)a.R.Cs * 32u + 4u * (i & (WAVE - 1)) + g);
+        f32x2* g2 = reinterpret_cast<f32x2*>(eg + 2 * g);
+        auto from = [&](uint32_t k) {
+            const float4 A = w.A[k];
+            const int d = w.D[k];
+            const bool use = (d & 0x7fffffff) != i && d >= 0;
+            return pair(A.x, A.y, A.z, w.B[k], use, [&] { return A.w; }, [&] { return w.C[k]; });
+        };
+        for (int b = 0; b < tl; b += 8) {
+            const uint32_t q = __builtin_nontemporal_load(l + (b >> 3) * 256);
+            const float t0 = from(q & 0xffffu);
+            const float t1 = from(q >> 16);
+            g2[(b >> 3) * 256] = f32x2{t0, t1};
+        }
+    } else {
+        for (int k = g; k < tl; k += PASS_LANES) {
+            const int j = (int)__builtin_nontemporal_load(e + slot_word<REC>(k));
+            const float4 xj = a.R.pos[j];
+            const float4 uj = a.u4[j];
+            const bool use = j != i && (ALLFLUID || a.mat[j] == a.fluid);
+            eg[slot_word<64u>(k)] = pair(xj.x, xj.y, xj.z, uj, use, [&] { return a.density[j]; },
+                                         [&] { return a.div[j]; });
         }
     }
     bx = group_sum<PASS_LANES>(bx);
