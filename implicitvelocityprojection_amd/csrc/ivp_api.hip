@@ -552,7 +552,8 @@ int ivp_set_option(ivp_ctx* c, int option, int value) {
         return IVP_OK;
     }
     if (option == IVP_OPT_BUILD_LANES) {
-        if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16) return IVP_ERR_ARG;
+        if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16 && value != 64)
+            return IVP_ERR_ARG;
         c->build_lanes = value;
         return IVP_OK;
     }

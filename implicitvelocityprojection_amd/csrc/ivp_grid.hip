@@ -479,6 +479,251 @@ __global__ __launch_bounds__(BLOCK) void k_scan(BuildArgs a) {
     }
 }
 
+// Candidate scan by cells (ids-only and float16 rows, one grid).  Every row of one cell visits
+// the same nine candidate runs (scan_ids: the runs of the 3x3 (y, z) neighbour cells, each over
+// the cells x - 1 .. x + 1), so a wave takes SCAN_ROWS consecutive rows of its block's 64-row tile,
+// cuts them into segments of equal cell, and for each segment loads the segment's candidates once
+// -- the nine runs laid end to end in scan_ids' visiting order, 64 per batch, up to SCAN_BATCHES
+// batches in registers -- and tests each row of the segment against them.  The row's hits of a
+// batch are ranked by the batch's wave mask, so slots fill in the one-lane order.  The slots are
+// collected in LDS (per row Cs words and 4 spare against bank conflicts; each wave owns its rows,
+// so no block barrier) and written out as whole 8-slot blocks of the paired layout, 512
+// contiguous bytes per wave and block.  Words of a written block outside the row's dynamic
+// [0, cnt) and static [Cs - cs, Cs) ranges are never read before the exact pass pads them.
+constexpr int SCAN_ROWS = 16;     // rows per wave: a block's four waves cover its tile
+constexpr int SCAN_BATCHES = 4;   // candidate batches of 64 held in registers per chunk
+
+// The candidate runs of a segment's cell, both populations at once: lane 16 p + k (k < 9) loads
+// population p's run k = 3 (dz + 1) + (dy + 1), the cells x - 1 .. x + 1 of row y + dy, layer
+// z + dz (empty outside the grid).  Returns, per lane, the exclusive prefix of the run lengths
+// within its population (rp) and the run's first particle minus it (rb: candidate v of run k is
+// particle v + rb); lane 16 p + 8 also returns the population's total in T.
+struct SegRuns {
+    int rp, rb, T;
+};
+__device__ __forceinline__ SegRuns seg_runs(const BuildArgs& a, int cx, int cy, int cz) {
+    const Grid& g = a.g;
+    const int lane = (int)(threadIdx.x & (WAVE - 1));
+    const int pop = lane >> 4, k = lane & 15;
+    const int zz = cz + k / 3 - 1, yy = cy + k % 3 - 1;
+    const bool live = pop < 2 && k < 9 && zz >= 0 && zz < g.dims[2] && yy >= 0 && yy < g.dims[1];
+    const int x0 = cx > 0 ? cx - 1 : 0;
+    const int x1 = cx < g.dims[0] - 1 ? cx + 1 : g.dims[0] - 1;
+    int s0 = 0, n = 0;
+    if (live) {
+        const int* start = pop == 0 ? a.dstart : a.sstart;
+        s0 = start[cell_key(g, x0, yy, zz)];
+        n = start[cell_key(g, x1, yy, zz) + 1] - s0;
+    }
+    // inclusive prefix over the 16 lanes of the population (DPP row shifts stay inside a row of
+    // 16 lanes and bring in 0 from outside it)
+    int inc = n;
+    inc += __builtin_amdgcn_update_dpp(0, inc, 0x111, 0xf, 0xf, false);   // row_shr:1
+    inc += __builtin_amdgcn_update_dpp(0, inc, 0x112, 0xf, 0xf, false);   // row_shr:2
+    inc += __builtin_amdgcn_update_dpp(0, inc, 0x114, 0xf, 0xf, false);   // row_shr:4
+    inc += __builtin_amdgcn_update_dpp(0, inc, 0x118, 0xf, 0xf, false);   // row_shr:8
+    return SegRuns{inc - n, s0 - (inc - n), inc};
+}
+
+// One row (position pr, global index row) against NBU batches of candidates (qx, qy, qz, qj; ok =
+// lanes holding a candidate): hit k of the row goes to slot k (dynamic population; an overflowing
+// row's hits past C all go to slot C, which no reader uses) or to slot Cs - 1 - k while k < lim
+// (static population), lanes without a stored hit write their own word of dum.  Returns the
+// row's hit count after these batches.  Without EXACT, candidates in the band [h2_lo, h2_hi) count
+// as misses and are flagged in near (the row is then done again with EXACT, which decides them by
+// the correctly rounded root as scan_group does).
+template <bool DYN, int NBU, bool EXACT>
+__device__ __forceinline__ int cell_row(const BuildArgs& a, const float (&qx)[SCAN_BATCHES],
+                                        const float (&qy)[SCAN_BATCHES], const float (&qz)[SCAN_BATCHES],
+                                        const int (&qj)[SCAN_BATCHES], const unsigned long long (&ok)[SCAN_BATCHES],
+                                        float3 pr, int row, int before, int lim, uint32_t* srow, uint32_t* dum,
+                                        unsigned long long& near) {
+#pragma unroll
+    for (int u = 0; u < NBU; ++u) {
+        const float dx = pr.x - qx[u], dy = pr.y - qy[u], dz = pr.z - qz[u];
+        const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        unsigned long long valid = ok[u];
+        if (DYN) valid &= __builtin_amdgcn_uicmp((unsigned)qj[u], (unsigned)row, ICMP_NE);
+        const unsigned long long lo = __builtin_amdgcn_fcmpf(d2, a.h2_lo, FCMP_OLT);
+        const unsigned long long hi = __builtin_amdgcn_fcmpf(d2, a.h2_hi, FCMP_OLT);
+        unsigned long long hit = lo & valid;
+        if (EXACT) {
+            const unsigned long long band = (hi & valid) & ~lo;
+            if (band) {
+                bool in = false;
+                if (__builtin_amdgcn_inverse_ballot_w64(band)) {
+                    float d = dx * dx + dy * dy + dz * dz;
+                    asm volatile("" : "+v"(d));
+                    in = sqrtf(d) < a.h;
+                }
+                hit |= __builtin_amdgcn_ballot_w64(in);
+            }
+        } else {
+            near |= hi ^ lo;
+        }
+        const int k = before + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(hit >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)hit, 0u));
+        uint32_t* p;
+        if (DYN) p = srow + (k < a.C ? k : a.C);
+        else p = k < lim ? srow + (a.Cs - 1 - k) : dum;
+        if (!__builtin_amdgcn_inverse_ballot_w64(hit)) p = dum;
+        *p = (uint32_t)qj[u];
+        before += (int)__popcll(hit);
+    }
+    return before;
+}
+
+// Rows rs .. re - 1 of a segment (global row row0 + r) against the chunk's NBU batches.  found
+// (lane r) counts row r's hits of the population; lim is C for the dynamic population and C - ndf
+// (lane r's dynamic total) for the static one.
+template <bool DYN, int NBU>
+__device__ __forceinline__ void cell_rows(const BuildArgs& a, const float (&qx)[SCAN_BATCHES],
+                                          const float (&qy)[SCAN_BATCHES], const float (&qz)[SCAN_BATCHES],
+                                          const int (&qj)[SCAN_BATCHES], const unsigned long long (&ok)[SCAN_BATCHES],
+                                          int cb, int rs, int re, int row0, float px, float py, float pz, int& found,
+                                          int ndf, uint32_t* st, uint32_t* dum, int S) {
+    for (int r = rs; r < re; ++r) {
+        const float3 pr = make_float3(
+            __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, px), r)),
+            __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, py), r)),
+            __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, pz), r)));
+        // a row's first chunk of a population starts at 0 (each row is in one segment)
+        const int before = cb == 0 ? 0 : __builtin_amdgcn_readlane(found, r);
+        int lim = DYN ? a.C : a.C - __builtin_amdgcn_readlane(ndf, r);
+        lim = lim > 0 ? lim : 0;
+        uint32_t* srow = st + r * S;
+        unsigned long long near = 0ull;
+        int after = cell_row<DYN, NBU, false>(a, qx, qy, qz, qj, ok, pr, row0 + r, before, lim, srow, dum, near);
+        if (near)   // a candidate in the band (rare): the row again, deciding the band exactly
+            after = cell_row<DYN, NBU, true>(a, qx, qy, qz, qj, ok, pr, row0 + r, before, lim, srow, dum, near);
+        found = (int)(threadIdx.x & (WAVE - 1)) == r ? after : found;
+    }
+}
+
+// One population of a segment: rows rs .. re - 1 of the wave (global row row0 + r) in the cell
+// (cx, cy, cz).  found (lane r) counts row r's hits of this population; the dynamic population
+// stores hit k at slot k while k < C, the static one at slot Cs - 1 - k while ndf + k < C (ndf =
+// lane r's dynamic total), as k_scan / scan_ids do.
+template <bool DYN>
+__device__ __forceinline__ void cell_pop_scan(const BuildArgs& a, const SegRuns& R, int T, int rs, int re, int row0,
+                                              float px, float py, float pz, int& found, int ndf, uint32_t* st,
+                                              uint32_t* dum, int S) {
+    constexpr int NB = SCAN_BATCHES;
+    constexpr int pl = DYN ? 0 : 16;
+    const int lane = (int)(threadIdx.x & (WAVE - 1));
+    const float4* __restrict__ ppos = DYN ? a.pos : a.pos + a.nd;
+    int rb[9], rp[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        rp[k] = __builtin_amdgcn_readlane(R.rp, pl + k);
+        rb[k] = __builtin_amdgcn_readlane(R.rb, pl + k);
+    }
+    // a particle of the population for lanes without a candidate (masked by ok)
+    const int jsafe = DYN ? row0 : a.ns - 1;
+    for (int cb = 0; cb < T; cb += WAVE * NB) {
+        float qx[NB], qy[NB], qz[NB];
+        int qj[NB];
+        unsigned long long ok[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            // candidate v = base + lane: the particle v + rb[k] of the last run k with rp[k] <= v
+            const int v = cb + WAVE * u + lane;
+            int off = rb[0];
+#pragma unroll
+            for (int k = 1; k < 9; ++k) off = v >= rp[k] ? rb[k] : off;
+            const bool in = v < T;
+            ok[u] = __builtin_amdgcn_ballot_w64(in);
+            qj[u] = in ? v + off : jsafe;
+        }
+        // every batch's loads in flight together (a batch past the candidates reads jsafe)
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const float4 q = ppos[qj[u]];
+            qx[u] = q.x; qy[u] = q.y; qz[u] = q.z;
+        }
+        const int nbu = (T - cb + WAVE - 1) / WAVE;
+        if (nbu >= 4)
+            cell_rows<DYN, 4>(a, qx, qy, qz, qj, ok, cb, rs, re, row0, px, py, pz, found, ndf, st, dum, S);
+        else if (nbu == 3)
+            cell_rows<DYN, 3>(a, qx, qy, qz, qj, ok, cb, rs, re, row0, px, py, pz, found, ndf, st, dum, S);
+        else if (nbu == 2)
+            cell_rows<DYN, 2>(a, qx, qy, qz, qj, ok, cb, rs, re, row0, px, py, pz, found, ndf, st, dum, S);
+        else
+            cell_rows<DYN, 1>(a, qx, qy, qz, qj, ok, cb, rs, re, row0, px, py, pz, found, ndf, st, dum, S);
+    }
+}
+
+// LDS of one k_scan_cells block: its tile's 64 rows of Cs + 4 words, then a discard word per
+// thread
+inline size_t scan_cells_lds(int Cs) { return ((size_t)WAVE * (size_t)(Cs + 4) + BLOCK) * sizeof(uint32_t); }
+constexpr size_t SCAN_CELLS_LDS_MAX = 64 * 1024;
+
+template <int FMT>
+__global__ __launch_bounds__(BLOCK) void k_scan_cells(BuildArgs a) {
+    static_assert(rec_words<FMT>() == 64u, "the cell scan writes the paired ids-only layout");
+    extern __shared__ uint32_t scan_stage[];
+    const int tile = xcd_block(blockIdx.x, gridDim.x);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE)), lane = (int)(threadIdx.x & (WAVE - 1));
+    const int row0 = tile * WAVE + wv * SCAN_ROWS;
+    if (row0 >= a.nd) return;   // whole waves; the block has no barrier
+    const int nrw = a.nd - row0 < SCAN_ROWS ? a.nd - row0 : SCAN_ROWS;
+    const int C = a.C, Cs = a.Cs, S = Cs + 4;
+    uint32_t* st = scan_stage + wv * SCAN_ROWS * S;
+    uint32_t* dum = scan_stage + WAVE * S + wv * WAVE + lane;   // the lane's discard word
+    const Grid& g = a.g;
+    // lane r < nrw: row row0 + r, its position and cell
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    int cx = 0, cy = 0, cz = 0, key = -1 - lane;
+    if (lane < nrw) {
+        const float4 p = a.pos[row0 + lane];
+        px = p.x; py = p.y; pz = p.z;
+        cx = cell_coord(g, px, 0); cy = cell_coord(g, py, 1); cz = cell_coord(g, pz, 2);
+        key = cell_key(g, cx, cy, cz);
+    }
+    int fd = 0, fs = 0;   // lane r: row r's dynamic and static hits
+    for (int s = 0; s < nrw;) {
+        const int k0 = __builtin_amdgcn_readlane(key, s);
+        const unsigned long long diff = __builtin_amdgcn_ballot_w64(lane > s && lane < nrw && key != k0);
+        const int e = diff ? (int)__builtin_ctzll(diff) : nrw;
+        const int scx = __builtin_amdgcn_readlane(cx, s), scy = __builtin_amdgcn_readlane(cy, s),
+                  scz = __builtin_amdgcn_readlane(cz, s);
+        const SegRuns R = seg_runs(a, scx, scy, scz);
+        const int Td = __builtin_amdgcn_readlane(R.T, 8), Ts = __builtin_amdgcn_readlane(R.T, 24);
+        if (Td > 0) cell_pop_scan<true>(a, R, Td, s, e, row0, px, py, pz, fd, 0, st, dum, S);
+        if (Ts > 0) cell_pop_scan<false>(a, R, Ts, s, e, row0, px, py, pz, fs, fd, st, dum, S);
+        s = e;
+    }
+    int cnt = 0, cs = 0;
+    if (lane < nrw) {
+        const int tot = fd + fs;
+        cnt = fd < C ? fd : C;
+        cs = (tot < C ? tot : C) - cnt;
+        a.rowcnt[row0 + lane] = cnt;
+        a.rowtot[row0 + lane] = tot;
+        if (tot > C) atomicOr(a.err_flag, 1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    // write-out: lane (q, sub) writes slots 8 b + sub and 8 b + sub + 4 of row q (words
+    // 512 b + 8 (row's place in the tile) + 2 sub), every block of [0, ceil8(cnt)) and of
+    // [floor8(Cs - cs), Cs); the slots of those blocks between the two ranges get the row's own id
+    // (the exact pass pads them so; until then it gathers them unused)
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    const int q = lane >> 2, sub = lane & 3;
+    const int qc = __shfl(cnt, q, WAVE), qs = __shfl(cs, q, WAVE);
+    if (q < nrw) {
+        const int nbd = (qc + 7) >> 3;
+        const int bs0 = qs > 0 ? (Cs - qs) >> 3 : (Cs >> 3);
+        const uint32_t* srow = st + q * S;
+        u32x2* dst = reinterpret_cast<u32x2*>(a.nb + (size_t)tile * (size_t)Cs * 64u +
+                                              8u * (uint32_t)(wv * SCAN_ROWS + q) + 2u * (uint32_t)sub);
+        const uint32_t self = (uint32_t)(row0 + q);
+        auto word = [&](int k) { return k < qc || k >= Cs - qs ? srow[k] : self; };
+        for (int b = 0; b < (Cs >> 3); ++b)
+            if (b < nbd || b >= bs0) dst[b * 256] = u32x2{word(8 * b + sub), word(8 * b + sub + 4)};
+    }
+}
+
 // Phase B after k_scan: one thread per row, the counts from k_scan
 template <int FMT>
 __global__ __launch_bounds__(BLOCK) void k_eval(BuildArgs a) {
@@ -1050,14 +1295,23 @@ int grid_load(ivp_ctx* c, int n, const float* x, const float* v, const float* m,
 // (build and sort per call): 8k rows 132 us with one fused lane, 92 with 16 lanes; 102k rows
 // 145 -> 115 with 4 lanes; 1M rows 501 -> 404 with 2 lanes and 4 batches in flight.  Fewer rows
 // leave fewer waves per SIMD, and the one-lane scan is then a chain of dependent load rounds.
+// 64 is the scan by cells (k_scan_cells), for ids-only and float16 rows of one grid whose
+// staged rows fit the LDS; elsewhere it falls back to 2 lanes per row.
 int build_lanes_for(const ivp_ctx* c) {
-    if (c->build_lanes > 0) return c->build_lanes;
     const int nd = c->nd;
-    return nd >= (1 << 18) ? 2 : nd >= (1 << 15) ? 4 : nd >= (1 << 12) ? 8 : 16;
+    int G = c->build_lanes > 0 ? c->build_lanes
+                               : nd >= (1 << 18) ? 64 : nd >= (1 << 15) ? 4 : nd >= (1 << 12) ? 8 : 16;
+    if (G == 64 && (c->rec_words != 64 || c->S > 0 || scan_cells_lds(c->Cs) > SCAN_CELLS_LDS_MAX)) G = 2;
+    return G;
 }
 
 template <int FMT, bool SCENES>
 int launch_scan(ivp_ctx* c, int G, const BuildArgs& ba) {
+    if constexpr (rec_words<FMT>() == 64u && !SCENES) {
+        if (G == 64)
+            return launch_shm(c, KC_SCAN, -1, 0.0, k_scan_cells<FMT>, dim3((c->nd + WAVE - 1) / WAVE),
+                              (unsigned)scan_cells_lds(c->Cs), ba);
+    }
     const dim3 grid(blocks_for(c->nd * G));
     switch (G) {
     case 2: return launch(c, KC_SCAN, -1, 0.0, k_scan<FMT, SCENES, 2>, grid, ba);

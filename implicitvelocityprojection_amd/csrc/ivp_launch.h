@@ -11,21 +11,28 @@ namespace ivp {
 // class the launch goes through hipExtLaunchKernelGGL with two events that the dispatch itself
 // stamps at kernel start and end, so the recorded time is the kernel's own execution time.
 // kclass < 0 marks helper launches that are never timed.
+// `shmem` bytes of dynamic LDS per block.
 template <typename... KArgs, typename... Args>
-inline int launch(ivp_ctx* c, int kclass, int tag, double bytes, void (*kernel)(KArgs...), dim3 grid,
-                  Args... args) {
+inline int launch_shm(ivp_ctx* c, int kclass, int tag, double bytes, void (*kernel)(KArgs...), dim3 grid,
+                      unsigned shmem, Args... args) {
     if (kclass >= 0 && c->prof && (c->prof_mask & (1u << kclass))) {
         ProfEvent ev;
         int rc = prof_take(c, kclass, tag, bytes, &ev);
         if (rc != IVP_OK) return rc;
-        hipExtLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, c->stream, ev.start, ev.stop, 0, args...);
+        hipExtLaunchKernelGGL(kernel, grid, dim3(BLOCK), shmem, c->stream, ev.start, ev.stop, 0, args...);
         rc = prof_push(c, &ev);
         if (rc != IVP_OK) return rc;
     } else {
-        hipLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, c->stream, args...);
+        hipLaunchKernelGGL(kernel, grid, dim3(BLOCK), shmem, c->stream, args...);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? IVP_OK : ivp_hip_fail(e, "kernel launch", __FILE__, __LINE__);
+}
+
+template <typename... KArgs, typename... Args>
+inline int launch(ivp_ctx* c, int kclass, int tag, double bytes, void (*kernel)(KArgs...), dim3 grid,
+                  Args... args) {
+    return launch_shm(c, kclass, tag, bytes, kernel, grid, 0u, args...);
 }
 
 }  // namespace ivp

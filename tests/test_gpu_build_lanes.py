@@ -1,5 +1,5 @@
-"""The neighbour build's candidate scan with 1, 2, 4, 8 or 16 lanes per row
-(IVP_OPT_BUILD_LANES) fills the rows identically: neighbour caches, density and k bitwise equal
+"""The neighbour build's candidate scan with 1, 2, 4, 8 or 16 lanes per row, or by cells (64;
+IVP_OPT_BUILD_LANES) fills the rows identically: neighbour caches, density and k bitwise equal
 to the oracle's in every setting, on the 8k and 102k columns and on a ragged count."""
 
 import numpy as np
@@ -26,7 +26,7 @@ def test_build_lanes_give_identical_rows(cfg_id, step, cache):
     mask = np.zeros(n * C, bool)
     for i in range(n):
         mask[i * C:i * C + sim.count[i]] = True
-    for lanes in (1, 2, 4, 8, 16, 0):
+    for lanes in (1, 2, 4, 8, 16, 64, 0):
         print(f"build lanes {lanes}: config {cfg_id} step {step} cache {cache}", flush=True)
         assert N.check(ctx.lib.ivp_set_option(ctx.h, N.OPT_BUILD_LANES, lanes)) == 0
         assert ctx.neighbors_build() == 0
@@ -57,7 +57,7 @@ def test_build_lanes_overflow_reported():
     ctx = Context(sim.h, cfg.domain_start, cfg.domain_size, n, cache_size=20)
     ctx.set_particles(t(sim.x), t(sim.v), t(sim.m), t(sim.m_inv), t(sim.density0), t(sim.is_dynamic, torch.int32),
                       t(sim.material, torch.int32))
-    for lanes in (1, 2, 8):
+    for lanes in (1, 2, 8, 64):
         N.check(ctx.lib.ivp_set_option(ctx.h, N.OPT_BUILD_LANES, lanes))
         assert ctx.neighbors_build() == N.IVP_NEIGHBOR_OVERFLOW
     ctx.close()
