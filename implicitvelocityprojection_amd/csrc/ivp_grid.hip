@@ -1295,13 +1295,16 @@ int grid_load(ivp_ctx* c, int n, const float* x, const float* v, const float* m,
 // (build and sort per call): 8k rows 132 us with one fused lane, 92 with 16 lanes; 102k rows
 // 145 -> 115 with 4 lanes; 1M rows 501 -> 404 with 2 lanes and 4 batches in flight.  Fewer rows
 // leave fewer waves per SIMD, and the one-lane scan is then a chain of dependent load rounds.
-// 64 is the scan by cells (k_scan_cells), for ids-only and float16 rows of one grid whose
-// staged rows fit the LDS; elsewhere it falls back to 2 lanes per row.
+// 64 is the scan by cells (k_scan_cells), for ids-only and float16 rows of one grid whose staged
+// rows fit the LDS: scan alone 1M rows 225 -> 186 us, 102k rows 32 -> 28 us, 8k rows 12-14 ->
+// 15 us (16 lanes stay there); where it does not apply, the count's lanes per row.
+static int lanes_by_count(int nd) { return nd >= (1 << 18) ? 2 : nd >= (1 << 15) ? 4 : nd >= (1 << 12) ? 8 : 16; }
+
 int build_lanes_for(const ivp_ctx* c) {
     const int nd = c->nd;
-    int G = c->build_lanes > 0 ? c->build_lanes
-                               : nd >= (1 << 18) ? 64 : nd >= (1 << 15) ? 4 : nd >= (1 << 12) ? 8 : 16;
-    if (G == 64 && (c->rec_words != 64 || c->S > 0 || scan_cells_lds(c->Cs) > SCAN_CELLS_LDS_MAX)) G = 2;
+    const bool cells = c->rec_words == 64 && c->S == 0 && scan_cells_lds(c->Cs) <= SCAN_CELLS_LDS_MAX;
+    int G = c->build_lanes > 0 ? c->build_lanes : cells && nd >= (1 << 15) ? 64 : lanes_by_count(nd);
+    if (G == 64 && !cells) G = lanes_by_count(nd);
     return G;
 }
 
