@@ -55,6 +55,33 @@ __device__ __forceinline__ float cubic_kernel(float r_norm, float h, float k) {
     return res;
 }
 
+// x / b rounded to nearest, given y = 1 / b rounded to nearest: q = RN(x y) is within an ulp of
+// x / b, and one correction with the exact residual x - b q gives the correctly rounded quotient
+// (Markstein's theorem; it needs y correctly rounded and no overflow or underflow, which holds for
+// the build's quotients: components of a vector over its length, a length over h).  Three
+// quotients by one divisor then cost one division and three corrections.
+__device__ __forceinline__ float div_rn(float x, float b, float y) {
+    const float q = x * y;
+    return fmaf(fmaf(-b, q, x), y, q);
+}
+
+// cubic_kernel with q = r_norm / h formed by div_rn from inv_h = 1 / h (rounded to nearest)
+__device__ __forceinline__ float cubic_kernel_rn(float r_norm, float h, float inv_h, float k) {
+    float res = 0.0f;
+    float q = div_rn(r_norm, h, inv_h);
+    if (q <= 1.0f) {
+        if (q <= 0.5f) {
+            float q2 = q * q;
+            float q3 = q2 * q;
+            res = k * (6.0f * q3 - 6.0f * q2 + 1.0f);
+        } else {
+            float factor = 1.0f - q;
+            res = k * 2.0f * factor * factor * factor;
+        }
+    }
+    return res;
+}
+
 __device__ __forceinline__ float len3(float x, float y, float z) {
     return sqrtf(x * x + y * y + z * z);
 }
@@ -65,6 +92,16 @@ __device__ __forceinline__ float3 spiky_grad(float rx, float ry, float rz, float
     float r_norm = len3(rx, ry, rz);
     if (r_norm < 1e-6f) r_norm = 1e-6f;
     float gx = rx / r_norm, gy = ry / r_norm, gz = rz / r_norm;
+    float t = h - r_norm;
+    float s = -k * (t * t);
+    return make_float3(s * gx, s * gy, s * gz);
+}
+
+// spiky_grad given r_norm = len3(rx, ry, rz), the three quotients by div_rn
+__device__ __forceinline__ float3 spiky_grad_rn(float rx, float ry, float rz, float r_norm, float h, float k) {
+    if (r_norm < 1e-6f) r_norm = 1e-6f;
+    const float y = 1.0f / r_norm;
+    float gx = div_rn(rx, r_norm, y), gy = div_rn(ry, r_norm, y), gz = div_rn(rz, r_norm, y);
     float t = h - r_norm;
     float s = -k * (t * t);
     return make_float3(s * gx, s * gy, s * gz);

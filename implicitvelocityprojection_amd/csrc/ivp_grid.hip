@@ -294,6 +294,7 @@ struct BuildArgs {
     int nd, ns, C, Cs;
     float h, h2_lo, h2_hi;
     float kc, ks;   // cubic and spiky kernel constants
+    float inv_h;    // 1 / h rounded to nearest (div_rn)
     float inv_gs;   // 1 / scale of float16 gradW records (a power of two)
     Grid g;
     const Grid* grids;   // independent scenes: grid of each scene, indexed by sid
@@ -766,7 +767,7 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
         }
         float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         float rn = len3(rx, ry, rz);
-        float3 gw = spiky_grad(rx, ry, rz, h, a.ks);
+        float3 gw = spiky_grad_rn(rx, ry, rz, rn, h, a.ks);
         if (FMT == IVP_ROWS_F32) {
             rec[64] = __float_as_uint(gw.x);
             rec[128] = __float_as_uint(gw.y);
@@ -775,7 +776,7 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
             h0 = pack_half2(gw.x * a.inv_gs, gw.y * a.inv_gs);
             h1 = pack_half2(gw.z * a.inv_gs, 0.0f);
         }
-        rho += pj.w * cubic_kernel(rn, h, a.kc);
+        rho += pj.w * cubic_kernel_rn(rn, h, a.inv_h, a.kc);
         float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
         aii += (Jx * Jx + Jy * Jy + Jz * Jz) * minv_j;
         jx -= Jx; jy -= Jy; jz -= Jz;
@@ -847,8 +848,8 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
         float4 pj = a.pos[a.nd + j];
         float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         float rn = len3(rx, ry, rz);
-        float3 gw = spiky_grad(rx, ry, rz, h, a.ks);
-        rho += pj.w * cubic_kernel(rn, h, a.kc);
+        float3 gw = spiky_grad_rn(rx, ry, rz, rn, h, a.ks);
+        rho += pj.w * cubic_kernel_rn(rn, h, a.inv_h, a.kc);
         float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
         jx -= Jx; jy -= Jy; jz -= Jz;
         sx += Jx; sy += Jy; sz += Jz;
@@ -1339,7 +1340,7 @@ int grid_build(ivp_ctx* c) {
         const double hh = (double)c->h * (double)c->h;
         float h2_lo = std::nextafter((float)(hh * (1.0 - 1.0 / 1048576.0)), 0.0f);
         float h2_hi = std::nextafter((float)(hh * (1.0 + 1.0 / 1048576.0)), 1.0f);
-        BuildArgs ba{c->nd, c->ns, c->C, c->Cs, c->h, h2_lo, h2_hi, c->kcubic, c->kspiky, 1.0f / c->gscale, c->grid,
+        BuildArgs ba{c->nd, c->ns, c->C, c->Cs, c->h, h2_lo, h2_hi, c->kcubic, c->kspiky, 1.0f / c->h, 1.0f / c->gscale, c->grid,
                      c->grids, c->sid, c->pos, c->minv, c->dstart, c->sstart, c->nb, c->g16, c->tile_len,
                      c->rowcnt, c->rowtot, c->density, c->kk, c->Ss, c->err_flag, c->partials, c->max_blocks};
         // bytes (pair count of the previous build): the pair records written, the ids read back
