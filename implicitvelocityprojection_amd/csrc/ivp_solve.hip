@@ -1333,10 +1333,11 @@ __global__ __launch_bounds__(BLOCK) void k_w18_apply(int n3, const float* __rest
 // ---- Jeske23 implicit surface tension (jeske23.py) -----------------------------------------
 
 // compute_Wst, jeske23.py:23-43: theta W_cubic(max(r, d); h_st), theta = 10/7 (same operation
-// order as the reference; norm = 8 / (pi h_st^3) is passed in)
-__device__ __forceinline__ float st_wst(float r, float d, float hst, float tn) {
+// order as the reference; norm = 8 / (pi h_st^3) is passed in; q = rc / h_st by div_rn from
+// inv_hst = 1 / h_st rounded to nearest, the correctly rounded quotient)
+__device__ __forceinline__ float st_wst(float r, float d, float hst, float inv_hst, float tn) {
     float rc = r > d ? r : d;
-    float q = rc / hst;
+    float q = div_rn(rc, hst, inv_hst);
     float fq = 0.0f;
     if (q <= 0.5f) {
         fq = 1.0f - 6.0f * q * q + 6.0f * q * q * q;
@@ -1348,9 +1349,9 @@ __device__ __forceinline__ float st_wst(float r, float d, float hst, float tn) {
 }
 
 // compute_grad_Wst, jeske23.py:46-66: returns s with grad = s * (x_ij / r); 0 for r <= d
-__device__ __forceinline__ float st_grad_scale(float r, float d, float hst, float tnh) {
+__device__ __forceinline__ float st_grad_scale(float r, float d, float hst, float inv_hst, float tnh) {
     if (r <= d || r < 1.0e-12f) return 0.0f;
-    float q = r / hst;
+    float q = div_rn(r, hst, inv_hst);
     if (q > 1.0f) return 0.0f;
     float fpq;
     if (q <= 0.5f) {
@@ -1374,6 +1375,7 @@ struct StArgs {
     float* pinv;       // 3x3 per row (diagonal)
     float* out;        // M u, or v - M v + b
     float sigma, dt, d, hst;
+    float inv_hst;     // 1 / h_st rounded to nearest
     float tn, tnh;     // theta * 8/(pi h_st^3), and that times 1/h_st
     double* partials;
     int stride;
@@ -1476,9 +1478,11 @@ __global__ __launch_bounds__(BLOCK) void k_st_setup(StArgs a) {
         if (use && r < a.hst) {
             float v0 = ui.x - uj.x, v1 = ui.y - uj.y, v2 = ui.z - uj.z;
             float rho_bar = 0.5f * (rho_i + rho_j());
-            float wst = st_wst(r, a.d, a.hst, a.tn);
-            float s = st_grad_scale(r, a.d, a.hst, a.tnh);
-            float g0 = s * (x0 / r), g1 = s * (x1 / r), g2 = s * (x2 / r);
+            float wst = st_wst(r, a.d, a.hst, a.inv_hst, a.tn);
+            float s = st_grad_scale(r, a.d, a.hst, a.inv_hst, a.tnh);
+            // x / r correctly rounded (div_rn; r = 0 gives NaN as the division does, zeroed below)
+            const float yr = 1.0f / r;
+            float g0 = s * div_rn(x0, r, yr), g1 = s * div_rn(x1, r, yr), g2 = s * div_rn(x2, r, yr);
             if (s == 0.0f) { g0 = 0.0f; g1 = 0.0f; g2 = 0.0f; }
             float g_ij = wst / rho_bar;
             float term_kernel = (g0 * v0 + g1 * v1 + g2 * v2) / rho_bar;
@@ -2067,7 +2071,7 @@ int jeske23_body(ivp_ctx* c, float sigma, float dt, int* iters_out) {
     const float tn = theta * norm;
     const float tnh = tn * (1.0f / hst);
     StArgs sa{R, c->cfg.material_fluid, c->mat, c->density, c->w_v4, c->st_div, c->st_gt, c->st_b, c->w_pinv,
-              c->w_rhs, sigma, dt, d, hst, tn, tnh, c->partials, stride, c->state};
+              c->w_rhs, sigma, dt, d, hst, 1.0f / hst, tn, tnh, c->partials, stride, c->state};
     const double pairs = (double)c->state_host->total_pairs;
     const double pb = pair_bytes<FM>();
     IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_st_prep, nb, nd, (const float*)c->vel, (const float4*)c->pos, c->w_v4));
