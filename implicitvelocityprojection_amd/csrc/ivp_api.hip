@@ -94,6 +94,9 @@ void free_all(ivp_ctx* c) {
     c->prof_free.clear();
     for (auto e : c->iter_ev) (void)hipEventDestroy(e);
     c->iter_ev.clear();
+    if (c->ip_end_ev) (void)hipEventDestroy(c->ip_end_ev);
+    c->ip_end_ev = nullptr;
+    c->ip_wait_pending = false;
 }
 
 int alloc_all(ivp_ctx* c) {
@@ -713,14 +716,18 @@ int ivp_step(ivp_ctx* c, const ivp_step_params* prm, ivp_ip_stats* stats, ivp_cg
             rc = visc_explicit(c, prm->viscosity_coeff, prm->dt);
         if (rc != IVP_OK) return rc;
     }
-    // pressure solve (framework.py:481-487; PRJ's divergence-free pass runs inside prj_solve)
-    rc = prm->pressure_method == 0 ? prj_solve(c, prm->dt, &prm->prj, stats, PRJ_BOTH) : ip_solve(c, prm->dt, &prm->ip, stats);
+    // pressure solve (framework.py:481-487; PRJ's divergence-free pass runs inside prj_solve).
+    // The IP solve's closing wait (its wall time and stats_iter_costs) comes after the rest of
+    // the step is enqueued, so the device does not idle while the host reads the clock.
+    rc = prm->pressure_method == 0 ? prj_solve(c, prm->dt, &prm->prj, stats, PRJ_BOTH)
+                                   : ip_solve(c, prm->dt, &prm->ip, stats, true);
     if (rc != IVP_OK) return truncated(rc);
     rc = glue_advect_boundary(c, prm->dt);
-    if (rc != IVP_OK) return rc;
     // the rebuild that ends the step, not waited for: the step's results come from the rows built
     // before it, and a row of the new build over the capacity stops the next step (above)
-    return grid_build(c);
+    if (rc == IVP_OK) rc = grid_build(c);
+    const int rw = ip_solve_wait(c, stats);
+    return rc != IVP_OK ? rc : rw;
 }
 
 int ivp_set_scenes(ivp_ctx* c, int n_scenes, const ivp_scene* scenes) {

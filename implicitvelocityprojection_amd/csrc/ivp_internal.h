@@ -26,6 +26,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -301,6 +302,12 @@ struct ivp_ctx {
 
     // stream marks at outer-iteration / pass boundaries (stats_iter_costs)
     std::vector<hipEvent_t> iter_ev;
+    // an IP solve whose closing wait was left to the caller (ivp_step enqueues the rest of the
+    // step first): its end mark, start time and outer-loop evaluation count
+    hipEvent_t ip_end_ev = nullptr;
+    bool ip_wait_pending = false;
+    std::chrono::steady_clock::time_point ip_t0;
+    int ip_n_err = 0;
 
     // profiling
     bool prof = false;
@@ -339,7 +346,10 @@ int store_read(ivp_ctx* c, int field, void* dst);
 int ip_solve_scenes(ivp_ctx* c, float dt, const ivp_ip_params* prm);
 
 // ivp_solve.hip
-int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* st);
+// defer: leave the closing synchronisation (and with it stats_iter_costs and solve_ms) to
+// ip_solve_wait, so that the caller can enqueue more work first
+int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* st, bool defer = false);
+int ip_solve_wait(ivp_ctx* c, ivp_ip_stats* st);
 int visc_explicit(ivp_ctx* c, float coeff, float dt);
 int visc_weiler18(ivp_ctx* c, float coeff, float dt, ivp_cg_stats* st);
 int st_jeske23(ivp_ctx* c, float sigma, float dt, ivp_cg_stats* st);

@@ -1887,19 +1887,32 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
         IVP_TRY(launch(c, KC_IP_PACK, -1, stream_bytes<FP>(c, total_pairs) + 68.0 * nd, k_ip_pack<FP, false>, np, pa));
         IVP_TRY(finalize<FIN_PACK>(c, np.x));
         IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        IVP_TRY(read_state(c));
-        if (*c->err_host & 4) return IVP_ERR_INTERNAL;
-        if (*c->err_host & 1) return IVP_NEIGHBOR_OVERFLOW;
-        err = c->state_host->err_sum / (double)nd;
-        if (st.n_density_errors < st.capacity && st.density_errors) st.density_errors[st.n_density_errors] = err;
-        st.n_density_errors++;
-        bool converged = err < prm->tol_opt && !prm->use_fixed_iters;
-        if ((converged && opt_iter >= 1) || opt_iter >= prm->max_iteration_opt) {
-            IVP_TRY(iter_mark(c, opt_iter + 1));
-            break;
+        // The first evaluation never ends the loop (the stop needs opt_iter >= 1, ip.py:374-376),
+        // so the first CG follows without a wait: its first chunk's synchronisation brings back the
+        // overflow flags and this pass's error sum and active counts (no later finalize writes them)
+        // before anything reads them.
+        auto record = [&]() -> int {
+            if (*c->err_host & 4) return IVP_ERR_INTERNAL;
+            if (*c->err_host & 1) return IVP_NEIGHBOR_OVERFLOW;
+            err = c->state_host->err_sum / (double)nd;
+            if (st.n_density_errors < st.capacity && st.density_errors) st.density_errors[st.n_density_errors] = err;
+            st.n_density_errors++;
+            return IVP_OK;
+        };
+        const bool deferred = opt_iter == 0 && prm->max_iteration_opt >= 1;
+        long long act_pairs = c->state_host->active_pairs;   // (deferred: the last evaluation's, for
+        int act_rows = c->state_host->n_active_rows;         //  the byte model until read back)
+        if (!deferred) {
+            IVP_TRY(read_state(c));
+            IVP_TRY(record());
+            bool converged = err < prm->tol_opt && !prm->use_fixed_iters;
+            if ((converged && opt_iter >= 1) || opt_iter >= prm->max_iteration_opt) {
+                IVP_TRY(iter_mark(c, opt_iter + 1));
+                break;
+            }
+            act_pairs = c->state_host->active_pairs;
+            act_rows = c->state_host->n_active_rows;
         }
-        long long act_pairs = c->state_host->active_pairs;
-        int act_rows = c->state_host->n_active_rows;
 
         RhsArgs ra{R, c->q4, c->Ss, prm->omega, c->b, c->nbact, c->rowcnt, c->partials, stride};
         IVP_TRY(launch(c, KC_IP_RHS, -1, stream_bytes<FP>(c, total_pairs) + 48.0 * nd, k_ip_rhs<FP, false>, np, ra));
@@ -1954,6 +1967,13 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
                               (const SolverState*)c->state, SceneRef{});
             };
             IVP_TRY(run_chunks(c, prm->max_iteration_pcg, c->cg_guess, one_iter, &iters, &st.cg_launch_chunks));
+            if (deferred) {
+                IVP_TRY(record());
+                act_pairs = c->state_host->active_pairs;
+                act_rows = c->state_host->n_active_rows;
+                prof_set_bytes(c, epoch, KC_CG_J,
+                               stream_bytes<FC>(c, (double)act_pairs) + (double)act_rows * (16 + 12 + 8) + (double)nd * 4);
+            }
             touch_pairs = c->state_host->touched_pairs;
             touch_rows = c->state_host->n_touched_rows;
             prof_set_bytes(c, epoch, KC_CG_JT,
@@ -2274,7 +2294,7 @@ int st_jeske23(ivp_ctx* c, float sigma, float dt, ivp_cg_stats* out) {
 
 // IP.solve with its wall-clock cost (stats_total_cost_ms, ip.py:335, 403-404: the reference
 // synchronises before reading the clock, and so does this).
-int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats) {
+int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats, bool defer) {
     if (!c->built) return IVP_ERR_STATE;
     if (!prm || !stats) return IVP_ERR_ARG;
     ivp_ip_stats& st = *stats;
@@ -2297,12 +2317,31 @@ int ip_solve(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* stats
     case IVP_ROWS_F16: rc = ip_solve_body<FmtF16Eval, FmtF16>(c, dt, prm, stats); break;
     default: rc = ip_solve_body<FmtIds, FmtIds>(c, dt, prm, stats); break;
     }
+    if (rc == IVP_OK && defer && c->nd > 0) {
+        if (!c->ip_end_ev) IVP_HIP(hipEventCreateWithFlags(&c->ip_end_ev, hipEventDisableTiming));
+        IVP_HIP(hipEventRecord(c->ip_end_ev, c->stream));
+        c->ip_wait_pending = true;
+        c->ip_t0 = t0;
+        c->ip_n_err = st.n_density_errors;
+        return rc;
+    }
     if (rc >= 0) {
         IVP_HIP(hipStreamSynchronize(c->stream));
         if (c->nd > 0) iter_costs(c, st.n_density_errors, stats);
     }
     stats->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
+}
+
+// The closing wait of a deferred ip_solve: the solve's end mark, then stats_iter_costs and the
+// solve's wall time as ip_solve would have filled them.
+int ip_solve_wait(ivp_ctx* c, ivp_ip_stats* stats) {
+    if (!c->ip_wait_pending) return IVP_OK;
+    c->ip_wait_pending = false;
+    IVP_HIP(hipEventSynchronize(c->ip_end_ev));
+    iter_costs(c, c->ip_n_err, stats);
+    stats->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c->ip_t0).count();
+    return IVP_OK;
 }
 
 // PRJ.solve (prj.py:292-340), followed by solve_df (prj.py:213-290) when part == PRJ_BOTH and
