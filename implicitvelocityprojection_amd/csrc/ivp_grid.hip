@@ -480,9 +480,9 @@ __global__ __launch_bounds__(BLOCK) void k_scan(BuildArgs a) {
     }
 }
 
-// Candidate scan by cells (ids-only and float16 rows, one grid).  Every row of one cell visits
-// the same nine candidate runs (scan_ids: the runs of the 3x3 (y, z) neighbour cells, each over
-// the cells x - 1 .. x + 1), so a wave takes SCAN_ROWS consecutive rows of its block's 64-row tile,
+// Candidate scan by cells (ids-only and float16 rows; one grid or a grid per scene).  Every row
+// of one cell visits the same nine candidate runs (scan_ids: the runs of the 3x3 (y, z) neighbour
+// cells, each over the cells x - 1 .. x + 1), so a wave takes SCAN_ROWS consecutive rows of its block's 64-row tile,
 // cuts them into segments of equal cell, and for each segment loads the segment's candidates once
 // -- the nine runs laid end to end in scan_ids' visiting order, 64 per batch, up to SCAN_BATCHES
 // batches in registers -- and tests each row of the segment against them.  The row's hits of a
@@ -502,8 +502,7 @@ constexpr int SCAN_BATCHES = 4;   // candidate batches of 64 held in registers p
 struct SegRuns {
     int rp, rb, T;
 };
-__device__ __forceinline__ SegRuns seg_runs(const BuildArgs& a, int cx, int cy, int cz) {
-    const Grid& g = a.g;
+__device__ __forceinline__ SegRuns seg_runs(const BuildArgs& a, const Grid& g, int cx, int cy, int cz) {
     const int lane = (int)(threadIdx.x & (WAVE - 1));
     const int pop = lane >> 4, k = lane & 15;
     const int zz = cz + k / 3 - 1, yy = cy + k % 3 - 1;
@@ -659,7 +658,7 @@ __device__ __forceinline__ void cell_pop_scan(const BuildArgs& a, const SegRuns&
 inline size_t scan_cells_lds(int Cs) { return ((size_t)WAVE * (size_t)(Cs + 4) + BLOCK) * sizeof(uint32_t); }
 constexpr size_t SCAN_CELLS_LDS_MAX = 64 * 1024;
 
-template <int FMT>
+template <int FMT, bool SCENES>
 __global__ __launch_bounds__(BLOCK) void k_scan_cells(BuildArgs a) {
     static_assert(rec_words<FMT>() == 64u, "the cell scan writes the paired ids-only layout");
     extern __shared__ uint32_t scan_stage[];
@@ -671,13 +670,15 @@ __global__ __launch_bounds__(BLOCK) void k_scan_cells(BuildArgs a) {
     const int C = a.C, Cs = a.Cs, S = Cs + 4;
     uint32_t* st = scan_stage + wv * SCAN_ROWS * S;
     uint32_t* dum = scan_stage + WAVE * S + wv * WAVE + lane;   // the lane's discard word
-    const Grid& g = a.g;
-    // lane r < nrw: row row0 + r, its position and cell
+    // lane r < nrw: row row0 + r, its position and cell (independent scenes: in its scene's
+    // grid, whose keys follow the previous scene's, so a segment never spans two scenes)
     float px = 0.0f, py = 0.0f, pz = 0.0f;
-    int cx = 0, cy = 0, cz = 0, key = -1 - lane;
+    int cx = 0, cy = 0, cz = 0, key = -1 - lane, sid = 0;
     if (lane < nrw) {
         const float4 p = a.pos[row0 + lane];
         px = p.x; py = p.y; pz = p.z;
+        if (SCENES) sid = a.sid[row0 + lane];
+        const Grid& g = SCENES ? a.grids[sid] : a.g;
         cx = cell_coord(g, px, 0); cy = cell_coord(g, py, 1); cz = cell_coord(g, pz, 2);
         key = cell_key(g, cx, cy, cz);
     }
@@ -688,7 +689,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan_cells(BuildArgs a) {
         const int e = diff ? (int)__builtin_ctzll(diff) : nrw;
         const int scx = __builtin_amdgcn_readlane(cx, s), scy = __builtin_amdgcn_readlane(cy, s),
                   scz = __builtin_amdgcn_readlane(cz, s);
-        const SegRuns R = seg_runs(a, scx, scy, scz);
+        const Grid& sg = SCENES ? a.grids[__builtin_amdgcn_readlane(sid, s)] : a.g;
+        const SegRuns R = seg_runs(a, sg, scx, scy, scz);
         const int Td = __builtin_amdgcn_readlane(R.T, 8), Ts = __builtin_amdgcn_readlane(R.T, 24);
         if (Td > 0) cell_pop_scan<true>(a, R, Td, s, e, row0, px, py, pz, fd, 0, st, dum, S);
         if (Ts > 0) cell_pop_scan<false>(a, R, Ts, s, e, row0, px, py, pz, fs, fd, st, dum, S);
@@ -1296,14 +1298,14 @@ int grid_load(ivp_ctx* c, int n, const float* x, const float* v, const float* m,
 // (build and sort per call): 8k rows 132 us with one fused lane, 92 with 16 lanes; 102k rows
 // 145 -> 115 with 4 lanes; 1M rows 501 -> 404 with 2 lanes and 4 batches in flight.  Fewer rows
 // leave fewer waves per SIMD, and the one-lane scan is then a chain of dependent load rounds.
-// 64 is the scan by cells (k_scan_cells), for ids-only and float16 rows of one grid whose staged
-// rows fit the LDS: scan alone 1M rows 225 -> 186 us, 102k rows 32 -> 28 us, 8k rows 12-14 ->
+// 64 is the scan by cells (k_scan_cells), for ids-only and float16 rows (one scene or many) whose
+// staged rows fit the LDS: scan alone 1M rows 225 -> 186 us, 102k rows 32 -> 28 us, 8k rows 12-14 ->
 // 15 us (16 lanes stay there); where it does not apply, the count's lanes per row.
 static int lanes_by_count(int nd) { return nd >= (1 << 18) ? 2 : nd >= (1 << 15) ? 4 : nd >= (1 << 12) ? 8 : 16; }
 
 int build_lanes_for(const ivp_ctx* c) {
     const int nd = c->nd;
-    const bool cells = c->rec_words == 64 && c->S == 0 && scan_cells_lds(c->Cs) <= SCAN_CELLS_LDS_MAX;
+    const bool cells = c->rec_words == 64 && scan_cells_lds(c->Cs) <= SCAN_CELLS_LDS_MAX;
     int G = c->build_lanes > 0 ? c->build_lanes : cells && nd >= (1 << 15) ? 64 : lanes_by_count(nd);
     if (G == 64 && !cells) G = lanes_by_count(nd);
     return G;
@@ -1311,9 +1313,9 @@ int build_lanes_for(const ivp_ctx* c) {
 
 template <int FMT, bool SCENES>
 int launch_scan(ivp_ctx* c, int G, const BuildArgs& ba) {
-    if constexpr (rec_words<FMT>() == 64u && !SCENES) {
+    if constexpr (rec_words<FMT>() == 64u) {
         if (G == 64)
-            return launch_shm(c, KC_SCAN, -1, 0.0, k_scan_cells<FMT>, dim3((c->nd + WAVE - 1) / WAVE),
+            return launch_shm(c, KC_SCAN, -1, 0.0, k_scan_cells<FMT, SCENES>, dim3((c->nd + WAVE - 1) / WAVE),
                               (unsigned)scan_cells_lds(c->Cs), ba);
     }
     const dim3 grid(blocks_for(c->nd * G));

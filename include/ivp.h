@@ -280,7 +280,7 @@ int ivp_scene_history(ivp_ctx *ctx, int scene, int capacity, double *density_err
  * count; 1 one lane per row, fused with the gradient/density/Aii pass; 2, 4, 8 or 16 lanes per row
  * in a separate scan (slots filled in the same order); 64 the scan by cells (a wave loads the
  * candidates of a cell once, 64 at a time, and tests every row of the cell against them; ids-only
- * and float16 rows of one grid, otherwise the count's choice).  The default takes the scan by
+ * and float16 rows, independent scenes included, otherwise the count's choice).  The default takes the scan by
  * cells from 32768 dynamic rows where it applies.  The rows are identical either way. */
 #define IVP_OPT_BUILD_LANES 5
 /* Weiler18's mutable solver attributes, read at every Weiler18 solve (the reference's GUI toggles

@@ -65,6 +65,33 @@ def test_scene_build_bit_exact_against_single_contexts():
     batch.close()
 
 
+@pytest.mark.parametrize("lanes", [1, 2, 64], ids=["one_lane", "two_lanes", "by_cells"])
+def test_scene_build_forms_give_identical_rows(lanes):
+    """Every candidate-scan form builds the batch's rows (each scene in its own grid) exactly as
+    the default does: counts, ids, density and k bitwise."""
+    from implicitvelocityprojection_amd import _native as N
+    from implicitvelocityprojection_amd.scene_batch import SceneBatch
+    batch = SceneBatch(_scenes())
+    C = 60
+    out = {}
+    for form in (0, lanes):
+        assert N.check(batch.ctx.lib.ivp_set_option(batch.ctx.h, N.OPT_BUILD_LANES, form)) == 0
+        assert batch.ctx.neighbors_build() == 0
+        cnt = torch.zeros(batch.n, dtype=torch.int32, device="cuda")
+        ids = torch.zeros(batch.n * C, dtype=torch.int32, device="cuda")
+        g = torch.zeros(batch.n * C * 3, dtype=torch.float32, device="cuda")
+        batch.ctx.neighbors_export(cnt, ids, g)
+        out[form] = (_np(cnt), _np(ids).reshape(batch.n, C), batch.read(2, (batch.n,)), batch.read(3, (batch.n,)))
+    c0, i0, d0, k0 = out[0]
+    c1, i1, d1, k1 = out[lanes]
+    assert np.array_equal(c0, c1)
+    live = np.arange(C)[None, :] < c0[:, None]
+    assert np.array_equal(i0[live], i1[live])
+    assert np.array_equal(d0.view(np.uint32), d1.view(np.uint32))
+    assert np.array_equal(k0.view(np.uint32), k1.view(np.uint32))
+    batch.close()
+
+
 def test_scene_neighbour_export_matches_single_contexts():
     from implicitvelocityprojection_amd.scene_batch import SceneBatch
     scenes = _scenes()[:3]
