@@ -480,17 +480,17 @@ __global__ __launch_bounds__(BLOCK) void k_scan(BuildArgs a) {
     }
 }
 
-// Candidate scan by cells (ids-only and float16 rows; one grid or a grid per scene).  Every row
-// of one cell visits the same nine candidate runs (scan_ids: the runs of the 3x3 (y, z) neighbour
-// cells, each over the cells x - 1 .. x + 1), so a wave takes SCAN_ROWS consecutive rows of its block's 64-row tile,
-// cuts them into segments of equal cell, and for each segment loads the segment's candidates once
-// -- the nine runs laid end to end in scan_ids' visiting order, 64 per batch, up to SCAN_BATCHES
-// batches in registers -- and tests each row of the segment against them.  The row's hits of a
-// batch are ranked by the batch's wave mask, so slots fill in the one-lane order.  The slots are
-// collected in LDS (per row Cs words and 4 spare against bank conflicts; each wave owns its rows,
-// so no block barrier) and written out as whole 8-slot blocks of the paired layout, 512
-// contiguous bytes per wave and block.  Words of a written block outside the row's dynamic
-// [0, cnt) and static [Cs - cs, Cs) ranges are never read before the exact pass pads them.
+// Candidate scan by cells (ids-only and float16 rows; one grid or a grid per scene).  Every row of
+// one cell visits the same nine candidate runs (scan_ids: the runs of the 3x3 (y, z) neighbour
+// cells, each over the cells x - 1 .. x + 1), so a wave takes SCAN_ROWS consecutive rows of its
+// block's 64-row tile, cuts them into segments of equal cell, and for each segment loads the
+// segment's candidates once -- the nine runs laid end to end in scan_ids' visiting order, 64 per
+// batch, up to SCAN_BATCHES batches in registers -- and tests each row of the segment against them.
+// The row's hits of a batch are ranked by the batch's wave mask, so slots fill in the one-lane
+// order.  The slots are collected in LDS (per row Cs words and 4 spare against bank conflicts; each
+// wave owns its rows, so no block barrier) and written out as whole 8-slot blocks of the paired
+// layout, 512 contiguous bytes per wave and block; words of a written block between the row's
+// dynamic [0, cnt) and static [Cs - cs, Cs) ranges get the row's own id, as the exact pass pads.
 constexpr int SCAN_ROWS = 16;     // rows per wave: a block's four waves cover its tile
 constexpr int SCAN_BATCHES = 4;   // candidate batches of 64 held in registers per chunk
 
