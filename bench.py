@@ -283,6 +283,11 @@ def scene_io_study(torch, repeats=5, cpu=True):
     return out
 
 
+def _progress(msg):
+    """A progress line on stderr (the result line stays alone on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def batched_study(n_scenes, steps, torch):
     """The batched form: n_scenes independent columns (synthetic_batch, scene-0 physics) stepped
     in one context, against the same scenes stepped one by one, each in its own context.
@@ -292,8 +297,10 @@ def batched_study(n_scenes, steps, torch):
     from implicitvelocityprojection_amd.config import SimConfig
     from implicitvelocityprojection_amd.scenes import load_particles, particle_masses, synthetic_batch
     scenes = synthetic_batch(n_scenes=n_scenes)
+    _progress(f"batched: {n_scenes} scenes loaded, autotuning")
     batch = SceneBatch(scenes)
     tune = batch.autotune(chunks=(1, 2, 4, 8), lanes=(1, 2, 4), steps=2)
+    _progress("batched: autotuned, timing")
     chosen = {"cg_chunk": batch.ctx.scene_cg_chunk, "lanes_per_row": batch.ctx.scene_lanes}
     for _ in range(2):
         batch.step()
@@ -312,6 +319,7 @@ def batched_study(n_scenes, steps, torch):
         max_err = max(max_err, max(s[3] for s in stats))
     torch.cuda.synchronize()
     wall_b = time.perf_counter() - t0
+    _progress(f"batched: {steps} steps timed; the scenes one by one")
     nd = batch.n_dynamic
     n_all = batch.n
     batch.close()
@@ -330,6 +338,8 @@ def batched_study(n_scenes, steps, torch):
         if ctx.neighbors_build() != 0:
             raise RuntimeError("neighbour overflow at load")
         ctxs.append((cfg, ctx))
+        if len(ctxs) % 64 == 0:
+            _progress(f"one by one: {len(ctxs)} contexts loaded")
     prm = _ip_params(ctxs[0][0])
     for cfg, ctx in ctxs:
         ctx.step(cfg.dt, cfg.gravity, 0, 0.0, prm)
