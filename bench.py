@@ -301,7 +301,8 @@ def batched_study(n_scenes, steps, torch):
     batch = SceneBatch(scenes)
     tune = batch.autotune(chunks=(1, 2, 4, 8), lanes=(1, 2, 4), steps=2)
     _progress("batched: autotuned, timing")
-    chosen = {"cg_chunk": batch.ctx.scene_cg_chunk, "lanes_per_row": batch.ctx.scene_lanes}
+    chosen = {"cg_chunk": batch.ctx.scene_cg_chunk, "lanes_per_row": batch.ctx.scene_lanes,
+              "build_lanes": batch.ctx.build_lanes}
     for _ in range(2):
         batch.step()
     torch.cuda.synchronize()

@@ -310,6 +310,16 @@ class Context:
         return v.value
 
     @property
+    def build_lanes(self):
+        """The neighbour build's candidate-scan form: 0 by the dynamic count, 1, 2, 4, 8 or 16 lanes
+        per row, or 64 for the scan by cells (the rows are identical in every form)."""
+        return self._opt(N.OPT_BUILD_LANES)
+
+    @build_lanes.setter
+    def build_lanes(self, k):
+        N.check(self.lib.ivp_set_option(self.h, N.OPT_BUILD_LANES, int(k)), "ivp_set_option")
+
+    @property
     def scene_cg_chunk(self):
         """CG iterations per replay of the scene solve's launch sequence."""
         return self._opt(N.OPT_SCENE_CG_CHUNK)

@@ -597,6 +597,10 @@ int ivp_get_option(ivp_ctx* c, int option, int* value) {
         *value = c->scene_lanes;
         return IVP_OK;
     }
+    if (option == IVP_OPT_BUILD_LANES) {
+        *value = c->build_lanes;
+        return IVP_OK;
+    }
     if (option == IVP_OPT_W18_WARM_START) {
         *value = c->w18_warm ? 1 : 0;
         return IVP_OK;

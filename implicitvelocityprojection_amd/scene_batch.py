@@ -93,12 +93,27 @@ class SceneBatch:
         if self.ctx.neighbors_build() != 0:
             raise RuntimeError("neighbour overflow at restore")
 
-    def autotune(self, chunks=(1, 2, 4, 8), lanes=(1, 2, 4), steps=2):
-        """Choose the CG iterations per replay and the lanes per neighbour row of the pair
-        passes: time `steps` steps from the same saved state for every combination, keep the
-        fastest, restore the state.  Returns {"chunk K, lanes G": ms per step}."""
+    def autotune(self, chunks=(1, 2, 4, 8), lanes=(1, 2, 4), steps=2, build_forms=(2, 4, 64), builds=3):
+        """Choose the neighbour build's candidate-scan form (rows identical in every form), then the
+        CG iterations per replay and the lanes per neighbour row of the pair passes: time `builds`
+        builds per form and `steps` steps per combination from the same saved state, keep the
+        fastest, restore the state.  Returns {"build lanes L": ms per build, "chunk K, lanes G": ms
+        per step}."""
         snap = self.snapshot()
         times = {}
+        build_ms = {}
+        for f in build_forms:
+            self.ctx.build_lanes = f
+            self.restore(snap)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(builds):
+                if self.ctx.neighbors_build() != 0:
+                    raise RuntimeError("neighbour overflow in the build autotune")
+            torch.cuda.synchronize()
+            build_ms[f] = (time.perf_counter() - t0) * 1e3 / builds
+        if build_ms:
+            self.ctx.build_lanes = min(build_ms, key=build_ms.get)
         for g in lanes:
             for k in chunks:
                 self.restore(snap)
@@ -115,7 +130,9 @@ class SceneBatch:
         best = min(times, key=times.get)
         self.ctx.scene_cg_chunk, self.ctx.scene_lanes = best
         self.restore(snap)
-        return {f"chunk {k}, lanes {g}": ms for (k, g), ms in times.items()}
+        out = {f"build lanes {f}": ms for f, ms in build_ms.items()}
+        out.update({f"chunk {k}, lanes {g}": ms for (k, g), ms in times.items()})
+        return out
 
     def close(self):
         self.ctx.close()

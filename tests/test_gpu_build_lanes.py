@@ -29,6 +29,7 @@ def test_build_lanes_give_identical_rows(cfg_id, step, cache):
     for lanes in (1, 2, 4, 8, 16, 64, 0):
         print(f"build lanes {lanes}: config {cfg_id} step {step} cache {cache}", flush=True)
         assert N.check(ctx.lib.ivp_set_option(ctx.h, N.OPT_BUILD_LANES, lanes)) == 0
+        assert ctx.build_lanes == lanes
         assert ctx.neighbors_build() == 0
         count = torch.zeros(n, dtype=torch.int32, device="cuda")
         ids = torch.zeros(n * C, dtype=torch.int32, device="cuda")
