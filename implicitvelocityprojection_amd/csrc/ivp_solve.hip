@@ -2367,8 +2367,8 @@ int prj_solve(ivp_ctx* c, float dt, const ivp_prj_params* prm, ivp_ip_stats* sta
     st.df_ms = 0.0;
     st.n_iter_costs = 0;
     IVP_TRY(prj_buffers(c, prm->max_iteration_opt + 1));
-    // a row that overflowed the neighbour capacity at the last build (the IP solve checks the
-    // same flag in its first constraint pass)
+    // a row that overflowed the neighbour capacity at the last build (the IP solve reads the
+    // same flag at its first synchronisation)
     IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     IVP_HIP(hipStreamSynchronize(c->stream));
     if (*c->err_host & 4) return IVP_ERR_INTERNAL;
