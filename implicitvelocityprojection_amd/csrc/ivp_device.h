@@ -86,6 +86,37 @@ __device__ __forceinline__ float len3(float x, float y, float z) {
     return sqrtf(x * x + y * y + z * z);
 }
 
+// sqrt(x) rounded to nearest from the hardware root s (v_sqrt_f32, within one ulp): s moves to a
+// neighbour when the neighbour's residual x - n s (one fused multiply-add) shows the root beyond
+// the midpoint.  Exact for every x >= 2^-96 and any s within one ulp (checked over every float of
+// [1, 4) and its three candidate roots); below 2^-96 (|r| < 2^-48) the result stays below 2^-48,
+// where the build's kernels do not depend on it (spiky clamps |r| to 1e-6; the cubic kernel is
+// exactly its constant for |r| / h < 7e-5).  The same as sqrtf for the build's pairs, without
+// sqrtf's scaling of small arguments.
+__device__ __forceinline__ float sqrt_rn(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u), sp = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rm = fmaf(-sm, s, x), rp = fmaf(-sp, s, x);
+    float r = rm <= 0.0f ? sm : s;
+    return rp > 0.0f ? sp : r;
+}
+
+__device__ __forceinline__ float len3_rn(float x, float y, float z) {
+    return sqrt_rn(x * x + y * y + z * z);
+}
+
+// 1/b rounded to nearest for normal b > 0 with a normal reciprocal: the hardware reciprocal y0
+// (v_rcp_f32, within one ulp) and one Newton step with fused multiply-adds, which rounds
+// correctly from any y0 within two ulps except when b's significand is all ones; there
+// 1/b = 2^-e-1 (1 + 2^-24 + ...) rounds to 2^-e-1 (1 + 2^-23), formed from b's exponent (checked
+// over every significand and five candidate y0).
+__device__ __forceinline__ float rcp_rn(float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float y1 = fmaf(fmaf(-b, y0, 1.0f), y0, y0);
+    const uint32_t u = __float_as_uint(b);
+    return (u & 0x7fffffu) == 0x7fffffu ? __uint_as_float((0x7e800000u - (u & 0x7f800000u)) | 1u) : y1;
+}
+
 // spiky_kernel_derivative, scripts/sim/sph_kernel.py:52-61 (same operation order);
 // k = spiky_constant(h)
 __device__ __forceinline__ float3 spiky_grad(float rx, float ry, float rz, float h, float k) {
@@ -100,7 +131,7 @@ __device__ __forceinline__ float3 spiky_grad(float rx, float ry, float rz, float
 // spiky_grad given r_norm = len3(rx, ry, rz), the three quotients by div_rn
 __device__ __forceinline__ float3 spiky_grad_rn(float rx, float ry, float rz, float r_norm, float h, float k) {
     if (r_norm < 1e-6f) r_norm = 1e-6f;
-    const float y = 1.0f / r_norm;
+    const float y = rcp_rn(r_norm);
     float gx = div_rn(rx, r_norm, y), gy = div_rn(ry, r_norm, y), gz = div_rn(rz, r_norm, y);
     float t = h - r_norm;
     float s = -k * (t * t);

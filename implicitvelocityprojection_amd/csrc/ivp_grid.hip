@@ -768,7 +768,7 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
             return;
         }
         float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        float rn = len3(rx, ry, rz);
+        float rn = len3_rn(rx, ry, rz);
         float3 gw = spiky_grad_rn(rx, ry, rz, rn, h, a.ks);
         if (FMT == IVP_ROWS_F32) {
             rec[64] = __float_as_uint(gw.x);
@@ -849,7 +849,7 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
         }
         float4 pj = a.pos[a.nd + j];
         float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        float rn = len3(rx, ry, rz);
+        float rn = len3_rn(rx, ry, rz);
         float3 gw = spiky_grad_rn(rx, ry, rz, rn, h, a.ks);
         rho += pj.w * cubic_kernel_rn(rn, h, a.inv_h, a.kc);
         float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
