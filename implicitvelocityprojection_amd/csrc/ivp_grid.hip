@@ -783,9 +783,9 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
         aii += (Jx * Jx + Jy * Jy + Jz * Jz) * minv_j;
         jx -= Jx; jy -= Jy; jz -= Jz;
     };
-    // Two slots in flight: both ids, then both neighbours' (x, m) and 1/m, then the two slots in
-    // order (the accumulation order of a one-slot loop).  An id past a row's count is read but
-    // never used, and the gathers take a clamped index.
+    // Two slots in flight (the cached-gradient formats): both ids, then both neighbours' (x, m)
+    // and 1/m, then the two slots in order (the accumulation order of a one-slot loop).  An id
+    // past a row's count is read but never used, and the gathers take a clamped index.
     auto dyn_pair = [&](uint32_t* r0, uint32_t* r1, int j0, int j1, int k, uint32_t (&hq)[4]) {
         // the empty asm statements keep each pair of loads together (the compiler would sink the
         // second slot's loads into its conditional use)
@@ -800,6 +800,25 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
                      "v"(p1.w), "v"(m1));
         if (valid && k < cnt) dyn_slot(r0, j0, p0, m0, hq[0], hq[1]);
         if (valid && k + 1 < cnt) dyn_slot(r1, j1, p1, m1, hq[2], hq[3]);
+    };
+    // Four slots in flight the same way (ids-only rows, whose eight ids of a block come from two
+    // loads)
+    auto dyn_quad = [&](const int* j, int k, uint32_t (&hq)[8]) {
+        asm volatile("" ::"v"(j[0]), "v"(j[1]), "v"(j[2]), "v"(j[3]));
+        float4 p[4];
+        float m[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = (unsigned)j[u] < (unsigned)a.nd ? j[u] : 0;
+            p[u] = a.pos[c];
+            m[u] = a.minv[c];
+        }
+        asm volatile("" ::"v"(p[0].x), "v"(p[0].y), "v"(p[0].z), "v"(p[0].w), "v"(m[0]), "v"(p[1].x), "v"(p[1].y),
+                     "v"(p[1].z), "v"(p[1].w), "v"(m[1]), "v"(p[2].x), "v"(p[2].y), "v"(p[2].z), "v"(p[2].w), "v"(m[2]),
+                     "v"(p[3].x), "v"(p[3].y), "v"(p[3].z), "v"(p[3].w), "v"(m[3]));
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (valid && k + u < cnt) dyn_slot(nullptr, j[u], p[u], m[u], hq[2 * u], hq[2 * u + 1]);
     };
     if constexpr (REC == 64u) {
         // ids-only rows: the row's 8 words of a block (slots 0, 4, 1, 5, 2, 6, 3, 7) with two
@@ -817,13 +836,15 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
 #pragma unroll
             for (int q = 0; q < 16; ++q) gq[q] = 0u;
 #pragma unroll
-            for (int t = 0; t < 8; t += 2) {
+            for (int t = 0; t < 8; t += 4) {
                 if (b + t >= tl) break;
-                uint32_t hq[4] = {0u, 0u, 0u, 0u};
-                dyn_pair(nullptr, nullptr, ids[t], ids[t + 1], b + t, hq);
-                const int p0 = ((t & 3) << 1) | (t >> 2), p1 = (((t + 1) & 3) << 1) | ((t + 1) >> 2);
-                gq[2 * p0] = hq[0]; gq[2 * p0 + 1] = hq[1];
-                gq[2 * p1] = hq[2]; gq[2 * p1 + 1] = hq[3];
+                uint32_t hq[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+                dyn_quad(ids + t, b + t, hq);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int sl = t + u, pw = ((sl & 3) << 1) | (sl >> 2);
+                    gq[2 * pw] = hq[2 * u]; gq[2 * pw + 1] = hq[2 * u + 1];
+                }
             }
             if (FMT == IVP_ROWS_F16 && valid) {
                 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -840,14 +861,12 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
             dyn_pair(r0, r1, (int)r0[0], (int)r1[0], k, hq);
         }
     }
-    for (int k = 0; k < ts; ++k) {
-        if (!valid || k >= cs) continue;
-        int j = (int)tbase[slot_word<REC>(Cs - 1 - k)];
+    // static slots Cs - 1, Cs - 2, ...: two ids, then both positions, in flight at once
+    auto st_slot = [&](int j, float4 pj) {
         if ((unsigned)j >= (unsigned)a.ns) {  // never taken: phase A stored a static id here
             atomicOr(a.err_flag, 4);
-            continue;
+            return;
         }
-        float4 pj = a.pos[a.nd + j];
         float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         float rn = len3_rn(rx, ry, rz);
         float3 gw = spiky_grad_rn(rx, ry, rz, rn, h, a.ks);
@@ -855,6 +874,16 @@ __device__ __forceinline__ void build_eval(const BuildArgs& a, int i, int lane, 
         float Jx = pj.w * gw.x, Jy = pj.w * gw.y, Jz = pj.w * gw.z;
         jx -= Jx; jy -= Jy; jz -= Jz;
         sx += Jx; sy += Jy; sz += Jz;
+    };
+    for (int k = 0; k < ts; k += 2) {
+        const int j0 = (int)tbase[slot_word<REC>(Cs - 1 - k)];
+        const int j1 = (int)tbase[slot_word<REC>(k + 1 < ts ? Cs - 2 - k : Cs - 1 - k)];
+        asm volatile("" ::"v"(j0), "v"(j1));
+        const float4 q0 = a.pos[a.nd + ((unsigned)j0 < (unsigned)a.ns ? j0 : 0)];
+        const float4 q1 = a.pos[a.nd + ((unsigned)j1 < (unsigned)a.ns ? j1 : 0)];
+        asm volatile("" ::"v"(q0.x), "v"(q0.y), "v"(q0.z), "v"(q0.w), "v"(q1.x), "v"(q1.y), "v"(q1.z), "v"(q1.w));
+        if (valid && k < cs) st_slot(j0, q0);
+        if (valid && k + 1 < cs) st_slot(j1, q1);
     }
     // Pad only after the static ids (slots Cs-cs .. Cs-1) have been consumed: the padded range
     // [cnt, tlp) follows the wave's longest row and can reach into them.
