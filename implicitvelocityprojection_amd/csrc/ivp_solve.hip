@@ -1074,6 +1074,15 @@ struct W18Args {
     const float* dv_prev;   // the last solve's correction in user order (warm starts)
 };
 
+// The operator pass's neighbour record: w18_record, with w = -10 ks V_j when the gradients are
+// evaluated (the constant of the folded pair scalar, w18_Av_row)
+template <class FM, bool ALLFLUID>
+__device__ __forceinline__ float4 w18_Av_record(const W18Args& a, int j) {
+    float4 r = w18_record<ALLFLUID>(a.R.pos, a.v4, a.mat, a.fluid, j);
+    if constexpr (FM::EVAL) r.w = r.w * (-10.0f * a.R.ks);
+    return r;
+}
+
 // (A u)_i = u_i - factor_i * 10 sum_fluid (m_j/rho_j) (u_ij . x_ij) / (r^2 + 0.01 h^2) gradW_ij
 // (compute_Av_weiler18, weiler18.py:103-172).  The caller stages the block's window.
 template <class FM, bool ALLFLUID, int G, class LA, class LB>
@@ -1083,6 +1092,28 @@ __device__ __forceinline__ float3 w18_Av_row(const W18Args& a, int i, float4 ui,
     float4 xi = a.R.pos[i];
     float lx = 0.0f, ly = 0.0f, lz = 0.0f;
     float eps2 = 0.01f * a.R.h * a.R.h;
+    if constexpr (FM::EVAL) {
+        // evaluated gradients: gradW_ij = -ks (h - |r|)^2 x_ij / |r| and the pair factor
+        // 10 V_j (u_ij . x_ij) / (r^2 + 0.01 h^2) fold into one scalar per pair, the constant
+        // -10 ks carried by the neighbour record's w (w18_Av_record); x_ij is scaled once
+        const float h = a.R.h;
+        row_pairs_w<FM, 3, G>(win, w, a.R, i, xi, loadA, loadB,
+                             [&](const float4& xj, const Vec<3>& uj, float, float, float) {
+                                 float x0 = xi.x - xj.x, x1 = xi.y - xj.y, x2 = xi.z - xj.z;
+                                 float v0 = ui.x - uj.v[0], v1 = ui.y - uj.v[1], v2 = ui.z - uj.v[2];
+                                 float r_sq = fmaf(x2, x2, fmaf(x1, x1, x0 * x0));
+                                 float vx = fmaf(v2, x2, fmaf(v1, x1, v0 * x0));
+                                 float inv = __builtin_amdgcn_rsqf(fmaxf(r_sq, 1e-12f));   // 1 / max(|r|, 1e-6)
+                                 float t = fmaf(-r_sq, inv, h);
+                                 float wp = xj.w * ((vx * __builtin_amdgcn_rcpf(r_sq + eps2)) * ((t * t) * inv));
+                                 lx = fmaf(wp, x0, lx); ly = fmaf(wp, x1, ly); lz = fmaf(wp, x2, lz);
+                             }, g);
+        lx = group_sum<G>(lx);
+        ly = group_sum<G>(ly);
+        lz = group_sum<G>(lz);
+        float f = a.fac[i];
+        return make_float3(ui.x - f * lx, ui.y - f * ly, ui.z - f * lz);
+    }
     float dd = 10.0f;
     row_pairs_w<FM, 3, G>(win, w, a.R, i, xi, loadA, loadB,
                          [&](const float4& xj, const Vec<3>& uj, float gx, float gy, float gz) {
@@ -1264,7 +1295,7 @@ __global__ __launch_bounds__(BLOCK) void k_w18_Av(W18Args a) {
     if (CHECK && a.st->active == 0) return;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = pass_row(blk), g = pass_lane();
-    auto loadA = [&](int j) { return w18_record<ALLFLUID>(a.R.pos, a.v4, a.mat, a.fluid, j); };
+    auto loadA = [&](int j) { return w18_Av_record<FM, ALLFLUID>(a, j); };
     auto loadB = [&](int j) { return vec3_of(a.v4, j); };
     __shared__ WinLds<FM, PASS_LANES> w;
     const bool win = stage_window(a.R, blk, w, loadA, [&](int j) { return a.v4[j]; });
