@@ -146,10 +146,11 @@ __global__ __launch_bounds__(BLOCK) void k_keys(int count, const float4* __restr
 }
 
 // slot = start[key] + (remaining count - 1); leaves ccount at zero for the next sort.  A run's
-// head takes the run's slots with one atomic and its lanes take them in order.
+// head takes the run's slots with one atomic and its lanes take them in order.  The slot receives
+// the particle's user id (the ranking below compares user ids within a cell).
 __global__ __launch_bounds__(BLOCK) void k_scatter(int count, const int* __restrict__ key,
                                                    const int* __restrict__ start, int* __restrict__ ccount,
-                                                   int* __restrict__ tmp_src) {
+                                                   const int* __restrict__ uid, int* __restrict__ cell_uid) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     bool valid = i < count;
     int k = valid ? key[i] : -1;
@@ -160,13 +161,13 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(int count, const int* __restr
     if (valid) {
         int lane = threadIdx.x & (WAVE - 1);
         int slot = start[k] + old - 1 - (lane - r.head_lane);
-        tmp_src[slot] = i;
+        cell_uid[slot] = uid[i];
     }
 }
 
 // final position = cell start + number of cell members with a smaller user id
 __global__ __launch_bounds__(BLOCK) void k_rank(int count, const int* __restrict__ key,
-                                                const int* __restrict__ start, const int* __restrict__ tmp_src,
+                                                const int* __restrict__ start, const int* __restrict__ cell_uid,
                                                 const int* __restrict__ uid, int* __restrict__ order) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= count) return;
@@ -174,7 +175,8 @@ __global__ __launch_bounds__(BLOCK) void k_rank(int count, const int* __restrict
     int lo = start[k], hi = start[k + 1];
     int u = uid[i];
     int rank = 0;
-    for (int s = lo; s < hi; ++s) rank += uid[tmp_src[s]] < u ? 1 : 0;
+#pragma unroll 4
+    for (int s = lo; s < hi; ++s) rank += cell_uid[s] < u ? 1 : 0;
     order[lo + rank] = i;
 }
 
@@ -1222,8 +1224,8 @@ int grid_sort_population(ivp_ctx* c, int offset, int count, int* start_table, bo
         IVP_TRY(launch(c, KC_SORT, -1, 20.0 * count, k_keys<false>, nb, count, (const float4*)(c->pos + offset),
                        c->grid, (const Grid*)nullptr, (const int*)nullptr, c->key, c->ccount));
     IVP_TRY(scan(c, c->ccount, start_table, c->ncell));
-    IVP_TRY(launch(c, KC_SORT, -1, 12.0 * count, k_scatter, nb, count, (const int*)c->key, (const int*)start_table,
-                   c->ccount, c->tmp_src));
+    IVP_TRY(launch(c, KC_SORT, -1, 16.0 * count, k_scatter, nb, count, (const int*)c->key, (const int*)start_table,
+                   c->ccount, (const int*)(c->uid + offset), c->tmp_src));
     IVP_TRY(launch(c, KC_SORT, -1, 20.0 * count, k_rank, nb, count, (const int*)c->key, (const int*)start_table,
                    (const int*)c->tmp_src, (const int*)(c->uid + offset), c->order));
     GatherArgs ga{count, offset, c->order, c->pos, c->pos_b, c->vel, c->vel_b, c->minv, c->minv_b,
