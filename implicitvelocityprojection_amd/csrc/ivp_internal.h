@@ -60,6 +60,7 @@ struct SolverState {
     int iter;           // CG iterations completed
     int max_iter;
     int active;         // 1 while CG continues
+    int dv_apply;       // Weiler18 / Jeske23: this iteration's dv += alpha p is left to the p pass
     int status;         // 0 ok, IVP_NOT_PD
     int n_active_rows;  // rows with dfdt > 0 (per outer iteration)
     int n_touched_rows; // rows whose J^T loop runs (active or with an active neighbour)

@@ -102,6 +102,7 @@ __device__ __forceinline__ void fin_apply(SolverState* st, const double (&t)[3],
         st->max_iter = max_iter;
         st->iter = 0;
         st->status = 0;
+        st->dv_apply = 0;
         st->active = (t[0] > floor_ && max_iter > 0) ? 1 : 0;
     } else if (OP == FIN_CG_PAP) {
         // ip.py:307-311
@@ -159,7 +160,9 @@ __device__ __forceinline__ void fin_apply(SolverState* st, const double (&t)[3],
             st->rr_old = t[1];
         }
     } else if (OP == FIN_W18_UPDATE) {
-        // weiler18.py:350-365 (t[0] = r.r, t[1] = r.z)
+        // weiler18.py:350-365 (t[0] = r.r, t[1] = r.z); the iteration's dv += alpha p follows in
+        // the p pass, also when this iteration ends the solve
+        st->dv_apply = 1;
         int it = st->iter + 1;
         st->iter = it;
         st->rr_last = t[0];
@@ -185,6 +188,9 @@ template <int OP>
 __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(FinArgs a) {
     constexpr int NV = fin_nv<OP>();
     SolverState* st = a.st;
+    // the p pass before this one applied the last iteration's dv update, if any (iterations
+    // enqueued past the stop must not apply it again)
+    if (OP == FIN_W18_PAP && threadIdx.x == 0) st->dv_apply = 0;
     if (OP == FIN_CG_PAP || OP == FIN_CG_UPDATE || OP == FIN_W18_PAP || OP == FIN_W18_UPDATE || OP == FIN_CG_REPLACE) {
         if (st->active == 0) return;
     }

@@ -1312,18 +1312,14 @@ __global__ __launch_bounds__(BLOCK) void k_w18_Av(W18Args a) {
     block_partials<1>(v, a.partials, a.stride);
 }
 
-// dv += alpha p, r -= alpha Ap, r.r, z = P r, r.z (weiler18.py:344-360)
+// r -= alpha Ap, r.r, z = P r, r.z (weiler18.py:344-360); dv += alpha p of the same lines
+// follows in the p pass, which reads p anyway
 __global__ __launch_bounds__(BLOCK) void k_w18_update(W18Args a) {
     if (a.st->active == 0) return;
-    float alpha = (float)a.st->alpha;
     float nalpha = (float)(-a.st->alpha);
     int i = blockIdx.x * BLOCK + threadIdx.x;
     double rr = 0.0, rz = 0.0;
     if (i < a.R.nd) {
-        float4 p = a.v4[i];
-        a.dv[3 * i + 0] = a.dv[3 * i + 0] + alpha * p.x;
-        a.dv[3 * i + 1] = a.dv[3 * i + 1] + alpha * p.y;
-        a.dv[3 * i + 2] = a.dv[3 * i + 2] + alpha * p.z;
         float r0 = a.r[3 * i + 0] + nalpha * a.out[3 * i + 0];
         float r1 = a.r[3 * i + 1] + nalpha * a.out[3 * i + 1];
         float r2 = a.r[3 * i + 2] + nalpha * a.out[3 * i + 2];
@@ -1340,18 +1336,28 @@ __global__ __launch_bounds__(BLOCK) void k_w18_update(W18Args a) {
     block_partials<2>(v, a.partials, a.stride);
 }
 
-// p = z + beta p
+// dv += alpha p (the iteration's update, weiler18.py:344-346), then p = z + beta p while the
+// solve continues
 __global__ __launch_bounds__(BLOCK) void k_w18_p(int nd, const float* __restrict__ z, float4* __restrict__ p4,
-                                                 const SolverState* __restrict__ st) {
-    if (st->active == 0) return;
+                                                 float* __restrict__ dv, const SolverState* __restrict__ st) {
+    const int apply = st->dv_apply, active = st->active;
+    if (apply == 0 && active == 0) return;
+    float alpha = (float)st->alpha;
     float beta = (float)st->beta;
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= nd) return;
     float4 p = p4[i];
-    p.x = z[3 * i + 0] + beta * p.x;
-    p.y = z[3 * i + 1] + beta * p.y;
-    p.z = z[3 * i + 2] + beta * p.z;
-    p4[i] = p;
+    if (apply) {
+        dv[3 * i + 0] = dv[3 * i + 0] + alpha * p.x;
+        dv[3 * i + 1] = dv[3 * i + 1] + alpha * p.y;
+        dv[3 * i + 2] = dv[3 * i + 2] + alpha * p.z;
+    }
+    if (active) {
+        p.x = z[3 * i + 0] + beta * p.x;
+        p.y = z[3 * i + 1] + beta * p.y;
+        p.z = z[3 * i + 2] + beta * p.z;
+        p4[i] = p;
+    }
 }
 
 // v += dv (weiler18.py:399-401)
@@ -2089,9 +2095,9 @@ int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
         auto one_iter = [&](int tag) -> int {
             IVP_TRY(launch(c, KC_W18_AV, tag, bAv, k_w18_Av<FM, ALLFLUID>, pass_grid(nd), cg));
             IVP_TRY(finalize<FIN_W18_PAP>(c, pass_grid(nd).x));
-            IVP_TRY(launch(c, KC_W18_UPDATE, tag, 124.0 * nd, k_w18_update, nb, cg));
+            IVP_TRY(launch(c, KC_W18_UPDATE, tag, 84.0 * nd, k_w18_update, nb, cg));
             IVP_TRY(finalize<FIN_W18_UPDATE>(c, nb.x));
-            return launch(c, KC_W18_P, tag, 44.0 * nd, k_w18_p, nb, nd, (const float*)c->w_z, c->w_p4,
+            return launch(c, KC_W18_P, tag, 68.0 * nd, k_w18_p, nb, nd, (const float*)c->w_z, c->w_p4, c->w_dv,
                           (const SolverState*)c->state);
         };
         c->prof_epoch++;
@@ -2146,9 +2152,9 @@ int jeske23_body(ivp_ctx* c, float sigma, float dt, int* iters_out) {
         auto one_iter = [&](int tag) -> int {
             IVP_TRY(launch(c, KC_ST_AV, tag, bAv, k_st_Av<FM, ALLFLUID, false>, np, cga));
             IVP_TRY(finalize<FIN_W18_PAP>(c, np.x));
-            IVP_TRY(launch(c, KC_W18_UPDATE, tag, 124.0 * nd, k_w18_update, nb, cg));
+            IVP_TRY(launch(c, KC_W18_UPDATE, tag, 84.0 * nd, k_w18_update, nb, cg));
             IVP_TRY(finalize<FIN_W18_UPDATE>(c, nb.x));
-            return launch(c, KC_W18_P, tag, 44.0 * nd, k_w18_p, nb, nd, (const float*)c->w_z, c->w_p4,
+            return launch(c, KC_W18_P, tag, 68.0 * nd, k_w18_p, nb, nd, (const float*)c->w_z, c->w_p4, c->w_dv,
                           (const SolverState*)c->state);
         };
         c->prof_epoch++;
