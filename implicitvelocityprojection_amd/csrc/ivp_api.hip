@@ -96,6 +96,8 @@ void free_all(ivp_ctx* c) {
     c->iter_ev.clear();
     if (c->ip_end_ev) (void)hipEventDestroy(c->ip_end_ev);
     c->ip_end_ev = nullptr;
+    if (c->wait_ev) (void)hipEventDestroy(c->wait_ev);
+    c->wait_ev = nullptr;
     c->ip_wait_pending = false;
 }
 

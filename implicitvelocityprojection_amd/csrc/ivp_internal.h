@@ -306,6 +306,7 @@ struct ivp_ctx {
     // an IP solve whose closing wait was left to the caller (ivp_step enqueues the rest of the
     // step first): its end mark, start time and outer-loop evaluation count
     hipEvent_t ip_end_ev = nullptr;
+    hipEvent_t wait_ev = nullptr;   // a chunk's state copy, when work follows it before the wait
     bool ip_wait_pending = false;
     std::chrono::steady_clock::time_point ip_t0;
     int ip_n_err = 0;

@@ -1258,7 +1258,9 @@ __global__ __launch_bounds__(BLOCK) void k_w18_warm_init(W18Args a, const float*
 // reference's self.dv, indexed by particle): one thread per user index, contiguous writes,
 // the internal row found through inv (static particles keep their zero)
 __global__ __launch_bounds__(BLOCK) void k_w18_save(int n, int nd, const float* __restrict__ dv,
-                                                    const int* __restrict__ inv, float* __restrict__ dv_prev) {
+                                                    const int* __restrict__ inv, float* __restrict__ dv_prev,
+                                                    const SolverState* __restrict__ st) {
+    if (st && st->active != 0) return;   // the solve continues: a later launch saves
     int u = blockIdx.x * BLOCK + threadIdx.x;
     if (u >= n) return;
     const int i = inv[u];
@@ -1360,8 +1362,10 @@ __global__ __launch_bounds__(BLOCK) void k_w18_p(int nd, const float* __restrict
     }
 }
 
-// v += dv (weiler18.py:399-401)
-__global__ __launch_bounds__(BLOCK) void k_w18_apply(int n3, const float* __restrict__ dv, float* __restrict__ vel) {
+// v += dv (weiler18.py:399-401), once the solve has stopped
+__global__ __launch_bounds__(BLOCK) void k_w18_apply(int n3, const float* __restrict__ dv, float* __restrict__ vel,
+                                                     const SolverState* __restrict__ st) {
+    if (st && st->active != 0) return;   // nullptr: unconditional
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n3) return;
     vel[i] = vel[i] + 1.0f * dv[i];
@@ -1860,8 +1864,23 @@ int read_state(ivp_ctx* c) {
 
 // Enqueue CG iterations in chunks until the device-side state says stop.  `launch_it(tag)`
 // enqueues one iteration.  Returns the iteration count in *iters.
-template <class L>
-int run_chunks(ivp_ctx* c, int max_iter, int guess, L&& launch_it, int* iters, int* chunks) {
+// The solver state copied back after the work enqueued so far, then `tail` enqueued (work that
+// follows a finished solve and returns at once while it continues), then a wait for the copy
+// alone: the device runs the tail while the host takes the state.
+template <class T>
+int read_state_then(ivp_ctx* c, T&& tail) {
+    IVP_HIP(hipMemcpyAsync(c->state_host, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+    if (!c->wait_ev) IVP_HIP(hipEventCreateWithFlags(&c->wait_ev, hipEventDisableTiming));
+    IVP_HIP(hipEventRecord(c->wait_ev, c->stream));
+    IVP_TRY(tail());
+    IVP_HIP(hipEventSynchronize(c->wait_ev));
+    return IVP_OK;
+}
+
+// With a tail, every chunk's read-back is followed by the tail; it has run on a finished solve
+// when the returned state is inactive.
+template <class L, class T>
+int run_chunks(ivp_ctx* c, int max_iter, int guess, L&& launch_it, int* iters, int* chunks, T&& tail) {
     int issued = 0;
     int chunk = std::max(2, std::min(guess, 64));
     for (;;) {
@@ -1869,7 +1888,7 @@ int run_chunks(ivp_ctx* c, int max_iter, int guess, L&& launch_it, int* iters, i
         if (todo <= 0) break;
         for (int q = 0; q < todo; ++q) IVP_TRY(launch_it(issued + q));
         issued += todo;
-        IVP_TRY(read_state(c));
+        IVP_TRY(read_state_then(c, tail));
         (*chunks)++;
         const SolverState& s = *c->state_host;
         if (s.active == 0) break;
@@ -1883,6 +1902,11 @@ int run_chunks(ivp_ctx* c, int max_iter, int guess, L&& launch_it, int* iters, i
     }
     *iters = c->state_host->iter;
     return IVP_OK;
+}
+
+template <class L>
+int run_chunks(ivp_ctx* c, int max_iter, int guess, L&& launch_it, int* iters, int* chunks) {
+    return run_chunks(c, max_iter, guess, launch_it, iters, chunks, []() { return (int)IVP_OK; });
 }
 
 // Bytes a pass of format FM reads from the pair stream: the id (4 B), plus 12 B of float32 or
@@ -1916,6 +1940,8 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
     IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_init, nb, nd, dt, (const float*)c->density, (const float*)c->rho0,
                    (const float*)c->vel, (const float4*)c->pos, c->cc, d4));
     int opt_iter = 0;
+    bool spec_finish = false;   // k_ip_finish enqueued before the last read-back
+    const float inv_dt = (float)(1.0 / (double)dt);
     double err = 0.0;
     IVP_TRY(iter_mark(c, 0));
     for (;;) {
@@ -1940,6 +1966,15 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
         long long act_pairs = c->state_host->active_pairs;   // (deferred: the last evaluation's, for
         int act_rows = c->state_host->n_active_rows;         //  the byte model until read back)
         if (!deferred) {
+            // An evaluation below the iteration limit ends the loop only when it converged, and
+            // then the solve closes with k_ip_finish on this d: enqueue it now, so that the
+            // device runs it while the host reads the state.  Should the loop go on, nothing in
+            // it reads the velocities or divergences it writes, and the closing pass writes them
+            // again.
+            spec_finish = opt_iter >= 1 && opt_iter < prm->max_iteration_opt && !prm->use_fixed_iters;
+            if (spec_finish)
+                IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_finish, nb, nd, inv_dt, (const float4*)d4, (const float*)c->jd,
+                               c->vel, c->divg));
             IVP_TRY(read_state(c));
             IVP_TRY(record());
             bool converged = err < prm->tol_opt && !prm->use_fixed_iters;
@@ -1947,6 +1982,7 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
                 IVP_TRY(iter_mark(c, opt_iter + 1));
                 break;
             }
+            spec_finish = false;
             act_pairs = c->state_host->active_pairs;
             act_rows = c->state_host->n_active_rows;
         }
@@ -2034,10 +2070,10 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
     }
     st.outer_iters = opt_iter;
     if (!prm->use_fixed_iters && err >= prm->tol_opt && opt_iter >= prm->max_iteration_opt) return IVP_NOT_CONVERGED;
-    float inv_dt = (float)(1.0 / (double)dt);
     // final_density_error (pressure_base.py:438-447) evaluates avg(max((J d + c)/rho0, 0)) at the
     // d of the last loop evaluation, with the same per-row arithmetic: it equals that err.
     st.final_density_error = err;
+    if (spec_finish) return IVP_OK;
     return launch(c, KC_GLUE, -1, 0.0, k_ip_finish, nb, nd, inv_dt, (const float4*)d4, (const float*)c->jd, c->vel,
                   c->divg);
 }
@@ -2100,18 +2136,27 @@ int weiler18_body(ivp_ctx* c, float coeff, float dt, int* iters_out) {
             return launch(c, KC_W18_P, tag, 68.0 * nd, k_w18_p, nb, nd, (const float*)c->w_z, c->w_p4, c->w_dv,
                           (const SolverState*)c->state);
         };
+        // the save of the correction and v += dv follow each chunk's state copy, running once the
+        // solve has stopped (while the host reads the state)
+        auto tail = [&]() -> int {
+            IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_w18_save, dim3(blocks_for(c->n)), c->n, nd, (const float*)c->w_dv,
+                           (const int*)c->inv, c->w_dv_prev, (const SolverState*)c->state));
+            return launch(c, KC_GLUE, -1, 0.0, k_w18_apply, dim3(blocks_for(3 * nd)), 3 * nd, (const float*)c->w_dv,
+                          c->vel, (const SolverState*)c->state);
+        };
         c->prof_epoch++;
         int epoch = c->prof_epoch;
-        IVP_TRY(run_chunks(c, 1000, c->w18_guess, one_iter, &iters, &chunks));
+        IVP_TRY(run_chunks(c, 1000, c->w18_guess, one_iter, &iters, &chunks, tail));
         // the pair count of the last build, now read back with the solver state
         prof_set_bytes(c, epoch, KC_W18_AV, stream_bytes<FM>(c, (double)c->state_host->total_pairs) + 48.0 * nd);
         prof_collect(c, iters, epoch);
         c->w18_guess = std::max(2, iters);
+        // the device stops the solve at the same 1000 iterations, so the last read-back shows it
+        // stopped and the tail after it has run
+        if (c->state_host->active != 0) return IVP_ERR_INTERNAL;
     }
     *iters_out = iters;
-    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_w18_save, dim3(blocks_for(c->n)), c->n, nd, (const float*)c->w_dv,
-                   (const int*)c->inv, c->w_dv_prev));
-    return launch(c, KC_GLUE, -1, 0.0, k_w18_apply, dim3(blocks_for(3 * nd)), 3 * nd, (const float*)c->w_dv, c->vel);
+    return IVP_OK;
 }
 
 template <class FM, bool ALLFLUID>
@@ -2165,7 +2210,8 @@ int jeske23_body(ivp_ctx* c, float sigma, float dt, int* iters_out) {
         c->st_guess = std::max(2, iters);
     }
     *iters_out = iters;
-    return launch(c, KC_GLUE, -1, 0.0, k_w18_apply, dim3(blocks_for(3 * nd)), 3 * nd, (const float*)c->w_dv, c->vel);
+    return launch(c, KC_GLUE, -1, 0.0, k_w18_apply, dim3(blocks_for(3 * nd)), 3 * nd, (const float*)c->w_dv, c->vel,
+                  (const SolverState*)nullptr);
 }
 
 // One pass loop of PRJ (solve_PJ, prj.py:162-211; DF: solve_df's loop, prj.py:238-281) on the
