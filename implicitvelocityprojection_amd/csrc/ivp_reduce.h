@@ -31,6 +31,7 @@ struct FinArgs {
     SolverState* st;
     double tol;
     int max_iter;
+    int gated;   // FIN_PACK queued behind a CG read-back: only once that CG stopped without failing
 };
 
 constexpr int FIN_BLOCK = 1024;
@@ -191,6 +192,7 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(FinArgs a) {
     // the p pass before this one applied the last iteration's dv update, if any (iterations
     // enqueued past the stop must not apply it again)
     if (OP == FIN_W18_PAP && threadIdx.x == 0) st->dv_apply = 0;
+    if (OP == FIN_PACK && a.gated && (st->active != 0 || st->status != 0)) return;
     if (OP == FIN_CG_PAP || OP == FIN_CG_UPDATE || OP == FIN_W18_PAP || OP == FIN_W18_UPDATE || OP == FIN_CG_REPLACE) {
         if (st->active == 0) return;
     }
@@ -202,8 +204,8 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(FinArgs a) {
 
 
 template <int OP>
-inline int finalize(ivp_ctx* c, int nblocks, double tol = 0.0, int max_iter = 0) {
-    FinArgs fa{c->partials, c->max_blocks, nblocks, c->state, tol, max_iter};
+inline int finalize(ivp_ctx* c, int nblocks, double tol = 0.0, int max_iter = 0, bool gated = false) {
+    FinArgs fa{c->partials, c->max_blocks, nblocks, c->state, tol, max_iter, gated ? 1 : 0};
     hipLaunchKernelGGL(k_finalize<OP>, dim3(1), dim3(FIN_BLOCK), 0, c->stream, fa);
     IVP_HIP(hipGetLastError());
     return IVP_OK;

@@ -381,13 +381,20 @@ struct PackArgs {
     double* partials;
     int stride;
     SceneRef sc;
+    // when set: the pass runs only if the CG of this state has stopped without failing (a pass
+    // queued behind a CG chunk's read-back)
+    const SolverState* gate = nullptr;
 };
+
+// A CG of state st has stopped, and not on a non-positive p.Ap
+__device__ __forceinline__ bool cg_stopped(const SolverState* st) { return st->active == 0 && st->status == 0; }
 
 // ip_pack_fused (ip.py:124-171) + coef_wise_div + compute_avg_density_error
 // (math_utils.py:76-91, pressure_base.py:187-203, 413-423).  J d in the form of
 // compute_J_x_kernel (pressure_base.py:111-127): sum_j m_j dot(d_i - d_j, gradW_ij) + dot(d_i, S_i).
 template <class FM, bool SC, int G = lanes_for<SC>()>
 __global__ __launch_bounds__(BLOCK) void k_ip_pack(PackArgs a) {
+    if (!SC && a.gate && !cg_stopped(a.gate)) return;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int i = pass_row<G>(blk), g = pass_lane<G>();
     Gate gt = gate_all(i, a.R.nd);
@@ -738,7 +745,8 @@ __global__ __launch_bounds__(BLOCK) void k_cg_refresh(int nd, const float* __res
 // d -= sol (add_vec3 with scale -1, ip.py:389)
 template <bool SC>
 __global__ __launch_bounds__(BLOCK) void k_ip_dupdate(int nd, float4* __restrict__ d4, const float* __restrict__ sol,
-                                                      SceneRef sc) {
+                                                      SceneRef sc, const SolverState* __restrict__ gate) {
+    if (gate && !cg_stopped(gate)) return;   // (see PackArgs::gate)
     const int blk = blockIdx.x;
     int i = blk * BLOCK + threadIdx.x;
     Gate gt = gate_all(i, nd);
@@ -1803,7 +1811,8 @@ int enqueue_scene_replay_g(ivp_ctx* c, hipStream_t st, float omega, int K) {
         hipLaunchKernelGGL(k_cg_p<true>, nb, blk, 0, st, nd, (const float*)c->r, c->p4,
                            (const SolverState*)c->state, sc);
     }
-    hipLaunchKernelGGL(k_ip_dupdate<true>, nb, blk, 0, st, nd, c->d4, (const float*)c->sol, sc);
+    hipLaunchKernelGGL(k_ip_dupdate<true>, nb, blk, 0, st, nd, c->d4, (const float*)c->sol, sc,
+                       (const SolverState*)nullptr);
     hipLaunchKernelGGL(k_scene_advance, dim3(1), blk, 0, st, c->S, c->ss, c->sactive);
     IVP_HIP(hipGetLastError());
     return IVP_OK;
@@ -1885,7 +1894,12 @@ int run_chunks(ivp_ctx* c, int max_iter, int guess, L&& launch_it, int* iters, i
     int chunk = std::max(2, std::min(guess, 64));
     for (;;) {
         int todo = std::min(chunk, max_iter - issued);
-        if (todo <= 0) break;
+        if (todo <= 0) {
+            // no iteration allowed at all: the set-up left the solve stopped; read it back (and
+            // run the tail) all the same
+            if (issued == 0) IVP_TRY(read_state_then(c, tail));
+            break;
+        }
         for (int q = 0; q < todo; ++q) IVP_TRY(launch_it(issued + q));
         issued += todo;
         IVP_TRY(read_state_then(c, tail));
@@ -1941,15 +1955,25 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
                    (const float*)c->vel, (const float4*)c->pos, c->cc, d4));
     int opt_iter = 0;
     bool spec_finish = false;   // k_ip_finish enqueued before the last read-back
+    bool pack_queued = false;   // this evaluation's passes were queued behind the CG's read-back
+    bool finish_queued = false; // and k_ip_finish with them
     const float inv_dt = (float)(1.0 / (double)dt);
     double err = 0.0;
     IVP_TRY(iter_mark(c, 0));
     for (;;) {
         PackArgs pa{R, d4, c->Ss, c->cc, c->kk, c->rho0, c->rowcnt, c->q4, c->sfac, c->jd, c->partials, stride};
         double total_pairs = (double)c->state_host->total_pairs;
-        IVP_TRY(launch(c, KC_IP_PACK, -1, stream_bytes<FP>(c, total_pairs) + 68.0 * nd, k_ip_pack<FP, false>, np, pa));
-        IVP_TRY(finalize<FIN_PACK>(c, np.x));
-        IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        // the constraint pass, its finalize and the flags' copy-back (queued in one piece)
+        auto queue_pack = [&](const SolverState* gate) -> int {
+            PackArgs pg = pa;
+            pg.gate = gate;
+            IVP_TRY(launch(c, KC_IP_PACK, -1, stream_bytes<FP>(c, total_pairs) + 68.0 * nd, k_ip_pack<FP, false>, np, pg));
+            IVP_TRY(finalize<FIN_PACK>(c, np.x, 0.0, 0, gate != nullptr));
+            IVP_HIP(hipMemcpyAsync(c->err_host, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            return IVP_OK;
+        };
+        if (!pack_queued) IVP_TRY(queue_pack(nullptr));
+        pack_queued = false;
         // The first evaluation never ends the loop (the stop needs opt_iter >= 1, ip.py:374-376),
         // so the first CG follows without a wait: its first chunk's synchronisation brings back the
         // overflow flags and this pass's error sum and active counts (no later finalize writes them)
@@ -1972,9 +1996,10 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
             // it reads the velocities or divergences it writes, and the closing pass writes them
             // again.
             spec_finish = opt_iter >= 1 && opt_iter < prm->max_iteration_opt && !prm->use_fixed_iters;
-            if (spec_finish)
+            if (spec_finish && !finish_queued)
                 IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_finish, nb, nd, inv_dt, (const float4*)d4, (const float*)c->jd,
                                c->vel, c->divg));
+            finish_queued = false;
             IVP_TRY(read_state(c));
             IVP_TRY(record());
             bool converged = err < prm->tol_opt && !prm->use_fixed_iters;
@@ -1999,6 +2024,7 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
         long long touch_pairs = 0;
         int touch_rows = 0;
         int iters = 0;
+        bool fin_next = false;   // the tail below queues k_ip_finish
         {
             c->prof_epoch++;
             int epoch = c->prof_epoch;
@@ -2039,7 +2065,24 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
                 return launch(c, KC_CG_P, tag, bP, k_cg_p<false>, nb, nd, (const float*)c->r, c->p4,
                               (const SolverState*)c->state, SceneRef{});
             };
-            IVP_TRY(run_chunks(c, prm->max_iteration_pcg, c->cg_guess, one_iter, &iters, &st.cg_launch_chunks));
+            // Behind each chunk's state copy: d -= x, the outer-iteration mark and the next constraint
+            // pass with its finalize, all returning at once unless the CG has stopped (and not
+            // failed), so the device works while the host reads the state; after the first CG of
+            // the solve (whose read-back also checks the rows for overflow) also the closing
+            // k_ip_finish that the next evaluation may need (see above).
+            fin_next = !deferred && opt_iter + 1 < prm->max_iteration_opt && !prm->use_fixed_iters;
+            auto tail = [&]() -> int {
+                const SolverState* gate = c->state;
+                IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_dupdate<false>, nb, nd, d4, (const float*)c->sol, SceneRef{},
+                               gate));
+                IVP_TRY(iter_mark(c, opt_iter + 1));
+                IVP_TRY(queue_pack(gate));
+                if (fin_next)
+                    IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_finish, nb, nd, inv_dt, (const float4*)d4,
+                                   (const float*)c->jd, c->vel, c->divg));
+                return IVP_OK;
+            };
+            IVP_TRY(run_chunks(c, prm->max_iteration_pcg, c->cg_guess, one_iter, &iters, &st.cg_launch_chunks, tail));
             if (deferred) {
                 IVP_TRY(record());
                 act_pairs = c->state_host->active_pairs;
@@ -2064,9 +2107,12 @@ int ip_solve_body(ivp_ctx* c, float dt, const ivp_ip_params* prm, ivp_ip_stats* 
         }
         if (st.n_pcg < st.capacity && st.pcg_iters) st.pcg_iters[st.n_pcg] = iters;
         st.n_pcg++;
-        IVP_TRY(launch(c, KC_GLUE, -1, 0.0, k_ip_dupdate<false>, nb, nd, d4, (const float*)c->sol, SceneRef{}));
+        // the CG stopped at the device's own limit at the latest, so its last read-back showed it
+        // stopped and the tail behind it (d -= x, the mark, the next constraint pass) has run
+        if (c->state_host->active != 0) return IVP_ERR_INTERNAL;
+        pack_queued = true;
+        finish_queued = fin_next;
         opt_iter++;
-        IVP_TRY(iter_mark(c, opt_iter));
     }
     st.outer_iters = opt_iter;
     if (!prm->use_fixed_iters && err >= prm->tol_opt && opt_iter >= prm->max_iteration_opt) return IVP_NOT_CONVERGED;
