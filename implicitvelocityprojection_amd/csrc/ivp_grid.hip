@@ -127,12 +127,15 @@ __device__ __forceinline__ KeyRun key_run(int k, bool valid) {
     return r;
 }
 
-// SCENES: particle i uses the grid of its scene, grids[sid[i]]
+// SCENES: particle i uses the grid of its scene, grids[sid[i]].  clear_flag (if set) is zeroed:
+// the build's overflow flags, set only by passes after this one.
 template <bool SCENES>
 __global__ __launch_bounds__(BLOCK) void k_keys(int count, const float4* __restrict__ pos, Grid g,
                                                 const Grid* __restrict__ grids, const int* __restrict__ sid,
-                                                int* __restrict__ key, int* __restrict__ ccount) {
+                                                int* __restrict__ key, int* __restrict__ ccount,
+                                                int* __restrict__ clear_flag) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (clear_flag && i == 0) *clear_flag = 0;
     bool valid = i < count;
     int k = -1;
     if (valid) {
@@ -1217,12 +1220,14 @@ int grid_sort_population(ivp_ctx* c, int offset, int count, int* start_table, bo
     // bytes: keys (read x 16, write key 4), scatter (key 4 + slot 4), rank (key, uid, cell members
     // ~ 3 x 4), gather (index 4 + every per-particle array read and written)
     const double per = dynamic ? 4 + 2 * (16 + 12 + 4 + 4 + 4 + 4 + 4) + 4 : 4 + 2 * (16 + 4 + 4 + 4 + 4) + 4;
+    // the dynamic population's sort starts a rebuild: its first pass clears the overflow flags
+    int* clear = dynamic ? c->err_flag : nullptr;
     if (c->S > 0)
         IVP_TRY(launch(c, KC_SORT, -1, 24.0 * count, k_keys<true>, nb, count, (const float4*)(c->pos + offset), c->grid,
-                       (const Grid*)c->grids, (const int*)(c->sid + offset), c->key, c->ccount));
+                       (const Grid*)c->grids, (const int*)(c->sid + offset), c->key, c->ccount, clear));
     else
         IVP_TRY(launch(c, KC_SORT, -1, 20.0 * count, k_keys<false>, nb, count, (const float4*)(c->pos + offset),
-                       c->grid, (const Grid*)nullptr, (const int*)nullptr, c->key, c->ccount));
+                       c->grid, (const Grid*)nullptr, (const int*)nullptr, c->key, c->ccount, clear));
     IVP_TRY(scan(c, c->ccount, start_table, c->ncell));
     IVP_TRY(launch(c, KC_SORT, -1, 16.0 * count, k_scatter, nb, count, (const int*)c->key, (const int*)start_table,
                    c->ccount, (const int*)(c->uid + offset), c->tmp_src));
@@ -1370,8 +1375,8 @@ int grid_build(ivp_ctx* c) {
         if (rc != IVP_OK) return rc;
     } else {
         IVP_HIP(hipMemsetAsync(c->dstart, 0, sizeof(int) * (size_t)(c->ncell + 1), c->stream));
+        IVP_HIP(hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream));   // (else cleared by the sort)
     }
-    IVP_HIP(hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream));
     if (c->nd > 0) {
         // float h2_lo <= h^2 (1 - 2^-20) and h2_hi >= h^2 (1 + 2^-20): the predicate bounds of scan_ids
         const double hh = (double)c->h * (double)c->h;
